@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Throughput of the production SQA sweep on one MI355X.
+
+Workload (one timed step = one full anneal): plain Trotterized SQA on a dense SK Ising, N=512,
+J_ij ~ N(0, 1/N) symmetric with zero diagonal, P=32 Trotter slices, 256 replicas, beta=32, Gamma
+linear 3.0 -> 1e-3 over 2000 sweeps (one sweep per schedule step), FP32 local fields, seeded
+synthetic instance. Attempts per step = 2000 * 256 * 32 * 512 = 8.39e9 single-spin Metropolis
+attempts.
+
+Prints one JSON line: whole-job attempts/s, the kernel's HBM-roofline fraction (algorithmic bytes
+10.25 B/attempt: fields read+write 8 B, spins read+write 2 B, J once per sweep 0.25 B, over the
+hipEvent-timed kernel duration), and the reference's CPU path timed on this host's cores over a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, P, R, SWEEPS = 512, 32, 256, 2000
+BETA, G0, G1 = 32.0, 3.0, 1e-3
+ATTEMPTS = SWEEPS * R * P * N
+# algorithmic bytes per sweep: fields r+w (2*R*P*N*4) + spins r+w (2*R*P*N) + J once (N*N*4)
+BYTES_PER_SWEEP = 2 * R * P * N * 4 + 2 * R * P * N + N * N * 4
+BYTES_PER_LAUNCH = BYTES_PER_SWEEP * SWEEPS
+HBM_PEAK_GBS = 8000.0
+METRIC = "spin-flip attempts/s; time-to-99%-target energy vs plain GPU SQA, 1 GPU"
+
+
+def instance(seed: int = 20261203):
+    rng = np.random.default_rng(seed)
+    J = rng.normal(0.0, 1.0 / np.sqrt(N), (N, N))
+    J = np.triu(J, 1)
+    return J + J.T
+
+
+def cpu_baseline(J, betas, gammas, sample_steps: int):
+    """Reference build (or the oracle restatement when it is absent) on the host cores."""
+    from oracle.orc import Oracle, Problem, REF_PORTABLE_SO, ORACLE_SO
+
+    path, kind = (REF_PORTABLE_SO, "reference") if os.path.exists(REF_PORTABLE_SO) else (ORACLE_SO, "port")
+    cores = min(16, os.cpu_count() or 1)
+    lib = Oracle(path)
+    lib.set_threads(cores)
+    p = Problem.dense(np.zeros(N), J)
+    t = time.perf_counter()
+    lib.sqa_run(p, betas[:sample_steps], gammas[:sample_steps], P, R, 1, 0, 0, 0, 1)
+    dt = time.perf_counter() - t
+    att = sample_steps * R * P * N
+    return {
+        "value": att / dt, "unit": "attempts/s", "cores": cores, "kind": kind,
+        "sample": (f"same N=512 P=32 R=256 instance, first {sample_steps} of {SWEEPS} schedule steps "
+                   f"(slice sweeps + the reference's serial per-step FP64 slice energies), {dt:.1f} s"),
+    }
+
+
+def traffic_from_profile():
+    f = os.path.join(ROOT, "profiles", "sqa_fp32_c2_traffic.json")
+    if os.path.exists(f):
+        try:
+            return json.load(open(f)).get("hbm_bytes_per_launch")
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every run here has a single MI355X, and the sweep shards by "
+              "independent replicas/instances (see DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import qlatanneal_amd as qa
+
+    J = instance()
+    betas = np.full(SWEEPS, BETA)
+    gammas = np.linspace(G0, G1, SWEEPS)
+    ham = qa.DenseIsing(np.zeros(N), J, 0.0)
+    sched = qa.SQASchedule.from_vectors(betas.tolist(), gammas.tolist())
+    ann = qa.SQAAnnealer(ham, sched, P, R, backend="hip")
+    for w in range(args.warmup):
+        ann.set_seed(100 + w)
+        ann.run(1, 0)
+        print(f"warmup {w} done", file=sys.stderr, flush=True)
+    kms = []
+    best = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ann.set_seed(1000 + k)
+        res = ann.run(1, 0)  # blocks until the device run has completed
+        kms.append(ann.last_kernel_ms())
+        best.append(res.best_energy)
+        print(f"step {k}: kernel {kms[-1]:.1f} ms best {res.best_energy:.4f}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    acc = ann.last_accept_rate()
+    value = args.steps * ATTEMPTS / wall
+    kern_s = float(np.mean(kms)) / 1e3
+    achieved = BYTES_PER_LAUNCH / kern_s / 1e9
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SK instance, J_ij ~ N(0, 1/N))",
+        "config": {"workload": "sqa_dense_sk_n512_p32_r256_2000sweeps_fp32", "N": N, "trotter_slices": P,
+                   "replicas": R, "sweeps": SWEEPS, "beta": BETA, "gamma": [G0, G1],
+                   "attempts_per_step": ATTEMPTS, "accept_rate": acc,
+                   "kernel_ms_mean": kern_s * 1e3, "best_energy_per_spin_mean": float(np.mean(best)) / N},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile()},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(J, betas, gammas, args.cpu_sample_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/*
+ * qanneal_hip.h — C-ABI of the gfx950 annealing-sweep library (qlatanneal_amd/libqanneal_hip.so).
+ *
+ * This is the device boundary of the annealing-sweep path. The reference reaches its engines
+ * through per-spin virtual calls (include/qanneal/backend.hpp:39-53) from inside the run loops;
+ * here the boundary sits one level up, at whole runs: the C++ annealer classes
+ * (the include/qanneal headers of this repository) hand a complete run to one of the qa_*_run entry
+ * points. Signatures use plain pointers and sizes only. Every buffer is caller-allocated host
+ * memory; the library never returns memory it allocated.
+ *
+ * Modes. QA_MODE_FP32 is the production sweep (FP32 local fields, Philox-4x32-10 counter RNG,
+ * lockstep replica groups). QA_MODE_FP64_VALIDATION reproduces the reference CPU trajectory: FP64
+ * fields and energies in the reference's summation order, and the reference's random stream
+ * (std::mt19937_64 draws, consumed exactly where the reference consumes them).
+ *
+ * Threading: calls on one qa_ctx are serialized by the caller (one context per host thread);
+ * every call blocks until its device work has completed.
+ */
+#ifndef QANNEAL_HIP_H
+#define QANNEAL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum qa_status {
+    QA_OK = 0,
+    QA_EINVAL = 1,        /* invalid argument (reference: std::invalid_argument)        */
+    QA_ENOMEM = 2,        /* device allocation failed                                    */
+    QA_EDEVICE = 3,       /* HIP runtime / kernel failure (reference: std::runtime_error) */
+    QA_EUNSUPPORTED = 4   /* valid request this build does not run                       */
+} qa_status;
+
+typedef enum qa_mode { QA_MODE_FP32 = 0, QA_MODE_FP64_VALIDATION = 1 } qa_mode;
+
+typedef struct qa_ctx qa_ctx;
+typedef struct qa_problem qa_problem;
+
+/* Message for the last failing call on this host thread. */
+const char *qa_last_error(void);
+
+/* Replaces make_backend(kind, ham) (backend.hpp:92-110) for the device part: binds a GPU. */
+qa_status qa_ctx_create(int device, qa_ctx **out);
+qa_status qa_ctx_destroy(qa_ctx *ctx);
+
+/* Device copies of the problem models. Each replaces the Hamiltonian object the reference's
+ * CPUBackend wraps (backend.hpp:55-90):
+ *   dense  — DenseIsing(h, J, n, c)          dense_ising.hpp:17-21; J row-major n*n
+ *   sparse — SparseIsing(h, edges, n, c)     sparse_ising.hpp:21-25; edges in caller order
+ *   hubo   — HigherOrderIsing after its ctor's consolidation (higher_order_ising.cpp:10-90):
+ *            linear h[n], terms as CSR (term_offset[nterms+1], term_vars), term_coeff[nterms].
+ * The problem is uploaded once and reused by every run. */
+qa_status qa_problem_dense(qa_ctx *ctx, int64_t n, const double *h, const double *J, double c,
+                           qa_problem **out);
+qa_status qa_problem_sparse(qa_ctx *ctx, int64_t n, const double *h, int64_t nedges,
+                            const int64_t *ei, const int64_t *ej, const double *ev, double c,
+                            qa_problem **out);
+qa_status qa_problem_hubo(qa_ctx *ctx, int64_t n, const double *h, int64_t nterms,
+                          const int64_t *term_offset, const uint32_t *term_vars,
+                          const double *term_coeff, double c, qa_problem **out);
+qa_status qa_problem_destroy(qa_problem *p);
+
+/* FP64 energies of `count` states (spins [count*n] in {-1,+1}), computed on the device in the
+ * reference's summation order, so the result equals Hamiltonian::energy bit for bit
+ * (dense_ising.cpp:29-44, sparse_ising.cpp:43-56, higher_order_ising.cpp:92-111). */
+qa_status qa_energy(const qa_problem *p, int64_t count, const int8_t *spins, double *out);
+
+/* ---- SA: replaces Annealer::run (annealer.cpp:33-75), `reads` independent chains ---------- */
+typedef struct qa_sa_params {
+    int64_t steps;               /* schedule length                                          */
+    const double *betas;         /* [steps]                                                  */
+    int64_t sweeps_per_beta;
+    int64_t reads;               /* independent chains run together                          */
+    int32_t mode;                /* qa_mode                                                  */
+    uint64_t seed;               /* FP32: Philox key                                         */
+    const int8_t *init_spins;    /* validation: [reads*n], State::random draws               */
+    const double *uniforms;      /* validation: [reads*uniforms_per_read], the engine's
+                                    uniform_real(0,1) stream after the init draws            */
+    int64_t uniforms_per_read;
+} qa_sa_params;
+
+typedef struct qa_sa_result {
+    int8_t *best_states;         /* [reads*n]                                                */
+    double *best_energies;       /* [reads]                                                  */
+    double *energy_trace;        /* [reads*steps] energy at the end of each step             */
+    int64_t *uniforms_used;      /* validation: [reads] (may be NULL)                        */
+    double *kernel_ms;           /* optional: device time of the sweep kernel                */
+} qa_sa_result;
+
+qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
+
+/* ---- SQA: replaces SQAAnnealer::run (sqa_annealer.cpp:87-477) ----------------------------- */
+typedef struct qa_sqa_params {
+    int64_t steps;
+    const double *betas;         /* [steps]                                                  */
+    const double *gammas;        /* [steps]                                                  */
+    int64_t trotter_slices;      /* P actually used (caller resolves continuous_time_slices) */
+    int64_t replicas;
+    int64_t sweeps_per_beta;
+    int64_t worldline_sweeps;
+    int64_t cluster_sweeps;
+    int32_t mode;                /* qa_mode                                                  */
+    uint64_t seed;               /* FP32: Philox key                                         */
+    const int8_t *init_spins;    /* validation: [replicas*P*n], SQAState::random draws       */
+    const uint64_t *replica_seeds; /* validation: [replicas], the per-replica engine seeds   */
+} qa_sqa_params;
+
+typedef struct qa_sqa_result {
+    int8_t *best_state;          /* [n]                                                      */
+    double *best_energy;         /* FP64 energy of best_state (recomputed on the device)     */
+    double *energy_trace;        /* [steps] mean slice energy at each step end               */
+    double *kernel_ms;           /* optional: device time of the sweep kernel(s)             */
+    double *accept_rate;         /* optional: accepted / attempted slice moves (FP32 mode)   */
+} qa_sqa_result;
+
+qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
+                     qa_sqa_result *res);
+
+/* ---- SQA parallel tempering: replaces SQAParallelTemperingAnnealer::run
+ *      (sqa_parallel_tempering.cpp:81-346) ------------------------------------------------- */
+typedef struct qa_sqapt_params {
+    int64_t rungs;               /* ladder length                                            */
+    const double *betas;         /* [rungs]                                                  */
+    const double *gammas;        /* [rungs]                                                  */
+    int64_t ladders;             /* independent ladders run together (FP32 mode)             */
+    int64_t trotter_slices;
+    int64_t sweeps_per_step;
+    int64_t worldline_sweeps;
+    int64_t cluster_sweeps;
+    int64_t steps;
+    int64_t swap_interval;
+    int32_t mode;
+    uint64_t seed;
+    const int8_t *init_spins;    /* validation: [rungs*P*n]                                  */
+    const uint64_t *rung_seeds;  /* validation: [rungs]                                      */
+    const double *swap_uniforms; /* validation: master-engine uniform stream for swaps       */
+    int64_t swap_uniforms_len;
+} qa_sqapt_params;
+
+typedef struct qa_sqapt_result {
+    int8_t *final_states;        /* [ladders*rungs*n] best slice per rung at the end         */
+    double *final_energies;      /* [ladders*rungs]                                          */
+    int8_t *best_state;          /* [ladders*n]                                              */
+    double *best_energy;         /* [ladders]                                                */
+    double *average_energy_trace;/* [ladders*steps]                                          */
+    double *swap_acceptance_trace;/* [ladders*steps]                                         */
+    int64_t *swap_uniforms_used; /* validation (may be NULL)                                 */
+    double *kernel_ms;
+} qa_sqapt_result;
+
+qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm,
+                       qa_sqapt_result *res);
+
+/* ---- sqa_chi: replaces SQAChiAnnealer::run_chi (sqa_chi_annealer.cpp:146-438) ------------ */
+typedef struct qa_chi_params {
+    int64_t trotter_slices;
+    int64_t replicas;
+    double beta, gamma_start, gamma_end;
+    int64_t num_steps, sweeps_per_step, scan_points, scan_sweeps, scan_burn;
+    double chi_floor_fraction, driver_A0, beta_ramp_fraction, beta_ramp_start;
+    int32_t mode;
+    uint64_t seed;
+    /* validation: the master-engine state is consumed by the library; the caller passes the
+     * seeds it would draw: [scan_points+1] fresh worldlines of R*P*n init spins and R*P cell
+     * seeds each, in draw order. */
+    const int8_t *init_spins;    /* [(scan_points+1)*R*P*n]                                   */
+    const uint64_t *cell_seeds;  /* [(scan_points+1)*R*P]                                     */
+    /* FP32 mode: when non-NULL the pilot scan is skipped and these chi_B values are used. */
+    const double *chi_override;  /* [scan_points]                                            */
+} qa_chi_params;
+
+typedef struct qa_chi_result {
+    int8_t *best_state;
+    double *best_energy;
+    double *energy_trace, *beta_schedule, *gamma_schedule, *s_schedule;   /* [num_steps]    */
+    double *scan_s, *scan_gamma, *scan_chi_B;                             /* [scan_points]  */
+    double *scalars;             /* [s_star, gamma_star, j_perp_star, chi_floor, driver_A0]  */
+    double *kernel_ms;
+} qa_chi_result;
+
+qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm,
+                         qa_chi_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""ctypes binding of the C-ABI in include/qanneal_hip.h (libqanneal_hip.so).
+
+This is the binding a Python caller of the reference would add to reach the device path without
+the C++ classes (INTEGRATION.md shows it beside the pybind11 route). It fails loudly when the
+library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libqanneal_hip.so")
+
+QA_OK, QA_EINVAL, QA_ENOMEM, QA_EDEVICE, QA_EUNSUPPORTED = range(5)
+MODE_FP32, MODE_FP64_VALIDATION = 0, 1
+
+_i64, _u64, _i32, _dbl = C.c_int64, C.c_uint64, C.c_int32, C.c_double
+_dp, _bp, _lp, _up, _u32p = (C.POINTER(C.c_double), C.POINTER(C.c_int8), C.POINTER(C.c_int64),
+                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint32))
+
+
+class SaParams(C.Structure):
+    _fields_ = [("steps", _i64), ("betas", _dp), ("sweeps_per_beta", _i64), ("reads", _i64),
+                ("mode", _i32), ("seed", _u64), ("init_spins", _bp), ("uniforms", _dp),
+                ("uniforms_per_read", _i64)]
+
+
+class SaResult(C.Structure):
+    _fields_ = [("best_states", _bp), ("best_energies", _dp), ("energy_trace", _dp),
+                ("uniforms_used", _lp), ("kernel_ms", _dp)]
+
+
+class SqaParams(C.Structure):
+    _fields_ = [("steps", _i64), ("betas", _dp), ("gammas", _dp), ("trotter_slices", _i64),
+                ("replicas", _i64), ("sweeps_per_beta", _i64), ("worldline_sweeps", _i64),
+                ("cluster_sweeps", _i64), ("mode", _i32), ("seed", _u64), ("init_spins", _bp),
+                ("replica_seeds", _up)]
+
+
+class SqaResult(C.Structure):
+    _fields_ = [("best_state", _bp), ("best_energy", _dp), ("energy_trace", _dp),
+                ("kernel_ms", _dp), ("accept_rate", _dp)]
+
+
+class SqaptParams(C.Structure):
+    _fields_ = [("rungs", _i64), ("betas", _dp), ("gammas", _dp), ("ladders", _i64),
+                ("trotter_slices", _i64), ("sweeps_per_step", _i64), ("worldline_sweeps", _i64),
+                ("cluster_sweeps", _i64), ("steps", _i64), ("swap_interval", _i64), ("mode", _i32),
+                ("seed", _u64), ("init_spins", _bp), ("rung_seeds", _up), ("swap_uniforms", _dp),
+                ("swap_uniforms_len", _i64)]
+
+
+class SqaptResult(C.Structure):
+    _fields_ = [("final_states", _bp), ("final_energies", _dp), ("best_state", _bp),
+                ("best_energy", _dp), ("average_energy_trace", _dp), ("swap_acceptance_trace", _dp),
+                ("swap_uniforms_used", _lp), ("kernel_ms", _dp)]
+
+
+class ChiParams(C.Structure):
+    _fields_ = [("trotter_slices", _i64), ("replicas", _i64), ("beta", _dbl), ("gamma_start", _dbl),
+                ("gamma_end", _dbl), ("num_steps", _i64), ("sweeps_per_step", _i64),
+                ("scan_points", _i64), ("scan_sweeps", _i64), ("scan_burn", _i64),
+                ("chi_floor_fraction", _dbl), ("driver_A0", _dbl), ("beta_ramp_fraction", _dbl),
+                ("beta_ramp_start", _dbl), ("mode", _i32), ("seed", _u64), ("init_spins", _bp),
+                ("cell_seeds", _up), ("chi_override", _dp)]
+
+
+class ChiResult(C.Structure):
+    _fields_ = [("best_state", _bp), ("best_energy", _dp), ("energy_trace", _dp),
+                ("beta_schedule", _dp), ("gamma_schedule", _dp), ("s_schedule", _dp),
+                ("scan_s", _dp), ("scan_gamma", _dp), ("scan_chi_B", _dp), ("scalars", _dp),
+                ("kernel_ms", _dp)]
+
+
+class QaError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+class QaInvalidArgument(QaError, ValueError):
+    pass
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built (run __graft_entry__.build()); "
+                              "qlatanneal_amd has no CPU fallback for the annealing sweep")
+        L = C.CDLL(LIB_PATH)
+        L.qa_last_error.restype = C.c_char_p
+        for name in ("qa_ctx_create", "qa_ctx_destroy", "qa_problem_dense", "qa_problem_sparse",
+                     "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_sa_run", "qa_sqa_run",
+                     "qa_sqapt_run", "qa_sqa_chi_run"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != QA_OK:
+        msg = lib().qa_last_error().decode()
+        if status == QA_EINVAL:
+            raise QaInvalidArgument(status, msg)
+        raise QaError(status, msg)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _b(a):
+    return a.ctypes.data_as(_bp) if a is not None else None
+
+
+class Context:
+    """One GPU (qa_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().qa_ctx_create(C.c_int(device), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qa_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceProblem:
+    """A problem model uploaded once (qa_problem)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, n: int):
+        self.ctx, self.h, self.n = ctx, handle, n
+
+    @staticmethod
+    def dense(ctx: Context, h, J, c: float = 0.0) -> "DeviceProblem":
+        h = np.ascontiguousarray(h, np.float64)
+        J = np.ascontiguousarray(J, np.float64)
+        n = h.shape[0]
+        if J.shape != (n, n):
+            raise ValueError("J must be n x n")
+        out = C.c_void_p()
+        check(lib().qa_problem_dense(ctx.h, _i64(n), _d(h), _d(J), _dbl(c), C.byref(out)))
+        return DeviceProblem(ctx, out, n)
+
+    @staticmethod
+    def sparse(ctx: Context, h, edges, c: float = 0.0) -> "DeviceProblem":
+        h = np.ascontiguousarray(h, np.float64)
+        n = h.shape[0]
+        ei = np.array([int(e[0]) for e in edges], np.int64)
+        ej = np.array([int(e[1]) for e in edges], np.int64)
+        ev = np.array([float(e[2]) for e in edges], np.float64)
+        out = C.c_void_p()
+        check(lib().qa_problem_sparse(ctx.h, _i64(n), _d(h), _i64(len(edges)), ei.ctypes.data_as(_lp),
+                                      ej.ctypes.data_as(_lp), _d(ev), _dbl(c), C.byref(out)))
+        return DeviceProblem(ctx, out, n)
+
+    @staticmethod
+    def hubo(ctx: Context, n: int, h, term_offset, term_vars, term_coeff, c: float = 0.0) -> "DeviceProblem":
+        h = np.ascontiguousarray(h, np.float64)
+        off = np.ascontiguousarray(term_offset, np.int64)
+        tv = np.ascontiguousarray(term_vars if len(term_vars) else [0], np.uint32)
+        tc = np.ascontiguousarray(term_coeff if len(term_coeff) else [0.0], np.float64)
+        out = C.c_void_p()
+        check(lib().qa_problem_hubo(ctx.h, _i64(n), _d(h), _i64(len(off) - 1), off.ctypes.data_as(_lp),
+                                    tv.ctypes.data_as(_u32p), _d(tc), _dbl(c), C.byref(out)))
+        return DeviceProblem(ctx, out, n)
+
+    def energy(self, spins) -> np.ndarray:
+        s = np.ascontiguousarray(spins, np.int8).reshape(-1, self.n)
+        out = np.zeros(s.shape[0])
+        check(lib().qa_energy(self.h, _i64(s.shape[0]), _b(s), _d(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().qa_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sa_run(prob: DeviceProblem, betas, sweeps_per_beta: int, reads: int = 1, mode: int = MODE_FP32,
+           seed: int = 0, init_spins=None, uniforms=None):
+    betas = np.ascontiguousarray(betas, np.float64)
+    steps, n = betas.shape[0], prob.n
+    p = SaParams(steps, _d(betas), sweeps_per_beta, reads, mode, seed & (2**64 - 1))
+    keep = []
+    if init_spins is not None:
+        init_spins = np.ascontiguousarray(init_spins, np.int8)
+        keep.append(init_spins)
+        p.init_spins = _b(init_spins)
+    if uniforms is not None:
+        uniforms = np.ascontiguousarray(uniforms, np.float64).reshape(reads, -1)
+        keep.append(uniforms)
+        p.uniforms = _d(uniforms)
+        p.uniforms_per_read = uniforms.shape[1]
+    bs, be, tr = np.zeros((reads, n), np.int8), np.zeros(reads), np.zeros((reads, steps))
+    used, ms = np.zeros(reads, np.int64), np.zeros(1)
+    r = SaResult(_b(bs), _d(be), _d(tr), used.ctypes.data_as(_lp), _d(ms))
+    check(lib().qa_sa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
+    return dict(best_states=bs, best_energies=be, energy_traces=tr, uniforms_used=used, kernel_ms=ms[0])
+
+
+def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: int, sweeps_per_beta: int,
+            worldline_sweeps: int = 0, cluster_sweeps: int = 0, mode: int = MODE_FP32, seed: int = 0,
+            init_spins=None, replica_seeds=None):
+    betas = np.ascontiguousarray(betas, np.float64)
+    gammas = np.ascontiguousarray(gammas, np.float64)
+    steps, n = betas.shape[0], prob.n
+    p = SqaParams(steps, _d(betas), _d(gammas), trotter_slices, replicas, sweeps_per_beta,
+                  worldline_sweeps, cluster_sweeps, mode, seed & (2**64 - 1))
+    keep = []
+    if init_spins is not None:
+        init_spins = np.ascontiguousarray(init_spins, np.int8)
+        keep.append(init_spins)
+        p.init_spins = _b(init_spins)
+    if replica_seeds is not None:
+        replica_seeds = np.ascontiguousarray(replica_seeds, np.uint64)
+        keep.append(replica_seeds)
+        p.replica_seeds = replica_seeds.ctypes.data_as(_up)
+    bs, be, tr, ms, ar = np.zeros(n, np.int8), np.zeros(1), np.zeros(steps), np.zeros(1), np.zeros(1)
+    r = SqaResult(_b(bs), _d(be), _d(tr), _d(ms), _d(ar))
+    check(lib().qa_sqa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
+    return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, kernel_ms=float(ms[0]),
+                accept_rate=float(ar[0]))

@@ -1,0 +1,494 @@
+// Annealer classes: argument checks as in the reference, then one whole run on the device through
+// the C-ABI (include/qanneal_hip.h). For FP64 validation runs the host draws exactly what the
+// reference's host code draws from the annealer's std::mt19937_64 (initial states, per-replica
+// seeds, uniform streams) and advances the engine by what the device consumed, so consecutive
+// runs and set_seed behave as in the reference.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <random>
+#include <stdexcept>
+
+#include "qanneal/annealer.hpp"
+#include "qanneal/sqa_annealer.hpp"
+#include "qanneal/sqa_chi_annealer.hpp"
+#include "qanneal/sqa_parallel_tempering.hpp"
+#include "qanneal_hip.h"
+
+namespace qanneal {
+
+namespace {
+
+HIPBackend &device_backend(const std::shared_ptr<Backend> &b) {
+    if (!b->is_device())
+        throw std::runtime_error(
+            "this build runs the annealing sweep on the GPU only: construct the annealer with "
+            "backend='hip' (or 'hip_fp64' for the FP64 validation replay)");
+    return static_cast<HIPBackend &>(*b);
+}
+
+int mode_of(const HIPBackend &b) {
+    return b.precision() == DevicePrecision::FP32 ? QA_MODE_FP32 : QA_MODE_FP64_VALIDATION;
+}
+
+std::vector<double> uniform_stream(const std::mt19937_64 &eng, std::size_t count) {
+    std::mt19937_64 copy = eng;
+    std::uniform_real_distribution<double> u(0.0, 1.0);
+    std::vector<double> out(count);
+    for (auto &v : out) v = u(copy);
+    return out;
+}
+
+State state_from(const int8_t *p, std::size_t n) {
+    State s(n);
+    std::memcpy(s.spins.data(), p, n);
+    return s;
+}
+
+}  // namespace
+
+// ======================================================================================= SA
+Annealer::Annealer(const Hamiltonian &hamiltonian, AnnealSchedule schedule)
+    : backend_(make_backend(BackendKind::HIP, hamiltonian)), schedule_(std::move(schedule)),
+      rng_(std::random_device{}()) {
+    if (schedule_.betas.empty()) throw std::invalid_argument("Schedule must contain at least one beta.");
+}
+
+Annealer::Annealer(std::shared_ptr<Backend> backend, AnnealSchedule schedule)
+    : backend_(std::move(backend)), schedule_(std::move(schedule)), rng_(std::random_device{}()) {
+    if (!backend_) throw std::invalid_argument("Annealer requires a backend.");
+    if (schedule_.betas.empty()) throw std::invalid_argument("Schedule must contain at least one beta.");
+}
+
+void Annealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
+
+AnnealResult Annealer::run(std::size_t sweeps_per_beta, Observer *observer) {
+    if (sweeps_per_beta == 0) throw std::invalid_argument("sweeps_per_beta must be > 0.");
+    if (observer) throw std::invalid_argument("observers are not called by device runs");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), steps = schedule_.size();
+    qa_sa_params prm{};
+    prm.steps = static_cast<int64_t>(steps);
+    prm.betas = schedule_.betas.data();
+    prm.sweeps_per_beta = static_cast<int64_t>(sweeps_per_beta);
+    prm.reads = 1;
+    prm.mode = mode_of(hb);
+    State init;
+    std::vector<double> uni;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng_();
+    } else {
+        init = State::random(n, rng_);  // annealer.cpp:40
+        uni = uniform_stream(rng_, steps * sweeps_per_beta * n);
+        prm.init_spins = init.spins.data();
+        prm.uniforms = uni.data();
+        prm.uniforms_per_read = static_cast<int64_t>(uni.size());
+    }
+    AnnealResult res;
+    res.best_state = State(n);
+    res.energy_trace.resize(steps);
+    int64_t used = 0;
+    qa_sa_result out{res.best_state.spins.data(), &res.best_energy, res.energy_trace.data(), &used, nullptr};
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    if (prm.mode != QA_MODE_FP32) rng_.discard(static_cast<unsigned long long>(used));
+    return res;
+}
+
+std::vector<AnnealResult> Annealer::run_reads(std::size_t sweeps_per_beta, std::size_t reads, std::uint64_t seed) {
+    if (sweeps_per_beta == 0) throw std::invalid_argument("sweeps_per_beta must be > 0.");
+    if (reads == 0) throw std::invalid_argument("reads must be > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), steps = schedule_.size();
+    qa_sa_params prm{};
+    prm.steps = static_cast<int64_t>(steps);
+    prm.betas = schedule_.betas.data();
+    prm.sweeps_per_beta = static_cast<int64_t>(sweeps_per_beta);
+    prm.reads = static_cast<int64_t>(reads);
+    prm.mode = mode_of(hb);
+    prm.seed = seed;
+    std::vector<int8_t> init;
+    std::vector<double> uni;
+    if (prm.mode != QA_MODE_FP32) {
+        const std::size_t per = steps * sweeps_per_beta * n;
+        init.resize(reads * n);
+        uni.resize(reads * per);
+        for (std::size_t r = 0; r < reads; ++r) {
+            std::mt19937_64 eng(seed + r);
+            State s = State::random(n, eng);
+            std::memcpy(&init[r * n], s.spins.data(), n);
+            auto u = uniform_stream(eng, per);
+            std::memcpy(&uni[r * per], u.data(), per * sizeof(double));
+        }
+        prm.init_spins = init.data();
+        prm.uniforms = uni.data();
+        prm.uniforms_per_read = static_cast<int64_t>(per);
+    }
+    std::vector<int8_t> best(reads * n);
+    std::vector<double> be(reads), tr(reads * steps);
+    qa_sa_result out{best.data(), be.data(), tr.data(), nullptr, nullptr};
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    std::vector<AnnealResult> all(reads);
+    for (std::size_t r = 0; r < reads; ++r) {
+        all[r].best_state = state_from(&best[r * n], n);
+        all[r].best_energy = be[r];
+        all[r].energy_trace.assign(tr.begin() + r * steps, tr.begin() + (r + 1) * steps);
+    }
+    return all;
+}
+
+// ====================================================================================== SQA
+namespace {
+void check_sqa_ctor(const SQASchedule &s, std::size_t slices, std::size_t replicas) {
+    if (s.betas.empty()) throw std::invalid_argument("SQA schedule must contain betas.");
+    if (s.betas.size() != s.gammas.size()) throw std::invalid_argument("SQA schedule betas/gammas length mismatch.");
+    if (slices == 0) throw std::invalid_argument("trotter_slices must be > 0.");
+    if (replicas == 0) throw std::invalid_argument("replicas must be > 0.");
+}
+}  // namespace
+
+SQAAnnealer::SQAAnnealer(const Hamiltonian &hamiltonian, SQASchedule schedule, std::size_t trotter_slices,
+                         std::size_t replicas)
+    : backend_(make_backend(BackendKind::HIP, hamiltonian)), schedule_(std::move(schedule)),
+      slices_(trotter_slices), replicas_(replicas), rng_(std::random_device{}()) {
+    check_sqa_ctor(schedule_, slices_, replicas_);
+}
+
+SQAAnnealer::SQAAnnealer(std::shared_ptr<Backend> backend, SQASchedule schedule, std::size_t trotter_slices,
+                         std::size_t replicas)
+    : backend_(std::move(backend)), schedule_(std::move(schedule)), slices_(trotter_slices),
+      replicas_(replicas), rng_(std::random_device{}()) {
+    if (!backend_) throw std::invalid_argument("SQAAnnealer requires a backend.");
+    check_sqa_ctor(schedule_, slices_, replicas_);
+}
+
+void SQAAnnealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
+
+SQAResult SQAAnnealer::run(std::size_t sweeps_per_beta, std::size_t worldline_sweeps, std::size_t cluster_sweeps,
+                           std::size_t continuous_time_slices, SQAObserver *observer) {
+    if (sweeps_per_beta == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+        throw std::invalid_argument(
+            "At least one of sweeps_per_beta, worldline_sweeps, or cluster_sweeps must be > 0.");
+    if (observer) throw std::invalid_argument("observers are not called by device runs");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size();
+    const std::size_t P = continuous_time_slices > slices_ ? continuous_time_slices : slices_;  // :97-99
+    const std::size_t R = replicas_, steps = schedule_.size();
+    qa_sqa_params prm{};
+    prm.steps = static_cast<int64_t>(steps);
+    prm.betas = schedule_.betas.data();
+    prm.gammas = schedule_.gammas.data();
+    prm.trotter_slices = static_cast<int64_t>(P);
+    prm.replicas = static_cast<int64_t>(R);
+    prm.sweeps_per_beta = static_cast<int64_t>(sweeps_per_beta);
+    prm.worldline_sweeps = static_cast<int64_t>(worldline_sweeps);
+    prm.cluster_sweeps = static_cast<int64_t>(cluster_sweeps);
+    prm.mode = mode_of(hb);
+    SQAState init;
+    std::vector<uint64_t> seeds;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng_();
+    } else {
+        init = SQAState::random(R, P, n, rng_);  // :100
+        seeds.resize(R);
+        for (auto &s : seeds) s = rng_();        // :108-111
+        prm.init_spins = init.data();
+        prm.replica_seeds = seeds.data();
+    }
+    SQAResult res;
+    res.best_state = State(n);
+    res.energy_trace.resize(steps);
+    double kms = 0.0, acc = 0.0;
+    qa_sqa_result out{res.best_state.spins.data(), &res.best_energy, res.energy_trace.data(), &kms, &acc};
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sqa_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    last_kernel_ms_ = kms;
+    last_accept_rate_ = acc;
+    return res;
+}
+
+SQAResult SQAAnnealer::run_optimal(double, double, double, double, double, std::size_t, std::size_t, std::size_t,
+                                   std::size_t, std::size_t, std::size_t, const std::string &, double, double) {
+    throw std::runtime_error("run_optimal (the bond-susceptibility adaptive schedule) is not in this build");
+}
+
+// ================================================================================== SQA-PT
+namespace {
+void check_pt_ctor(const std::vector<double> &b, const std::vector<double> &g, std::size_t slices) {
+    if (b.size() < 2 || b.size() != g.size())
+        throw std::invalid_argument("SQA parallel tempering requires >=2 beta/gamma pairs.");
+    if (slices == 0) throw std::invalid_argument("trotter_slices must be > 0.");
+}
+}  // namespace
+
+SQAParallelTemperingAnnealer::SQAParallelTemperingAnnealer(const Hamiltonian &hamiltonian, std::vector<double> betas,
+                                                           std::vector<double> gammas, std::size_t trotter_slices)
+    : backend_(make_backend(BackendKind::HIP, hamiltonian)), betas_(std::move(betas)), gammas_(std::move(gammas)),
+      slices_(trotter_slices), rng_(std::random_device{}()) {
+    check_pt_ctor(betas_, gammas_, slices_);
+}
+
+SQAParallelTemperingAnnealer::SQAParallelTemperingAnnealer(std::shared_ptr<Backend> backend,
+                                                           std::vector<double> betas, std::vector<double> gammas,
+                                                           std::size_t trotter_slices)
+    : backend_(std::move(backend)), betas_(std::move(betas)), gammas_(std::move(gammas)), slices_(trotter_slices),
+      rng_(std::random_device{}()) {
+    if (!backend_) throw std::invalid_argument("SQAParallelTemperingAnnealer requires a backend.");
+    check_pt_ctor(betas_, gammas_, slices_);
+}
+
+void SQAParallelTemperingAnnealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
+
+SQAParallelTemperingResult SQAParallelTemperingAnnealer::run(std::size_t sweeps_per_step,
+                                                             std::size_t worldline_sweeps, std::size_t steps,
+                                                             std::size_t swap_interval, std::size_t cluster_sweeps,
+                                                             std::size_t continuous_time_slices) {
+    if (steps == 0) throw std::invalid_argument("steps must be > 0.");
+    if (swap_interval == 0) throw std::invalid_argument("swap_interval must be > 0.");
+    if (sweeps_per_step == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+        throw std::invalid_argument("At least one of sweeps_per_step/worldline_sweeps/cluster_sweeps must be > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), R = betas_.size();
+    const std::size_t P = continuous_time_slices > slices_ ? continuous_time_slices : slices_;
+    qa_sqapt_params prm{};
+    prm.rungs = static_cast<int64_t>(R);
+    prm.betas = betas_.data();
+    prm.gammas = gammas_.data();
+    prm.ladders = 1;
+    prm.trotter_slices = static_cast<int64_t>(P);
+    prm.sweeps_per_step = static_cast<int64_t>(sweeps_per_step);
+    prm.worldline_sweeps = static_cast<int64_t>(worldline_sweeps);
+    prm.cluster_sweeps = static_cast<int64_t>(cluster_sweeps);
+    prm.steps = static_cast<int64_t>(steps);
+    prm.swap_interval = static_cast<int64_t>(swap_interval);
+    prm.mode = mode_of(hb);
+    std::vector<int8_t> init;
+    std::vector<uint64_t> seeds;
+    std::vector<double> swap_u;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng_();
+    } else {
+        init.resize(R * P * n);
+        for (std::size_t r = 0; r < R; ++r) {  // sqa_parallel_tempering.cpp:103-107
+            SQAState s = SQAState::random(1, P, n, rng_);
+            std::memcpy(&init[r * P * n], s.data(), P * n);
+        }
+        seeds.resize(R);
+        for (auto &s : seeds) s = rng_();  // :109-112
+        swap_u = uniform_stream(rng_, steps * (R - 1));
+        prm.init_spins = init.data();
+        prm.rung_seeds = seeds.data();
+        prm.swap_uniforms = swap_u.data();
+        prm.swap_uniforms_len = static_cast<int64_t>(swap_u.size());
+    }
+    std::vector<int8_t> fs(R * n), bs(n);
+    std::vector<double> fe(R), at(steps), st(steps);
+    double be = 0.0;
+    int64_t used = 0;
+    qa_sqapt_result out{fs.data(), fe.data(), bs.data(), &be, at.data(), st.data(), &used, nullptr};
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sqapt_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    if (prm.mode != QA_MODE_FP32) rng_.discard(static_cast<unsigned long long>(used));
+    SQAParallelTemperingResult res;
+    for (std::size_t r = 0; r < R; ++r) res.final_states.push_back(state_from(&fs[r * n], n));
+    res.final_energies = fe;
+    res.best_state = state_from(bs.data(), n);
+    res.best_energy = be;
+    res.average_energy_trace = at;
+    res.swap_acceptance_trace = st;
+    return res;
+}
+
+std::vector<SQAParallelTemperingResult> SQAParallelTemperingAnnealer::run_ladders(
+    std::size_t ladders, std::size_t sweeps_per_step, std::size_t worldline_sweeps, std::size_t steps,
+    std::size_t swap_interval, std::size_t cluster_sweeps) {
+    if (ladders == 0) throw std::invalid_argument("ladders must be > 0.");
+    if (steps == 0) throw std::invalid_argument("steps must be > 0.");
+    if (swap_interval == 0) throw std::invalid_argument("swap_interval must be > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    if (hb.precision() != DevicePrecision::FP32)
+        throw std::invalid_argument("run_ladders is an FP32-mode entry point");
+    const std::size_t n = hb.size(), R = betas_.size(), L = ladders;
+    qa_sqapt_params prm{};
+    prm.rungs = static_cast<int64_t>(R);
+    prm.betas = betas_.data();
+    prm.gammas = gammas_.data();
+    prm.ladders = static_cast<int64_t>(L);
+    prm.trotter_slices = static_cast<int64_t>(slices_);
+    prm.sweeps_per_step = static_cast<int64_t>(sweeps_per_step);
+    prm.worldline_sweeps = static_cast<int64_t>(worldline_sweeps);
+    prm.cluster_sweeps = static_cast<int64_t>(cluster_sweeps);
+    prm.steps = static_cast<int64_t>(steps);
+    prm.swap_interval = static_cast<int64_t>(swap_interval);
+    prm.mode = QA_MODE_FP32;
+    prm.seed = rng_();
+    std::vector<int8_t> fs(L * R * n), bs(L * n);
+    std::vector<double> fe(L * R), be(L), at(L * steps), st(L * steps);
+    qa_sqapt_result out{fs.data(), fe.data(), bs.data(), be.data(), at.data(), st.data(), nullptr, nullptr};
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sqapt_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    std::vector<SQAParallelTemperingResult> all(L);
+    for (std::size_t l = 0; l < L; ++l) {
+        auto &r = all[l];
+        for (std::size_t q = 0; q < R; ++q) r.final_states.push_back(state_from(&fs[(l * R + q) * n], n));
+        r.final_energies.assign(fe.begin() + l * R, fe.begin() + (l + 1) * R);
+        r.best_state = state_from(&bs[l * n], n);
+        r.best_energy = be[l];
+        r.average_energy_trace.assign(at.begin() + l * steps, at.begin() + (l + 1) * steps);
+        r.swap_acceptance_trace.assign(st.begin() + l * steps, st.begin() + (l + 1) * steps);
+    }
+    return all;
+}
+
+SQAParallelTemperingResult SQAParallelTemperingAnnealer::run_optimal(std::size_t, std::size_t, std::size_t, double,
+                                                                     double, double, std::size_t, std::size_t,
+                                                                     std::size_t, std::size_t, std::size_t,
+                                                                     const std::string &) {
+    throw std::runtime_error("run_optimal (the adaptive shared-J_perp schedule) is not in this build");
+}
+
+// ================================================================================= sqa_chi
+SQAChiAnnealer::SQAChiAnnealer(const Hamiltonian &hamiltonian, std::size_t trotter_slices, std::size_t replicas)
+    : backend_(make_backend(BackendKind::HIP, hamiltonian)), slices_(trotter_slices), replicas_(replicas),
+      rng_(std::random_device{}()) {
+    if (slices_ == 0) throw std::invalid_argument("trotter_slices must be > 0.");
+    if (replicas_ == 0) throw std::invalid_argument("replicas must be > 0.");
+}
+
+SQAChiAnnealer::SQAChiAnnealer(std::shared_ptr<Backend> backend, std::size_t trotter_slices, std::size_t replicas)
+    : backend_(std::move(backend)), slices_(trotter_slices), replicas_(replicas), rng_(std::random_device{}()) {
+    if (!backend_) throw std::invalid_argument("SQAChiAnnealer requires a backend.");
+    if (slices_ == 0) throw std::invalid_argument("trotter_slices must be > 0.");
+    if (replicas_ == 0) throw std::invalid_argument("replicas must be > 0.");
+}
+
+void SQAChiAnnealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
+
+namespace {
+SQAChiResult chi_run(HIPBackend &hb, std::mt19937_64 &rng, std::size_t P, std::size_t R, qa_chi_params prm) {
+    const std::size_t n = hb.size();
+    const std::size_t NS = static_cast<std::size_t>(prm.num_steps), SP = static_cast<std::size_t>(prm.scan_points);
+    std::vector<int8_t> init;
+    std::vector<uint64_t> seeds;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng();
+    } else {
+        // every fresh worldline draws R*P*n spins then R*P cell seeds (sqa_chi_annealer.cpp:228-240,
+        // 394-405): scan_points pilot worldlines, then the production one
+        const std::size_t blocks = SP + 1;
+        init.resize(blocks * R * P * n);
+        seeds.resize(blocks * R * P);
+        for (std::size_t b = 0; b < blocks; ++b) {
+            SQAState s = SQAState::random(R, P, n, rng);
+            std::memcpy(&init[b * R * P * n], s.data(), R * P * n);
+            for (std::size_t k = 0; k < R * P; ++k) seeds[b * R * P + k] = rng();
+        }
+        prm.init_spins = init.data();
+        prm.cell_seeds = seeds.data();
+    }
+    SQAChiResult res;
+    res.best_state = State(n);
+    res.energy_trace.resize(NS);
+    res.beta_schedule.resize(NS);
+    res.gamma_schedule.resize(NS);
+    res.s_schedule.resize(NS);
+    res.scan_s.resize(SP);
+    res.scan_gamma.resize(SP);
+    res.scan_chi_B.resize(SP);
+    double sc[5] = {0, 0, 0, 0, 0};
+    qa_chi_result out{res.best_state.spins.data(), &res.best_energy, res.energy_trace.data(),
+                      res.beta_schedule.data(), res.gamma_schedule.data(), res.s_schedule.data(),
+                      res.scan_s.data(), res.scan_gamma.data(), res.scan_chi_B.data(), sc, nullptr};
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sqa_chi_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    res.s_star = sc[0];
+    res.gamma_star = sc[1];
+    res.j_perp_star = sc[2];
+    res.chi_floor = sc[3];
+    res.driver_A0 = sc[4];
+    return res;
+}
+}  // namespace
+
+SQAChiResult SQAChiAnnealer::run_chi(double beta, double gamma_start, double gamma_end, std::size_t num_steps,
+                                     std::size_t sweeps_per_step, std::size_t scan_points, std::size_t scan_sweeps,
+                                     std::size_t scan_burn, double chi_floor_fraction, double driver_A0,
+                                     double beta_ramp_fraction, double beta_ramp_start, const std::string &) {
+    // sqa_chi_annealer.cpp:159-179 argument checks
+    if (num_steps < 2) throw std::invalid_argument("num_steps must be >= 2.");
+    if (sweeps_per_step == 0) throw std::invalid_argument("sweeps_per_step must be > 0.");
+    if (!(beta > 0.0)) throw std::invalid_argument("beta must be > 0.");
+    if (!(gamma_end > 0.0) || !(gamma_start > gamma_end))
+        throw std::invalid_argument("Require gamma_start > gamma_end > 0.");
+    if (scan_points < 4) throw std::invalid_argument("scan_points must be >= 4.");
+    if (scan_sweeps == 0) throw std::invalid_argument("scan_sweeps must be > 0.");
+    if (chi_floor_fraction < 0.0) throw std::invalid_argument("chi_floor_fraction must be >= 0.");
+    HIPBackend &hb = device_backend(backend_);
+    qa_chi_params prm{};
+    prm.trotter_slices = static_cast<int64_t>(slices_);
+    prm.replicas = static_cast<int64_t>(replicas_);
+    prm.beta = beta;
+    prm.gamma_start = gamma_start;
+    prm.gamma_end = gamma_end;
+    prm.num_steps = static_cast<int64_t>(num_steps);
+    prm.sweeps_per_step = static_cast<int64_t>(sweeps_per_step);
+    prm.scan_points = static_cast<int64_t>(scan_points);
+    prm.scan_sweeps = static_cast<int64_t>(scan_sweeps);
+    prm.scan_burn = static_cast<int64_t>(scan_burn);
+    prm.chi_floor_fraction = chi_floor_fraction;
+    prm.driver_A0 = driver_A0;
+    prm.beta_ramp_fraction = beta_ramp_fraction;
+    prm.beta_ramp_start = beta_ramp_start;
+    prm.mode = mode_of(hb);
+    return chi_run(hb, rng_, slices_, replicas_, prm);
+}
+
+SQAChiResult SQAChiAnnealer::run_chi_with_profile(const std::vector<double> &chi_B, double beta, double gamma_start,
+                                                  double gamma_end, std::size_t num_steps,
+                                                  std::size_t sweeps_per_step, double chi_floor_fraction,
+                                                  double driver_A0, double beta_ramp_fraction,
+                                                  double beta_ramp_start) {
+    if (chi_B.size() < 4) throw std::invalid_argument("scan_points must be >= 4.");
+    if (num_steps < 2) throw std::invalid_argument("num_steps must be >= 2.");
+    if (sweeps_per_step == 0) throw std::invalid_argument("sweeps_per_step must be > 0.");
+    if (!(beta > 0.0)) throw std::invalid_argument("beta must be > 0.");
+    if (!(gamma_end > 0.0) || !(gamma_start > gamma_end))
+        throw std::invalid_argument("Require gamma_start > gamma_end > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    if (hb.precision() != DevicePrecision::FP32)
+        throw std::invalid_argument("a supplied chi_B profile is an FP32-mode path");
+    qa_chi_params prm{};
+    prm.trotter_slices = static_cast<int64_t>(slices_);
+    prm.replicas = static_cast<int64_t>(replicas_);
+    prm.beta = beta;
+    prm.gamma_start = gamma_start;
+    prm.gamma_end = gamma_end;
+    prm.num_steps = static_cast<int64_t>(num_steps);
+    prm.sweeps_per_step = static_cast<int64_t>(sweeps_per_step);
+    prm.scan_points = static_cast<int64_t>(chi_B.size());
+    prm.scan_sweeps = 1;
+    prm.scan_burn = 0;
+    prm.chi_floor_fraction = chi_floor_fraction;
+    prm.driver_A0 = driver_A0;
+    prm.beta_ramp_fraction = beta_ramp_fraction;
+    prm.beta_ramp_start = beta_ramp_start;
+    prm.mode = QA_MODE_FP32;
+    prm.chi_override = chi_B.data();
+    return chi_run(hb, rng_, slices_, replicas_, prm);
+}
+
+}  // namespace qanneal

@@ -1,0 +1,345 @@
+// Host FP64 methods of the problem models, QUBO conversion, and the HIP backend's device plumbing.
+// Semantics follow dense_ising.cpp, sparse_ising.cpp, higher_order_ising.cpp and qubo.cpp.
+#include <algorithm>
+#include <cctype>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "qanneal/backend.hpp"
+#include "qanneal/dense_ising.hpp"
+#include "qanneal/higher_order_ising.hpp"
+#include "qanneal/qubo.hpp"
+#include "qanneal/sparse_ising.hpp"
+#include "qanneal_hip.h"
+
+namespace qanneal {
+
+namespace {
+void need_size(std::size_t n, std::size_t expect) {
+    if (n != expect) throw std::invalid_argument("State size mismatch.");
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------------- DenseIsing
+DenseIsing::DenseIsing(std::vector<double> h, std::vector<double> J, std::size_t n, double c)
+    : h_(std::move(h)), J_(std::move(J)), n_(n), c_(c) {
+    if (n_ == 0) throw std::invalid_argument("DenseIsing size must be > 0.");
+    if (h_.size() != n_) throw std::invalid_argument("DenseIsing h size mismatch.");
+    if (J_.size() != n_ * n_) throw std::invalid_argument("DenseIsing J size mismatch.");
+}
+
+double DenseIsing::energy(const int8_t *s, std::size_t n) const {
+    need_size(n, n_);
+    validate_spins(s, n_);
+    double e = c_;
+    for (std::size_t i = 0; i < n_; ++i) e += h_[i] * static_cast<double>(s[i]);
+    for (std::size_t i = 0; i < n_; ++i) {
+        const double si = static_cast<double>(s[i]);
+        for (std::size_t j = i + 1; j < n_; ++j) e += J_[i * n_ + j] * si * static_cast<double>(s[j]);
+    }
+    return e;
+}
+
+double DenseIsing::delta_energy(const int8_t *s, std::size_t n, std::size_t flip) const {
+    need_size(n, n_);
+    if (flip >= n_) throw std::invalid_argument("Flip index out of range.");
+    double local = h_[flip];
+    const double *row = &J_[flip * n_];
+    for (std::size_t j = 0; j < n_; ++j)
+        if (j != flip) local += row[j] * static_cast<double>(s[j]);
+    return -2.0 * static_cast<double>(s[flip]) * local;
+}
+
+void DenseIsing::compute_local_fields(const int8_t *s, std::size_t n, double *f) const {
+    need_size(n, n_);
+    for (std::size_t i = 0; i < n_; ++i) {
+        const double *row = &J_[i * n_];
+        double acc = h_[i];
+        for (std::size_t j = 0; j < n_; ++j) acc += row[j] * static_cast<double>(s[j]);
+        f[i] = acc - row[i] * static_cast<double>(s[i]);
+    }
+}
+
+void DenseIsing::update_local_fields_after_flip(double *f, const int8_t *, std::size_t n, std::size_t flip,
+                                                int8_t old_spin) const {
+    need_size(n, n_);
+    const double ds = -2.0 * static_cast<double>(old_spin);
+    const double *row = &J_[flip * n_];
+    for (std::size_t j = 0; j < n_; ++j) f[j] += row[j] * ds;
+    f[flip] -= row[flip] * ds;
+}
+
+// --------------------------------------------------------------------------------- SparseIsing
+SparseIsing::SparseIsing(std::vector<double> h, std::vector<SparseEdge> edges, std::size_t n, double c)
+    : h_(std::move(h)), edges_(std::move(edges)), n_(n), c_(c) {
+    if (n_ == 0) throw std::invalid_argument("SparseIsing size must be > 0.");
+    if (h_.size() != n_) throw std::invalid_argument("SparseIsing h size mismatch.");
+    for (const auto &e : edges_) {
+        if (e.i >= n_ || e.j >= n_) throw std::invalid_argument("SparseIsing edge index out of range.");
+        if (e.i == e.j) throw std::invalid_argument("SparseIsing self-edge not allowed.");
+    }
+    adj_.assign(n_, {});
+    for (const auto &e : edges_) {
+        adj_[e.i].push_back({e.j, e.value});
+        adj_[e.j].push_back({e.i, e.value});
+    }
+}
+
+double SparseIsing::energy(const int8_t *s, std::size_t n) const {
+    need_size(n, n_);
+    validate_spins(s, n_);
+    double e = c_;
+    for (std::size_t i = 0; i < n_; ++i) e += h_[i] * static_cast<double>(s[i]);
+    for (const auto &ed : edges_) e += ed.value * static_cast<double>(s[ed.i]) * static_cast<double>(s[ed.j]);
+    return e;
+}
+
+double SparseIsing::delta_energy(const int8_t *s, std::size_t n, std::size_t flip) const {
+    need_size(n, n_);
+    if (flip >= n_) throw std::invalid_argument("Flip index out of range.");
+    double local = h_[flip];
+    for (const auto &nb : adj_[flip]) local += nb.second * static_cast<double>(s[nb.first]);
+    return -2.0 * static_cast<double>(s[flip]) * local;
+}
+
+void SparseIsing::compute_local_fields(const int8_t *s, std::size_t n, double *f) const {
+    need_size(n, n_);
+    for (std::size_t i = 0; i < n_; ++i) f[i] = h_[i];
+    for (const auto &ed : edges_) {
+        f[ed.i] += ed.value * static_cast<double>(s[ed.j]);
+        f[ed.j] += ed.value * static_cast<double>(s[ed.i]);
+    }
+}
+
+void SparseIsing::update_local_fields_after_flip(double *f, const int8_t *, std::size_t n, std::size_t flip,
+                                                 int8_t old_spin) const {
+    need_size(n, n_);
+    const double ds = -2.0 * static_cast<double>(old_spin);
+    for (const auto &nb : adj_[flip]) f[nb.first] += nb.second * ds;
+}
+
+// ---------------------------------------------------------------------------- HigherOrderIsing
+HigherOrderIsing::HigherOrderIsing(std::size_t n, const std::vector<RawTerm> &terms, double c)
+    : h_(n, 0.0), n_(n), c_(c) {
+    if (n_ == 0) throw std::invalid_argument("HigherOrderIsing size must be > 0.");
+    std::map<std::vector<std::uint32_t>, double> support;  // ordered by sorted variable list
+    for (const auto &t : terms) {
+        if (t.second == 0.0) continue;
+        std::vector<std::size_t> v = t.first;
+        for (std::size_t x : v)
+            if (x >= n_) throw std::invalid_argument("HigherOrderIsing term variable index out of range.");
+        std::sort(v.begin(), v.end());
+        std::vector<std::uint32_t> odd;
+        for (std::size_t a = 0; a < v.size();) {
+            std::size_t b = a;
+            while (b < v.size() && v[b] == v[a]) ++b;
+            if ((b - a) & 1u) odd.push_back(static_cast<std::uint32_t>(v[a]));
+            a = b;
+        }
+        if (odd.empty()) c_ += t.second;
+        else if (odd.size() == 1) h_[odd[0]] += t.second;
+        else support[odd] += t.second;
+    }
+    offset_.push_back(0);
+    incident_.assign(n_, {});
+    for (const auto &kv : support) {
+        if (kv.second == 0.0) continue;
+        const auto id = static_cast<std::uint32_t>(coeff_.size());
+        coeff_.push_back(kv.second);
+        for (std::uint32_t x : kv.first) {
+            vars_.push_back(x);
+            incident_[x].push_back(id);
+        }
+        offset_.push_back(vars_.size());
+        max_degree_ = std::max(max_degree_, kv.first.size());
+    }
+    if (max_degree_ == 0 && std::any_of(h_.begin(), h_.end(), [](double x) { return x != 0.0; })) max_degree_ = 1;
+}
+
+double HigherOrderIsing::energy(const int8_t *s, std::size_t n) const {
+    need_size(n, n_);
+    validate_spins(s, n_);
+    double e = c_;
+    for (std::size_t i = 0; i < n_; ++i) e += h_[i] * static_cast<double>(s[i]);
+    return e + coupling_energy(s, n);
+}
+
+double HigherOrderIsing::coupling_energy(const int8_t *s, std::size_t n) const {
+    need_size(n, n_);
+    double e = 0.0;
+    for (std::size_t t = 0; t < coeff_.size(); ++t) {
+        double prod = coeff_[t];
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k) prod *= static_cast<double>(s[vars_[k]]);
+        e += prod;
+    }
+    return e;
+}
+
+double HigherOrderIsing::delta_energy(const int8_t *s, std::size_t n, std::size_t flip) const {
+    need_size(n, n_);
+    if (flip >= n_) throw std::invalid_argument("Flip index out of range.");
+    double local = h_[flip];
+    for (std::uint32_t t : incident_[flip]) {
+        double prod = coeff_[t];
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k)
+            if (vars_[k] != flip) prod *= static_cast<double>(s[vars_[k]]);
+        local += prod;
+    }
+    return -2.0 * static_cast<double>(s[flip]) * local;
+}
+
+void HigherOrderIsing::compute_local_fields(const int8_t *s, std::size_t n, double *f) const {
+    need_size(n, n_);
+    for (std::size_t i = 0; i < n_; ++i) f[i] = h_[i];
+    for (std::size_t t = 0; t < coeff_.size(); ++t) {
+        double prod = coeff_[t];
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k) prod *= static_cast<double>(s[vars_[k]]);
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k) f[vars_[k]] += prod * static_cast<double>(s[vars_[k]]);
+    }
+}
+
+void HigherOrderIsing::update_local_fields_after_flip(double *f, const int8_t *ns, std::size_t n,
+                                                      std::size_t flip, int8_t) const {
+    need_size(n, n_);
+    for (std::uint32_t t : incident_[flip]) {
+        double prod = coeff_[t];
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k) prod *= static_cast<double>(ns[vars_[k]]);
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k)
+            if (vars_[k] != flip) f[vars_[k]] += 2.0 * prod * static_cast<double>(ns[vars_[k]]);
+    }
+}
+
+// --------------------------------------------------------------------------------------- QUBO
+QUBO::QUBO(std::vector<double> q, std::size_t n) : q_(std::move(q)), n_(n) {
+    if (n_ == 0) throw std::invalid_argument("QUBO size must be > 0.");
+    if (q_.size() != n_ * n_) throw std::invalid_argument("QUBO matrix size mismatch.");
+}
+
+QUBO::QUBO(const std::vector<std::pair<std::pair<std::size_t, std::size_t>, double>> &entries, std::size_t n)
+    : q_(n * n, 0.0), n_(n) {
+    if (n_ == 0) throw std::invalid_argument("QUBO size must be > 0.");
+    for (const auto &e : entries) {
+        if (e.first.first >= n_ || e.first.second >= n_) throw std::invalid_argument("QUBO entry index out of range.");
+        q_[e.first.first * n_ + e.first.second] += e.second;
+    }
+}
+
+// With W = (Q + Q^T)/2 and x = (1 + s)/2: x^T Q x = 1/4 s^T W s + 1/2 s^T W 1 + 1/4 1^T W 1, so
+// h = 1/2 W 1, J = 1/2 W (full matrix; energy() reads i<j only) and c = 1/4 (tr W + 1^T W 1).
+DenseIsing QUBO::to_ising() const {
+    const std::size_t n = n_;
+    std::vector<double> W(n * n), h(n, 0.0), J(n * n, 0.0);
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t j = 0; j < n; ++j) W[i * n + j] = 0.5 * (q_[i * n + j] + q_[j * n + i]);
+    for (std::size_t i = 0; i < n; ++i) {
+        double rs = 0.0;
+        for (std::size_t j = 0; j < n; ++j) rs += W[i * n + j];
+        h[i] = 0.5 * rs;
+    }
+    double c = 0.0;
+    for (std::size_t i = 0; i < n; ++i) {
+        c += 0.25 * W[i * n + i];
+        for (std::size_t j = 0; j < n; ++j) {
+            J[i * n + j] = 0.5 * W[i * n + j];
+            c += 0.25 * W[i * n + j];
+        }
+    }
+    return DenseIsing(std::move(h), std::move(J), n, c);
+}
+
+// --------------------------------------------------------------------------------- backends
+BackendSpec parse_backend(std::string_view name) {
+    std::string s(name);
+    std::transform(s.begin(), s.end(), s.begin(), [](unsigned char ch) { return std::tolower(ch); });
+    if (s == "cpu") return {BackendKind::CPU, DevicePrecision::FP32};
+    if (s == "cuda") return {BackendKind::CUDA, DevicePrecision::FP32};
+    if (s == "hip" || s == "hip_fp32" || s == "rocm") return {BackendKind::HIP, DevicePrecision::FP32};
+    if (s == "hip_fp64" || s == "hip-fp64" || s == "hip_validate")
+        return {BackendKind::HIP, DevicePrecision::FP64Validation};
+    throw std::invalid_argument("Unknown backend: " + std::string(name));
+}
+
+void throw_on_status(int status) {
+    if (status == QA_OK) return;
+    const std::string msg = qa_last_error();
+    if (status == QA_EINVAL) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+namespace {
+struct DeviceContext {
+    qa_ctx *ctx = nullptr;
+    std::mutex run_mu;
+};
+
+DeviceContext &device_context() {
+    static DeviceContext dc;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *env = std::getenv("QANNEAL_HIP_DEVICE");
+        const int dev = env ? std::atoi(env) : 0;
+        throw_on_status(qa_ctx_create(dev, &dc.ctx));
+    });
+    return dc;
+}
+}  // namespace
+
+HIPBackend::HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision)
+    : CPUBackend(std::move(ham)), precision_(precision) {}
+
+HIPBackend::~HIPBackend() {
+    if (problem_) qa_problem_destroy(problem_);
+}
+
+qa_ctx *HIPBackend::context() const { return device_context().ctx; }
+
+std::mutex &HIPBackend::device_mutex() const { return device_context().run_mu; }
+
+qa_problem *HIPBackend::device_problem() const {
+    std::lock_guard<std::mutex> lock(upload_mu_);
+    if (problem_) return problem_;
+    qa_ctx *ctx = context();
+    const Hamiltonian *h = ham_.get();
+    qa_problem *p = nullptr;
+    if (auto *d = dynamic_cast<const DenseIsing *>(h)) {
+        throw_on_status(qa_problem_dense(ctx, static_cast<int64_t>(d->size()), d->h().data(), d->J().data(),
+                                         d->constant(), &p));
+    } else if (auto *sp = dynamic_cast<const SparseIsing *>(h)) {
+        std::vector<int64_t> ei, ej;
+        std::vector<double> ev;
+        for (const auto &e : sp->edges()) {
+            ei.push_back(static_cast<int64_t>(e.i));
+            ej.push_back(static_cast<int64_t>(e.j));
+            ev.push_back(e.value);
+        }
+        throw_on_status(qa_problem_sparse(ctx, static_cast<int64_t>(sp->size()), sp->h().data(),
+                                          static_cast<int64_t>(ev.size()), ei.data(), ej.data(), ev.data(),
+                                          sp->constant(), &p));
+    } else if (auto *hb = dynamic_cast<const HigherOrderIsing *>(h)) {
+        std::vector<int64_t> off(hb->term_offsets().begin(), hb->term_offsets().end());
+        throw_on_status(qa_problem_hubo(ctx, static_cast<int64_t>(hb->size()), hb->h().data(),
+                                        static_cast<int64_t>(hb->num_terms()), off.data(),
+                                        hb->term_variables().data(), hb->term_coefficients().data(),
+                                        hb->constant(), &p));
+    } else {
+        throw std::invalid_argument("the HIP backend runs DenseIsing, SparseIsing and HigherOrderIsing models");
+    }
+    problem_ = p;
+    return problem_;
+}
+
+std::shared_ptr<Backend> make_backend(BackendSpec spec, std::shared_ptr<const Hamiltonian> ham) {
+    switch (spec.kind) {
+        case BackendKind::CPU: return std::make_shared<CPUBackend>(std::move(ham));
+        case BackendKind::CUDA: throw std::runtime_error("CUDA backend not enabled at build time.");
+        case BackendKind::HIP:
+            if (!ham) throw std::invalid_argument("HIPBackend requires a Hamiltonian.");
+            return std::make_shared<HIPBackend>(std::move(ham), spec.precision);
+    }
+    throw std::runtime_error("Unknown backend kind.");
+}
+
+}  // namespace qanneal

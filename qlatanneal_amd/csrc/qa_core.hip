@@ -1,0 +1,393 @@
+// Context, problem upload, FP64 energy kernel and the C-ABI entry points of libqanneal_hip.so.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qa_internal.hpp"
+
+namespace qa {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+double trotter_coupling(double beta, double gamma, int64_t slices) {
+    const double x = std::max(beta * gamma / static_cast<double>(slices), 1e-12);
+    return 0.5 * std::log(1.0 / std::tanh(x));
+}
+
+template <class F>
+qa_status guarded(F &&f) {
+    try {
+        f();
+        return QA_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return QA_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return QA_EDEVICE;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 energy, one thread per state, in the reference's summation order:
+//   dense  E = c + sum_i h_i s_i + sum_{i<j} J_ij s_i s_j        (dense_ising.cpp:29-44)
+//   sparse E = c + sum_i h_i s_i + sum_edges v s_i s_j           (sparse_ising.cpp:43-56)
+//   hubo   E = c + sum_i h_i s_i + sum_t J_t prod s              (higher_order_ising.cpp:92-111)
+// Every product is an FP64 value times +-1, so contraction cannot change a rounding; the sum is
+// sequential, hence bit-identical to the host.
+// ---------------------------------------------------------------------------------------------
+__global__ void energy_kernel(ProblemView pv, const int8_t *__restrict__ spins, int64_t count,
+                              double *__restrict__ out) {
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k >= count) return;
+    const int n = pv.n;
+    const int8_t *s = spins + k * n;
+    double e = pv.c;
+    for (int i = 0; i < n; ++i) e += pv.h64[i] * static_cast<double>(s[i]);
+    if (pv.kind == QA_KIND_DENSE) {
+        for (int i = 0; i < n; ++i) {
+            const double si = static_cast<double>(s[i]);
+            const double *row = pv.J64 + static_cast<int64_t>(i) * n;
+            for (int j = i + 1; j < n; ++j) e += row[j] * si * static_cast<double>(s[j]);
+        }
+    } else if (pv.kind == QA_KIND_SPARSE) {
+        for (int q = 0; q < pv.nedges; ++q)
+            e += pv.ev64[q] * static_cast<double>(s[pv.ei[q]]) * static_cast<double>(s[pv.ej[q]]);
+    } else {
+        for (int t = 0; t < pv.nterms; ++t) {
+            double prod = pv.t_coef64[t];
+            for (int q = pv.t_off[t]; q < pv.t_off[t + 1]; ++q) prod *= static_cast<double>(s[pv.t_vars[q]]);
+            e += prod;
+        }
+    }
+    out[k] = e;
+}
+
+void device_energies(const qa_problem *p, const int8_t *d_spins, int64_t count, double *d_out,
+                     hipStream_t s) {
+    if (count <= 0) return;
+    const int block = 64;
+    const int64_t grid = (count + block - 1) / block;
+    hipLaunchKernelGGL(energy_kernel, dim3(static_cast<unsigned>(grid)), dim3(block), 0, s, view_of(p),
+                       d_spins, count, d_out);
+    QA_HIP(hipGetLastError());
+}
+
+static void check_spins(const int8_t *s, int64_t len) {
+    for (int64_t i = 0; i < len; ++i)
+        if (s[i] != 1 && s[i] != -1) throw Error{QA_EINVAL, "State spins must be -1 or +1."};
+}
+
+static void check_finite(const double *v, int64_t len, const char *what) {
+    for (int64_t i = 0; i < len; ++i)
+        if (!std::isfinite(v[i])) throw Error{QA_EINVAL, std::string(what) + " must be finite."};
+}
+
+}  // namespace qa
+
+using namespace qa;
+
+extern "C" {
+
+const char *qa_last_error(void) { return g_last_error.c_str(); }
+
+qa_status qa_ctx_create(int device, qa_ctx **out) {
+    return guarded([&] {
+        QA_REQUIRE(out != nullptr, "qa_ctx_create: out is NULL");
+        int count = 0;
+        QA_HIP(hipGetDeviceCount(&count));
+        QA_REQUIRE(device >= 0 && device < count, "qa_ctx_create: no such device");
+        QA_HIP(hipSetDevice(device));
+        auto *c = new qa_ctx();
+        c->device = device;
+        hipDeviceProp_t prop;
+        QA_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount;
+        c->lds_per_block = prop.sharedMemPerBlock;
+        QA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        QA_HIP(hipEventCreate(&c->ev0));
+        QA_HIP(hipEventCreate(&c->ev1));
+        *out = c;
+    });
+}
+
+qa_status qa_ctx_destroy(qa_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+qa_status qa_problem_dense(qa_ctx *ctx, int64_t n, const double *h, const double *J, double c,
+                           qa_problem **out) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && out && h && J, "qa_problem_dense: NULL argument");
+        QA_REQUIRE(n > 0, "DenseIsing size must be > 0.");
+        QA_REQUIRE(n <= (1 << 20), "DenseIsing size too large for the device path.");
+        check_finite(h, n, "h");
+        check_finite(J, n * n, "J");
+        QA_HIP(hipSetDevice(ctx->device));
+        auto p = std::make_unique<qa_problem>();
+        p->ctx = ctx;
+        p->kind = QA_KIND_DENSE;
+        p->n = n;
+        p->c = c;
+        std::vector<float> hf(n), Jf(static_cast<size_t>(n * n));
+        for (int64_t i = 0; i < n; ++i) hf[i] = static_cast<float>(h[i]);
+        bool sym = true;
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = 0; j < n; ++j) {
+                const double v = J[i * n + j];
+                Jf[i * n + j] = (i == j) ? 0.0f : static_cast<float>(v);
+                if (v != J[j * n + i]) sym = false;
+            }
+        p->symmetric = sym;
+        p->h64.alloc(n);
+        p->h32.alloc(n);
+        p->J64.alloc(n * n);
+        p->J32.alloc(n * n);
+        p->h64.upload(h, n, ctx->stream);
+        p->h32.upload(hf.data(), n, ctx->stream);
+        p->J64.upload(J, n * n, ctx->stream);
+        p->J32.upload(Jf.data(), n * n, ctx->stream);
+        QA_HIP(hipStreamSynchronize(ctx->stream));
+        *out = p.release();
+    });
+}
+
+qa_status qa_problem_sparse(qa_ctx *ctx, int64_t n, const double *h, int64_t nedges,
+                            const int64_t *ei, const int64_t *ej, const double *ev, double c,
+                            qa_problem **out) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && out && h, "qa_problem_sparse: NULL argument");
+        QA_REQUIRE(n > 0, "SparseIsing size must be > 0.");
+        QA_REQUIRE(nedges >= 0 && (nedges == 0 || (ei && ej && ev)), "qa_problem_sparse: bad edge arrays");
+        check_finite(h, n, "h");
+        check_finite(ev, nedges, "edge values");
+        // sparse_ising.cpp:17-41: range and self-edge checks; adjacency in edge insertion order
+        std::vector<int32_t> deg(n, 0);
+        for (int64_t e = 0; e < nedges; ++e) {
+            QA_REQUIRE(ei[e] >= 0 && ei[e] < n && ej[e] >= 0 && ej[e] < n, "SparseIsing edge index out of range.");
+            QA_REQUIRE(ei[e] != ej[e], "SparseIsing self-edge not allowed.");
+            ++deg[ei[e]];
+            ++deg[ej[e]];
+        }
+        std::vector<int32_t> ptr(n + 1, 0), fill(n, 0), idx(2 * nedges), e_i(nedges), e_j(nedges);
+        for (int64_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + deg[i];
+        std::vector<double> val(2 * nedges);
+        std::vector<float> valf(2 * nedges);
+        int maxdeg = 0;
+        for (int64_t i = 0; i < n; ++i) maxdeg = std::max(maxdeg, deg[i]);
+        for (int64_t e = 0; e < nedges; ++e) {
+            const int64_t a = ei[e], b = ej[e];
+            e_i[e] = static_cast<int32_t>(a);
+            e_j[e] = static_cast<int32_t>(b);
+            int32_t pa = ptr[a] + fill[a]++;
+            idx[pa] = static_cast<int32_t>(b);
+            val[pa] = ev[e];
+            int32_t pb = ptr[b] + fill[b]++;
+            idx[pb] = static_cast<int32_t>(a);
+            val[pb] = ev[e];
+        }
+        for (size_t k = 0; k < val.size(); ++k) valf[k] = static_cast<float>(val[k]);
+        std::vector<float> hf(n);
+        for (int64_t i = 0; i < n; ++i) hf[i] = static_cast<float>(h[i]);
+        QA_HIP(hipSetDevice(ctx->device));
+        auto p = std::make_unique<qa_problem>();
+        p->ctx = ctx;
+        p->kind = QA_KIND_SPARSE;
+        p->n = n;
+        p->c = c;
+        p->nedges = nedges;
+        p->max_degree = maxdeg;
+        hipStream_t s = ctx->stream;
+        p->h64.alloc(n); p->h64.upload(h, n, s);
+        p->h32.alloc(n); p->h32.upload(hf.data(), n, s);
+        p->ei.alloc(std::max<int64_t>(nedges, 1)); p->ei.upload(e_i.data(), nedges, s);
+        p->ej.alloc(std::max<int64_t>(nedges, 1)); p->ej.upload(e_j.data(), nedges, s);
+        p->ev64.alloc(std::max<int64_t>(nedges, 1)); p->ev64.upload(ev, nedges, s);
+        p->adj_ptr.alloc(n + 1); p->adj_ptr.upload(ptr.data(), n + 1, s);
+        p->adj_idx.alloc(std::max<int64_t>(2 * nedges, 1)); p->adj_idx.upload(idx.data(), 2 * nedges, s);
+        p->adj_val64.alloc(std::max<int64_t>(2 * nedges, 1)); p->adj_val64.upload(val.data(), 2 * nedges, s);
+        p->adj_val32.alloc(std::max<int64_t>(2 * nedges, 1)); p->adj_val32.upload(valf.data(), 2 * nedges, s);
+        QA_HIP(hipStreamSynchronize(s));
+        *out = p.release();
+    });
+}
+
+qa_status qa_problem_hubo(qa_ctx *ctx, int64_t n, const double *h, int64_t nterms,
+                          const int64_t *term_offset, const uint32_t *term_vars,
+                          const double *term_coeff, double c, qa_problem **out) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && out && h, "qa_problem_hubo: NULL argument");
+        QA_REQUIRE(n > 0, "HigherOrderIsing size must be > 0.");
+        QA_REQUIRE(nterms >= 0 && (nterms == 0 || (term_offset && term_vars && term_coeff)),
+                   "qa_problem_hubo: bad term arrays");
+        check_finite(h, n, "h");
+        check_finite(term_coeff, nterms, "term coefficients");
+        QA_REQUIRE(nterms == 0 || term_offset[0] == 0, "term_offset[0] must be 0");
+        const int64_t nv = nterms ? term_offset[nterms] : 0;
+        std::vector<int32_t> off(nterms + 1), vars(std::max<int64_t>(nv, 1));
+        std::vector<int32_t> deg(n, 0);
+        for (int64_t t = 0; t <= nterms; ++t) off[t] = static_cast<int32_t>(nterms ? term_offset[t] : 0);
+        for (int64_t t = 0; t < nterms; ++t) {
+            QA_REQUIRE(term_offset[t + 1] >= term_offset[t] + 2, "HUBO terms must have degree >= 2 after consolidation");
+            for (int64_t q = term_offset[t]; q < term_offset[t + 1]; ++q) {
+                QA_REQUIRE(term_vars[q] < static_cast<uint64_t>(n), "HigherOrderIsing term variable index out of range.");
+                vars[q] = static_cast<int32_t>(term_vars[q]);
+                ++deg[term_vars[q]];
+            }
+        }
+        std::vector<int32_t> iptr(n + 1, 0), fill(n, 0), iidx(std::max<int64_t>(nv, 1));
+        for (int64_t i = 0; i < n; ++i) iptr[i + 1] = iptr[i] + deg[i];
+        int maxdeg = 0;
+        for (int64_t i = 0; i < n; ++i) maxdeg = std::max(maxdeg, deg[i]);
+        for (int64_t t = 0; t < nterms; ++t)
+            for (int64_t q = term_offset[t]; q < term_offset[t + 1]; ++q) {
+                const uint32_t v = term_vars[q];
+                iidx[iptr[v] + fill[v]++] = static_cast<int32_t>(t);
+            }
+        std::vector<float> cf(std::max<int64_t>(nterms, 1)), hf(n);
+        for (int64_t t = 0; t < nterms; ++t) cf[t] = static_cast<float>(term_coeff[t]);
+        for (int64_t i = 0; i < n; ++i) hf[i] = static_cast<float>(h[i]);
+        QA_HIP(hipSetDevice(ctx->device));
+        auto p = std::make_unique<qa_problem>();
+        p->ctx = ctx;
+        p->kind = QA_KIND_HUBO;
+        p->n = n;
+        p->c = c;
+        p->nterms = nterms;
+        p->max_degree = maxdeg;
+        hipStream_t s = ctx->stream;
+        p->h64.alloc(n); p->h64.upload(h, n, s);
+        p->h32.alloc(n); p->h32.upload(hf.data(), n, s);
+        p->t_off.alloc(nterms + 1); p->t_off.upload(off.data(), nterms + 1, s);
+        p->t_vars.alloc(std::max<int64_t>(nv, 1)); p->t_vars.upload(vars.data(), nv, s);
+        p->t_coef64.alloc(std::max<int64_t>(nterms, 1)); p->t_coef64.upload(term_coeff, nterms, s);
+        p->t_coef32.alloc(std::max<int64_t>(nterms, 1)); p->t_coef32.upload(cf.data(), nterms, s);
+        p->inc_ptr.alloc(n + 1); p->inc_ptr.upload(iptr.data(), n + 1, s);
+        p->inc_idx.alloc(std::max<int64_t>(nv, 1)); p->inc_idx.upload(iidx.data(), nv, s);
+        QA_HIP(hipStreamSynchronize(s));
+        *out = p.release();
+    });
+}
+
+qa_status qa_problem_destroy(qa_problem *p) {
+    return guarded([&] {
+        if (!p) return;
+        (void)hipSetDevice(p->ctx->device);
+        delete p;
+    });
+}
+
+qa_status qa_energy(const qa_problem *p, int64_t count, const int8_t *spins, double *out) {
+    return guarded([&] {
+        QA_REQUIRE(p && spins && out && count >= 0, "qa_energy: bad argument");
+        check_spins(spins, count * p->n);
+        QA_HIP(hipSetDevice(p->ctx->device));
+        hipStream_t s = p->ctx->stream;
+        DevBuf<int8_t> ds(count * p->n);
+        DevBuf<double> de(count);
+        ds.upload(spins, count * p->n, s);
+        device_energies(p, ds.ptr, count, de.ptr, s);
+        de.download(out, count, s);
+        QA_HIP(hipStreamSynchronize(s));
+    });
+}
+
+qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sa_run: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sa_run: problem belongs to another context");
+        QA_REQUIRE(prm->steps > 0 && prm->betas, "Schedule must contain at least one beta.");
+        QA_REQUIRE(prm->sweeps_per_beta > 0, "sweeps_per_beta must be > 0.");
+        QA_REQUIRE(prm->reads > 0, "reads must be > 0.");
+        QA_REQUIRE(res->best_states && res->best_energies && res->energy_trace, "qa_sa_run: NULL result array");
+        check_finite(prm->betas, prm->steps, "betas");
+        QA_HIP(hipSetDevice(ctx->device));
+        if (prm->mode == QA_MODE_FP64_VALIDATION) sa_validation(ctx, p, prm, res);
+        else if (prm->mode == QA_MODE_FP32) sa_fp32(ctx, p, prm, res);
+        else throw Error{QA_EINVAL, "unknown mode"};
+    });
+}
+
+qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sqa_run: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sqa_run: problem belongs to another context");
+        QA_REQUIRE(prm->steps > 0 && prm->betas && prm->gammas, "SQA schedule must contain betas.");
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->replicas > 0, "replicas must be > 0.");
+        QA_REQUIRE(prm->sweeps_per_beta >= 0 && prm->worldline_sweeps >= 0 && prm->cluster_sweeps >= 0,
+                   "sweep counts must be >= 0.");
+        QA_REQUIRE(prm->sweeps_per_beta > 0 || prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0,
+                   "At least one of sweeps_per_beta, worldline_sweeps, or cluster_sweeps must be > 0.");
+        QA_REQUIRE(res->best_state && res->best_energy && res->energy_trace, "qa_sqa_run: NULL result array");
+        check_finite(prm->betas, prm->steps, "betas");
+        check_finite(prm->gammas, prm->steps, "gammas");
+        QA_HIP(hipSetDevice(ctx->device));
+        if (prm->mode == QA_MODE_FP64_VALIDATION) sqa_validation(ctx, p, prm, res);
+        else if (prm->mode == QA_MODE_FP32) sqa_fp32(ctx, p, prm, res);
+        else throw Error{QA_EINVAL, "unknown mode"};
+    });
+}
+
+qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sqapt_run: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sqapt_run: problem belongs to another context");
+        QA_REQUIRE(prm->rungs >= 2 && prm->betas && prm->gammas, "SQA parallel tempering requires >=2 beta/gamma pairs.");
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->steps > 0, "steps must be > 0.");
+        QA_REQUIRE(prm->swap_interval > 0, "swap_interval must be > 0.");
+        QA_REQUIRE(prm->ladders > 0, "ladders must be > 0.");
+        QA_REQUIRE(prm->sweeps_per_step >= 0 && prm->worldline_sweeps >= 0 && prm->cluster_sweeps >= 0,
+                   "sweep counts must be >= 0.");
+        QA_REQUIRE(prm->sweeps_per_step > 0 || prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0,
+                   "At least one of sweeps_per_step/worldline_sweeps/cluster_sweeps must be > 0.");
+        check_finite(prm->betas, prm->rungs, "betas");
+        check_finite(prm->gammas, prm->rungs, "gammas");
+        QA_HIP(hipSetDevice(ctx->device));
+        if (prm->mode == QA_MODE_FP64_VALIDATION) sqapt_validation(ctx, p, prm, res);
+        else if (prm->mode == QA_MODE_FP32) sqapt_fp32(ctx, p, prm, res);
+        else throw Error{QA_EINVAL, "unknown mode"};
+    });
+}
+
+qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sqa_chi_run: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sqa_chi_run: problem belongs to another context");
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->replicas > 0, "replicas must be > 0.");
+        QA_REQUIRE(prm->num_steps >= 2, "num_steps must be >= 2.");
+        QA_REQUIRE(prm->sweeps_per_step > 0, "sweeps_per_step must be > 0.");
+        QA_REQUIRE(prm->beta > 0.0, "beta must be > 0.");
+        QA_REQUIRE(prm->gamma_end > 0.0 && prm->gamma_start > prm->gamma_end, "Require gamma_start > gamma_end > 0.");
+        QA_REQUIRE(prm->scan_points >= 4, "scan_points must be >= 4.");
+        QA_REQUIRE(prm->scan_sweeps > 0, "scan_sweeps must be > 0.");
+        QA_REQUIRE(prm->scan_burn >= 0, "scan_burn must be >= 0.");
+        QA_REQUIRE(prm->chi_floor_fraction >= 0.0, "chi_floor_fraction must be >= 0.");
+        QA_HIP(hipSetDevice(ctx->device));
+        if (prm->mode == QA_MODE_FP64_VALIDATION) chi_validation(ctx, p, prm, res);
+        else if (prm->mode == QA_MODE_FP32) chi_fp32(ctx, p, prm, res);
+        else throw Error{QA_EINVAL, "unknown mode"};
+    });
+}
+
+}  // extern "C"

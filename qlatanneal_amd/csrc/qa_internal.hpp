@@ -1,0 +1,178 @@
+// Internal declarations of libqanneal_hip.so (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "qanneal_hip.h"
+
+namespace qa {
+
+// ---- error plumbing ------------------------------------------------------------------------
+void set_error(const std::string &msg);
+
+struct Error {
+    qa_status status;
+    std::string msg;
+};
+
+#define QA_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::qa::Error{e_ == hipErrorOutOfMemory ? QA_ENOMEM : QA_EDEVICE,          \
+                              std::string(#call) + ": " + hipGetErrorString(e_)};         \
+    } while (0)
+
+#define QA_REQUIRE(cond, msg)                                                              \
+    do {                                                                                   \
+        if (!(cond)) throw ::qa::Error{QA_EINVAL, (msg)};                                  \
+    } while (0)
+
+// ---- device buffers ------------------------------------------------------------------------
+template <class T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); ptr = o.ptr; count = o.count; o.ptr = nullptr; o.count = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        count = n;
+        if (n) QA_HIP(hipMalloc(&ptr, n * sizeof(T)));
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+    void upload(const T *src, size_t n, hipStream_t s) {
+        if (n) QA_HIP(hipMemcpyAsync(ptr, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *dst, size_t n, hipStream_t s) const {
+        if (n) QA_HIP(hipMemcpyAsync(dst, ptr, n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+}  // namespace qa
+
+struct qa_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cus = 256;
+    size_t lds_per_block = 65536;
+};
+
+enum { QA_KIND_DENSE = 0, QA_KIND_SPARSE = 1, QA_KIND_HUBO = 2 };
+
+// Device-resident problem. FP64 copies serve validation runs and exact energies; FP32 copies
+// (diagonal of J cleared: the dynamics never read it) serve the production sweep.
+struct qa_problem {
+    qa_ctx *ctx = nullptr;
+    int kind = QA_KIND_DENSE;
+    int64_t n = 0;
+    double c = 0.0;
+    bool symmetric = true;
+    qa::DevBuf<double> h64;      // [n]
+    qa::DevBuf<float> h32;       // [n]
+    // dense
+    qa::DevBuf<double> J64;      // [n*n] as given
+    qa::DevBuf<float> J32;       // [n*n] zero diagonal
+    // sparse: undirected edges (energy order) + adjacency CSR in reference insertion order
+    int64_t nedges = 0;
+    qa::DevBuf<int32_t> ei, ej;
+    qa::DevBuf<double> ev64;
+    qa::DevBuf<int32_t> adj_ptr, adj_idx;  // [n+1], [2*nedges]
+    qa::DevBuf<double> adj_val64;
+    qa::DevBuf<float> adj_val32;
+    // hubo: consolidated terms + incidence CSR (term ids per variable, ascending)
+    int64_t nterms = 0;
+    qa::DevBuf<int32_t> t_off, t_vars;
+    qa::DevBuf<double> t_coef64;
+    qa::DevBuf<float> t_coef32;
+    qa::DevBuf<int32_t> inc_ptr, inc_idx;
+    int max_degree = 0;          // sparse: max adjacency length; hubo: max incident terms
+};
+
+// Views passed to kernels (by value).
+struct ProblemView {
+    int kind;
+    int n;
+    double c;
+    const double *h64;
+    const float *h32;
+    const double *J64;
+    const float *J32;
+    int nedges;
+    const int32_t *ei, *ej;
+    const double *ev64;
+    const int32_t *adj_ptr, *adj_idx;
+    const double *adj_val64;
+    const float *adj_val32;
+    int nterms;
+    const int32_t *t_off, *t_vars;
+    const double *t_coef64;
+    const float *t_coef32;
+    const int32_t *inc_ptr, *inc_idx;
+};
+
+inline ProblemView view_of(const qa_problem *p) {
+    ProblemView v{};
+    v.kind = p->kind;
+    v.n = static_cast<int>(p->n);
+    v.c = p->c;
+    v.h64 = p->h64.ptr;
+    v.h32 = p->h32.ptr;
+    v.J64 = p->J64.ptr;
+    v.J32 = p->J32.ptr;
+    v.nedges = static_cast<int>(p->nedges);
+    v.ei = p->ei.ptr;
+    v.ej = p->ej.ptr;
+    v.ev64 = p->ev64.ptr;
+    v.adj_ptr = p->adj_ptr.ptr;
+    v.adj_idx = p->adj_idx.ptr;
+    v.adj_val64 = p->adj_val64.ptr;
+    v.adj_val32 = p->adj_val32.ptr;
+    v.nterms = static_cast<int>(p->nterms);
+    v.t_off = p->t_off.ptr;
+    v.t_vars = p->t_vars.ptr;
+    v.t_coef64 = p->t_coef64.ptr;
+    v.t_coef32 = p->t_coef32.ptr;
+    v.inc_ptr = p->inc_ptr.ptr;
+    v.inc_idx = p->inc_idx.ptr;
+    return v;
+}
+
+namespace qa {
+
+// Trotter coupling J_perp = 0.5 ln(1 / tanh(max(beta*Gamma/P, 1e-12))) — the formula of
+// sqa_annealer.cpp:66-70, evaluated on the host in FP64 for every step.
+double trotter_coupling(double beta, double gamma, int64_t slices);
+
+// Energies on the device (FP64, reference order) for `count` states already resident.
+void device_energies(const qa_problem *p, const int8_t *d_spins, int64_t count, double *d_out,
+                     hipStream_t s);
+
+// Engine entry points (implemented per engine file).
+void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
+void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
+void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
+void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
+void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
+void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
+void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
+void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
+
+}  // namespace qa

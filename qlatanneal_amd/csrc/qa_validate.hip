@@ -1,0 +1,475 @@
+// FP64 validation mode: the device replays the reference CPU trajectory.
+//
+// Arithmetic follows the reference operation by operation (FP64 fields, the same summation
+// orders, de = fma(beta/P, dE_classical, dE_trotter) where the reference build contracts), and
+// the random stream is the reference's: std::mt19937_64 outputs drawn on the host, consumed on
+// the device exactly where the reference consumes them (a uniform only when dE > 0, slice picks
+// through Lemire's rejection). Shuffles of the visit order are drawn on the host with
+// std::shuffle on the same engine, so libstdc++ <random> defines every draw.
+//
+// Residual difference: exp() on the device (ROCm device libm) and in glibc may differ in the
+// last ulp; a decision flips only if the uniform falls inside that ulp (~1e-16 per uphill move).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "qa_internal.hpp"
+#include "qa_rng.hpp"
+
+namespace qa {
+namespace {
+
+// ---- FP64 Hamiltonian pieces on the device (reference order) ------------------------------
+__device__ double delta_fp64(const ProblemView &pv, const int8_t *s, int f) {
+    // dense_ising.cpp:46-62 / sparse_ising.cpp:58-71 / higher_order_ising.cpp:113-136
+    double local = pv.h64[f];
+    if (pv.kind == QA_KIND_DENSE) {
+        const double *row = pv.J64 + static_cast<int64_t>(f) * pv.n;
+        for (int j = 0; j < pv.n; ++j)
+            if (j != f) local += row[j] * static_cast<double>(s[j]);
+    } else if (pv.kind == QA_KIND_SPARSE) {
+        for (int q = pv.adj_ptr[f]; q < pv.adj_ptr[f + 1]; ++q)
+            local += pv.adj_val64[q] * static_cast<double>(s[pv.adj_idx[q]]);
+    } else {
+        for (int q = pv.inc_ptr[f]; q < pv.inc_ptr[f + 1]; ++q) {
+            const int t = pv.inc_idx[q];
+            double prod = pv.t_coef64[t];
+            for (int k = pv.t_off[t]; k < pv.t_off[t + 1]; ++k)
+                if (pv.t_vars[k] != f) prod *= static_cast<double>(s[pv.t_vars[k]]);
+            local += prod;
+        }
+    }
+    return -2.0 * static_cast<double>(s[f]) * local;
+}
+
+// fields[i] for one state; i-parallel over the calling threads (dense_ising.cpp:64-83,
+// sparse_ising.cpp:73-85, higher_order_ising.cpp:138-160; per element the contributions arrive
+// in the same order as in the reference's loops).
+__device__ void fields_fp64(const ProblemView &pv, const int8_t *s, double *f, int lane, int lanes) {
+    for (int i = lane; i < pv.n; i += lanes) {
+        double acc = pv.h64[i];
+        if (pv.kind == QA_KIND_DENSE) {
+            const double *row = pv.J64 + static_cast<int64_t>(i) * pv.n;
+            for (int j = 0; j < pv.n; ++j) acc += row[j] * static_cast<double>(s[j]);
+            acc -= row[i] * static_cast<double>(s[i]);
+        } else if (pv.kind == QA_KIND_SPARSE) {
+            for (int q = pv.adj_ptr[i]; q < pv.adj_ptr[i + 1]; ++q)
+                acc += pv.adj_val64[q] * static_cast<double>(s[pv.adj_idx[q]]);
+        } else {
+            for (int q = pv.inc_ptr[i]; q < pv.inc_ptr[i + 1]; ++q) {
+                const int t = pv.inc_idx[q];
+                double prod = pv.t_coef64[t];
+                for (int k = pv.t_off[t]; k < pv.t_off[t + 1]; ++k) prod *= static_cast<double>(s[pv.t_vars[k]]);
+                acc += prod * static_cast<double>(s[i]);
+            }
+        }
+        f[i] = acc;
+    }
+}
+
+// update_local_fields_after_flip, wave-cooperative (dense_ising.cpp:88-106; sparse and hubo
+// update lists may touch one element twice, so they run in list order on lane 0).
+__device__ void update_fp64(const ProblemView &pv, double *f, const int8_t *ns, int flip, int8_t old,
+                            int lane, int lanes) {
+    if (pv.kind == QA_KIND_DENSE) {
+        const double ds = -2.0 * static_cast<double>(old);
+        const double *row = pv.J64 + static_cast<int64_t>(flip) * pv.n;
+        for (int j = lane; j < pv.n; j += lanes) {
+            double v = f[j] + row[j] * ds;
+            if (j == flip) v -= row[flip] * ds;
+            f[j] = v;
+        }
+    } else if (lane == 0) {
+        if (pv.kind == QA_KIND_SPARSE) {
+            const double ds = -2.0 * static_cast<double>(old);
+            for (int q = pv.adj_ptr[flip]; q < pv.adj_ptr[flip + 1]; ++q)
+                f[pv.adj_idx[q]] += pv.adj_val64[q] * ds;
+        } else {
+            for (int q = pv.inc_ptr[flip]; q < pv.inc_ptr[flip + 1]; ++q) {
+                const int t = pv.inc_idx[q];
+                double prod = pv.t_coef64[t];
+                for (int k = pv.t_off[t]; k < pv.t_off[t + 1]; ++k) prod *= static_cast<double>(ns[pv.t_vars[k]]);
+                for (int k = pv.t_off[t]; k < pv.t_off[t + 1]; ++k) {
+                    const int v = pv.t_vars[k];
+                    if (v != flip) f[v] += 2.0 * prod * static_cast<double>(ns[v]);
+                }
+            }
+        }
+    }
+}
+
+__device__ double energy_fp64(const ProblemView &pv, const int8_t *s) {
+    double e = pv.c;
+    for (int i = 0; i < pv.n; ++i) e += pv.h64[i] * static_cast<double>(s[i]);
+    if (pv.kind == QA_KIND_DENSE) {
+        for (int i = 0; i < pv.n; ++i) {
+            const double si = static_cast<double>(s[i]);
+            const double *row = pv.J64 + static_cast<int64_t>(i) * pv.n;
+            for (int j = i + 1; j < pv.n; ++j) e += row[j] * si * static_cast<double>(s[j]);
+        }
+    } else if (pv.kind == QA_KIND_SPARSE) {
+        for (int q = 0; q < pv.nedges; ++q)
+            e += pv.ev64[q] * static_cast<double>(s[pv.ei[q]]) * static_cast<double>(s[pv.ej[q]]);
+    } else {
+        for (int t = 0; t < pv.nterms; ++t) {
+            double prod = pv.t_coef64[t];
+            for (int q = pv.t_off[t]; q < pv.t_off[t + 1]; ++q) prod *= static_cast<double>(s[pv.t_vars[q]]);
+            e += prod;
+        }
+    }
+    return e;
+}
+
+// ---- SA (annealer.cpp:33-75): one thread per read ----------------------------------------
+__global__ void sa_validation_kernel(ProblemView pv, int reads, int steps, const double *__restrict__ betas,
+                                     int sweeps, int8_t *spins, int8_t *best, double *best_e,
+                                     double *trace, const double *__restrict__ uni, int64_t ucap,
+                                     int64_t *used, int *status) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= reads) return;
+    const int n = pv.n;
+    int8_t *s = spins + static_cast<int64_t>(r) * n;
+    int8_t *b = best + static_cast<int64_t>(r) * n;
+    const double *u = uni + static_cast<int64_t>(r) * ucap;
+    double e = energy_fp64(pv, s);
+    double be = e;
+    for (int i = 0; i < n; ++i) b[i] = s[i];
+    int64_t cur = 0;
+    for (int st = 0; st < steps; ++st) {
+        const double beta = betas[st];
+        for (int sw = 0; sw < sweeps; ++sw)
+            for (int i = 0; i < n; ++i) {
+                const double d = delta_fp64(pv, s, i);
+                bool acc = d <= 0.0;
+                if (!acc) {
+                    if (cur >= ucap) { status[r] = 1; return; }
+                    acc = u[cur++] < exp(-beta * d);
+                }
+                if (acc) {
+                    s[i] = static_cast<int8_t>(-s[i]);
+                    e += d;
+                    if (e < be) {
+                        be = e;
+                        for (int k = 0; k < n; ++k) b[k] = s[k];
+                    }
+                }
+            }
+        trace[static_cast<int64_t>(r) * steps + st] = e;
+    }
+    best_e[r] = be;
+    used[r] = cur;
+}
+
+// ---- SQA sweep phases (sqa_annealer.cpp:176-315), one wave per replica --------------------
+enum { PH_SLICE = 0, PH_CLUSTER = 1, PH_WORLDLINE = 2 };
+
+struct SqaPhaseArgs {
+    ProblemView pv;
+    int P;
+    int phase;
+    double bs, jp, join;
+    int8_t *spins;         // [R][P][n]
+    double *fields;        // [R][P][n]
+    const int32_t *order;  // [R][n]
+    const uint64_t *stream;// [R][cap]
+    int64_t cap;
+    int64_t *used;         // [R]
+    int *status;           // [R]
+};
+
+__global__ void __launch_bounds__(64) sqa_phase_kernel(SqaPhaseArgs a) {
+    extern __shared__ unsigned char lds[];
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n = a.pv.n, P = a.P;
+    int8_t *S = a.spins + static_cast<int64_t>(r) * P * n;
+    double *F = a.fields + static_cast<int64_t>(r) * P * n;
+    const int32_t *ord = a.order + static_cast<int64_t>(r) * n;
+    const uint64_t *str = a.stream + static_cast<int64_t>(r) * a.cap;
+    char *incl = reinterpret_cast<char *>(lds);                     // [P]
+    int *stk = reinterpret_cast<int *>(lds + ((P + 15) / 16) * 16);  // [P]
+    int64_t cur = 0;
+    bool ok = true;
+    auto sl = [&](int t) { return S + static_cast<int64_t>(t) * n; };
+    auto fl = [&](int t) { return F + static_cast<int64_t>(t) * n; };
+
+    if (a.phase == PH_SLICE) {
+        for (int t = 0; t < P && ok; ++t) {
+            int8_t *cs = sl(t);
+            double *f = fl(t);
+            const int8_t *pv_ = sl(t == 0 ? P - 1 : t - 1);
+            const int8_t *nx = sl((t + 1) % P);
+            for (int k = 0; k < n; ++k) {
+                const int i = ord[k];
+                const int8_t s = cs[i];
+                const double dcl = -2.0 * static_cast<double>(s) * f[i];
+                const double dtr = 2.0 * a.jp * static_cast<double>(s) * static_cast<double>(pv_[i] + nx[i]);
+                const double de = __fma_rn(a.bs, dcl, dtr);  // sqa_annealer.cpp:200
+                bool acc = de <= 0.0;
+                if (!acc) {
+                    if (cur >= a.cap) { ok = false; break; }
+                    acc = canonical_from_u64(str[cur++]) < exp(-de);
+                }
+                if (acc) {
+                    __syncthreads();
+                    if (lane == 0) cs[i] = static_cast<int8_t>(-s);
+                    __syncthreads();
+                    update_fp64(a.pv, f, cs, i, s, lane, 64);
+                    __syncthreads();
+                }
+            }
+        }
+    } else if (a.phase == PH_CLUSTER) {
+        for (int k = 0; k < n && ok; ++k) {
+            const int i = ord[k];
+            const uint64_t seed = lemire_pick(str, &cur, a.cap, static_cast<uint64_t>(P), &ok);
+            if (!ok) break;
+            // Growth along the imaginary-time ring (:219-246): LIFO, prev before next. Every lane
+            // runs the same sequence on identical data; lane 0 stores.
+            if (lane == 0) for (int t = 0; t < P; ++t) incl[t] = 0;
+            __syncthreads();
+            int top = 0;
+            if (lane == 0) { stk[0] = static_cast<int>(seed); incl[seed] = 1; }
+            __syncthreads();
+            top = 1;
+            const int8_t s0 = sl(static_cast<int>(seed))[i];
+            while (top > 0 && ok) {
+                const int t = stk[--top];
+                const int pvt = t == 0 ? P - 1 : t - 1, nxt = (t + 1) % P;
+                bool join_p = false, join_n = false;
+                if (!incl[pvt] && sl(pvt)[i] == s0) {
+                    if (cur >= a.cap) { ok = false; break; }
+                    join_p = canonical_from_u64(str[cur++]) < a.join;
+                }
+                __syncthreads();
+                if (join_p) {
+                    if (lane == 0) { incl[pvt] = 1; stk[top] = pvt; }
+                    ++top;
+                }
+                __syncthreads();
+                if (!incl[nxt] && sl(nxt)[i] == s0) {
+                    if (cur >= a.cap) { ok = false; break; }
+                    join_n = canonical_from_u64(str[cur++]) < a.join;
+                }
+                __syncthreads();
+                if (join_n) {
+                    if (lane == 0) { incl[nxt] = 1; stk[top] = nxt; }
+                    ++top;
+                }
+                __syncthreads();
+            }
+            if (!ok) break;
+            double dcl = 0.0, dtm = 0.0;
+            for (int t = 0; t < P; ++t) {
+                if (!incl[t]) continue;
+                dcl += -2.0 * static_cast<double>(s0) * fl(t)[i];
+                const int pvt = t == 0 ? P - 1 : t - 1, nxt = (t + 1) % P;
+                if (!incl[pvt]) dtm += 2.0 * a.jp * static_cast<double>(s0) * static_cast<double>(sl(pvt)[i]);
+                if (!incl[nxt]) dtm += 2.0 * a.jp * static_cast<double>(s0) * static_cast<double>(sl(nxt)[i]);
+            }
+            const double de = __fma_rn(a.bs, dcl, dtm);  // sqa_annealer.cpp:275
+            bool acc = de <= 0.0;
+            if (!acc) {
+                if (cur >= a.cap) { ok = false; break; }
+                acc = canonical_from_u64(str[cur++]) < exp(-de);
+            }
+            if (acc) {
+                for (int t = 0; t < P; ++t) {
+                    if (!incl[t]) continue;
+                    __syncthreads();
+                    if (lane == 0) sl(t)[i] = static_cast<int8_t>(-s0);
+                    __syncthreads();
+                    update_fp64(a.pv, fl(t), sl(t), i, s0, lane, 64);
+                    __syncthreads();
+                }
+            }
+        }
+    } else {
+        for (int k = 0; k < n; ++k) {
+            const int i = ord[k];
+            double dcl = 0.0;
+            for (int t = 0; t < P; ++t) dcl += -2.0 * static_cast<double>(sl(t)[i]) * fl(t)[i];
+            const double de = a.bs * dcl;
+            bool acc = de <= 0.0;
+            if (!acc) {
+                if (cur >= a.cap) { ok = false; break; }
+                acc = canonical_from_u64(str[cur++]) < exp(-de);
+            }
+            if (acc) {
+                for (int t = 0; t < P; ++t) {
+                    const int8_t old = sl(t)[i];
+                    __syncthreads();
+                    if (lane == 0) sl(t)[i] = static_cast<int8_t>(-old);
+                    __syncthreads();
+                    update_fp64(a.pv, fl(t), sl(t), i, old, lane, 64);
+                    __syncthreads();
+                }
+            }
+        }
+    }
+    if (lane == 0) {
+        a.used[r] = cur;
+        if (!ok) a.status[r] = 1;
+    }
+}
+
+__global__ void fields_kernel(ProblemView pv, const int8_t *spins, double *fields, int64_t states) {
+    const int64_t k = blockIdx.x;
+    if (k >= states) return;
+    fields_fp64(pv, spins + k * pv.n, fields + k * pv.n, threadIdx.x, blockDim.x);
+}
+
+}  // namespace
+
+// Host: fields for `states` states in FP64.
+void fields_fp64_launch(const qa_problem *p, const int8_t *d_spins, double *d_fields, int64_t states,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(fields_kernel, dim3(static_cast<unsigned>(states)), dim3(64), 0, s, view_of(p),
+                       d_spins, d_fields, states);
+    QA_HIP(hipGetLastError());
+}
+
+void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res) {
+    QA_REQUIRE(prm->init_spins && prm->uniforms, "validation mode needs init_spins and uniforms");
+    QA_REQUIRE(prm->uniforms_per_read >= 0, "uniforms_per_read must be >= 0");
+    const int64_t n = p->n, R = prm->reads;
+    for (int64_t k = 0; k < R * n; ++k)
+        QA_REQUIRE(prm->init_spins[k] == 1 || prm->init_spins[k] == -1, "State spins must be -1 or +1.");
+    hipStream_t s = ctx->stream;
+    DevBuf<int8_t> spins(R * n), best(R * n);
+    DevBuf<double> betas(prm->steps), be(R), tr(R * prm->steps), uni(std::max<int64_t>(R * prm->uniforms_per_read, 1));
+    DevBuf<int64_t> used(R);
+    DevBuf<int> status(R);
+    spins.upload(prm->init_spins, R * n, s);
+    betas.upload(prm->betas, prm->steps, s);
+    uni.upload(prm->uniforms, R * prm->uniforms_per_read, s);
+    QA_HIP(hipMemsetAsync(status.ptr, 0, R * sizeof(int), s));
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    const int block = 64;
+    hipLaunchKernelGGL(sa_validation_kernel, dim3(static_cast<unsigned>((R + block - 1) / block)), dim3(block), 0, s,
+                       view_of(p), static_cast<int>(R), static_cast<int>(prm->steps), betas.ptr,
+                       static_cast<int>(prm->sweeps_per_beta), spins.ptr, best.ptr, be.ptr, tr.ptr, uni.ptr,
+                       prm->uniforms_per_read, used.ptr, status.ptr);
+    QA_HIP(hipGetLastError());
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    std::vector<int> st(R);
+    std::vector<int64_t> us(R);
+    status.download(st.data(), R, s);
+    used.download(us.data(), R, s);
+    best.download(res->best_states, R * n, s);
+    be.download(res->best_energies, R, s);
+    tr.download(res->energy_trace, R * prm->steps, s);
+    QA_HIP(hipStreamSynchronize(s));
+    for (int64_t r = 0; r < R; ++r)
+        if (st[r]) throw Error{QA_EINVAL, "validation uniform stream exhausted (uniforms_per_read too small)"};
+    if (res->uniforms_used) std::memcpy(res->uniforms_used, us.data(), R * sizeof(int64_t));
+    if (res->kernel_ms) {
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *res->kernel_ms = ms;
+    }
+}
+
+// SQA validation: per sweep phase the host shuffles each replica's persistent visit order with
+// std::shuffle on that replica's engine (sqa_annealer.cpp:181,216,295), hands the device a
+// chunk of raw engine outputs drawn from a copy of the engine, and afterwards advances the
+// engine by exactly the number of outputs the device consumed.
+void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    QA_REQUIRE(prm->init_spins && prm->replica_seeds, "validation mode needs init_spins and replica_seeds");
+    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
+    QA_REQUIRE(P <= 4096, "validation mode supports at most 4096 Trotter slices");
+    for (int64_t k = 0; k < R * P * n; ++k)
+        QA_REQUIRE(prm->init_spins[k] == 1 || prm->init_spins[k] == -1, "State spins must be -1 or +1.");
+    hipStream_t s = ctx->stream;
+    DevBuf<int8_t> spins(R * P * n);
+    DevBuf<double> fields(R * P * n), energies(R * P);
+    DevBuf<int32_t> order(R * n);
+    DevBuf<int64_t> used(R);
+    DevBuf<int> status(R);
+    spins.upload(prm->init_spins, R * P * n, s);
+    fields_fp64_launch(p, spins.ptr, fields.ptr, R * P, s);
+    QA_HIP(hipMemsetAsync(status.ptr, 0, R * sizeof(int), s));
+
+    std::vector<std::mt19937_64> eng(R);
+    for (int64_t r = 0; r < R; ++r) eng[r].seed(prm->replica_seeds[r]);
+    std::vector<std::vector<std::size_t>> ord(R, std::vector<std::size_t>(n));
+    for (auto &o : ord) std::iota(o.begin(), o.end(), std::size_t{0});
+    const int64_t cap_max = std::max<int64_t>({P * n, n * (2 * P + 2) + 64, n});
+    DevBuf<uint64_t> stream(R * cap_max);
+    std::vector<uint64_t> hstream(R * cap_max);
+    std::vector<int32_t> hord(R * n);
+    std::vector<int64_t> hused(R);
+    std::vector<int> hst(R);
+    std::vector<double> hen(R * P);
+    double best_e = std::numeric_limits<double>::infinity();
+    std::vector<int8_t> best(n, 1);
+    float total_ms = 0.0f;
+
+    auto phase = [&](int ph, double bs, double jp, double join) {
+        const int64_t cap = ph == PH_SLICE ? P * n : (ph == PH_CLUSTER ? n * (2 * P + 2) + 64 : n);
+        for (int64_t r = 0; r < R; ++r) {
+            std::shuffle(ord[r].begin(), ord[r].end(), eng[r]);
+            for (int64_t k = 0; k < n; ++k) hord[r * n + k] = static_cast<int32_t>(ord[r][k]);
+            std::mt19937_64 copy = eng[r];
+            for (int64_t k = 0; k < cap; ++k) hstream[r * cap + k] = copy();
+        }
+        order.upload(hord.data(), R * n, s);
+        stream.upload(hstream.data(), R * cap, s);
+        SqaPhaseArgs a{view_of(p), static_cast<int>(P), ph, bs, jp, join, spins.ptr, fields.ptr, order.ptr,
+                       stream.ptr, cap, used.ptr, status.ptr};
+        const size_t lds = ((P + 15) / 16) * 16 + P * sizeof(int);
+        QA_HIP(hipEventRecord(ctx->ev0, s));
+        hipLaunchKernelGGL(sqa_phase_kernel, dim3(static_cast<unsigned>(R)), dim3(64), lds, s, a);
+        QA_HIP(hipGetLastError());
+        QA_HIP(hipEventRecord(ctx->ev1, s));
+        used.download(hused.data(), R, s);
+        status.download(hst.data(), R, s);
+        QA_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        total_ms += ms;
+        for (int64_t r = 0; r < R; ++r) {
+            if (hst[r]) throw Error{QA_EDEVICE, "validation stream exhausted inside a sweep phase"};
+            eng[r].discard(static_cast<unsigned long long>(hused[r]));
+        }
+    };
+
+    for (int64_t step = 0; step < prm->steps; ++step) {
+        const double beta = prm->betas[step], gamma = prm->gammas[step];
+        const double jp = trotter_coupling(beta, gamma, P);
+        const double bs = beta / static_cast<double>(P);
+        const double join = 1.0 - std::exp(-2.0 * jp);
+        for (int64_t k = 0; k < prm->sweeps_per_beta; ++k) phase(PH_SLICE, bs, jp, join);
+        for (int64_t k = 0; k < prm->cluster_sweeps; ++k) phase(PH_CLUSTER, bs, jp, join);
+        for (int64_t k = 0; k < prm->worldline_sweeps; ++k) phase(PH_WORLDLINE, bs, jp, join);
+        // step tail (sqa_annealer.cpp:455-473): FP64 slice energies on the device, then the
+        // reference's ordered scan for the mean and the first strict minimum.
+        device_energies(p, spins.ptr, R * P, energies.ptr, s);
+        energies.download(hen.data(), R * P, s);
+        QA_HIP(hipStreamSynchronize(s));
+        double avg = 0.0;
+        int64_t arg = -1;
+        for (int64_t k = 0; k < R * P; ++k) {
+            avg += hen[k];
+            if (hen[k] < best_e) { best_e = hen[k]; arg = k; }
+        }
+        if (arg >= 0) {
+            QA_HIP(hipMemcpyAsync(best.data(), spins.ptr + arg * n, n, hipMemcpyDeviceToHost, s));
+            QA_HIP(hipStreamSynchronize(s));
+        }
+        res->energy_trace[step] = avg / static_cast<double>(R * P);
+    }
+    std::memcpy(res->best_state, best.data(), n);
+    *res->best_energy = best_e;
+    if (res->kernel_ms) *res->kernel_ms = total_ms;
+    if (res->accept_rate) *res->accept_rate = -1.0;
+}
+
+}  // namespace qa
