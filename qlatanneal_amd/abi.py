@@ -242,3 +242,70 @@ def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: i
     check(lib().qa_sqa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, kernel_ms=float(ms[0]),
                 accept_rate=float(ar[0]))
+
+
+def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_per_step: int,
+              worldline_sweeps: int, steps: int, swap_interval: int = 1, cluster_sweeps: int = 0,
+              ladders: int = 1, mode: int = MODE_FP32, seed: int = 0, init_spins=None, rung_seeds=None,
+              swap_uniforms=None):
+    betas = np.ascontiguousarray(betas, np.float64)
+    gammas = np.ascontiguousarray(gammas, np.float64)
+    R, n = betas.shape[0], prob.n
+    p = SqaptParams(R, _d(betas), _d(gammas), ladders, trotter_slices, sweeps_per_step, worldline_sweeps,
+                    cluster_sweeps, steps, swap_interval, mode, seed & (2**64 - 1))
+    keep = []
+    if init_spins is not None:
+        init_spins = np.ascontiguousarray(init_spins, np.int8)
+        keep.append(init_spins)
+        p.init_spins = _b(init_spins)
+    if rung_seeds is not None:
+        rung_seeds = np.ascontiguousarray(rung_seeds, np.uint64)
+        keep.append(rung_seeds)
+        p.rung_seeds = rung_seeds.ctypes.data_as(_up)
+    if swap_uniforms is not None:
+        swap_uniforms = np.ascontiguousarray(swap_uniforms, np.float64)
+        keep.append(swap_uniforms)
+        p.swap_uniforms = _d(swap_uniforms)
+        p.swap_uniforms_len = swap_uniforms.shape[0]
+    L = ladders
+    fs, fe = np.zeros((L, R, n), np.int8), np.zeros((L, R))
+    bs, be = np.zeros((L, n), np.int8), np.zeros(L)
+    at, st = np.zeros((L, steps)), np.zeros((L, steps))
+    used, ms = np.zeros(1, np.int64), np.zeros(1)
+    r = SqaptResult(_b(fs), _d(fe), _b(bs), _d(be), _d(at), _d(st), used.ctypes.data_as(_lp), _d(ms))
+    check(lib().qa_sqapt_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
+    return dict(final_states=fs, final_energies=fe, best_state=bs, best_energy=be, average_energy_trace=at,
+                swap_acceptance_trace=st, swap_uniforms_used=int(used[0]), kernel_ms=float(ms[0]))
+
+
+def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: float, gamma_start: float,
+                gamma_end: float, num_steps: int = 100, sweeps_per_step: int = 20, scan_points: int = 16,
+                scan_sweeps: int = 30, scan_burn: int = 10, chi_floor_fraction: float = 1e-6,
+                driver_A0: float = 0.0, beta_ramp_fraction: float = 0.3, beta_ramp_start: float = 0.1,
+                mode: int = MODE_FP32, seed: int = 0, init_spins=None, cell_seeds=None, chi_override=None):
+    p = ChiParams(trotter_slices, replicas, beta, gamma_start, gamma_end, num_steps, sweeps_per_step, scan_points,
+                  scan_sweeps, scan_burn, chi_floor_fraction, driver_A0, beta_ramp_fraction, beta_ramp_start,
+                  mode, seed & (2**64 - 1))
+    keep = []
+    if init_spins is not None:
+        init_spins = np.ascontiguousarray(init_spins, np.int8)
+        keep.append(init_spins)
+        p.init_spins = _b(init_spins)
+    if cell_seeds is not None:
+        cell_seeds = np.ascontiguousarray(cell_seeds, np.uint64)
+        keep.append(cell_seeds)
+        p.cell_seeds = cell_seeds.ctypes.data_as(_up)
+    if chi_override is not None:
+        chi_override = np.ascontiguousarray(chi_override, np.float64)
+        keep.append(chi_override)
+        p.chi_override = _d(chi_override)
+    n = prob.n
+    bs, be = np.zeros(n, np.int8), np.zeros(1)
+    tr, bsch, gsch, ssch = (np.zeros(num_steps) for _ in range(4))
+    ss, sg, sc = (np.zeros(scan_points) for _ in range(3))
+    sc5, ms = np.zeros(5), np.zeros(1)
+    r = ChiResult(_b(bs), _d(be), _d(tr), _d(bsch), _d(gsch), _d(ssch), _d(ss), _d(sg), _d(sc), _d(sc5), _d(ms))
+    check(lib().qa_sqa_chi_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
+    return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, beta_schedule=bsch, gamma_schedule=gsch,
+                s_schedule=ssch, scan_s=ss, scan_gamma=sg, scan_chi_B=sc, s_star=sc5[0], gamma_star=sc5[1],
+                j_perp_star=sc5[2], chi_floor=sc5[3], driver_A0=sc5[4], kernel_ms=float(ms[0]))

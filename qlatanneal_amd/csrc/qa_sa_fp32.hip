@@ -1,0 +1,254 @@
+// Production SA (FP32 local fields, Philox-4x32-10), annealer.cpp:33-75 semantics for `reads`
+// independent chains in one launch.
+//
+// A workgroup owns up to 64 reads (lane = read for the Metropolis decision). Spins and FP32
+// fields of its reads stay in LDS when they fit, else in global memory. Sites are visited in
+// index order 0..n-1 as in the reference; after each step the whole workgroup applies
+// f += J[i,:] * (-2 s_old) for the reads that flipped (all of them use row i, loaded once).
+// Energies: E starts from an FP64 evaluation (fields accumulated in FP64) and accumulates the
+// accepted dE; the best state is copied after every improving accepted flip, as in the
+// reference, and the returned best energy is recomputed in FP64 from the returned state.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "qa_internal.hpp"
+#include "qa_rng.hpp"
+
+namespace qa {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxReads = 64;
+enum : uint32_t { TAG_SA_INIT = 11, TAG_SA_STEP = 12 };
+
+struct SaArgs {
+    ProblemView pv;
+    int reads, steps, sweeps;
+    const float *betas;     // [steps]
+    uint32_t k0, k1;
+    int8_t *gS;             // [reads][n] (storage when not in LDS)
+    float *gF;              // [reads][n]
+    int8_t *best;           // [reads][n]
+    double *best_e;         // [reads]  (running-energy value at the best state)
+    double *trace;          // [reads][steps]
+    int ldf;
+};
+
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x;
+    const int n = a.pv.n;
+    const int r0 = blockIdx.x * kMaxReads;
+    const int G = min(kMaxReads, a.reads - r0);
+    const int ldf = a.ldf;
+    unsigned char *p = smem;
+    float *F;
+    int8_t *S;
+    if constexpr (kLds) {
+        F = reinterpret_cast<float *>(p);
+        p += static_cast<size_t>(kMaxReads) * ldf * sizeof(float);
+        S = reinterpret_cast<int8_t *>(p);
+        p += ((static_cast<size_t>(kMaxReads) * n + 15) / 16) * 16;
+    } else {
+        F = a.gF + static_cast<int64_t>(r0) * n;
+        S = a.gS + static_cast<int64_t>(r0) * n;
+    }
+    float *DS = reinterpret_cast<float *>(p);
+    p += kMaxReads * sizeof(float);
+    double *E = reinterpret_cast<double *>(p);
+    p += kMaxReads * sizeof(double);
+    unsigned long long *masks = reinterpret_cast<unsigned long long *>(p);  // [0] flips, [1] improved
+
+    for (int e = tid; e < G * n; e += kThreads) {
+        const int c = e / n, i = e - c * n;
+        const Philox4 r = philox4x32_10(static_cast<uint32_t>(r0 + c), static_cast<uint32_t>(i), 0u, TAG_SA_INIT,
+                                        a.k0, a.k1);
+        S[static_cast<int64_t>(c) * n + i] = (r.x & 1u) ? 1 : -1;
+    }
+    if (tid < kMaxReads) E[tid] = 0.0;
+    __syncthreads();
+    // fields in FP64 (stored as FP32) and the exact starting energy
+    for (int c = 0; c < G; ++c) {
+        const int8_t *sc = S + static_cast<int64_t>(c) * n;
+        double part = 0.0;
+        for (int i = tid; i < n; i += kThreads) {
+            double acc = a.pv.h64[i];
+            if (a.pv.kind == QA_KIND_DENSE) {
+                const double *row = a.pv.J64 + static_cast<int64_t>(i) * n;
+                for (int j = 0; j < n; ++j)
+                    if (j != i) acc += row[j] * sc[j];
+            } else if (a.pv.kind == QA_KIND_SPARSE) {
+                for (int q = a.pv.adj_ptr[i]; q < a.pv.adj_ptr[i + 1]; ++q) acc += a.pv.adj_val64[q] * sc[a.pv.adj_idx[q]];
+            } else {
+                for (int q = a.pv.inc_ptr[i]; q < a.pv.inc_ptr[i + 1]; ++q) {
+                    const int t = a.pv.inc_idx[q];
+                    double prod = a.pv.t_coef64[t];
+                    for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k)
+                        if (a.pv.t_vars[k] != i) prod *= sc[a.pv.t_vars[k]];
+                    acc += prod;
+                }
+            }
+            F[static_cast<int64_t>(c) * ldf + i] = static_cast<float>(acc);
+            if (a.pv.kind != QA_KIND_HUBO) part += 0.5 * sc[i] * (a.pv.h64[i] + acc);
+        }
+        if (a.pv.kind == QA_KIND_HUBO) {
+            for (int i = tid; i < n; i += kThreads) part += a.pv.h64[i] * sc[i];
+            for (int t = tid; t < a.pv.nterms; t += kThreads) {
+                double prod = a.pv.t_coef64[t];
+                for (int q = a.pv.t_off[t]; q < a.pv.t_off[t + 1]; ++q) prod *= sc[a.pv.t_vars[q]];
+                part += prod;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        if ((tid & 63) == 0) atomicAdd(&E[c], part);
+    }
+    __syncthreads();
+    double ecur = 0.0, ebest = 0.0;
+    if (tid < G) {
+        ecur = a.pv.c + E[tid];
+        ebest = ecur;
+    }
+    for (int i = tid; i < G * n; i += kThreads) a.best[static_cast<int64_t>(r0) * n + i] = S[(i / n) * static_cast<int64_t>(n) + (i % n)];
+    __syncthreads();
+
+    uint32_t ctr = 0;
+    for (int step = 0; step < a.steps; ++step) {
+        const float beta = a.betas[step];
+        for (int sw = 0; sw < a.sweeps; ++sw) {
+            Philox4 rnd{0, 0, 0, 0};
+            for (int i = 0; i < n; ++i, ++ctr) {
+                if (tid < kMaxReads) {
+                    bool acc = false, imp = false;
+                    if ((ctr & 3) == 0)
+                        rnd = philox4x32_10(static_cast<uint32_t>(r0 + tid), ctr >> 2, 0u, TAG_SA_STEP, a.k0, a.k1);
+                    if (tid < G) {
+                        int8_t *sc = S + static_cast<int64_t>(tid) * n;
+                        const float fs = static_cast<float>(sc[i]);
+                        const float de = -2.0f * fs * F[static_cast<int64_t>(tid) * ldf + i];
+                        const uint32_t q = ctr & 3;
+                        const uint32_t w = q == 0 ? rnd.x : q == 1 ? rnd.y : q == 2 ? rnd.z : rnd.w;
+                        acc = de <= 0.0f || u01(w) < __expf(-beta * de);
+                        DS[tid] = -2.0f * fs;
+                        if (acc) {
+                            sc[i] = static_cast<int8_t>(-sc[i]);
+                            ecur += static_cast<double>(de);
+                            if (ecur < ebest) {
+                                ebest = ecur;
+                                imp = true;
+                            }
+                        }
+                    }
+                    const unsigned long long fm = __ballot(acc), im = __ballot(imp);
+                    if (tid == 0) {
+                        masks[0] = fm;
+                        masks[1] = im;
+                    }
+                }
+                __syncthreads();
+                unsigned long long m = masks[0];
+                if (m) {
+                    if (a.pv.kind == QA_KIND_DENSE) {
+                        const float *row = a.pv.J32 + static_cast<int64_t>(i) * n;
+                        for (int j = tid; j < n; j += kThreads) {
+                            const float rj = row[j];
+                            unsigned long long mm = m;
+                            while (mm) {
+                                const int c = __ffsll(static_cast<long long>(mm)) - 1;
+                                mm &= mm - 1ull;
+                                float *fc = F + static_cast<int64_t>(c) * ldf;
+                                fc[j] = fmaf(DS[c], rj, fc[j]);
+                            }
+                        }
+                    } else if (tid < G && ((m >> tid) & 1ull)) {
+                        float *fc = F + static_cast<int64_t>(tid) * ldf;
+                        const int8_t *sc = S + static_cast<int64_t>(tid) * n;
+                        if (a.pv.kind == QA_KIND_SPARSE) {
+                            for (int q = a.pv.adj_ptr[i]; q < a.pv.adj_ptr[i + 1]; ++q)
+                                fc[a.pv.adj_idx[q]] += a.pv.adj_val32[q] * DS[tid];
+                        } else {
+                            for (int q = a.pv.inc_ptr[i]; q < a.pv.inc_ptr[i + 1]; ++q) {
+                                const int t = a.pv.inc_idx[q];
+                                float prod = a.pv.t_coef32[t];
+                                for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k) prod *= sc[a.pv.t_vars[k]];
+                                for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k) {
+                                    const int v = a.pv.t_vars[k];
+                                    if (v != i) fc[v] += 2.0f * prod * sc[v];
+                                }
+                            }
+                        }
+                    }
+                    unsigned long long im = masks[1];
+                    while (im) {
+                        const int c = __ffsll(static_cast<long long>(im)) - 1;
+                        im &= im - 1ull;
+                        for (int j = tid; j < n; j += kThreads)
+                            a.best[static_cast<int64_t>(r0 + c) * n + j] = S[static_cast<int64_t>(c) * n + j];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (tid < G) a.trace[static_cast<int64_t>(r0 + tid) * a.steps + step] = ecur;
+    }
+    if (tid < G) a.best_e[r0 + tid] = ebest;
+}
+
+}  // namespace
+
+void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res) {
+    if (p->kind == QA_KIND_DENSE && !p->symmetric)
+        throw Error{QA_EUNSUPPORTED, "FP32 mode needs a symmetric J; use the FP64 validation mode for an asymmetric J"};
+    const int64_t n = p->n, R = prm->reads;
+    const int ldf = static_cast<int>(n + 1);
+    const size_t small = kMaxReads * (4 + 8) + 32;
+    const size_t big = small + static_cast<size_t>(kMaxReads) * ldf * 4 + ((static_cast<size_t>(kMaxReads) * n + 15) / 16) * 16;
+    const bool use_lds = big <= 160 * 1024;
+    const size_t shmem = use_lds ? big : small;
+    hipStream_t s = ctx->stream;
+    std::vector<float> hb(prm->steps);
+    for (int64_t k = 0; k < prm->steps; ++k) hb[k] = static_cast<float>(prm->betas[k]);
+    DevBuf<float> betas(prm->steps), gF(use_lds ? 1 : R * n);
+    DevBuf<int8_t> gS(use_lds ? 1 : R * n), best(R * n);
+    DevBuf<double> be(R), tr(R * prm->steps), ex(R);
+    betas.upload(hb.data(), prm->steps, s);
+    SaArgs a{};
+    a.pv = view_of(p);
+    a.reads = static_cast<int>(R);
+    a.steps = static_cast<int>(prm->steps);
+    a.sweeps = static_cast<int>(prm->sweeps_per_beta);
+    a.betas = betas.ptr;
+    a.k0 = static_cast<uint32_t>(prm->seed);
+    a.k1 = static_cast<uint32_t>(prm->seed >> 32);
+    a.gS = gS.ptr;
+    a.gF = gF.ptr;
+    a.best = best.ptr;
+    a.best_e = be.ptr;
+    a.trace = tr.ptr;
+    a.ldf = use_lds ? ldf : static_cast<int>(n);
+    auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
+    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(shmem)));
+    const unsigned grid = static_cast<unsigned>((R + kMaxReads - 1) / kMaxReads);
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), shmem, s, a);
+    QA_HIP(hipGetLastError());
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    device_energies(p, best.ptr, R, ex.ptr, s);
+    best.download(res->best_states, R * n, s);
+    ex.download(res->best_energies, R, s);
+    tr.download(res->energy_trace, R * prm->steps, s);
+    QA_HIP(hipStreamSynchronize(s));
+    if (res->uniforms_used)
+        for (int64_t r = 0; r < R; ++r) res->uniforms_used[r] = 0;
+    if (res->kernel_ms) {
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *res->kernel_ms = ms;
+    }
+}
+
+}  // namespace qa

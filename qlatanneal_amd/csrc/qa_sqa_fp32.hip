@@ -33,7 +33,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxChains = 64;
 
-enum : uint32_t { TAG_INIT = 1, TAG_ORDER = 2, TAG_SLICE = 3, TAG_WORLD = 4, TAG_CLUSTER = 5 };
+enum : uint32_t { TAG_INIT = 1, TAG_ORDER = 2, TAG_SLICE = 3, TAG_WORLD = 4, TAG_CLUSTER = 5, TAG_CHECKER = 6 };
 
 struct SqaArgs {
     ProblemView pv;
@@ -51,6 +51,19 @@ struct SqaArgs {
     int8_t *best_state;    // [ngroups][n]
     unsigned long long *accepted;  // [ngroups]
     int ldf;               // row stride of F (LDS: n+1)
+    int ring;              // J rows staged in LDS (0: read rows from global memory)
+    int pfd;               // prefetch distance of the row ring, in steps
+    const int *param_of_rep;  // non-null: replica rep uses bs/jp/join[param_of_rep[rep]] (ladder rungs)
+    int resume;            // 1: continue from gS/gF instead of a fresh random state
+    int save_fields;       // 1: write F back to gF at the end (LDS layout)
+    int tail_mode;         // 0: group minimum + trace partials; 1: per-chain energies and bond sums
+    double *chain_e;       // tail_mode 1: [R*P] slice energies at the end of the launch
+    int *chain_b;          // tail_mode 1: [R*P] sum_i s_t,i s_t+1,i
+    uint32_t sweep_base;   // sweep counter offset of this launch (random-stream continuity)
+    int sweep_kind;        // 0: SQA slice sweeps (wavefront); 1: sqa_chi parity checkerboard
+    int mag_burn;          // checkerboard: sweeps before magnetisation is accumulated
+    double *mag_m2;        // [R] sum over measured sweeps of m^2 (null: no measurement)
+    double *mag_abs;       // [R] sum over measured sweeps of |m|
 };
 
 template <bool kLds>
@@ -79,6 +92,10 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     }
     int *order = reinterpret_cast<int *>(p);
     p += static_cast<size_t>(n) * sizeof(int);
+    float *Jring = reinterpret_cast<float *>(p);  // [ring][n]: J rows of upcoming positions
+    p += static_cast<size_t>(a.ring) * n * sizeof(float);
+    int *RSLOT = reinterpret_cast<int *>(p);
+    p += kMaxChains * sizeof(int);
     float *DS = reinterpret_cast<float *>(p);
     p += kMaxChains * sizeof(float);
     int *ROW = reinterpret_cast<int *>(p);
@@ -97,34 +114,45 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
 
     const uint32_t cbase = static_cast<uint32_t>(r0 * P);  // global chain id of chain 0
 
-    // ---- initial spins (SQAState::random analogue) and fields ------------------------------
-    for (int e = tid; e < G * n; e += kThreads) {
-        const int c = e / n, i = e - c * n;
-        const Philox4 r = philox4x32_10(cbase + c, static_cast<uint32_t>(i), 0u, TAG_INIT, a.k0, a.k1);
-        S[static_cast<int64_t>(c) * n + i] = (r.x & 1u) ? 1 : -1;
-    }
-    __syncthreads();
-    for (int e = tid; e < G * n; e += kThreads) {
-        const int c = e / n, i = e - c * n;
-        const int8_t *sc = S + static_cast<int64_t>(c) * n;
-        double acc = a.pv.h32[i];
-        if (a.pv.kind == QA_KIND_DENSE) {
-            for (int j = 0; j < n; ++j) acc += static_cast<double>(a.pv.J32[static_cast<int64_t>(j) * n + i]) * sc[j];
-        } else if (a.pv.kind == QA_KIND_SPARSE) {
-            for (int q = a.pv.adj_ptr[i]; q < a.pv.adj_ptr[i + 1]; ++q)
-                acc += static_cast<double>(a.pv.adj_val32[q]) * sc[a.pv.adj_idx[q]];
-        } else {
-            for (int q = a.pv.inc_ptr[i]; q < a.pv.inc_ptr[i + 1]; ++q) {
-                const int t = a.pv.inc_idx[q];
-                double prod = a.pv.t_coef32[t];
-                for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k)
-                    if (a.pv.t_vars[k] != i) prod *= sc[a.pv.t_vars[k]];
-                acc += prod;
+    if (a.resume) {
+        if constexpr (kLds) {
+            for (int e = tid; e < G * n; e += kThreads) {
+                const int c = e / n, i = e - c * n;
+                S[e] = a.gS[static_cast<int64_t>(r0) * P * n + e];
+                F[static_cast<int64_t>(c) * ldf + i] = a.gF[static_cast<int64_t>(r0) * P * n + e];
             }
         }
-        F[static_cast<int64_t>(c) * ldf + i] = static_cast<float>(acc);
+        __syncthreads();
+    } else {
+        // ---- initial spins (SQAState::random analogue) and fields ------------------------------
+        for (int e = tid; e < G * n; e += kThreads) {
+            const int c = e / n, i = e - c * n;
+            const Philox4 r = philox4x32_10(cbase + c, static_cast<uint32_t>(i), 0u, TAG_INIT, a.k0, a.k1);
+            S[static_cast<int64_t>(c) * n + i] = (r.x & 1u) ? 1 : -1;
+        }
+        __syncthreads();
+        for (int e = tid; e < G * n; e += kThreads) {
+            const int c = e / n, i = e - c * n;
+            const int8_t *sc = S + static_cast<int64_t>(c) * n;
+            double acc = a.pv.h32[i];
+            if (a.pv.kind == QA_KIND_DENSE) {
+                for (int j = 0; j < n; ++j) acc += static_cast<double>(a.pv.J32[static_cast<int64_t>(j) * n + i]) * sc[j];
+            } else if (a.pv.kind == QA_KIND_SPARSE) {
+                for (int q = a.pv.adj_ptr[i]; q < a.pv.adj_ptr[i + 1]; ++q)
+                    acc += static_cast<double>(a.pv.adj_val32[q]) * sc[a.pv.adj_idx[q]];
+            } else {
+                for (int q = a.pv.inc_ptr[i]; q < a.pv.inc_ptr[i + 1]; ++q) {
+                    const int t = a.pv.inc_idx[q];
+                    double prod = a.pv.t_coef32[t];
+                    for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k)
+                        if (a.pv.t_vars[k] != i) prod *= sc[a.pv.t_vars[k]];
+                    acc += prod;
+                }
+            }
+            F[static_cast<int64_t>(c) * ldf + i] = static_cast<float>(acc);
+        }
     }
-    double gbest = (tid == 0) ? a.best_e[g] : 0.0;
+    double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
     unsigned long long nacc = 0;
     __syncthreads();
 
@@ -143,7 +171,8 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 const int c = __ffsll(static_cast<long long>(m)) - 1;
                 m &= m - 1ull;
                 const float ds = DS[c];
-                const float *row = a.pv.J32 + static_cast<int64_t>(ROW[c]) * n;
+                const float *row = a.ring ? Jring + static_cast<int64_t>(RSLOT[c]) * n
+                                          : a.pv.J32 + static_cast<int64_t>(ROW[c]) * n;
                 float *fc = F + static_cast<int64_t>(c) * ldf;
                 for (int j = tid; j < n; j += kThreads) fc[j] = fmaf(ds, row[j], fc[j]);
             }
@@ -188,9 +217,43 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
         __syncthreads();
     };
 
-    uint32_t sweep_id = 0;
+    // Row ring: the J rows of order positions [k, k + pfd] are staged in LDS slot (position % ring)
+    // before they are needed; position k + pfd is fetched into registers during step k and stored
+    // after that step's update, into the slot of a position no chain reads any more.
+    constexpr int kPre = 8;  // registers per thread for one prefetched row (n <= 256 * kPre)
+    float pre[kPre];
+    auto ring_preload = [&]() {
+        if (!a.ring) return;
+        const int m = min(a.pfd, n);
+        for (int q = 0; q < m; ++q) {
+            const float *row = a.pv.J32 + static_cast<int64_t>(order[q]) * n;
+            for (int j = tid; j < n; j += kThreads) Jring[static_cast<int64_t>(q % a.ring) * n + j] = row[j];
+        }
+        __syncthreads();
+    };
+    auto ring_fetch = [&](int pos) {
+        if (!a.ring || pos >= n) return;
+        const float *row = a.pv.J32 + static_cast<int64_t>(order[pos]) * n;
+#pragma unroll
+        for (int q = 0; q < kPre; ++q) {
+            const int j = tid + q * kThreads;
+            if (j < n) pre[q] = row[j];
+        }
+    };
+    auto ring_store = [&](int pos) {
+        if (!a.ring || pos >= n) return;
+        float *dst = Jring + static_cast<int64_t>(pos % a.ring) * n;
+#pragma unroll
+        for (int q = 0; q < kPre; ++q) {
+            const int j = tid + q * kThreads;
+            if (j < n) dst[j] = pre[q];
+        }
+    };
+
+    uint32_t sweep_id = a.sweep_base;
     for (int step = 0; step < a.steps; ++step) {
-        const float bs = a.bs[step], jp = a.jp[step], join = a.join[step];
+        const int pidx = a.param_of_rep ? (decider ? a.param_of_rep[r0 + rr_me] : 0) : step;
+        const float bs = a.bs[pidx], jp = a.jp[pidx], join = a.join[pidx];
         // ---------------- slice sweeps (sqa_annealer.cpp:180-209) --------------------------
         // Skewed wavefront: at step tau, slice t works on order position k = tau - t. Slice t-1
         // has already passed position k (so s_{t-1}[i_k] is its value after this sweep) and slice
@@ -198,11 +261,82 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
         // what the reference's slice-by-slice loop sees, including the ring closure where slice
         // P-1 reads slice 0's swept value. Simultaneous updates always touch distinct spin
         // indices, so they never interact. A sweep takes n + P - 1 steps.
+        if (a.sweep_kind == 1) {
+            // sqa_chi checkerboard (sqa_chi_annealer.cpp:81-140): each parity class of slices
+            // sweeps sites 0..n-1 in order while the other class is held fixed; for odd P the
+            // last slice (parity 0, neighbour of slice 0) forms its own phase after the other
+            // even slices, which is the reference's single-thread cell order.
+            for (int i = tid; i < n; i += kThreads) order[i] = i;
+            __syncthreads();
+            const int nph = P == 1 ? 1 : ((P & 1) ? 3 : 2);
+            const int colour = P == 1 ? 0 : ((P & 1) ? (t_me == P - 1 ? 1 : ((t_me & 1) ? 2 : 0)) : (t_me & 1));
+            for (int sw = 0; sw < a.spb; ++sw, ++sweep_id) {
+                Philox4 rnd{0, 0, 0, 0};
+                for (int ph = 0; ph < nph; ++ph) {
+                    ring_preload();
+                    for (int k = 0; k < n; ++k) {
+                        ring_fetch(k + a.pfd);
+                        const int tau = ph * n + k;
+                        if (tid < kMaxChains) {
+                            const bool active = decider && colour == ph;
+                            bool acc = false;
+                            if ((tau & 3) == 0)
+                                rnd = philox4x32_10(cbase + c_me, sweep_id, static_cast<uint32_t>(tau >> 2), TAG_CHECKER,
+                                                    a.k0, a.k1);
+                            if (active) {
+                                int8_t *sc = S + static_cast<int64_t>(c_me) * n;
+                                const int8_t s = sc[k];
+                                const float f = F[static_cast<int64_t>(c_me) * ldf + k];
+                                const int nb = S[static_cast<int64_t>(cprev) * n + k] + S[static_cast<int64_t>(cnext) * n + k];
+                                const float fs = static_cast<float>(s);
+                                const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(nb));
+                                const uint32_t q = tau & 3;
+                                const uint32_t word = q == 0 ? rnd.x : q == 1 ? rnd.y : q == 2 ? rnd.z : rnd.w;
+                                acc = de <= 0.0f || u01(word) < __expf(-de);
+                                DS[c_me] = -2.0f * fs;
+                                ROW[c_me] = k;
+                                if (a.ring) RSLOT[c_me] = k % a.ring;
+                                if (acc) sc[k] = static_cast<int8_t>(-s);
+                            }
+                            const unsigned long long m = __ballot(acc);
+                            if (tid == 0) *flipmask = m;
+                            nacc += acc ? 1ull : 0ull;
+                        }
+                        __syncthreads();
+                        apply_updates();
+                        ring_store(k + a.pfd);
+                        __syncthreads();
+                    }
+                }
+                if (a.mag_m2 && sw >= a.mag_burn) {
+                    // m_r = sum of the replica's spins / (n P), accumulated as m^2 and |m| (:124-139)
+                    const int wave = tid >> 6, lane = tid & 63;
+                    for (int c = wave; c < G; c += kThreads / 64) {
+                        const int8_t *sc = S + static_cast<int64_t>(c) * n;
+                        int sum = 0;
+                        for (int i = lane; i < n; i += 64) sum += sc[i];
+                        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+                        if (lane == 0) ered[c] = static_cast<double>(sum);
+                    }
+                    __syncthreads();
+                    if (tid < nrep) {
+                        double tot = 0.0;
+                        for (int t = 0; t < P; ++t) tot += ered[tid * P + t];
+                        const double m = tot / (static_cast<double>(n) * P);
+                        a.mag_m2[r0 + tid] += m * m;
+                        a.mag_abs[r0 + tid] += fabs(m);
+                    }
+                    __syncthreads();
+                }
+            }
+        } else
         for (int sw = 0; sw < a.spb; ++sw, ++sweep_id) {
             make_order(sweep_id);
+            ring_preload();
             Philox4 rnd{0, 0, 0, 0};
             const int nsteps = n + P - 1;
             for (int tau = 0; tau < nsteps; ++tau) {
+                ring_fetch(tau + a.pfd);
                 if (tid < kMaxChains) {
                     const int k = tau - t_me;
                     const bool active = decider && k >= 0 && k < n;
@@ -223,6 +357,7 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                         acc = de <= 0.0f || u01(word) < __expf(-de);
                         DS[c_me] = -2.0f * fs;
                         ROW[c_me] = i;
+                        if (a.ring) RSLOT[c_me] = k % a.ring;
                         if (acc) sc[i] = static_cast<int8_t>(-s);
                     }
                     const unsigned long long m = __ballot(acc);
@@ -231,14 +366,17 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 }
                 __syncthreads();
                 apply_updates();
+                ring_store(tau + a.pfd);
                 __syncthreads();
             }
         }
         // ---------------- cluster sweeps (sqa_annealer.cpp:215-288) ------------------------
         for (int sw = 0; sw < a.cl; ++sw, ++sweep_id) {
             make_order(sweep_id);
+            ring_preload();
             for (int k = 0; k < n; ++k) {
                 const int i = order[k];
+                ring_fetch(k + a.pfd);
                 if (decider && t_me == 0) {
                     // leader lane of replica rr_me grows the imaginary-time cluster (LIFO, prev
                     // before next), membership as a bit mask over the replica's P <= 64 slices
@@ -282,6 +420,7 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                     const int8_t s = sc[i];
                     DS[c_me] = -2.0f * s;
                     ROW[c_me] = i;
+                    if (a.ring) RSLOT[c_me] = k % a.ring;
                     if (acc) sc[i] = static_cast<int8_t>(-s);
                 }
                 if (tid < kMaxChains) {
@@ -290,14 +429,17 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 }
                 __syncthreads();
                 apply_updates();
+                ring_store(k + a.pfd);
                 __syncthreads();
             }
         }
         // ---------------- worldline sweeps (sqa_annealer.cpp:294-315) -----------------------
         for (int sw = 0; sw < a.wl; ++sw, ++sweep_id) {
             make_order(sweep_id);
+            ring_preload();
             for (int k = 0; k < n; ++k) {
                 const int i = order[k];
+                ring_fetch(k + a.pfd);
                 if (decider) wred[c_me] = -2.0f * S[static_cast<int64_t>(c_me) * n + i] * F[static_cast<int64_t>(c_me) * ldf + i];
                 __syncthreads();
                 if (decider && t_me == 0) {
@@ -316,6 +458,7 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                     const int8_t s = sc[i];
                     DS[c_me] = -2.0f * s;
                     ROW[c_me] = i;
+                    if (a.ring) RSLOT[c_me] = k % a.ring;
                     if (acc) sc[i] = static_cast<int8_t>(-s);
                 }
                 if (tid < kMaxChains) {
@@ -324,11 +467,12 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 }
                 __syncthreads();
                 apply_updates();
+                ring_store(k + a.pfd);
                 __syncthreads();
             }
         }
         // ---------------- step tail: slice energies, group minimum, trace partial ----------
-        {
+        if (a.tail_mode != 2) {
             const int wave = tid >> 6, lane = tid & 63;
             for (int c = wave; c < G; c += kThreads / 64) {
                 const int8_t *sc = S + static_cast<int64_t>(c) * n;
@@ -348,9 +492,20 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 }
                 for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
                 if (lane == 0) ered[c] = a.pv.c + acc;
+                if (a.tail_mode == 1) {
+                    const int t = c % P;
+                    const int8_t *sn = S + static_cast<int64_t>((c / P) * P + (t + 1 == P ? 0 : t + 1)) * n;
+                    int b = 0;
+                    for (int i = lane; i < n; i += 64) b += sc[i] * sn[i];
+                    for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+                    if (lane == 0) {
+                        a.chain_e[static_cast<int64_t>(r0) * P + c] = a.pv.c + acc;
+                        a.chain_b[static_cast<int64_t>(r0) * P + c] = b;
+                    }
+                }
             }
             __syncthreads();
-            if (tid == 0) {
+            if (tid == 0 && a.tail_mode == 0) {
                 double sum = 0.0, mn = gbest;
                 int arg = -1;
                 for (int c = 0; c < G; ++c) {
@@ -367,7 +522,7 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 }
             }
             __syncthreads();
-            const int arg = scal[0];
+            const int arg = a.tail_mode == 0 ? scal[0] : -1;
             if (arg >= 0)
                 for (int i = tid; i < n; i += kThreads)
                     a.best_state[static_cast<int64_t>(g) * n + i] = S[static_cast<int64_t>(arg) * n + i];
@@ -376,7 +531,13 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     }
     // final spins out (for the caller's state readback / diagnostics)
     if constexpr (kLds) {
-        for (int e = tid; e < G * n; e += kThreads) a.gS[static_cast<int64_t>(r0) * P * n + e] = S[e];
+        for (int e = tid; e < G * n; e += kThreads) {
+            a.gS[static_cast<int64_t>(r0) * P * n + e] = S[e];
+            if (a.save_fields) {
+                const int c = e / n, i = e - c * n;
+                a.gF[static_cast<int64_t>(r0) * P * n + e] = F[static_cast<int64_t>(c) * ldf + i];
+            }
+        }
     }
     if (tid < kMaxChains) {
         for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
@@ -384,32 +545,164 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     }
 }
 
+
+// ---- SQA-PT exchange (sqa_parallel_tempering.cpp:299-335), one workgroup per ladder ----------
+// States stay where they are; the rung <-> state-slot map moves. For adjacent rungs a, b holding
+// states with C = sum_t E_t and B = sum_{t,i} s_t,i s_t+1,i, the reference's 4-action difference
+// (e_ab + e_ba) - (e_aa + e_bb) equals ((beta_a - beta_b)/P)(C_b - C_a) - (J_a - J_b)(B_b - B_a).
+struct PtSwapArgs {
+    int R, P, n, steps, step, do_swap;
+    const double *chain_e;
+    const int *chain_b;
+    int *slot_of_rung;  // [L][R]
+    int *rung_of_slot;  // [L*R]
+    const double *bp;   // [R] beta / P
+    const double *jpr;  // [R] J_perp
+    uint32_t k0, k1;
+    const int8_t *gS;
+    double *best_e;     // [L]
+    int8_t *best_state; // [L][n]
+    double *avg_trace;  // [L][steps]
+    double *swap_trace; // [L][steps]
+};
+
+__global__ void __launch_bounds__(64) pt_swap_kernel(PtSwapArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int l = blockIdx.x, tid = threadIdx.x, R = a.R, P = a.P;
+    double *C = reinterpret_cast<double *>(smem);
+    double *E = C + R;
+    long long *B = reinterpret_cast<long long *>(E + R);
+    int *src = reinterpret_cast<int *>(B + R);  // [0]: chain to copy into best_state, or -1
+    int *sl = a.slot_of_rung + static_cast<int64_t>(l) * R;
+    for (int r = tid; r < R; r += 64) {
+        const int base = sl[r] * P;
+        double cs = 0.0, mn = 1e308;
+        long long bs = 0;
+        for (int t = 0; t < P; ++t) {
+            const double e = a.chain_e[base + t];
+            cs += e;
+            bs += a.chain_b[base + t];
+            if (e < mn) mn = e;
+        }
+        C[r] = cs;
+        E[r] = mn;
+        B[r] = bs;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        src[0] = -1;
+        double be = a.best_e[l];
+        for (int r = 0; r < R; ++r)
+            if (E[r] < be) {
+                be = E[r];
+                const int base = sl[r] * P;
+                int bt = 0;
+                for (int t = 0; t < P; ++t)
+                    if (a.chain_e[base + t] == E[r]) { bt = t; break; }
+                src[0] = base + bt;
+            }
+        a.best_e[l] = be;
+        double acc = 0.0, att = 0.0;
+        if (a.do_swap) {
+            for (int r = 0; r + 1 < R; ++r) {
+                const double d = (a.bp[r] - a.bp[r + 1]) * (C[r + 1] - C[r]) -
+                                 (a.jpr[r] - a.jpr[r + 1]) * static_cast<double>(B[r + 1] - B[r]);
+                att += 1.0;
+                bool ok = d <= 0.0;
+                if (!ok) {
+                    const Philox4 q = philox4x32_10(static_cast<uint32_t>(l), static_cast<uint32_t>(a.step),
+                                                    static_cast<uint32_t>(r), 0x5A5A0000u, a.k0, a.k1);
+                    ok = static_cast<double>(u01(q.x)) < exp(-d);
+                }
+                if (ok) {
+                    int ts = sl[r]; sl[r] = sl[r + 1]; sl[r + 1] = ts;
+                    double tc = C[r]; C[r] = C[r + 1]; C[r + 1] = tc;
+                    double te = E[r]; E[r] = E[r + 1]; E[r + 1] = te;
+                    long long tb = B[r]; B[r] = B[r + 1]; B[r + 1] = tb;
+                    acc += 1.0;
+                }
+            }
+        }
+        double avg = 0.0;
+        for (int r = 0; r < R; ++r) avg += E[r];
+        a.avg_trace[static_cast<int64_t>(l) * a.steps + a.step] = avg / R;
+        a.swap_trace[static_cast<int64_t>(l) * a.steps + a.step] = att > 0.0 ? acc / att : 0.0;
+        for (int r = 0; r < R; ++r) a.rung_of_slot[sl[r]] = r;
+    }
+    __syncthreads();
+    const int c = src[0];
+    if (c >= 0)
+        for (int i = tid; i < a.n; i += 64)
+            a.best_state[static_cast<int64_t>(l) * a.n + i] = a.gS[static_cast<int64_t>(c) * a.n + i];
+}
+
+__global__ void gather_states(const int8_t *gS, const int *chain_of, int count, int n, int8_t *out) {
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        out[static_cast<int64_t>(k) * n + i] = gS[static_cast<int64_t>(chain_of[k]) * n + i];
+}
 }  // namespace
 
-void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
-    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
+// Launch plan of the FP32 sweep kernel: replicas per workgroup, state home (LDS or global), J-row
+// ring, dynamic LDS size.
+struct SqaPlan {
+    int64_t rg = 1, ngroups = 0;
+    bool use_lds = false;
+    int ring = 0, pfd = 4, ldf = 0;
+    size_t shmem = 0;
+};
+
+SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P) {
+    const int64_t n = p->n;
     if (P > kMaxChains)
         throw Error{QA_EUNSUPPORTED, "FP32 mode runs at most 64 Trotter slices per replica in this build"};
     if (p->kind == QA_KIND_DENSE && !p->symmetric)
         throw Error{QA_EUNSUPPORTED, "FP32 mode needs a symmetric J (the reference's field updates assume it); "
                                      "use the FP64 validation mode for an asymmetric J"};
     QA_REQUIRE(n < (1 << 24), "problem too large");
+    SqaPlan pl;
     // replicas per group: fill a 64-lane wave when there are enough replicas for every CU
-    int64_t rg = std::max<int64_t>(1, kMaxChains / P);
-    while (rg > 1 && (R + rg - 1) / rg < ctx->num_cus) --rg;
-    const int ldf = static_cast<int>(n + 1);
+    pl.rg = std::max<int64_t>(1, kMaxChains / P);
+    while (pl.rg > 1 && (R + pl.rg - 1) / pl.rg < ctx->num_cus) --pl.rg;
+    pl.ldf = static_cast<int>(n + 1);
+    pl.ring = (p->kind == QA_KIND_DENSE && n <= 2048) ? static_cast<int>(P) + pl.pfd + 1 : 0;
     auto lds_bytes = [&](int64_t r, bool lds) {
-        size_t b = static_cast<size_t>(n) * 4 + kMaxChains * (4 + 4 + 8 + 4 + 8 + 4) + 16 + 16;
-        if (lds) b += static_cast<size_t>(r * P) * ldf * 4 + ((static_cast<size_t>(r * P * n) + 15) / 16) * 16;
+        size_t b = static_cast<size_t>(n) * 4 + kMaxChains * (4 + 4 + 8 + 4 + 8 + 4 + 4) + 16 + 16 +
+                   static_cast<size_t>(pl.ring) * n * 4;
+        if (lds) b += static_cast<size_t>(r * P) * pl.ldf * 4 + ((static_cast<size_t>(r * P * n) + 15) / 16) * 16;
         return b;
     };
     const size_t lds_cap = 160 * 1024;
-    while (rg > 1 && lds_bytes(rg, true) > lds_cap) --rg;
-    const bool use_lds = lds_bytes(rg, true) <= lds_cap;
-    const int64_t ngroups = (R + rg - 1) / rg;
-    const size_t shmem = lds_bytes(rg, use_lds);
-    QA_REQUIRE(shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
+    while (pl.rg > 1 && lds_bytes(pl.rg, true) > lds_cap) --pl.rg;
+    pl.use_lds = lds_bytes(pl.rg, true) <= lds_cap;
+    if (!pl.use_lds && pl.ring) {  // keep the state in LDS in preference to the row ring
+        pl.ring = 0;
+        pl.rg = std::max<int64_t>(1, kMaxChains / P);
+        while (pl.rg > 1 && (R + pl.rg - 1) / pl.rg < ctx->num_cus) --pl.rg;
+        while (pl.rg > 1 && lds_bytes(pl.rg, true) > lds_cap) --pl.rg;
+        pl.use_lds = lds_bytes(pl.rg, true) <= lds_cap;
+    }
+    pl.ngroups = (R + pl.rg - 1) / pl.rg;
+    pl.shmem = lds_bytes(pl.rg, pl.use_lds);
+    QA_REQUIRE(pl.shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
+    if (!pl.use_lds) pl.ldf = static_cast<int>(n);
+    return pl;
+}
 
+void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &a) {
+    auto kern = pl.use_lds ? sqa_fp32_kernel<true> : sqa_fp32_kernel<false>;
+    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(pl.shmem)));
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(pl.ngroups)), dim3(kThreads), pl.shmem, ctx->stream, a);
+    QA_HIP(hipGetLastError());
+}
+
+void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
+    const SqaPlan pl = plan_sqa(ctx, p, R, P);
+    const int64_t rg = pl.rg, ngroups = pl.ngroups;
+    const bool use_lds = pl.use_lds;
     hipStream_t s = ctx->stream;
     std::vector<float> hbs(prm->steps), hjp(prm->steps), hjoin(prm->steps);
     for (int64_t k = 0; k < prm->steps; ++k) {
@@ -453,14 +746,12 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     a.best_chain = bchain.ptr;
     a.best_state = bstate.ptr;
     a.accepted = accd.ptr;
-    a.ldf = use_lds ? ldf : static_cast<int>(n);
+    a.ldf = pl.ldf;
+    a.ring = pl.ring;
+    a.pfd = pl.pfd;
 
-    auto kern = use_lds ? sqa_fp32_kernel<true> : sqa_fp32_kernel<false>;
-    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
     QA_HIP(hipEventRecord(ctx->ev0, s));
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(ngroups)), dim3(kThreads), shmem, s, a);
-    QA_HIP(hipGetLastError());
+    launch_sqa(ctx, pl, a);
     QA_HIP(hipEventRecord(ctx->ev1, s));
 
     std::vector<double> hpart(prm->steps * ngroups), hbe(ngroups);
@@ -498,6 +789,286 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
         const double att = static_cast<double>(prm->steps) * prm->sweeps_per_beta * R * P * n;
         *res->accept_rate = att > 0 ? static_cast<double>(tot) / att : 0.0;
     }
+}
+
+
+void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res) {
+    QA_REQUIRE(res->final_states && res->final_energies && res->best_state && res->best_energy &&
+                   res->average_energy_trace && res->swap_acceptance_trace, "qa_sqapt_run: NULL result array");
+    const int64_t n = p->n, R = prm->rungs, L = prm->ladders, P = prm->trotter_slices, steps = prm->steps;
+    const int64_t NS = L * R;  // state slots, one replica each
+    const SqaPlan pl = plan_sqa(ctx, p, NS, P);
+    hipStream_t s = ctx->stream;
+    std::vector<float> hbs(R), hjp(R), hjoin(R);
+    std::vector<double> hbp(R), hjpd(R);
+    for (int64_t r = 0; r < R; ++r) {
+        hjpd[r] = trotter_coupling(prm->betas[r], prm->gammas[r], P);
+        hbp[r] = prm->betas[r] / static_cast<double>(P);
+        hbs[r] = static_cast<float>(hbp[r]);
+        hjp[r] = static_cast<float>(hjpd[r]);
+        hjoin[r] = static_cast<float>(1.0 - std::exp(-2.0 * hjpd[r]));
+    }
+    std::vector<int> sor(NS), ros(NS);
+    for (int64_t l = 0; l < L; ++l)
+        for (int64_t r = 0; r < R; ++r) {
+            sor[l * R + r] = static_cast<int>(l * R + r);
+            ros[l * R + r] = static_cast<int>(r);
+        }
+    DevBuf<float> dbs(R), djp(R), djoin(R), gF(NS * P * n);
+    DevBuf<double> dbp(R), djpd(R), chain_e(NS * P), be(L), at(L * steps), st(L * steps);
+    DevBuf<int> chain_b(NS * P), dsor(NS), dros(NS);
+    DevBuf<int8_t> gS(NS * P * n), bstate(L * n);
+    DevBuf<unsigned long long> accd(pl.ngroups);
+    dbs.upload(hbs.data(), R, s);
+    djp.upload(hjp.data(), R, s);
+    djoin.upload(hjoin.data(), R, s);
+    dbp.upload(hbp.data(), R, s);
+    djpd.upload(hjpd.data(), R, s);
+    dsor.upload(sor.data(), NS, s);
+    dros.upload(ros.data(), NS, s);
+    std::vector<double> inf(L, std::numeric_limits<double>::infinity());
+    be.upload(inf.data(), L, s);
+
+    SqaArgs a{};
+    a.pv = view_of(p);
+    a.P = static_cast<int>(P);
+    a.R = static_cast<int>(NS);
+    a.rg = static_cast<int>(pl.rg);
+    a.steps = 1;
+    a.spb = static_cast<int>(prm->sweeps_per_step);
+    a.wl = static_cast<int>(prm->worldline_sweeps);
+    a.cl = static_cast<int>(prm->cluster_sweeps);
+    a.bs = dbs.ptr;
+    a.jp = djp.ptr;
+    a.join = djoin.ptr;
+    a.k0 = static_cast<uint32_t>(prm->seed);
+    a.k1 = static_cast<uint32_t>(prm->seed >> 32);
+    a.gS = gS.ptr;
+    a.gF = gF.ptr;
+    a.accepted = accd.ptr;
+    a.ldf = pl.ldf;
+    a.ring = pl.ring;
+    a.pfd = pl.pfd;
+    a.param_of_rep = dros.ptr;
+    a.save_fields = 1;
+    a.tail_mode = 1;
+    a.chain_e = chain_e.ptr;
+    a.chain_b = chain_b.ptr;
+    PtSwapArgs w{};
+    w.R = static_cast<int>(R);
+    w.P = static_cast<int>(P);
+    w.n = static_cast<int>(n);
+    w.steps = static_cast<int>(steps);
+    w.chain_e = chain_e.ptr;
+    w.chain_b = chain_b.ptr;
+    w.slot_of_rung = dsor.ptr;
+    w.rung_of_slot = dros.ptr;
+    w.bp = dbp.ptr;
+    w.jpr = djpd.ptr;
+    w.k0 = a.k0 ^ 0x13579BDFu;
+    w.k1 = a.k1;
+    w.gS = gS.ptr;
+    w.best_e = be.ptr;
+    w.best_state = bstate.ptr;
+    w.avg_trace = at.ptr;
+    w.swap_trace = st.ptr;
+    const size_t wsh = static_cast<size_t>(R) * 24 + 16;
+    const uint32_t per_step = static_cast<uint32_t>(prm->sweeps_per_step + prm->worldline_sweeps + prm->cluster_sweeps);
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    for (int64_t step = 0; step < steps; ++step) {
+        a.resume = step > 0 ? 1 : 0;
+        a.sweep_base = static_cast<uint32_t>(step) * per_step;
+        launch_sqa(ctx, pl, a);
+        w.step = static_cast<int>(step);
+        w.do_swap = ((step + 1) % prm->swap_interval == 0) ? 1 : 0;
+        hipLaunchKernelGGL(pt_swap_kernel, dim3(static_cast<unsigned>(L)), dim3(64), wsh, s, w);
+        QA_HIP(hipGetLastError());
+    }
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    std::vector<double> he(NS * P);
+    chain_e.download(he.data(), NS * P, s);
+    dsor.download(sor.data(), NS, s);
+    QA_HIP(hipStreamSynchronize(s));
+    // final states: per rung the projected (lowest-energy) slice of the state it holds
+    std::vector<int> pick(NS);
+    for (int64_t k = 0; k < NS; ++k) {
+        const int base = sor[k] * static_cast<int>(P);
+        int bt = 0;
+        for (int t = 1; t < P; ++t)
+            if (he[base + t] < he[base + bt]) bt = t;
+        pick[k] = base + bt;
+    }
+    DevBuf<int> dpick(NS);
+    DevBuf<int8_t> fs(NS * n);
+    DevBuf<double> fe(NS), bex(L);
+    dpick.upload(pick.data(), NS, s);
+    hipLaunchKernelGGL(gather_states, dim3(static_cast<unsigned>(NS)), dim3(256), 0, s, gS.ptr, dpick.ptr,
+                       static_cast<int>(NS), static_cast<int>(n), fs.ptr);
+    QA_HIP(hipGetLastError());
+    device_energies(p, fs.ptr, NS, fe.ptr, s);
+    device_energies(p, bstate.ptr, L, bex.ptr, s);
+    fs.download(res->final_states, NS * n, s);
+    fe.download(res->final_energies, NS, s);
+    bstate.download(res->best_state, L * n, s);
+    bex.download(res->best_energy, L, s);
+    at.download(res->average_energy_trace, L * steps, s);
+    st.download(res->swap_acceptance_trace, L * steps, s);
+    QA_HIP(hipStreamSynchronize(s));
+    // a ladder whose best was a final-state slice keeps the lower of the two exact energies
+    for (int64_t l = 0; l < L; ++l)
+        for (int64_t r = 0; r < R; ++r)
+            if (res->final_energies[l * R + r] < res->best_energy[l]) {
+                res->best_energy[l] = res->final_energies[l * R + r];
+                std::memcpy(res->best_state + l * n, res->final_states + (l * R + r) * n, n);
+            }
+    if (res->swap_uniforms_used) *res->swap_uniforms_used = 0;
+    if (res->kernel_ms) {
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *res->kernel_ms = ms;
+    }
+}
+
+
+// sqa_chi pilot scan: `points` x R replicas in one launch; replica rep belongs to point rep / R
+// and sweeps (burn + sweeps) checkerboard sweeps on a fresh worldline at that point's J_perp,
+// accumulating sum m^2 and sum |m| over the measured sweeps.
+void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t points,
+                    const std::vector<double> &bs, const std::vector<double> &jp, int64_t burn, int64_t sweeps,
+                    uint64_t seed, std::vector<double> &m2, std::vector<double> &mabs, float *ms) {
+    const int64_t n = p->n, NS = points * R;
+    const SqaPlan pl = plan_sqa(ctx, p, NS, P);
+    hipStream_t s = ctx->stream;
+    std::vector<float> hbs(points), hjp(points), hjoin(points);
+    std::vector<int> pof(NS);
+    for (int64_t q = 0; q < points; ++q) {
+        hbs[q] = static_cast<float>(bs[q]);
+        hjp[q] = static_cast<float>(jp[q]);
+        hjoin[q] = static_cast<float>(1.0 - std::exp(-2.0 * jp[q]));
+    }
+    for (int64_t k = 0; k < NS; ++k) pof[k] = static_cast<int>(k / R);
+    DevBuf<float> dbs(points), djp(points), djoin(points), gF(pl.use_lds ? 1 : NS * P * n);
+    DevBuf<int> dpof(NS);
+    DevBuf<int8_t> gS(NS * P * n);
+    DevBuf<double> dm2(NS), dma(NS);
+    DevBuf<unsigned long long> accd(pl.ngroups);
+    dbs.upload(hbs.data(), points, s);
+    djp.upload(hjp.data(), points, s);
+    djoin.upload(hjoin.data(), points, s);
+    dpof.upload(pof.data(), NS, s);
+    QA_HIP(hipMemsetAsync(dm2.ptr, 0, NS * sizeof(double), s));
+    QA_HIP(hipMemsetAsync(dma.ptr, 0, NS * sizeof(double), s));
+    SqaArgs a{};
+    a.pv = view_of(p);
+    a.P = static_cast<int>(P);
+    a.R = static_cast<int>(NS);
+    a.rg = static_cast<int>(pl.rg);
+    a.steps = 1;
+    a.spb = static_cast<int>(burn + sweeps);
+    a.bs = dbs.ptr;
+    a.jp = djp.ptr;
+    a.join = djoin.ptr;
+    a.k0 = static_cast<uint32_t>(seed);
+    a.k1 = static_cast<uint32_t>(seed >> 32);
+    a.gS = gS.ptr;
+    a.gF = gF.ptr;
+    a.accepted = accd.ptr;
+    a.ldf = pl.ldf;
+    a.ring = pl.ring;
+    a.pfd = pl.pfd;
+    a.param_of_rep = dpof.ptr;
+    a.tail_mode = 2;
+    a.sweep_kind = 1;
+    a.mag_burn = static_cast<int>(burn);
+    a.mag_m2 = dm2.ptr;
+    a.mag_abs = dma.ptr;
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    launch_sqa(ctx, pl, a);
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    m2.resize(NS);
+    mabs.resize(NS);
+    dm2.download(m2.data(), NS, s);
+    dma.download(mabs.data(), NS, s);
+    QA_HIP(hipStreamSynchronize(s));
+    QA_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+}
+
+// sqa_chi production anneal: R replicas, checkerboard sweeps along (beta_k, Gamma_k).
+void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t steps,
+                             const double *betas, const double *gammas, int64_t sweeps, uint64_t seed,
+                             int8_t *best_state, double *best_energy, double *trace, float *ms) {
+    const int64_t n = p->n;
+    const SqaPlan pl = plan_sqa(ctx, p, R, P);
+    const int64_t ngroups = pl.ngroups;
+    hipStream_t s = ctx->stream;
+    std::vector<float> hbs(steps), hjp(steps), hjoin(steps);
+    for (int64_t k = 0; k < steps; ++k) {
+        const double jp = trotter_coupling(betas[k], gammas[k], P);
+        hbs[k] = static_cast<float>(betas[k] / static_cast<double>(P));
+        hjp[k] = static_cast<float>(jp);
+        hjoin[k] = static_cast<float>(1.0 - std::exp(-2.0 * jp));
+    }
+    DevBuf<float> dbs(steps), djp(steps), djoin(steps), gF(pl.use_lds ? 1 : R * P * n);
+    DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
+    DevBuf<double> tpart(steps * ngroups), be(ngroups), ex(1);
+    DevBuf<int> bstep(ngroups), bchain(ngroups);
+    DevBuf<unsigned long long> accd(ngroups);
+    dbs.upload(hbs.data(), steps, s);
+    djp.upload(hjp.data(), steps, s);
+    djoin.upload(hjoin.data(), steps, s);
+    std::vector<double> inf(ngroups, std::numeric_limits<double>::infinity());
+    be.upload(inf.data(), ngroups, s);
+    QA_HIP(hipMemsetAsync(bchain.ptr, 0xff, ngroups * sizeof(int), s));
+    SqaArgs a{};
+    a.pv = view_of(p);
+    a.P = static_cast<int>(P);
+    a.R = static_cast<int>(R);
+    a.rg = static_cast<int>(pl.rg);
+    a.steps = static_cast<int>(steps);
+    a.spb = static_cast<int>(sweeps);
+    a.bs = dbs.ptr;
+    a.jp = djp.ptr;
+    a.join = djoin.ptr;
+    a.k0 = static_cast<uint32_t>(seed);
+    a.k1 = static_cast<uint32_t>(seed >> 32);
+    a.gS = gS.ptr;
+    a.gF = gF.ptr;
+    a.trace_part = tpart.ptr;
+    a.best_e = be.ptr;
+    a.best_step = bstep.ptr;
+    a.best_chain = bchain.ptr;
+    a.best_state = bstate.ptr;
+    a.accepted = accd.ptr;
+    a.ldf = pl.ldf;
+    a.ring = pl.ring;
+    a.pfd = pl.pfd;
+    a.sweep_kind = 1;
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    launch_sqa(ctx, pl, a);
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    std::vector<double> hpart(steps * ngroups), hbe(ngroups);
+    std::vector<int> hstep(ngroups), hchain(ngroups);
+    tpart.download(hpart.data(), hpart.size(), s);
+    be.download(hbe.data(), ngroups, s);
+    bstep.download(hstep.data(), ngroups, s);
+    bchain.download(hchain.data(), ngroups, s);
+    QA_HIP(hipStreamSynchronize(s));
+    QA_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    for (int64_t k = 0; k < steps; ++k) {
+        double sum = 0.0;
+        for (int64_t gi = 0; gi < ngroups; ++gi) sum += hpart[k * ngroups + gi];
+        trace[k] = sum / static_cast<double>(R * P);
+    }
+    int64_t bg = -1;
+    for (int64_t gi = 0; gi < ngroups; ++gi) {
+        if (hchain[gi] < 0) continue;
+        if (bg < 0 || hbe[gi] < hbe[bg] || (hbe[gi] == hbe[bg] && hstep[gi] < hstep[bg])) bg = gi;
+    }
+    QA_REQUIRE(bg >= 0, "no finite slice energy was produced");
+    QA_HIP(hipMemcpyAsync(best_state, bstate.ptr + bg * n, n, hipMemcpyDeviceToHost, s));
+    device_energies(p, bstate.ptr + bg * n, 1, ex.ptr, s);
+    ex.download(best_energy, 1, s);
+    QA_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace qa

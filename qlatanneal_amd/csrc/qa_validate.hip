@@ -320,6 +320,157 @@ __global__ void __launch_bounds__(64) sqa_phase_kernel(SqaPhaseArgs a) {
     }
 }
 
+// ---- SQA-PT sweep phases (sqa_parallel_tempering.cpp:208-296): no field cache, the
+// classical dE of every attempt is the O(n) delta_energy; one wave per rung.
+struct PtPhaseArgs {
+    ProblemView pv;
+    int P;
+    int phase;
+    const double *bs, *jp, *join;  // [rungs]
+    const int *slot;               // [rungs] state buffer holding rung r
+    int8_t *spins;                 // [rungs][P][n] (indexed by slot)
+    const int32_t *order;          // [rungs][n]
+    const uint64_t *stream;        // [rungs][cap]
+    int64_t cap;
+    int64_t *used;
+    int *status;
+};
+
+__global__ void __launch_bounds__(64) pt_phase_kernel(PtPhaseArgs a) {
+    extern __shared__ unsigned char lds[];
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n = a.pv.n, P = a.P;
+    int8_t *S = a.spins + static_cast<int64_t>(a.slot[r]) * P * n;
+    const int32_t *ord = a.order + static_cast<int64_t>(r) * n;
+    const uint64_t *str = a.stream + static_cast<int64_t>(r) * a.cap;
+    const double bs = a.bs[r], jp = a.jp[r], join = a.join[r];
+    char *incl = reinterpret_cast<char *>(lds);
+    int *stk = reinterpret_cast<int *>(lds + ((P + 15) / 16) * 16);
+    int64_t cur = 0;
+    bool ok = true;
+    auto sl = [&](int t) { return S + static_cast<int64_t>(t) * n; };
+    if (a.phase == PH_SLICE) {
+        for (int t = 0; t < P && ok; ++t) {
+            int8_t *cs = sl(t);
+            const int8_t *pv_ = sl(t == 0 ? P - 1 : t - 1), *nx = sl((t + 1) % P);
+            for (int k = 0; k < n; ++k) {
+                const int i = ord[k];
+                const double dcl = delta_fp64(a.pv, cs, i);
+                const double dtr = 2.0 * jp * static_cast<double>(cs[i]) * static_cast<double>(pv_[i] + nx[i]);
+                const double de = __fma_rn(bs, dcl, dtr);  // sqa_parallel_tempering.cpp:215
+                bool acc = de <= 0.0;
+                if (!acc) {
+                    if (cur >= a.cap) { ok = false; break; }
+                    acc = canonical_from_u64(str[cur++]) < exp(-de);
+                }
+                __syncthreads();
+                if (acc && lane == 0) cs[i] = static_cast<int8_t>(-cs[i]);
+                __syncthreads();
+            }
+        }
+    } else if (a.phase == PH_CLUSTER) {
+        for (int k = 0; k < n && ok; ++k) {
+            const int i = ord[k];
+            const uint64_t seed = lemire_pick(str, &cur, a.cap, static_cast<uint64_t>(P), &ok);
+            if (!ok) break;
+            if (lane == 0) {
+                for (int t = 0; t < P; ++t) incl[t] = 0;
+                stk[0] = static_cast<int>(seed);
+                incl[seed] = 1;
+            }
+            __syncthreads();
+            int top = 1;
+            const int8_t s0 = sl(static_cast<int>(seed))[i];
+            while (top > 0 && ok) {
+                const int t = stk[--top];
+                const int pvt = t == 0 ? P - 1 : t - 1, nxt = (t + 1) % P;
+                bool jp_ = false, jn_ = false;
+                if (!incl[pvt] && sl(pvt)[i] == s0) {
+                    if (cur >= a.cap) { ok = false; break; }
+                    jp_ = canonical_from_u64(str[cur++]) < join;
+                }
+                __syncthreads();
+                if (jp_) {
+                    if (lane == 0) { incl[pvt] = 1; stk[top] = pvt; }
+                    ++top;
+                }
+                __syncthreads();
+                if (!incl[nxt] && sl(nxt)[i] == s0) {
+                    if (cur >= a.cap) { ok = false; break; }
+                    jn_ = canonical_from_u64(str[cur++]) < join;
+                }
+                __syncthreads();
+                if (jn_) {
+                    if (lane == 0) { incl[nxt] = 1; stk[top] = nxt; }
+                    ++top;
+                }
+                __syncthreads();
+            }
+            if (!ok) break;
+            double dcl = 0.0, dtm = 0.0;
+            for (int t = 0; t < P; ++t) {
+                if (!incl[t]) continue;
+                dcl += delta_fp64(a.pv, sl(t), i);
+                const int pvt = t == 0 ? P - 1 : t - 1, nxt = (t + 1) % P;
+                const double st = static_cast<double>(sl(t)[i]);
+                if (!incl[pvt]) dtm += 2.0 * jp * st * static_cast<double>(sl(pvt)[i]);
+                if (!incl[nxt]) dtm += 2.0 * jp * st * static_cast<double>(sl(nxt)[i]);
+            }
+            const double de = __fma_rn(bs, dcl, dtm);  // sqa_parallel_tempering.cpp:268
+            bool acc = de <= 0.0;
+            if (!acc) {
+                if (cur >= a.cap) { ok = false; break; }
+                acc = canonical_from_u64(str[cur++]) < exp(-de);
+            }
+            __syncthreads();
+            if (acc && lane == 0)
+                for (int t = 0; t < P; ++t)
+                    if (incl[t]) sl(t)[i] = static_cast<int8_t>(-sl(t)[i]);
+            __syncthreads();
+        }
+    } else {
+        for (int k = 0; k < n; ++k) {
+            const int i = ord[k];
+            double dcl = 0.0;
+            for (int t = 0; t < P; ++t) dcl += delta_fp64(a.pv, sl(t), i);
+            const double de = bs * dcl;
+            bool acc = de <= 0.0;
+            if (!acc) {
+                if (cur >= a.cap) { ok = false; break; }
+                acc = canonical_from_u64(str[cur++]) < exp(-de);
+            }
+            __syncthreads();
+            if (acc && lane == 0)
+                for (int t = 0; t < P; ++t) sl(t)[i] = static_cast<int8_t>(-sl(t)[i]);
+            __syncthreads();
+        }
+    }
+    if (lane == 0) {
+        a.used[r] = cur;
+        if (!ok) a.status[r] = 1;
+    }
+}
+
+// Integer imaginary-time bond sum B = sum_t sum_i s_t,i s_t+1,i of one state per block.
+__global__ void bond_kernel(const int8_t *spins, int P, int n, int64_t *out) {
+    const int8_t *S = spins + static_cast<int64_t>(blockIdx.x) * P * n;
+    int64_t acc = 0;
+    for (int e = threadIdx.x; e < P * n; e += blockDim.x) {
+        const int t = e / n, i = e - t * n;
+        acc += static_cast<int64_t>(S[e]) * S[static_cast<int64_t>((t + 1) % P) * n + i];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    __shared__ int64_t part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s = 0;
+        for (int w = 0; w < static_cast<int>(blockDim.x) / 64; ++w) s += part[w];
+        out[blockIdx.x] = s;
+    }
+}
+
 __global__ void fields_kernel(ProblemView pv, const int8_t *spins, double *fields, int64_t states) {
     const int64_t k = blockIdx.x;
     if (k >= states) return;
@@ -471,5 +622,171 @@ void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, 
     if (res->kernel_ms) *res->kernel_ms = total_ms;
     if (res->accept_rate) *res->accept_rate = -1.0;
 }
+
+
+// SQA-PT validation: rung r keeps its engine (seeded from rung_seeds[r]) and its visit order
+// while states move between rungs on accepted swaps (sqa_parallel_tempering.cpp:320-321); the
+// host keeps the rung -> state-buffer map. Swap decisions use the caller's master-engine uniform
+// stream, consumed only when the 4-action dE is positive.
+#pragma clang fp contract(off)
+void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res) {
+    QA_REQUIRE(prm->ladders == 1, "validation mode runs one ladder");
+    QA_REQUIRE(prm->init_spins && prm->rung_seeds, "validation mode needs init_spins and rung_seeds");
+    QA_REQUIRE(prm->swap_uniforms || prm->swap_uniforms_len == 0, "validation mode needs swap_uniforms");
+    const int64_t n = p->n, R = prm->rungs, P = prm->trotter_slices, steps = prm->steps;
+    QA_REQUIRE(P <= 4096, "validation mode supports at most 4096 Trotter slices");
+    for (int64_t k = 0; k < R * P * n; ++k)
+        QA_REQUIRE(prm->init_spins[k] == 1 || prm->init_spins[k] == -1, "State spins must be -1 or +1.");
+    QA_REQUIRE(res->final_states && res->final_energies && res->best_state && res->best_energy &&
+                   res->average_energy_trace && res->swap_acceptance_trace, "qa_sqapt_run: NULL result array");
+    hipStream_t s = ctx->stream;
+    DevBuf<int8_t> spins(R * P * n);
+    DevBuf<double> energies(R * P), dbs(R), djp(R), djoin(R);
+    DevBuf<int64_t> bonds(R), used(R);
+    DevBuf<int32_t> order(R * n), dslot(R);
+    DevBuf<int> status(R);
+    spins.upload(prm->init_spins, R * P * n, s);
+    QA_HIP(hipMemsetAsync(status.ptr, 0, R * sizeof(int), s));
+    std::vector<double> hbs(R), hjp(R), hjoin(R);
+    for (int64_t r = 0; r < R; ++r) {
+        hbs[r] = prm->betas[r] / static_cast<double>(P);
+        hjp[r] = trotter_coupling(prm->betas[r], prm->gammas[r], P);
+        hjoin[r] = 1.0 - std::exp(-2.0 * hjp[r]);
+    }
+    dbs.upload(hbs.data(), R, s);
+    djp.upload(hjp.data(), R, s);
+    djoin.upload(hjoin.data(), R, s);
+    std::vector<int32_t> slot(R);
+    std::iota(slot.begin(), slot.end(), 0);
+    std::vector<std::mt19937_64> eng(R);
+    for (int64_t r = 0; r < R; ++r) eng[r].seed(prm->rung_seeds[r]);
+    std::vector<std::vector<std::size_t>> ord(R, std::vector<std::size_t>(n));
+    for (auto &o : ord) std::iota(o.begin(), o.end(), std::size_t{0});
+    const int64_t cap_max = std::max<int64_t>({P * n, n * (2 * P + 2) + 64, n});
+    DevBuf<uint64_t> stream(R * cap_max);
+    std::vector<uint64_t> hstream(R * cap_max);
+    std::vector<int32_t> hord(R * n);
+    std::vector<int64_t> hused(R), hbond(R);
+    std::vector<int> hst(R);
+    std::vector<double> hen(R * P);
+    int64_t ucur = 0;
+    float total_ms = 0.0f;
+
+    auto phase = [&](int ph) {
+        const int64_t cap = ph == PH_SLICE ? P * n : (ph == PH_CLUSTER ? n * (2 * P + 2) + 64 : n);
+        for (int64_t r = 0; r < R; ++r) {
+            std::shuffle(ord[r].begin(), ord[r].end(), eng[r]);
+            for (int64_t k = 0; k < n; ++k) hord[r * n + k] = static_cast<int32_t>(ord[r][k]);
+            std::mt19937_64 copy = eng[r];
+            for (int64_t k = 0; k < cap; ++k) hstream[r * cap + k] = copy();
+        }
+        order.upload(hord.data(), R * n, s);
+        stream.upload(hstream.data(), R * cap, s);
+        dslot.upload(slot.data(), R, s);
+        PtPhaseArgs a{view_of(p), static_cast<int>(P), ph, dbs.ptr, djp.ptr, djoin.ptr, dslot.ptr, spins.ptr,
+                      order.ptr, stream.ptr, cap, used.ptr, status.ptr};
+        const size_t lds = ((P + 15) / 16) * 16 + P * sizeof(int);
+        QA_HIP(hipEventRecord(ctx->ev0, s));
+        hipLaunchKernelGGL(pt_phase_kernel, dim3(static_cast<unsigned>(R)), dim3(64), lds, s, a);
+        QA_HIP(hipGetLastError());
+        QA_HIP(hipEventRecord(ctx->ev1, s));
+        used.download(hused.data(), R, s);
+        status.download(hst.data(), R, s);
+        QA_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        total_ms += ms;
+        for (int64_t r = 0; r < R; ++r) {
+            if (hst[r]) throw Error{QA_EDEVICE, "validation stream exhausted inside a sweep phase"};
+            eng[r].discard(static_cast<unsigned long long>(hused[r]));
+        }
+    };
+    auto measure = [&]() {
+        device_energies(p, spins.ptr, R * P, energies.ptr, s);
+        hipLaunchKernelGGL(bond_kernel, dim3(static_cast<unsigned>(R)), dim3(256), 0, s, spins.ptr,
+                           static_cast<int>(P), static_cast<int>(n), bonds.ptr);
+        QA_HIP(hipGetLastError());
+        energies.download(hen.data(), R * P, s);
+        bonds.download(hbond.data(), R, s);
+        QA_HIP(hipStreamSynchronize(s));
+    };
+    // projected energy of the state in buffer b: first strict minimum over slices (:156-170)
+    auto projected = [&](int32_t b, int64_t *best_t) {
+        double m = std::numeric_limits<double>::infinity();
+        int64_t bt = 0;
+        for (int64_t t = 0; t < P; ++t)
+            if (hen[b * P + t] < m) { m = hen[b * P + t]; bt = t; }
+        *best_t = bt;
+        return m;
+    };
+    // action (:128-154): (beta/P) sum_t E_t - J_perp B, contracted as the reference build does
+    auto action = [&](int32_t b, double beta, double gamma) {
+        double cs = 0.0;
+        for (int64_t t = 0; t < P; ++t) cs += hen[b * P + t];
+        const double jpv = trotter_coupling(beta, gamma, P);
+        return std::fma(beta / static_cast<double>(P), cs, -(jpv * static_cast<double>(hbond[b])));
+    };
+    double best_e = std::numeric_limits<double>::infinity();
+    std::vector<int8_t> best(n, 1);
+    std::vector<double> en(R);
+    auto take_best = [&]() {
+        for (int64_t r = 0; r < R; ++r) {
+            int64_t bt;
+            en[r] = projected(slot[r], &bt);
+            if (en[r] < best_e) {
+                best_e = en[r];
+                QA_HIP(hipMemcpyAsync(best.data(), spins.ptr + (static_cast<int64_t>(slot[r]) * P + bt) * n, n,
+                                      hipMemcpyDeviceToHost, s));
+                QA_HIP(hipStreamSynchronize(s));
+            }
+        }
+    };
+    measure();
+    take_best();
+    for (int64_t step = 0; step < steps; ++step) {
+        for (int64_t k = 0; k < prm->sweeps_per_step; ++k) phase(PH_SLICE);
+        for (int64_t k = 0; k < prm->cluster_sweeps; ++k) phase(PH_CLUSTER);
+        for (int64_t k = 0; k < prm->worldline_sweeps; ++k) phase(PH_WORLDLINE);
+        measure();
+        take_best();
+        double acc = 0.0, att = 0.0;
+        if ((step + 1) % prm->swap_interval == 0) {
+            for (int64_t r = 0; r + 1 < R; ++r) {
+                const double eaa = action(slot[r], prm->betas[r], prm->gammas[r]);
+                const double ebb = action(slot[r + 1], prm->betas[r + 1], prm->gammas[r + 1]);
+                const double eab = action(slot[r + 1], prm->betas[r], prm->gammas[r]);
+                const double eba = action(slot[r], prm->betas[r + 1], prm->gammas[r + 1]);
+                const double d = (eab + eba) - (eaa + ebb);
+                att += 1.0;
+                bool ok = d <= 0.0;
+                if (!ok) {
+                    if (ucur >= prm->swap_uniforms_len) throw Error{QA_EINVAL, "swap uniform stream exhausted"};
+                    ok = prm->swap_uniforms[ucur++] < std::exp(-d);
+                }
+                if (ok) {
+                    std::swap(slot[r], slot[r + 1]);
+                    std::swap(en[r], en[r + 1]);
+                    acc += 1.0;
+                }
+            }
+        }
+        double avg = 0.0;
+        for (double e : en) avg += e;
+        res->average_energy_trace[step] = avg / static_cast<double>(R);
+        res->swap_acceptance_trace[step] = att > 0.0 ? acc / att : 0.0;
+    }
+    for (int64_t r = 0; r < R; ++r) {
+        int64_t bt;
+        res->final_energies[r] = projected(slot[r], &bt);
+        QA_HIP(hipMemcpyAsync(res->final_states + r * n, spins.ptr + (static_cast<int64_t>(slot[r]) * P + bt) * n, n,
+                              hipMemcpyDeviceToHost, s));
+    }
+    QA_HIP(hipStreamSynchronize(s));
+    std::memcpy(res->best_state, best.data(), n);
+    *res->best_energy = best_e;
+    if (res->swap_uniforms_used) *res->swap_uniforms_used = ucur;
+    if (res->kernel_ms) *res->kernel_ms = total_ms;
+}
+#pragma clang fp contract(on)
 
 }  // namespace qa

@@ -134,3 +134,104 @@ def test_sqa_fp32_stationary_mean(ctx):
     est = float(np.mean(r["energy_trace"][200:]))
     print("stationary exact", exact, "fp32", est, flush=True)
     assert abs(est - exact) < 0.02 * (1 + abs(exact))
+
+
+def test_sqapt_validation_golden(ctx):
+    from qlatanneal_amd import abi
+    for c in load("sqapt.json")["cases"]:
+        print("sqapt case", c["seed"], flush=True)
+        dp = device_problem(ctx, c["problem"])
+        betas, gammas = dec(c["betas"]), dec(c["gammas"])
+        R = len(betas)
+        P = max(c["trotter_slices"], c["continuous_time_slices"])
+        g = MT64(c["seed"])
+        init = g.spins(R * P * dp.n)
+        seeds = g.raw(R)
+        swap_u = g.uniforms(c["steps"] * (R - 1))
+        r = abi.sqapt_run(dp, betas, gammas, P, c["sweeps_per_step"], c["worldline_sweeps"], c["steps"],
+                          c["swap_interval"], c["cluster_sweeps"], 1, abi.MODE_FP64_VALIDATION, 0, init, seeds, swap_u)
+        exp = out(c["out"])
+        assert bits_equal(r["final_states"][0], exp["final_states"].reshape(R, -1))
+        assert bits_equal(r["final_energies"][0], exp["final_energies"])
+        assert bits_equal(r["best_state"][0], exp["best_state"])
+        assert bits_equal(r["best_energy"], exp["best_energy"])
+        assert bits_equal(r["average_energy_trace"][0], exp["average_energy_trace"])
+        assert bits_equal(r["swap_acceptance_trace"][0], exp["swap_acceptance_trace"])
+
+
+def test_sqapt_fp32_exact_energies_and_distribution(ctx):
+    from scipy.stats import ks_2samp
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    rng = np.random.default_rng(23)
+    n, P, steps = 32, 8, 25
+    J = _sk(rng, n)
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+    betas, gammas = np.geomspace(0.3, 3.0, 6), np.geomspace(3.0, 0.05, 6)
+    orc = Oracle(ORACLE_SO)
+    orc.set_threads(8)
+    prob = Problem.dense(np.zeros(n), J)
+    # one launch with 256 independent ladders vs 256 seeded reference runs
+    r = abi.sqapt_run(dp, betas, gammas, P, 1, 1, steps, 1, 0, ladders=256, mode=abi.MODE_FP32, seed=3)
+    for l in range(4):
+        assert r["best_energy"][l] == orc.energy(prob, r["best_state"][l])[0]
+        for q in range(len(betas)):
+            assert r["final_energies"][l, q] == orc.energy(prob, r["final_states"][l, q])[0]
+    cpu = np.array([orc.sqapt_run(prob, betas, gammas, P, 1, 1, steps, 1, 0, 0, 900 + k)["best_energy"] for k in range(256)])
+    p = ks_2samp(r["best_energy"], cpu).pvalue
+    print("PT KS p", p, r["best_energy"].mean(), cpu.mean(), "swap acc", r["swap_acceptance_trace"].mean(), flush=True)
+    assert p > 0.01
+
+
+def test_sqa_chi_validation_golden(ctx):
+    from qlatanneal_amd import abi
+    for c in load("sqa_chi.json")["cases"]:
+        print("chi case", c["seed"], c["trotter_slices"], flush=True)
+        dp = device_problem(ctx, c["problem"])
+        P, R, SP = c["trotter_slices"], c["replicas"], c["scan_points"]
+        g = MT64(c["seed"])
+        init, seeds = [], []
+        for _ in range(SP + 1):
+            init.append(g.spins(R * P * dp.n))
+            seeds.append(g.raw(R * P))
+        r = abi.sqa_chi_run(dp, P, R, c["beta"], c["gamma_start"], c["gamma_end"], c["num_steps"],
+                            c["sweeps_per_step"], SP, c["scan_sweeps"], c["scan_burn"],
+                            mode=abi.MODE_FP64_VALIDATION, init_spins=np.concatenate(init),
+                            cell_seeds=np.concatenate(seeds))
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(r[k], exp[k]), (c["seed"], k)
+
+
+def test_sqa_chi_fp32(ctx):
+    from scipy.stats import ks_2samp
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    # dense ferromagnetic chain: exact ground state -(n-1) (tests/test_sqa_chi_annealer.cpp:241-255)
+    n = 12
+    J = np.zeros((n, n))
+    for i in range(n - 1):
+        J[i, i + 1] = J[i + 1, i] = -1.0
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+    r = abi.sqa_chi_run(dp, 8, 4, 4.0, 3.0, 0.05, 100, 10, 10, 20, 6, seed=7)
+    assert r["best_energy"] == -(n - 1)
+    assert np.all(np.diff(r["gamma_schedule"]) <= 1e-12)
+    assert abs(r["gamma_schedule"][0] - 3.0) < 1e-9 and abs(r["gamma_schedule"][-1] - 0.05) < 1e-6
+    assert np.all(r["scan_chi_B"] >= 0) and r["scan_chi_B"].max() > r["scan_chi_B"].min()
+    # SK instance: residual energies vs the oracle's run_chi over 256 seeds
+    rng = np.random.default_rng(31)
+    n = 24
+    J = _sk(rng, n)
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+    prob = Problem.dense(np.zeros(n), J)
+    orc = Oracle(ORACLE_SO)
+    orc.set_threads(8)
+    kw = dict(beta=2.0, gamma_start=3.0, gamma_end=0.05, num_steps=20, sweeps_per_step=2, scan_points=8,
+              scan_sweeps=6, scan_burn=2)
+    gpu = np.array([abi.sqa_chi_run(dp, 4, 2, seed=100 + k, **kw)["best_energy"] for k in range(256)])
+    cpu = np.array([orc.sqa_chi_run(prob, 4, 2, seed=700 + k, **kw)["best_energy"] for k in range(256)])
+    for e, s in [(gpu[0], None)]:
+        assert np.isfinite(e)
+    p = ks_2samp(gpu, cpu).pvalue
+    print("chi KS p", p, gpu.mean(), cpu.mean(), flush=True)
+    assert p > 0.01
