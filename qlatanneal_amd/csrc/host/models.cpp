@@ -162,9 +162,16 @@ HigherOrderIsing::HigherOrderIsing(std::size_t n, const std::vector<RawTerm> &te
 double HigherOrderIsing::energy(const int8_t *s, std::size_t n) const {
     need_size(n, n_);
     validate_spins(s, n_);
+    // one running sum: c, then the linear terms, then every coupling term (the order the
+    // reference accumulates in, so the FP64 result is identical)
     double e = c_;
     for (std::size_t i = 0; i < n_; ++i) e += h_[i] * static_cast<double>(s[i]);
-    return e + coupling_energy(s, n);
+    for (std::size_t t = 0; t < coeff_.size(); ++t) {
+        double prod = coeff_[t];
+        for (std::size_t k = offset_[t]; k < offset_[t + 1]; ++k) prod *= static_cast<double>(s[vars_[k]]);
+        e += prod;
+    }
+    return e;
 }
 
 double HigherOrderIsing::coupling_energy(const int8_t *s, std::size_t n) const {
