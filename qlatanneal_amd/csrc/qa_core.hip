@@ -165,6 +165,14 @@ qa_status qa_problem_dense(qa_ctx *ctx, int64_t n, const double *h, const double
         p->h32.upload(hf.data(), n, ctx->stream);
         p->J64.upload(J, n * n, ctx->stream);
         p->J32.upload(Jf.data(), n * n, ctx->stream);
+        if (n % 4 != 0) {
+            const int64_t rn = (n + 3) / 4 * 4;
+            std::vector<float> Jp(static_cast<size_t>(n * rn), 0.0f);
+            for (int64_t i = 0; i < n; ++i)
+                std::copy(Jf.begin() + i * n, Jf.begin() + (i + 1) * n, Jp.begin() + i * rn);
+            p->J32p.alloc(n * rn);
+            p->J32p.upload(Jp.data(), n * rn, ctx->stream);
+        }
         QA_HIP(hipStreamSynchronize(ctx->stream));
         *out = p.release();
     });

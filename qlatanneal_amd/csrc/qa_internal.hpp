@@ -90,6 +90,7 @@ struct qa_problem {
     // dense
     qa::DevBuf<double> J64;      // [n*n] as given
     qa::DevBuf<float> J32;       // [n*n] zero diagonal
+    qa::DevBuf<float> J32p;      // [n][n rounded up to 4], zero padded; only when n % 4 != 0
     // sparse: undirected edges (energy order) + adjacency CSR in reference insertion order
     int64_t nedges = 0;
     qa::DevBuf<int32_t> ei, ej;
@@ -115,6 +116,7 @@ struct ProblemView {
     const float *h32;
     const double *J64;
     const float *J32;
+    const float *J32p;  // J32 with rows padded to a multiple of 4 floats (J32 itself when n % 4 == 0)
     int nedges;
     const int32_t *ei, *ej;
     const double *ev64;
@@ -137,6 +139,7 @@ inline ProblemView view_of(const qa_problem *p) {
     v.h32 = p->h32.ptr;
     v.J64 = p->J64.ptr;
     v.J32 = p->J32.ptr;
+    v.J32p = (p->n % 4 == 0) ? p->J32.ptr : p->J32p.ptr;
     v.nedges = static_cast<int>(p->nedges);
     v.ei = p->ei.ptr;
     v.ej = p->ej.ptr;

@@ -28,24 +28,47 @@ __host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
     return static_cast<uint32_t>((static_cast<uint64_t>(a) * b) >> 32);
 }
 
+// A Philox-4x32 evaluation in progress: counter words and the round key.
+struct PhiloxRun {
+    uint32_t c0, c1, c2, c3, k0, k1;
+
+    __host__ __device__ __forceinline__ void start(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t key0,
+                                                   uint32_t key1) {
+        c0 = a;
+        c1 = b;
+        c2 = c;
+        c3 = d;
+        k0 = key0;
+        k1 = key1;
+    }
+    // Apply R rounds (a full evaluation is 10; evaluations may be spread over several calls).
+    template <int R>
+    __host__ __device__ __forceinline__ void rounds() {
+        constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+        constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t hi0 = mulhi32(M0, c0), lo0 = M0 * c0;
+            const uint32_t hi1 = mulhi32(M1, c2), lo1 = M1 * c2;
+            const uint32_t n0 = hi1 ^ c1 ^ k0;
+            const uint32_t n2 = hi0 ^ c3 ^ k1;
+            c0 = n0;
+            c1 = lo1;
+            c2 = n2;
+            c3 = lo0;
+            k0 += W0;
+            k1 += W1;
+        }
+    }
+    __host__ __device__ __forceinline__ Philox4 result() const { return Philox4{c0, c1, c2, c3}; }
+};
+
 __host__ __device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                                   uint32_t k0, uint32_t k1) {
-    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = mulhi32(M0, c0), lo0 = M0 * c0;
-        const uint32_t hi1 = mulhi32(M1, c2), lo1 = M1 * c2;
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = lo1;
-        c2 = n2;
-        c3 = lo0;
-        k0 += W0;
-        k1 += W1;
-    }
-    return Philox4{c0, c1, c2, c3};
+    PhiloxRun r;
+    r.start(c0, c1, c2, c3, k0, k1);
+    r.rounds<10>();
+    return r.result();
 }
 
 // Uniform float in [0, 1) with 24 random bits.

@@ -20,8 +20,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "qa_internal.hpp"
@@ -30,8 +33,31 @@
 namespace qa {
 namespace {
 
-constexpr int kThreads = 256;
+// Pipelined slice sweeps give the waves roles: wave 0 decides (one lane per chain), waves 1-4
+// apply field updates, wave 5 stages J rows and shuffles the next visit order. The other sweep
+// kinds use all threads alike.
+constexpr int kThreads = 384;
+constexpr int kUpd = 256;        // update threads (tid 64..319)
+constexpr int kHelperWave = 5;
+constexpr int kPreH = 32;        // helper registers for one staged row (n <= 64 * kPreH)
 constexpr int kMaxChains = 64;
+
+// LDS regions start on 16-byte boundaries (the per-chain double arrays need 8).
+__host__ __device__ constexpr size_t lds_align16(size_t b) { return (b + 15) / 16 * 16; }
+
+// Four floats as a plain vector value (16-byte LDS and global accesses).
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// fn(std::integral_constant<int, 0>) ... fn(std::integral_constant<int, N-1>): register arrays
+// indexed this way stay in registers.
+template <typename Fn, int... Is>
+__device__ __forceinline__ void static_for_impl(Fn &&fn, std::integer_sequence<int, Is...>) {
+    (fn(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn &&fn) {
+    static_for_impl(fn, std::make_integer_sequence<int, N>{});
+}
 
 enum : uint32_t { TAG_INIT = 1, TAG_ORDER = 2, TAG_SLICE = 3, TAG_WORLD = 4, TAG_CLUSTER = 5, TAG_CHECKER = 6 };
 
@@ -50,7 +76,8 @@ struct SqaArgs {
     int *best_chain;       // [ngroups]
     int8_t *best_state;    // [ngroups][n]
     unsigned long long *accepted;  // [ngroups]
-    int ldf;               // row stride of F (LDS: n+1)
+    int ldf;               // row stride of F (LDS layout: n rounded up to 4; global layout: n)
+    int ldr;               // row stride of the J-row ring (n rounded up to 4)
     int ring;              // J rows staged in LDS (0: read rows from global memory)
     int pfd;               // prefetch distance of the row ring, in steps
     const int *param_of_rep;  // non-null: replica rep uses bs/jp/join[param_of_rep[rep]] (ladder rungs)
@@ -66,72 +93,100 @@ struct SqaArgs {
     double *mag_abs;       // [R] sum over measured sweeps of |m|
 };
 
-template <bool kLds>
-__global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int g = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int n = a.pv.n, P = a.P;
-    const int r0 = g * a.rg;
-    const int nrep = min(a.rg, a.R - r0);
-    const int G = nrep * P;
-    const int ldf = a.ldf;
 
-    // LDS carve-up
-    unsigned char *p = smem;
-    float *F;
-    int8_t *S;
-    if constexpr (kLds) {
-        F = reinterpret_cast<float *>(p);
-        p += static_cast<size_t>(a.rg) * P * ldf * sizeof(float);
-        S = reinterpret_cast<int8_t *>(p);
-        p += ((static_cast<size_t>(a.rg) * P * n + 15) / 16) * 16;
-    } else {
-        F = a.gF + static_cast<int64_t>(r0) * P * n;
-        S = a.gS + static_cast<int64_t>(r0) * P * n;
+// ---- pieces shared by the sweep kernels -----------------------------------------------------
+// Group state: F[G][ldf] fields and S[G][n] spins of chains r0*P .. r0*P + G - 1 (LDS or global).
+
+// LDS carve-up shared by both sweep kernels (sizes as computed by plan_sqa).
+struct SqaLds {
+    float *F;          // [rg*P][ldf] fields (LDS layout)
+    int8_t *S;         // [rg*P][n] spins (LDS layout)
+    int *order;        // two visit-order buffers of obstride ints
+    int obstride;
+    float *Jring;      // [ring][ldf]: J rows of upcoming order positions
+    int *RSLOT;        // [kMaxChains] ring slot of each chain's site (two-barrier sweeps)
+    float *DS;         // [kMaxChains] -2 s_old of each chain's attempt (two-barrier sweeps)
+    int *ROW;          // [kMaxChains] site of each chain's attempt (two-barrier sweeps)
+    double *ered;      // [kMaxChains] slice energies of the step tail
+    float *wred;       // [kMaxChains] worldline partial sums
+    unsigned long long *cmask;  // [kMaxChains] cluster memberships
+    int *rflag;        // [kMaxChains] worldline decisions
+    unsigned long long *flipmask;  // two-barrier: [0]; pipelined: [b] flips, [2 + b] flips from s = +1
+    int *scal;         // [0]: best chain of the step
+};
+
+__device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) {
+    const int n = a.pv.n;
+    SqaLds L{};
+    if (lds_state) {
+        L.F = reinterpret_cast<float *>(p);
+        p += lds_align16(static_cast<size_t>(a.rg) * a.P * a.ldf * sizeof(float));
+        L.S = reinterpret_cast<int8_t *>(p);
+        p += lds_align16(static_cast<size_t>(a.rg) * a.P * n);
     }
-    int *order = reinterpret_cast<int *>(p);
-    p += static_cast<size_t>(n) * sizeof(int);
-    float *Jring = reinterpret_cast<float *>(p);  // [ring][n]: J rows of upcoming positions
-    p += static_cast<size_t>(a.ring) * n * sizeof(float);
-    int *RSLOT = reinterpret_cast<int *>(p);
+    L.order = reinterpret_cast<int *>(p);
+    L.obstride = static_cast<int>(lds_align16(static_cast<size_t>(n) * sizeof(int)) / sizeof(int));
+    p += 2 * lds_align16(static_cast<size_t>(n) * sizeof(int));
+    L.Jring = reinterpret_cast<float *>(p);
+    p += lds_align16(static_cast<size_t>(a.ring) * a.ldr * sizeof(float));
+    L.RSLOT = reinterpret_cast<int *>(p);
     p += kMaxChains * sizeof(int);
-    float *DS = reinterpret_cast<float *>(p);
+    L.DS = reinterpret_cast<float *>(p);
     p += kMaxChains * sizeof(float);
-    int *ROW = reinterpret_cast<int *>(p);
+    L.ROW = reinterpret_cast<int *>(p);
     p += kMaxChains * sizeof(int);
-    double *ered = reinterpret_cast<double *>(p);
+    L.ered = reinterpret_cast<double *>(p);
     p += kMaxChains * sizeof(double);
-    float *wred = reinterpret_cast<float *>(p);
+    L.wred = reinterpret_cast<float *>(p);
     p += kMaxChains * sizeof(float);
-    unsigned long long *cmask = reinterpret_cast<unsigned long long *>(p);
+    L.cmask = reinterpret_cast<unsigned long long *>(p);
     p += kMaxChains * sizeof(unsigned long long);
-    int *rflag = reinterpret_cast<int *>(p);
+    L.rflag = reinterpret_cast<int *>(p);
     p += kMaxChains * sizeof(int);
-    unsigned long long *flipmask = reinterpret_cast<unsigned long long *>(p);
-    p += 16;
-    int *scal = reinterpret_cast<int *>(p);  // [0]: best chain this step
+    L.flipmask = reinterpret_cast<unsigned long long *>(p);
+    p += 32;
+    L.scal = reinterpret_cast<int *>(p);
+    return L;
+}
 
-    const uint32_t cbase = static_cast<uint32_t>(r0 * P);  // global chain id of chain 0
+// Random permutation of 0..n-1 for sweep sweep_id of the group whose first chain is cbase
+// (Fisher-Yates by one thread, Philox draws: four per counter block); the caller synchronises.
+__device__ void shuffle_order(int *order, int n, uint32_t cbase, uint32_t sweep_id, uint32_t k0, uint32_t k1) {
+    for (int i = 0; i < n; ++i) order[i] = i;
+    for (int i = n - 1; i > 0; i -= 4) {
+        const Philox4 r = philox4x32_10(cbase, sweep_id, static_cast<uint32_t>(i), TAG_ORDER, k0, k1);
+        const uint32_t d[4] = {r.x, r.y, r.z, r.w};
+        for (int q = 0; q < 4 && i - q > 0; ++q) {
+            const int ii = i - q;
+            const int j = static_cast<int>(below(d[q], static_cast<uint32_t>(ii + 1)));
+            const int tmp = order[ii];
+            order[ii] = order[j];
+            order[j] = tmp;
+        }
+    }
+}
 
+
+// Initial spins (SQAState::random analogue) and FP32 fields, or the state of a previous launch.
+__device__ void group_init(const SqaArgs &a, float *F, int8_t *S, int ldf, int G, int r0, bool lds_state) {
+    const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, P = a.P;
+    const uint32_t cbase = static_cast<uint32_t>(r0 * P);
     if (a.resume) {
-        if constexpr (kLds) {
-            for (int e = tid; e < G * n; e += kThreads) {
+        if (lds_state) {
+            for (int e = tid; e < G * n; e += nt) {
                 const int c = e / n, i = e - c * n;
                 S[e] = a.gS[static_cast<int64_t>(r0) * P * n + e];
                 F[static_cast<int64_t>(c) * ldf + i] = a.gF[static_cast<int64_t>(r0) * P * n + e];
             }
         }
-        __syncthreads();
     } else {
-        // ---- initial spins (SQAState::random analogue) and fields ------------------------------
-        for (int e = tid; e < G * n; e += kThreads) {
+        for (int e = tid; e < G * n; e += nt) {
             const int c = e / n, i = e - c * n;
             const Philox4 r = philox4x32_10(cbase + c, static_cast<uint32_t>(i), 0u, TAG_INIT, a.k0, a.k1);
             S[static_cast<int64_t>(c) * n + i] = (r.x & 1u) ? 1 : -1;
         }
         __syncthreads();
-        for (int e = tid; e < G * n; e += kThreads) {
+        for (int e = tid; e < G * n; e += nt) {
             const int c = e / n, i = e - c * n;
             const int8_t *sc = S + static_cast<int64_t>(c) * n;
             double acc = a.pv.h32[i];
@@ -151,10 +206,120 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
             }
             F[static_cast<int64_t>(c) * ldf + i] = static_cast<float>(acc);
         }
+        // row padding (columns n .. ldf-1) stays zero
+        for (int e = tid; e < G * (ldf - n); e += nt) {
+            const int c = e / (ldf - n);
+            F[static_cast<int64_t>(c) * ldf + n + (e - c * (ldf - n))] = 0.0f;
+        }
     }
+    __syncthreads();
+}
+
+// Step tail: slice energies E = c + 1/2 sum_i s_i (h_i + f_i) in FP64 from the FP32 fields (direct
+// sum for HUBO). tail_mode 0 keeps the group's first strict minimum (best_e/best_step/best_chain/
+// best_state; gbest lives in thread 0) and writes the group's partial of the energy trace;
+// tail_mode 1 writes per-chain energies and bond sums sum_i s_t,i s_t+1,i.
+__device__ void group_tail(const SqaArgs &a, const float *F, const int8_t *S, int ldf, int G, int r0, int step,
+                           double *ered, int *scal, double &gbest) {
+    const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, P = a.P, g = blockIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int c = wave; c < G; c += nt / 64) {
+        const int8_t *sc = S + static_cast<int64_t>(c) * n;
+        const float *fc = F + static_cast<int64_t>(c) * ldf;
+        double acc = 0.0;
+        if (a.pv.kind != QA_KIND_HUBO) {
+            for (int i = lane; i < n; i += 64)
+                acc += static_cast<double>(sc[i]) * (static_cast<double>(a.pv.h32[i]) + static_cast<double>(fc[i]));
+            acc *= 0.5;
+        } else {
+            for (int i = lane; i < n; i += 64) acc += static_cast<double>(a.pv.h32[i]) * sc[i];
+            for (int t = lane; t < a.pv.nterms; t += 64) {
+                double prod = a.pv.t_coef32[t];
+                for (int q = a.pv.t_off[t]; q < a.pv.t_off[t + 1]; ++q) prod *= sc[a.pv.t_vars[q]];
+                acc += prod;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) ered[c] = a.pv.c + acc;
+        if (a.tail_mode == 1) {
+            const int t = c % P;
+            const int8_t *sn = S + static_cast<int64_t>((c / P) * P + (t + 1 == P ? 0 : t + 1)) * n;
+            int b = 0;
+            for (int i = lane; i < n; i += 64) b += sc[i] * sn[i];
+            for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+            if (lane == 0) {
+                a.chain_e[static_cast<int64_t>(r0) * P + c] = a.pv.c + acc;
+                a.chain_b[static_cast<int64_t>(r0) * P + c] = b;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && a.tail_mode == 0) {
+        double sum = 0.0, mn = gbest;
+        int arg = -1;
+        for (int c = 0; c < G; ++c) {
+            sum += ered[c];
+            if (ered[c] < mn) { mn = ered[c]; arg = c; }
+        }
+        a.trace_part[static_cast<int64_t>(step) * gridDim.x + g] = sum;
+        scal[0] = arg;
+        if (arg >= 0) {
+            gbest = mn;
+            a.best_e[g] = mn;
+            a.best_step[g] = step;
+            a.best_chain[g] = r0 * P + arg;
+        }
+    }
+    __syncthreads();
+    const int arg = a.tail_mode == 0 ? scal[0] : -1;
+    if (arg >= 0)
+        for (int i = tid; i < n; i += nt) a.best_state[static_cast<int64_t>(g) * n + i] = S[static_cast<int64_t>(arg) * n + i];
+    __syncthreads();
+}
+
+// Final spins (and, for a resumed sequence of launches, fields) of an LDS-resident group.
+__device__ void group_save(const SqaArgs &a, const float *F, const int8_t *S, int ldf, int G, int r0) {
+    const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, P = a.P;
+    for (int e = tid; e < G * n; e += nt) {
+        a.gS[static_cast<int64_t>(r0) * P * n + e] = S[e];
+        if (a.save_fields) {
+            const int c = e / n, i = e - c * n;
+            a.gF[static_cast<int64_t>(r0) * P * n + e] = F[static_cast<int64_t>(c) * ldf + i];
+        }
+    }
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int g = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int n = a.pv.n, P = a.P;
+    const int r0 = g * a.rg;
+    const int nrep = min(a.rg, a.R - r0);
+    const int G = nrep * P;
+    const int ldf = a.ldf;
+
+    const SqaLds L = carve_lds(a, smem, kLds);
+    float *F = kLds ? L.F : a.gF + static_cast<int64_t>(r0) * P * n;
+    int8_t *S = kLds ? L.S : a.gS + static_cast<int64_t>(r0) * P * n;
+    int *order = L.order;
+    float *Jring = L.Jring;
+    int *RSLOT = L.RSLOT;
+    float *DS = L.DS;
+    int *ROW = L.ROW;
+    double *ered = L.ered;
+    float *wred = L.wred;
+    unsigned long long *cmask = L.cmask;
+    int *rflag = L.rflag;
+    unsigned long long *flipmask = L.flipmask;
+    int *scal = L.scal;
+
+    const uint32_t cbase = static_cast<uint32_t>(r0 * P);  // global chain id of chain 0
+
+    group_init(a, F, S, ldf, G, r0, kLds);
     double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
     unsigned long long nacc = 0;
-    __syncthreads();
 
     const bool decider = tid < G;
     const int c_me = tid;
@@ -163,20 +328,51 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     const int cnext = rr_me * P + (t_me + 1 == P ? 0 : t_me + 1);
 
     // Apply the field updates of the chains flagged in flipmask (DS/ROW hold ds and spin index).
+    // Dense: the flipped chains are taken eight at a time with a wave-uniform mask; every LDS
+    // load of a batch (ring row value, field) is issued before any of its stores, so one batch
+    // costs one LDS round trip instead of one per chain and column.
     auto apply_updates = [&]() {
-        unsigned long long m = *flipmask;
-        if (m == 0ull) return;
+        const unsigned long long m_all = *flipmask;
+        if (m_all == 0ull) return;
         if (a.pv.kind == QA_KIND_DENSE) {
+            uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
+            uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
+            unsigned long long m = (static_cast<unsigned long long>(mhi) << 32) | mlo;
             while (m) {
-                const int c = __ffsll(static_cast<long long>(m)) - 1;
-                m &= m - 1ull;
-                const float ds = DS[c];
-                const float *row = a.ring ? Jring + static_cast<int64_t>(RSLOT[c]) * n
-                                          : a.pv.J32 + static_cast<int64_t>(ROW[c]) * n;
-                float *fc = F + static_cast<int64_t>(c) * ldf;
-                for (int j = tid; j < n; j += kThreads) fc[j] = fmaf(ds, row[j], fc[j]);
+                int cs[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (m) {
+                        cs[q] = __ffsll(static_cast<long long>(m)) - 1;
+                        m &= m - 1ull;
+                    } else {
+                        cs[q] = -1;
+                    }
+                }
+                float dsv[8];
+                const float *rows[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (cs[q] >= 0) {
+                        dsv[q] = DS[cs[q]];
+                        rows[q] = a.ring ? Jring + static_cast<int64_t>(RSLOT[cs[q]]) * n
+                                         : a.pv.J32 + static_cast<int64_t>(ROW[cs[q]]) * n;
+                    }
+                }
+                for (int j = tid; j < n; j += kThreads) {
+                    float rv[8], fv[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (cs[q] >= 0) {
+                            rv[q] = rows[q][j];
+                            fv[q] = F[static_cast<int64_t>(cs[q]) * ldf + j];
+                        }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (cs[q] >= 0) F[static_cast<int64_t>(cs[q]) * ldf + j] = fmaf(dsv[q], rv[q], fv[q]);
+                }
             }
-        } else if (tid < G && ((m >> tid) & 1ull)) {
+        } else if (tid < G && ((m_all >> tid) & 1ull)) {
             const int c = tid, i = ROW[c];
             float *fc = F + static_cast<int64_t>(c) * ldf;
             const int8_t *sc = S + static_cast<int64_t>(c) * n;
@@ -199,30 +395,18 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     };
 
     // Random permutation of 0..n-1 for this sweep (Fisher-Yates on one lane, Philox draws).
-    auto make_order = [&](uint32_t sweep_id) {
-        if (tid == 0) {
-            for (int i = 0; i < n; ++i) order[i] = i;
-            for (int i = n - 1; i > 0; i -= 4) {
-                const Philox4 r = philox4x32_10(cbase, sweep_id, static_cast<uint32_t>(i), TAG_ORDER, a.k0, a.k1);
-                const uint32_t d[4] = {r.x, r.y, r.z, r.w};
-                for (int q = 0; q < 4 && i - q > 0; ++q) {
-                    const int ii = i - q;
-                    const int j = static_cast<int>(below(d[q], static_cast<uint32_t>(ii + 1)));
-                    const int tmp = order[ii];
-                    order[ii] = order[j];
-                    order[j] = tmp;
-                }
-            }
-        }
+    auto make_order_into = [&](int *ord, uint32_t sweep_id) {
+        if (tid == 0) shuffle_order(ord, n, cbase, sweep_id, a.k0, a.k1);
         __syncthreads();
     };
+    auto make_order = [&](uint32_t sweep_id) { make_order_into(order, sweep_id); };
 
     // Row ring: the J rows of order positions [k, k + pfd] are staged in LDS slot (position % ring)
     // before they are needed; position k + pfd is fetched into registers during step k and stored
     // after that step's update, into the slot of a position no chain reads any more.
     constexpr int kPre = 8;  // registers per thread for one prefetched row (n <= 256 * kPre)
     float pre[kPre];
-    auto ring_preload = [&]() {
+    auto ring_preload_from = [&](const int *order) {
         if (!a.ring) return;
         const int m = min(a.pfd, n);
         for (int q = 0; q < m; ++q) {
@@ -231,6 +415,7 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
         }
         __syncthreads();
     };
+    auto ring_preload = [&]() { ring_preload_from(order); };
     auto ring_fetch = [&](int pos) {
         if (!a.ring || pos >= n) return;
         const float *row = a.pv.J32 + static_cast<int64_t>(order[pos]) * n;
@@ -249,6 +434,8 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
             if (j < n) dst[j] = pre[q];
         }
     };
+
+    const int wave = tid >> 6, lane = tid & 63;
 
     uint32_t sweep_id = a.sweep_base;
     for (int step = 0; step < a.steps; ++step) {
@@ -310,7 +497,6 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 }
                 if (a.mag_m2 && sw >= a.mag_burn) {
                     // m_r = sum of the replica's spins / (n P), accumulated as m^2 and |m| (:124-139)
-                    const int wave = tid >> 6, lane = tid & 63;
                     for (int c = wave; c < G; c += kThreads / 64) {
                         const int8_t *sc = S + static_cast<int64_t>(c) * n;
                         int sum = 0;
@@ -471,80 +657,298 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
                 __syncthreads();
             }
         }
-        // ---------------- step tail: slice energies, group minimum, trace partial ----------
-        if (a.tail_mode != 2) {
-            const int wave = tid >> 6, lane = tid & 63;
-            for (int c = wave; c < G; c += kThreads / 64) {
-                const int8_t *sc = S + static_cast<int64_t>(c) * n;
-                const float *fc = F + static_cast<int64_t>(c) * ldf;
-                double acc = 0.0;
-                if (a.pv.kind != QA_KIND_HUBO) {
-                    for (int i = lane; i < n; i += 64)
-                        acc += static_cast<double>(sc[i]) * (static_cast<double>(a.pv.h32[i]) + static_cast<double>(fc[i]));
-                    acc *= 0.5;
-                } else {
-                    for (int i = lane; i < n; i += 64) acc += static_cast<double>(a.pv.h32[i]) * sc[i];
-                    for (int t = lane; t < a.pv.nterms; t += 64) {
-                        double prod = a.pv.t_coef32[t];
-                        for (int q = a.pv.t_off[t]; q < a.pv.t_off[t + 1]; ++q) prod *= sc[a.pv.t_vars[q]];
-                        acc += prod;
-                    }
-                }
-                for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-                if (lane == 0) ered[c] = a.pv.c + acc;
-                if (a.tail_mode == 1) {
-                    const int t = c % P;
-                    const int8_t *sn = S + static_cast<int64_t>((c / P) * P + (t + 1 == P ? 0 : t + 1)) * n;
-                    int b = 0;
-                    for (int i = lane; i < n; i += 64) b += sc[i] * sn[i];
-                    for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
-                    if (lane == 0) {
-                        a.chain_e[static_cast<int64_t>(r0) * P + c] = a.pv.c + acc;
-                        a.chain_b[static_cast<int64_t>(r0) * P + c] = b;
-                    }
-                }
-            }
-            __syncthreads();
-            if (tid == 0 && a.tail_mode == 0) {
-                double sum = 0.0, mn = gbest;
-                int arg = -1;
-                for (int c = 0; c < G; ++c) {
-                    sum += ered[c];
-                    if (ered[c] < mn) { mn = ered[c]; arg = c; }
-                }
-                a.trace_part[static_cast<int64_t>(step) * gridDim.x + g] = sum;
-                scal[0] = arg;
-                if (arg >= 0) {
-                    gbest = mn;
-                    a.best_e[g] = mn;
-                    a.best_step[g] = step;
-                    a.best_chain[g] = r0 * P + arg;
-                }
-            }
-            __syncthreads();
-            const int arg = a.tail_mode == 0 ? scal[0] : -1;
-            if (arg >= 0)
-                for (int i = tid; i < n; i += kThreads)
-                    a.best_state[static_cast<int64_t>(g) * n + i] = S[static_cast<int64_t>(arg) * n + i];
-            __syncthreads();
-        }
+        // ---------------- step tail ---------------------------------------------------------
+        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, step, ered, scal, gbest);
     }
     // final spins out (for the caller's state readback / diagnostics)
-    if constexpr (kLds) {
-        for (int e = tid; e < G * n; e += kThreads) {
-            a.gS[static_cast<int64_t>(r0) * P * n + e] = S[e];
-            if (a.save_fields) {
-                const int c = e / n, i = e - c * n;
-                a.gF[static_cast<int64_t>(r0) * P * n + e] = F[static_cast<int64_t>(c) * ldf + i];
-            }
-        }
-    }
+    if constexpr (kLds) group_save(a, F, S, ldf, G, r0);
     if (tid < kMaxChains) {
         for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
         if (tid == 0) a.accepted[g] = nacc;
     }
 }
 
+
+// ---- pipelined slice sweeps (dense couplings, LDS-resident state) ---------------------------
+// The same visit orders, random words and FP32 arithmetic as the two-barrier slice sweep of
+// sqa_fp32_kernel, hence bit-identical results, with one barrier per lockstep step and the
+// workgroup's waves in fixed roles:
+//   wave 0     decides step tau: lane c = chain c (sqa_annealer.cpp:188-208 acceptance);
+//   waves 1-4  apply the field updates f_c += J[i,:] ds_c of step tau-1, the flipped chains
+//              dealt round-robin to the four waves, 16-byte LDS accesses;
+//   wave 5     stages the J row of order position tau + pfd into the LDS ring (loaded during
+//              step tau, stored during step tau + 1) and takes one Fisher-Yates step of the next
+//              sweep's visit order, whose draw blocks it evaluates 64 at a time.
+// The decision of chain c at site i = order[k] needs f_c[i] including c's own flip at step
+// tau-1 (site order[k-1]). The updaters skip the 4-float group holding i in that update and
+// wave 0 applies the same fmaf to the group itself, reading J[order[k-1]] from the ring, so no
+// LDS word is written by two waves in one step. NQ: 4-float groups per lane in a row (ldf <= 256 NQ).
+template <int NQ>
+__global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int kB = NQ >= 8 ? 1 : 8 / NQ;  // flipped chains per update batch
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: role branches stay scalar
+    const int n = a.pv.n, P = a.P;
+    const int r0 = g * a.rg;
+    const int nrep = min(a.rg, a.R - r0);
+    const int G = nrep * P;
+    const int ldf = a.ldf;  // == a.ldr, a multiple of 4
+    const int ngr = ldf >> 2;
+    const int W = a.ring;
+    const SqaLds L = carve_lds(a, smem, true);
+    // Philox words of the current and the next four-step block, [block parity][word][chain]; the
+    // two-barrier sweeps' per-chain scratch (RSLOT .. rflag, 2304 contiguous bytes) is unused here
+    uint32_t *RW = reinterpret_cast<uint32_t *>(L.RSLOT);
+    float *F = L.F;
+    int8_t *S = L.S;
+    f4 *F4 = reinterpret_cast<f4 *>(L.F);
+    const f4 *J4 = reinterpret_cast<const f4 *>(L.Jring);
+    f4 *Jring4 = reinterpret_cast<f4 *>(L.Jring);
+    unsigned long long *flipmask = L.flipmask;
+    const uint32_t cbase = static_cast<uint32_t>(r0 * P);
+
+    group_init(a, F, S, ldf, G, r0, true);
+    double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
+    unsigned long long nacc = 0;
+
+    const bool decider = tid < G;
+    const int c_me = tid;
+    const int rr_me = c_me / P, t_me = c_me - (c_me / P) * P;
+    const int cprev = rr_me * P + (t_me == 0 ? P - 1 : t_me - 1);
+    const int cnext = rr_me * P + (t_me + 1 == P ? 0 : t_me + 1);
+    const int t_lane = lane % P;
+
+    // wave 5: one staged row, NQ groups per lane
+    f4 hpre[NQ];
+    // (every lane loads NQ groups, clamped into the row, so that hpre stays in registers)
+    auto ring_fetch = [&](const int *ord, int pos) {
+        const f4 *row = reinterpret_cast<const f4 *>(a.pv.J32p + static_cast<int64_t>(ord[min(pos, n - 1)]) * ldf);
+        static_for<NQ>([&](auto r) { hpre[r] = row[min(lane + 64 * r, ngr - 1)]; });
+    };
+    auto ring_store = [&](int pos, int slot) {
+        if (pos >= n) return;
+        static_for<NQ>([&](auto r) {
+            const int gi = lane + 64 * r;
+            if (gi < ngr) Jring4[slot * ngr + gi] = hpre[r];
+        });
+    };
+
+    // waves 1-4: the updates of step tau-1 (flip masks of parity bb). Chain c was at order
+    // position kc = tau - 1 - t_c (ring slot slot_l on lane c) and decides site order[kc + 1]
+    // next; its group is left to wave 0.
+    auto apply = [&](int bb, int tau, const int *ord, int slot_l) {
+        const unsigned long long m_all = flipmask[bb];
+        const unsigned long long u_all = flipmask[2 + bb];
+        int skip_l = -1;
+        {
+            const int kc = tau - 1 - t_lane;
+            if (lane < G && kc >= 0 && kc + 1 < n) skip_l = ord[kc + 1] >> 2;
+        }
+        const uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
+        const uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
+        const uint32_t ulo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u_all));
+        const uint32_t uhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u_all >> 32));
+        const unsigned long long up = (static_cast<unsigned long long>(uhi) << 32) | ulo;
+        // this wave's share: the flipped chains c with c % 4 == wave - 1
+        unsigned long long mine = ((static_cast<unsigned long long>(mhi) << 32) | mlo) & (0x1111111111111111ull << (wave - 1));
+        auto batch = [&](auto bsz) {
+            constexpr int B = decltype(bsz)::value;
+            float dsv[B];
+            int roff[B], foff[B], skp[B];
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                const int c = __ffsll(static_cast<long long>(mine)) - 1;
+                mine &= mine - 1ull;
+                dsv[q] = ((up >> c) & 1ull) ? -2.0f : 2.0f;
+                roff[q] = __builtin_amdgcn_readlane(slot_l, c) * ngr;
+                skp[q] = __builtin_amdgcn_readlane(skip_l, c);
+                foff[q] = c * ngr;
+            }
+            // every lane loads (clamped into the row) so that rv/fv stay in registers
+            f4 rv[B][NQ], fv[B][NQ];
+            static_for<B>([&](auto q) {
+                static_for<NQ>([&](auto r) {
+                    const int gi = min(lane + 64 * r, ngr - 1);
+                    rv[q][r] = J4[roff[q] + gi];
+                    fv[q][r] = F4[foff[q] + gi];
+                });
+            });
+            static_for<B>([&](auto q) {
+                static_for<NQ>([&](auto r) {
+                    const int gi = lane + 64 * r;
+                    if (gi < ngr && gi != skp[q]) {
+                        const f4 x = rv[q][r], y = fv[q][r];
+                        F4[foff[q] + gi] = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                                              fmaf(dsv[q], x.w, y.w)};
+                    }
+                });
+            });
+        };
+        while (mine) {
+            const int cnt = __popcll(mine);
+            if (kB >= 8 && cnt >= 8) batch(std::integral_constant<int, (kB >= 8 ? 8 : 1)>{});
+            else if (kB >= 4 && cnt >= 4) batch(std::integral_constant<int, (kB >= 4 ? 4 : 1)>{});
+            else if (kB >= 2 && cnt >= 2) batch(std::integral_constant<int, (kB >= 2 ? 2 : 1)>{});
+            else batch(std::integral_constant<int, 1>{});
+        }
+    };
+
+    uint32_t sweep_id = a.sweep_base;
+    bool have_next = false;  // order buffer cur already holds the order of sweep next_id
+    uint32_t next_id = 0;
+    int cur = 0;
+    for (int step = 0; step < a.steps; ++step) {
+        const int pidx = a.param_of_rep ? (decider ? a.param_of_rep[r0 + rr_me] : 0) : step;
+        const float bs = a.bs[pidx], jp = a.jp[pidx];
+        for (int sw = 0; sw < a.spb; ++sw, ++sweep_id) {
+            int *ocur = L.order + cur * L.obstride;
+            int *onext = L.order + (cur ^ 1) * L.obstride;
+            if (!(have_next && next_id == sweep_id)) {
+                if (tid == 0) shuffle_order(ocur, n, cbase, sweep_id, a.k0, a.k1);
+                __syncthreads();
+            }
+            const bool pre_next = sw + 1 < a.spb || step + 1 < a.steps;
+            if (pre_next)
+                for (int i = tid; i < n; i += kThreads) onext[i] = i;
+            if (wave == kHelperWave) {  // Philox block 0 of chain = lane
+                const Philox4 r = philox4x32_10(cbase + lane, sweep_id, 0u, TAG_SLICE, a.k0, a.k1);
+                RW[lane] = r.x;
+                RW[64 + lane] = r.y;
+                RW[128 + lane] = r.z;
+                RW[192 + lane] = r.w;
+            }
+            // ring rows of positions 0 .. pfd-1
+            for (int e = tid; e < min(a.pfd, n) * ngr; e += kThreads) {
+                const int q = e / ngr, gi = e - q * ngr;
+                Jring4[q * ngr + gi] = reinterpret_cast<const f4 *>(a.pv.J32p + static_cast<int64_t>(ocur[q]) * ldf)[gi];
+            }
+            __syncthreads();
+
+            const int nsteps = n + P - 1;
+            bool fl_prev = false;        // wave 0: chain flipped at the previous step
+            float ds_prev = 0.0f;        // wave 0: -2 s_old of that flip
+            int i_cur = ocur[0];         // wave 0: site of this step (once active)
+            int sl = W - 1 - t_lane;     // waves 0-4: ring slot of position tau - 1 - t of lane's chain
+            PhiloxRun nxt{};                   // wave 5: block of the next four steps for chain = lane
+            int sh_ii = pre_next ? n - 1 : 0;  // wave 5: next Fisher-Yates iteration
+            Philox4 shb{0, 0, 0, 0};            // wave 5: this lane's draw block of the batch
+            int hslot = a.pfd % W;              // wave 5: slot of position tau - 1 + pfd
+            for (int tau = 0; tau <= nsteps; ++tau) {
+                const int b = tau & 1;
+                if (wave == 0) {
+                    if (tau < nsteps) {
+                        const int k = tau - t_me;
+                        const bool active = decider && k >= 0 && k < n;
+                        const int q = tau & 3;
+                        bool acc = false, upflip = false;
+                        int i_nxt = i_cur;
+                        if (active) {
+                            const int i = i_cur;
+                            int8_t *sp = S + c_me * n + i;
+                            const int8_t s = *sp;
+                            const int nb = S[cprev * n + i] + S[cnext * n + i];
+                            if (k + 1 < n) i_nxt = ocur[k + 1];
+                            float f;
+                            if (fl_prev) {
+                                f4 fv = F4[c_me * ngr + (i >> 2)];
+                                const f4 jv = J4[sl * ngr + (i >> 2)];
+                                fv = f4{fmaf(ds_prev, jv.x, fv.x), fmaf(ds_prev, jv.y, fv.y), fmaf(ds_prev, jv.z, fv.z),
+                                        fmaf(ds_prev, jv.w, fv.w)};
+                                F4[c_me * ngr + (i >> 2)] = fv;
+                                const int e = i & 3;
+                                f = e == 0 ? fv.x : e == 1 ? fv.y : e == 2 ? fv.z : fv.w;
+                            } else {
+                                f = F[c_me * ldf + i];
+                            }
+                            const float fs = static_cast<float>(s);
+                            const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(nb));
+                            const uint32_t word = RW[((tau >> 2) & 1) * 256 + q * 64 + c_me];
+                            acc = de <= 0.0f || u01(word) < __expf(-de);
+                            if (acc) *sp = static_cast<int8_t>(-s);
+                            upflip = s > 0;
+                            ds_prev = -2.0f * fs;
+                        } else if (k == -1) {
+                            i_nxt = ocur[0];
+                        }
+                        i_cur = i_nxt;
+                        fl_prev = acc;
+                        sl = sl + 1 == W ? 0 : sl + 1;
+                        const unsigned long long m = __ballot(acc);
+                        const unsigned long long mu = __ballot(acc && upflip);
+                        if (tid == 0) {
+                            flipmask[b] = m;
+                            flipmask[2 + b] = mu;
+                        }
+                        nacc += acc ? 1ull : 0ull;
+                    }
+                } else if (wave <= 4) {
+                    if (tau > 0) apply(b ^ 1, tau, ocur, sl);
+                    sl = sl + 1 == W ? 0 : sl + 1;
+                } else {
+                    // Philox block m + 1 of chain = lane (counter (chain, sweep, m + 1, TAG_SLICE)),
+                    // evaluated 3+3+2+2 rounds over the steps of block m and published at its last
+                    {
+                        const int q = tau & 3;
+                        if (q == 0) {
+                            nxt.start(cbase + lane, sweep_id, static_cast<uint32_t>((tau >> 2) + 1), TAG_SLICE, a.k0, a.k1);
+                            nxt.rounds<3>();
+                        } else if (q == 1) {
+                            nxt.rounds<3>();
+                        } else if (q == 2) {
+                            nxt.rounds<2>();
+                        } else {
+                            nxt.rounds<2>();
+                            uint32_t *dst = RW + (((tau >> 2) + 1) & 1) * 256 + lane;
+                            dst[0] = nxt.c0;
+                            dst[64] = nxt.c1;
+                            dst[128] = nxt.c2;
+                            dst[192] = nxt.c3;
+                        }
+                    }
+                    // store the row fetched last step, shuffle, fetch this step's row: the loads stay
+                    // in flight for a whole step
+                    if (tau > 0) {
+                        ring_store(tau - 1 + a.pfd, hslot);
+                        hslot = hslot + 1 == W ? 0 : hslot + 1;
+                    }
+                    if (sh_ii > 0) {
+                        // iteration sh_ii of shuffle_order(onext, ..., sweep_id + 1, ...)
+                        const int it = n - 1 - sh_ii;
+                        if ((it & 255) == 0) {
+                            const int i0 = n - 1 - 4 * ((it >> 2) + lane);
+                            if (i0 > 0)
+                                shb = philox4x32_10(cbase, sweep_id + 1, static_cast<uint32_t>(i0), TAG_ORDER, a.k0, a.k1);
+                        }
+                        const int qq = it & 3;
+                        const uint32_t wv = qq == 0 ? shb.x : qq == 1 ? shb.y : qq == 2 ? shb.z : shb.w;
+                        const uint32_t d = __builtin_amdgcn_readlane(wv, (it >> 2) & 63);
+                        if (lane == 0) {
+                            const int j = static_cast<int>(below(d, static_cast<uint32_t>(sh_ii + 1)));
+                            const int tmp = onext[sh_ii];
+                            onext[sh_ii] = onext[j];
+                            onext[j] = tmp;
+                        }
+                        --sh_ii;
+                    }
+                    ring_fetch(ocur, tau + a.pfd);
+                }
+                __syncthreads();
+            }
+            have_next = pre_next;
+            if (pre_next) {
+                next_id = sweep_id + 1;
+                cur ^= 1;
+            }
+        }
+        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, step, L.ered, L.scal, gbest);
+    }
+    group_save(a, F, S, ldf, G, r0);
+    if (tid < kMaxChains) {
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        if (tid == 0) a.accepted[g] = nacc;
+    }
+}
 
 // ---- SQA-PT exchange (sqa_parallel_tempering.cpp:299-335), one workgroup per ladder ----------
 // States stay where they are; the rung <-> state-slot map moves. For adjacent rungs a, b holding
@@ -646,10 +1050,12 @@ __global__ void gather_states(const int8_t *gS, const int *chain_of, int count, 
 
 // Launch plan of the FP32 sweep kernel: replicas per workgroup, state home (LDS or global), J-row
 // ring, dynamic LDS size.
+constexpr bool n_fits_ring(int n) { return n <= 2048; }
+
 struct SqaPlan {
     int64_t rg = 1, ngroups = 0;
     bool use_lds = false;
-    int ring = 0, pfd = 4, ldf = 0;
+    int ring = 0, pfd = 1, ldf = 0, ldr = 0, pipe = 0;
     size_t shmem = 0;
 };
 
@@ -662,39 +1068,65 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P) {
                                      "use the FP64 validation mode for an asymmetric J"};
     QA_REQUIRE(n < (1 << 24), "problem too large");
     SqaPlan pl;
+    const int64_t rn = (n + 3) / 4 * 4;  // LDS row stride of fields and staged J rows
     // replicas per group: fill a 64-lane wave when there are enough replicas for every CU
     pl.rg = std::max<int64_t>(1, kMaxChains / P);
     while (pl.rg > 1 && (R + pl.rg - 1) / pl.rg < ctx->num_cus) --pl.rg;
-    pl.ldf = static_cast<int>(n + 1);
-    pl.ring = (p->kind == QA_KIND_DENSE && n <= 2048) ? static_cast<int>(P) + pl.pfd + 1 : 0;
-    auto lds_bytes = [&](int64_t r, bool lds) {
-        size_t b = static_cast<size_t>(n) * 4 + kMaxChains * (4 + 4 + 8 + 4 + 8 + 4 + 4) + 16 + 16 +
-                   static_cast<size_t>(pl.ring) * n * 4;
-        if (lds) b += static_cast<size_t>(r * P) * pl.ldf * 4 + ((static_cast<size_t>(r * P * n) + 15) / 16) * 16;
+    const size_t lds_cap = 160 * 1024;
+    // LDS without the row ring: two visit orders, per-chain scratch, and (LDS layout) the state
+    auto fixed = [&](int64_t r, bool lds) {
+        size_t b = 2 * lds_align16(static_cast<size_t>(n) * 4) + kMaxChains * (4 + 4 + 4 + 8 + 4 + 8 + 4) + 32 + 16;
+        if (lds) b += lds_align16(static_cast<size_t>(r * P * rn) * 4) + lds_align16(static_cast<size_t>(r * P * n));
         return b;
     };
-    const size_t lds_cap = 160 * 1024;
-    while (pl.rg > 1 && lds_bytes(pl.rg, true) > lds_cap) --pl.rg;
-    pl.use_lds = lds_bytes(pl.rg, true) <= lds_cap;
-    if (!pl.use_lds && pl.ring) {  // keep the state in LDS in preference to the row ring
-        pl.ring = 0;
-        pl.rg = std::max<int64_t>(1, kMaxChains / P);
-        while (pl.rg > 1 && (R + pl.rg - 1) / pl.rg < ctx->num_cus) --pl.rg;
-        while (pl.rg > 1 && lds_bytes(pl.rg, true) > lds_cap) --pl.rg;
-        pl.use_lds = lds_bytes(pl.rg, true) <= lds_cap;
-    }
+    // J-row ring slots beside an LDS state of r replicas: P + 2 .. P + 5 (prefetch distance 1..4)
+    const bool dense_ring = p->kind == QA_KIND_DENSE && n_fits_ring(static_cast<int>(n));
+    auto ring_rows = [&](int64_t r) -> int {
+        if (!dense_ring) return 0;
+        const size_t f = fixed(r, true);
+        if (f + 16 >= lds_cap) return 0;
+        const int64_t w = std::min<int64_t>(P + 5, static_cast<int64_t>((lds_cap - f - 16) / (static_cast<size_t>(rn) * 4)));
+        return w >= P + 2 ? static_cast<int>(w) : 0;
+    };
+    while (pl.rg > 1 && (fixed(pl.rg, true) > lds_cap || (dense_ring && ring_rows(pl.rg) == 0))) --pl.rg;
+    pl.use_lds = fixed(pl.rg, true) <= lds_cap;
+    pl.ring = pl.use_lds ? ring_rows(pl.rg) : 0;
+    pl.pfd = pl.ring ? pl.ring - static_cast<int>(P) - 1 : 1;
+    const char *env = std::getenv("QANNEAL_SQA_PIPELINE");
+    pl.pipe = (pl.use_lds && pl.ring > 0 && !(env && env[0] == '0')) ? 1 : 0;
     pl.ngroups = (R + pl.rg - 1) / pl.rg;
-    pl.shmem = lds_bytes(pl.rg, pl.use_lds);
+    pl.shmem = fixed(pl.rg, pl.use_lds) + lds_align16(static_cast<size_t>(pl.ring) * rn * 4);
     QA_REQUIRE(pl.shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
-    if (!pl.use_lds) pl.ldf = static_cast<int>(n);
+    pl.ldf = static_cast<int>(pl.use_lds ? rn : n);
+    pl.ldr = static_cast<int>(rn);
     return pl;
 }
 
-void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &a) {
-    auto kern = pl.use_lds ? sqa_fp32_kernel<true> : sqa_fp32_kernel<false>;
-    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(pl.shmem)));
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(pl.ngroups)), dim3(kThreads), pl.shmem, ctx->stream, a);
+void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
+    SqaArgs a = args;
+    a.ldf = pl.ldf;
+    a.ldr = pl.ldr;
+    a.ring = pl.ring;
+    a.pfd = pl.pfd;
+    // the pipelined kernel runs plain slice sweeps over dense couplings with an LDS state
+    const bool pipe = pl.pipe && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 && a.pv.kind == QA_KIND_DENSE;
+    const void *kern;
+    if (pipe) {
+        QA_REQUIRE(a.ring >= a.P + 2 && a.ldf == a.ldr && a.ldf % 4 == 0 && n_fits_ring(a.pv.n),
+                   "pipelined sweep needs the dense row ring");
+        const int ngr = a.ldf / 4;
+        kern = ngr <= 64 ? reinterpret_cast<const void *>(sqa_pipe_kernel<1>)
+             : ngr <= 128 ? reinterpret_cast<const void *>(sqa_pipe_kernel<2>)
+             : ngr <= 256 ? reinterpret_cast<const void *>(sqa_pipe_kernel<4>)
+                          : reinterpret_cast<const void *>(sqa_pipe_kernel<8>);
+    } else {
+        kern = pl.use_lds ? reinterpret_cast<const void *>(sqa_fp32_kernel<true>)
+                          : reinterpret_cast<const void *>(sqa_fp32_kernel<false>);
+    }
+    QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pl.shmem)));
+    void *params[] = {&a};
+    QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(pl.ngroups)), dim3(kThreads), params, pl.shmem,
+                           ctx->stream));
     QA_HIP(hipGetLastError());
 }
 
@@ -746,9 +1178,6 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     a.best_chain = bchain.ptr;
     a.best_state = bstate.ptr;
     a.accepted = accd.ptr;
-    a.ldf = pl.ldf;
-    a.ring = pl.ring;
-    a.pfd = pl.pfd;
 
     QA_HIP(hipEventRecord(ctx->ev0, s));
     launch_sqa(ctx, pl, a);
@@ -846,9 +1275,6 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     a.gS = gS.ptr;
     a.gF = gF.ptr;
     a.accepted = accd.ptr;
-    a.ldf = pl.ldf;
-    a.ring = pl.ring;
-    a.pfd = pl.pfd;
     a.param_of_rep = dros.ptr;
     a.save_fields = 1;
     a.tail_mode = 1;
@@ -973,9 +1399,6 @@ void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int6
     a.gS = gS.ptr;
     a.gF = gF.ptr;
     a.accepted = accd.ptr;
-    a.ldf = pl.ldf;
-    a.ring = pl.ring;
-    a.pfd = pl.pfd;
     a.param_of_rep = dpof.ptr;
     a.tail_mode = 2;
     a.sweep_kind = 1;
@@ -1039,9 +1462,6 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
     a.best_chain = bchain.ptr;
     a.best_state = bstate.ptr;
     a.accepted = accd.ptr;
-    a.ldf = pl.ldf;
-    a.ring = pl.ring;
-    a.pfd = pl.pfd;
     a.sweep_kind = 1;
     QA_HIP(hipEventRecord(ctx->ev0, s));
     launch_sqa(ctx, pl, a);
