@@ -113,6 +113,7 @@ struct SqaLds {
     int *rflag;        // [kMaxChains] worldline decisions
     unsigned long long *flipmask;  // two-barrier: [0]; pipelined: [b] flips, [2 + b] flips from s = +1
     int *scal;         // [0]: best chain of the step
+    float *junk;       // [64][4]: target of stores a lane must not make (never read)
 };
 
 __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) {
@@ -146,6 +147,8 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
     L.flipmask = reinterpret_cast<unsigned long long *>(p);
     p += 32;
     L.scal = reinterpret_cast<int *>(p);
+    p += 16;
+    L.junk = reinterpret_cast<float *>(p);
     return L;
 }
 
@@ -737,14 +740,10 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
     // waves 1-4: the updates of step tau-1 (flip masks of parity bb). Chain c was at order
     // position kc = tau - 1 - t_c (ring slot slot_l on lane c) and decides site order[kc + 1]
     // next; its group is left to wave 0.
-    auto apply = [&](int bb, int tau, const int *ord, int slot_l) {
+    f4 *junk4 = reinterpret_cast<f4 *>(L.junk) + lane;
+    auto apply = [&](int bb, int slot_l, int skip_l) {
         const unsigned long long m_all = flipmask[bb];
         const unsigned long long u_all = flipmask[2 + bb];
-        int skip_l = -1;
-        {
-            const int kc = tau - 1 - t_lane;
-            if (lane < G && kc >= 0 && kc + 1 < n) skip_l = ord[kc + 1] >> 2;
-        }
         const uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
         const uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
         const uint32_t ulo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u_all));
@@ -765,7 +764,8 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
                 skp[q] = __builtin_amdgcn_readlane(skip_l, c);
                 foff[q] = c * ngr;
             }
-            // every lane loads (clamped into the row) so that rv/fv stay in registers
+            // every load and store is unconditional (loads clamped into the row, excluded stores
+            // sent to junk), so a batch is one LDS round trip with no branches
             f4 rv[B][NQ], fv[B][NQ];
             static_for<B>([&](auto q) {
                 static_for<NQ>([&](auto r) {
@@ -777,11 +777,10 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
             static_for<B>([&](auto q) {
                 static_for<NQ>([&](auto r) {
                     const int gi = lane + 64 * r;
-                    if (gi < ngr && gi != skp[q]) {
-                        const f4 x = rv[q][r], y = fv[q][r];
-                        F4[foff[q] + gi] = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
-                                              fmaf(dsv[q], x.w, y.w)};
-                    }
+                    const f4 x = rv[q][r], y = fv[q][r];
+                    f4 *dst = (gi < ngr && gi != skp[q]) ? F4 + foff[q] + gi : junk4;
+                    *dst = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                              fmaf(dsv[q], x.w, y.w)};
                 });
             });
         };
@@ -830,6 +829,7 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
             float ds_prev = 0.0f;        // wave 0: -2 s_old of that flip
             int i_cur = ocur[0];         // wave 0: site of this step (once active)
             int sl = W - 1 - t_lane;     // waves 0-4: ring slot of position tau - 1 - t of lane's chain
+            int skn = -1;                // waves 1-4: skip group of lane's chain in the next step
             PhiloxRun nxt{};                   // wave 5: block of the next four steps for chain = lane
             int sh_ii = pre_next ? n - 1 : 0;  // wave 5: next Fisher-Yates iteration
             Philox4 shb{0, 0, 0, 0};            // wave 5: this lane's draw block of the batch
@@ -838,52 +838,51 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
                 const int b = tau & 1;
                 if (wave == 0) {
                     if (tau < nsteps) {
+                        // every lane issues all of its loads at once (addresses kept valid: lanes
+                        // without a chain read chain 0's), then decides with selects
                         const int k = tau - t_me;
                         const bool active = decider && k >= 0 && k < n;
-                        const int q = tau & 3;
-                        bool acc = false, upflip = false;
-                        int i_nxt = i_cur;
-                        if (active) {
-                            const int i = i_cur;
-                            int8_t *sp = S + c_me * n + i;
-                            const int8_t s = *sp;
-                            const int nb = S[cprev * n + i] + S[cnext * n + i];
-                            if (k + 1 < n) i_nxt = ocur[k + 1];
-                            float f;
-                            if (fl_prev) {
-                                f4 fv = F4[c_me * ngr + (i >> 2)];
-                                const f4 jv = J4[sl * ngr + (i >> 2)];
-                                fv = f4{fmaf(ds_prev, jv.x, fv.x), fmaf(ds_prev, jv.y, fv.y), fmaf(ds_prev, jv.z, fv.z),
-                                        fmaf(ds_prev, jv.w, fv.w)};
-                                F4[c_me * ngr + (i >> 2)] = fv;
-                                const int e = i & 3;
-                                f = e == 0 ? fv.x : e == 1 ? fv.y : e == 2 ? fv.z : fv.w;
-                            } else {
-                                f = F[c_me * ldf + i];
-                            }
-                            const float fs = static_cast<float>(s);
-                            const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(nb));
-                            const uint32_t word = RW[((tau >> 2) & 1) * 256 + q * 64 + c_me];
-                            acc = de <= 0.0f || u01(word) < __expf(-de);
-                            if (acc) *sp = static_cast<int8_t>(-s);
-                            upflip = s > 0;
-                            ds_prev = -2.0f * fs;
-                        } else if (k == -1) {
-                            i_nxt = ocur[0];
-                        }
-                        i_cur = i_nxt;
+                        const int cc = decider ? c_me : 0;
+                        const int ccp = decider ? cprev : 0, ccn = decider ? cnext : 0;
+                        const int i = i_cur;
+                        const int gq = cc * ngr + (i >> 2);
+                        const f4 fv = F4[gq];
+                        const f4 jv = J4[sl * ngr + (i >> 2)];
+                        const int8_t s = S[cc * n + i];
+                        const int nb = S[ccp * n + i] + S[ccn * n + i];
+                        const uint32_t word = RW[((tau >> 2) & 1) * 256 + (tau & 3) * 64 + cc];
+                        const int i_nx = ocur[min(max(k + 1, 0), n - 1)];
+                        const f4 fc = f4{fmaf(ds_prev, jv.x, fv.x), fmaf(ds_prev, jv.y, fv.y), fmaf(ds_prev, jv.z, fv.z),
+                                         fmaf(ds_prev, jv.w, fv.w)};
+                        const f4 fu = fl_prev ? fc : fv;
+                        const int e = i & 3;
+                        const float f = e == 0 ? fu.x : e == 1 ? fu.y : e == 2 ? fu.z : fu.w;
+                        const float fs = static_cast<float>(s);
+                        const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(nb));
+                        const bool acc = active && (de <= 0.0f || u01(word) < __expf(-de));
+                        if (active && fl_prev) F4[gq] = fc;
+                        if (acc) S[cc * n + i] = static_cast<int8_t>(-s);
+                        if (k + 1 >= 0 && k + 1 < n) i_cur = i_nx;
+                        if (active) ds_prev = -2.0f * fs;
                         fl_prev = acc;
                         sl = sl + 1 == W ? 0 : sl + 1;
                         const unsigned long long m = __ballot(acc);
-                        const unsigned long long mu = __ballot(acc && upflip);
+                        const unsigned long long mu = __ballot(acc && s > 0);
                         if (tid == 0) {
                             flipmask[b] = m;
                             flipmask[2 + b] = mu;
                         }
-                        nacc += acc ? 1ull : 0ull;
+                        if (tid == 0) nacc += __popcll(m);
                     }
                 } else if (wave <= 4) {
-                    if (tau > 0) apply(b ^ 1, tau, ocur, sl);
+                    // lane's chain was at position tau - 1 - t in step tau - 1 and decides the site
+                    // at position tau - t next: its 4-float group is wave 0's in this step
+                    const int skip_l = skn;
+                    {
+                        const int kn = tau + 1 - t_lane;  // position after next, for the next step
+                        skn = (lane < G && kn >= 1 && kn < n) ? ocur[kn] >> 2 : -1;
+                    }
+                    if (tau > 0) apply(b ^ 1, sl, skip_l);
                     sl = sl + 1 == W ? 0 : sl + 1;
                 } else {
                     // Philox block m + 1 of chain = lane (counter (chain, sweep, m + 1, TAG_SLICE)),
@@ -1075,7 +1074,8 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P) {
     const size_t lds_cap = 160 * 1024;
     // LDS without the row ring: two visit orders, per-chain scratch, and (LDS layout) the state
     auto fixed = [&](int64_t r, bool lds) {
-        size_t b = 2 * lds_align16(static_cast<size_t>(n) * 4) + kMaxChains * (4 + 4 + 4 + 8 + 4 + 8 + 4) + 32 + 16;
+        size_t b = 2 * lds_align16(static_cast<size_t>(n) * 4) + kMaxChains * (4 + 4 + 4 + 8 + 4 + 8 + 4) + 32 + 16 +
+                   64 * 16;
         if (lds) b += lds_align16(static_cast<size_t>(r * P * rn) * 4) + lds_align16(static_cast<size_t>(r * P * n));
         return b;
     };
