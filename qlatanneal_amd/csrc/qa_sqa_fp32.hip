@@ -949,6 +949,7 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
     }
 }
 
+
 // ---- SQA-PT exchange (sqa_parallel_tempering.cpp:299-335), one workgroup per ladder ----------
 // States stay where they are; the rung <-> state-slot map moves. For adjacent rungs a, b holding
 // states with C = sum_t E_t and B = sum_{t,i} s_t,i s_t+1,i, the reference's 4-action difference
@@ -1111,6 +1112,7 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     // the pipelined kernel runs plain slice sweeps over dense couplings with an LDS state
     const bool pipe = pl.pipe && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 && a.pv.kind == QA_KIND_DENSE;
     const void *kern;
+    unsigned threads = kThreads;
     if (pipe) {
         QA_REQUIRE(a.ring >= a.P + 2 && a.ldf == a.ldr && a.ldf % 4 == 0 && n_fits_ring(a.pv.n),
                    "pipelined sweep needs the dense row ring");
@@ -1125,7 +1127,7 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     }
     QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pl.shmem)));
     void *params[] = {&a};
-    QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(pl.ngroups)), dim3(kThreads), params, pl.shmem,
+    QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(pl.ngroups)), dim3(threads), params, pl.shmem,
                            ctx->stream));
     QA_HIP(hipGetLastError());
 }

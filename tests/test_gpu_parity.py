@@ -83,42 +83,54 @@ def test_sqa_fp32_returned_energy_exact(ctx):
         assert np.all(np.isfinite(r["energy_trace"]))
 
 
-def test_sqa_fp32_pipelined_sweep_bit_identical(ctx, monkeypatch):
-    """The one-barrier pipelined slice sweep (decisions of step t overlapped with the field
-    updates of step t-1) must reproduce the two-barrier lockstep sweep bit for bit: same spins,
-    same FP32 fields, hence the same accept count, energy trace and best sample."""
+# sweep implementations selected by an environment switch: the two-barrier lockstep sweep and the
+# pipelined LDS-field sweep (the register-field sweep is switched off in the library)
+_SWEEP_PATHS = {"lockstep": {"QANNEAL_SQA_PIPELINE": "0"},
+                "pipelined": {"QANNEAL_SQA_PIPELINE": "1"}}
+
+
+def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
+    """The one-barrier pipelined sweep (decisions of step t overlapped with the field updates of
+    step t-1) must reproduce the two-barrier lockstep sweep bit for bit: same spins, same FP32
+    fields, hence the same accept count, energy trace and best sample."""
     from qlatanneal_amd import abi
     rng = np.random.default_rng(41)
     cases = [(200, 8, 16, 1, 0, 0), (97, 5, 7, 3, 1, 1), (512, 32, 4, 2, 0, 0), (33, 1, 3, 2, 0, 0),
-             (64, 64, 2, 1, 0, 1), (130, 16, 9, 2, 1, 0)]
+             (64, 64, 2, 1, 0, 1), (130, 16, 9, 2, 1, 0), (128, 12, 5, 2, 0, 0), (64, 4, 1024, 1, 0, 0),
+             (301, 20, 3, 1, 0, 0)]
     for n, P, R, spb, wl, cl in cases:
-        print("pipelined-vs-lockstep case", n, P, R, spb, wl, cl, flush=True)
+        print("sweep-path case", n, P, R, spb, wl, cl, flush=True)
         J = _sk(rng, n)
         h = rng.normal(0, 0.2, n)
         dp = abi.DeviceProblem.dense(ctx, h, J, -0.25)
         betas, gammas = np.full(12, 3.0), np.geomspace(3.0, 0.02, 12)
         res = {}
-        for flag in ("0", "1"):
-            monkeypatch.setenv("QANNEAL_SQA_PIPELINE", flag)
-            res[flag] = abi.sqa_run(dp, betas, gammas, P, R, spb, wl, cl, abi.MODE_FP32, 99)
-        a, b = res["0"], res["1"]
-        assert a["accept_rate"] == b["accept_rate"]
-        assert np.array_equal(a["best_state"], b["best_state"])
-        assert bits_equal([a["best_energy"]], [b["best_energy"]])
-        assert bits_equal(a["energy_trace"], b["energy_trace"])
-    # SQA-PT runs every sweep launch through the same kernel (resume + per-chain tails)
+        for name, env in _SWEEP_PATHS.items():
+            for k, v in env.items():
+                monkeypatch.setenv(k, v)
+            res[name] = abi.sqa_run(dp, betas, gammas, P, R, spb, wl, cl, abi.MODE_FP32, 99)
+        a = res["lockstep"]
+        for name in ("pipelined",):
+            b = res[name]
+            assert a["accept_rate"] == b["accept_rate"], name
+            assert np.array_equal(a["best_state"], b["best_state"]), name
+            assert bits_equal([a["best_energy"]], [b["best_energy"]]), name
+            assert bits_equal(a["energy_trace"], b["energy_trace"]), name
+    # SQA-PT runs every sweep launch through the same kernels (resume + per-chain tails)
     n, P = 48, 8
     J = _sk(rng, n)
     dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
     res = {}
-    for flag in ("0", "1"):
-        monkeypatch.setenv("QANNEAL_SQA_PIPELINE", flag)
-        res[flag] = abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, 2, 0, 6, 1, 0,
+    for name, env in _SWEEP_PATHS.items():
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        res[name] = abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, 2, 0, 6, 1, 0,
                                   ladders=8, mode=abi.MODE_FP32, seed=5)
-    for key in ("final_states", "best_state"):
-        assert np.array_equal(res["0"][key], res["1"][key])
-    for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
-        assert bits_equal(res["0"][key].ravel(), res["1"][key].ravel())
+    for name in ("pipelined",):
+        for key in ("final_states", "best_state"):
+            assert np.array_equal(res["lockstep"][key], res[name][key]), (name, key)
+        for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
+            assert bits_equal(res["lockstep"][key].ravel(), res[name][key].ravel()), (name, key)
 
 
 def test_sqa_fp32_distribution_vs_oracle(ctx):
