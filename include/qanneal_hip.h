@@ -67,6 +67,13 @@ qa_status qa_problem_destroy(qa_problem *p);
  * (dense_ising.cpp:29-44, sparse_ising.cpp:43-56, higher_order_ising.cpp:92-111). */
 qa_status qa_energy(const qa_problem *p, int64_t count, const int8_t *spins, double *out);
 
+/* FP32 local fields f_i = h_i + sum_{j != i} J_ij s_j of `count` states (spins [count*n]) into
+ * fields[count*n]: DenseIsing::compute_local_fields (dense_ising.cpp:64-83) for many states at
+ * once. Symmetric dense couplings run on the matrix cores (bf16 hi/lo split of the FP32 J, FP32
+ * accumulation; couplings with <= 16 significant bits are exact); other models round the FP64
+ * fields of the reference order (sparse_ising.cpp:73-85, higher_order_ising.cpp:138-160). */
+qa_status qa_local_fields(const qa_problem *p, int64_t count, const int8_t *spins, float *fields);
+
 /* ---- SA: replaces Annealer::run (annealer.cpp:33-75), `reads` independent chains ---------- */
 typedef struct qa_sa_params {
     int64_t steps;               /* schedule length                                          */

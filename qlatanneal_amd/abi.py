@@ -98,7 +98,7 @@ def lib() -> C.CDLL:
         L = C.CDLL(LIB_PATH)
         L.qa_last_error.restype = C.c_char_p
         for name in ("qa_ctx_create", "qa_ctx_destroy", "qa_problem_dense", "qa_problem_sparse",
-                     "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_sa_run", "qa_sqa_run",
+                     "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_local_fields", "qa_sa_run", "qa_sqa_run",
                      "qa_sqapt_run", "qa_sqa_chi_run"):
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -184,6 +184,14 @@ class DeviceProblem:
         s = np.ascontiguousarray(spins, np.int8).reshape(-1, self.n)
         out = np.zeros(s.shape[0])
         check(lib().qa_energy(self.h, _i64(s.shape[0]), _b(s), _d(out)))
+        return out
+
+    def local_fields(self, spins) -> np.ndarray:
+        """FP32 local fields of each state (matrix cores for symmetric dense couplings)."""
+        s = np.ascontiguousarray(spins, np.int8).reshape(-1, self.n)
+        out = np.zeros(s.shape, np.float32)
+        check(lib().qa_local_fields(self.h, _i64(s.shape[0]), _b(s),
+                                    out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
     def close(self):

@@ -27,7 +27,6 @@
 
 namespace qa {
 
-void fields_fp64_launch(const qa_problem *p, const int8_t *d_spins, double *d_fields, int64_t states, hipStream_t s);
 void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t points,
                     const std::vector<double> &bs, const std::vector<double> &jp, int64_t burn, int64_t sweeps,
                     uint64_t seed, std::vector<double> &m2, std::vector<double> &mabs, float *ms);

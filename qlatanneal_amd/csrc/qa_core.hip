@@ -165,6 +165,7 @@ qa_status qa_problem_dense(qa_ctx *ctx, int64_t n, const double *h, const double
         p->h32.upload(hf.data(), n, ctx->stream);
         p->J64.upload(J, n * n, ctx->stream);
         p->J32.upload(Jf.data(), n * n, ctx->stream);
+        if (sym) build_bf16_split(p.get(), Jf.data(), ctx->stream);
         if (n % 4 != 0) {
             const int64_t rn = (n + 3) / 4 * 4;
             std::vector<float> Jp(static_cast<size_t>(n * rn), 0.0f);
@@ -314,6 +315,21 @@ qa_status qa_energy(const qa_problem *p, int64_t count, const int8_t *spins, dou
         ds.upload(spins, count * p->n, s);
         device_energies(p, ds.ptr, count, de.ptr, s);
         de.download(out, count, s);
+        QA_HIP(hipStreamSynchronize(s));
+    });
+}
+
+qa_status qa_local_fields(const qa_problem *p, int64_t count, const int8_t *spins, float *fields) {
+    return guarded([&] {
+        QA_REQUIRE(p && spins && fields && count >= 0, "qa_local_fields: bad argument");
+        check_spins(spins, count * p->n);
+        QA_HIP(hipSetDevice(p->ctx->device));
+        hipStream_t s = p->ctx->stream;
+        DevBuf<int8_t> ds(count * p->n);
+        DevBuf<float> df(count * p->n);
+        ds.upload(spins, count * p->n, s);
+        fields_fp32_launch(p, ds.ptr, count, df.ptr, p->n, s);
+        df.download(fields, count * p->n, s);
         QA_HIP(hipStreamSynchronize(s));
     });
 }

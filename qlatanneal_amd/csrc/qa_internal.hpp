@@ -91,6 +91,8 @@ struct qa_problem {
     qa::DevBuf<double> J64;      // [n*n] as given
     qa::DevBuf<float> J32;       // [n*n] zero diagonal
     qa::DevBuf<float> J32p;      // [n][n rounded up to 4], zero padded; only when n % 4 != 0
+    qa::DevBuf<uint16_t> Jbf_hi, Jbf_lo;  // bf16 split of J32 for the matrix-core field refresh
+    int ldk_bf = 0;                        //   [n rounded up to 64][ldk_bf = n rounded up to 16]
     // sparse: undirected edges (energy order) + adjacency CSR in reference insertion order
     int64_t nedges = 0;
     qa::DevBuf<int32_t> ei, ej;
@@ -167,6 +169,17 @@ double trotter_coupling(double beta, double gamma, int64_t slices);
 // Energies on the device (FP64, reference order) for `count` states already resident.
 void device_energies(const qa_problem *p, const int8_t *d_spins, int64_t count, double *d_out,
                      hipStream_t s);
+
+// Matrix-core field refresh (qa_fields.hip): bf16 split of the FP32 couplings (dense, symmetric),
+// FP32 fields of `count` resident states (row stride ldf), and the max |FP32 - FP64| field error of
+// resident states against an FP64 recompute in the reference order.
+void build_bf16_split(qa_problem *p, const float *J32, hipStream_t s);
+void fields_fp32_launch(const qa_problem *p, const int8_t *d_spins, int64_t count, float *d_fields, int64_t ldf,
+                        hipStream_t s);
+double field_drift(const qa_problem *p, const int8_t *d_spins, const float *d_fields, int64_t ldf, int64_t count,
+                   hipStream_t s);
+void fields_fp64_launch(const qa_problem *p, const int8_t *d_spins, double *d_fields, int64_t states,
+                        hipStream_t s);
 
 // Engine entry points (implemented per engine file).
 void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);

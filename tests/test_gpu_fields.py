@@ -1,0 +1,96 @@
+"""Matrix-core local fields (qa_local_fields) against the oracle's FP64 fields.
+
+Dense symmetric couplings run as an FP32-accumulated bf16 GEMM with J split into hi + lo parts.
+Each field is checked against the oracle (the reference's DenseIsing::compute_local_fields order
+in FP64) within a bound built from the actual split residuals: sum_k |J_jk - hi_jk - lo_jk| (the
+split), sum_k |J_jk - fl32(J_jk)| (FP32 couplings) and an FP32 accumulation term. Couplings with
+at most 16 significant bits (QUBO-derived half-integers) must come out exact. Sparse and HUBO
+models return the FP64 reference-order fields rounded once to FP32, bit for bit.
+"""
+import numpy as np
+import pytest
+
+from tests.golden_io import load, problem, device_problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from qlatanneal_amd.abi import Context
+    return Context(0)
+
+
+def _bf16_rne(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def _sk(rng, n):
+    J = rng.normal(0, 1 / np.sqrt(n), (n, n))
+    J = np.triu(J, 1)
+    return J + J.T
+
+
+def test_local_fields_dense_within_split_bound(ctx):
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    orc = Oracle(ORACLE_SO)
+    rng = np.random.default_rng(7)
+    for n, count in [(64, 64), (100, 70), (257, 130), (512, 33), (31, 5)]:
+        print("dense fields case", n, count, flush=True)
+        J = _sk(rng, n)
+        h = rng.normal(0, 0.3, n)
+        dp = abi.DeviceProblem.dense(ctx, h, J, 0.0)
+        S = rng.choice(np.array([-1, 1], np.int8), size=(count, n))
+        F = dp.local_fields(S).astype(np.float64)
+        prob = Problem.dense(h, J)
+        ref = np.stack([orc.local_fields(prob, S[m]) for m in range(count)])
+        J32 = J.astype(np.float32)
+        np.fill_diagonal(J32, 0.0)
+        hi = _bf16_rne(J32)
+        lo = _bf16_rne(J32 - hi)
+        res = J32.astype(np.float64) - hi - lo
+        r32 = J - J32.astype(np.float64)
+        np.fill_diagonal(r32, 0.0)
+        absS = np.abs(S.astype(np.float64))
+        bound = (absS @ (np.abs(res) + np.abs(r32)).T + 2.0 ** -22 * (n + 2) * (np.abs(h) + absS @ np.abs(J).T)
+                 + np.abs(h - h.astype(np.float32)))
+        err = np.abs(F - ref)
+        print("  max err", err.max(), "max bound", bound.max(), flush=True)
+        assert np.all(err <= bound)
+        # a transposed or shifted tile would be off by O(|J| sqrt(n)) = O(1), far above the bound
+        assert bound.max() < 0.01
+
+
+def test_local_fields_exact_for_half_integer_couplings(ctx):
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(11)
+    for n, count in [(96, 40), (200, 65)]:
+        Q = rng.integers(-100, 101, (n, n)).astype(np.float64)
+        J = np.triu(Q, 1) / 2.0
+        J = J + J.T
+        h = rng.integers(-400, 401, n) / 4.0
+        dp = abi.DeviceProblem.dense(ctx, h, J, 0.0)
+        S = rng.choice(np.array([-1, 1], np.int8), size=(count, n))
+        F = dp.local_fields(S).astype(np.float64)
+        exact = h[None, :] + S.astype(np.float64) @ J.T
+        assert np.array_equal(F, exact)
+
+
+def test_local_fields_sparse_hubo_reference_order(ctx):
+    from oracle.orc import Oracle, ORACLE_SO, DENSE
+    orc = Oracle(ORACLE_SO)
+    rng = np.random.default_rng(5)
+    for c in load("hamiltonian.json")["cases"]:
+        p = problem(c["problem"])
+        if p.kind == DENSE:
+            continue
+        dp = device_problem(ctx, c["problem"])
+        S = rng.choice(np.array([-1, 1], np.int8), size=(6, p.n))
+        F = dp.local_fields(S)
+        for m in range(6):
+            want = orc.local_fields(p, S[m]).astype(np.float32)
+            assert np.array_equal(F[m], want)
