@@ -112,6 +112,9 @@ typedef struct qa_sqa_params {
     uint64_t seed;               /* FP32: Philox key                                         */
     const int8_t *init_spins;    /* validation: [replicas*P*n], SQAState::random draws       */
     const uint64_t *replica_seeds; /* validation: [replicas], the per-replica engine seeds   */
+    int64_t refresh_interval;    /* FP32: sweeps between recomputations of all FP32 fields on
+                                    the matrix cores from the spins (0: never; no reference
+                                    counterpart, the reference keeps FP64 fields)             */
 } qa_sqa_params;
 
 typedef struct qa_sqa_result {
@@ -120,6 +123,9 @@ typedef struct qa_sqa_result {
     double *energy_trace;        /* [steps] mean slice energy at each step end               */
     double *kernel_ms;           /* optional: device time of the sweep kernel(s)             */
     double *accept_rate;         /* optional: accepted / attempted slice moves (FP32 mode)   */
+    double *field_drift;         /* optional [2], FP32 with refresh_interval > 0: max |f32 -
+                                    f64| over up to 64 chains before each refresh, and after
+                                    it (f64: the reference-order FP64 fields); NaN if none    */
 } qa_sqa_result;
 
 qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
@@ -144,6 +150,7 @@ typedef struct qa_sqapt_params {
     const uint64_t *rung_seeds;  /* validation: [rungs]                                      */
     const double *swap_uniforms; /* validation: master-engine uniform stream for swaps       */
     int64_t swap_uniforms_len;
+    int64_t refresh_interval;    /* FP32: as in qa_sqa_params (sweeps between field refreshes) */
 } qa_sqapt_params;
 
 typedef struct qa_sqapt_result {
@@ -155,6 +162,7 @@ typedef struct qa_sqapt_result {
     double *swap_acceptance_trace;/* [ladders*steps]                                         */
     int64_t *swap_uniforms_used; /* validation (may be NULL)                                 */
     double *kernel_ms;
+    double *field_drift;         /* optional [2]: as in qa_sqa_result                         */
 } qa_sqapt_result;
 
 qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm,

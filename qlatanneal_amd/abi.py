@@ -38,12 +38,12 @@ class SqaParams(C.Structure):
     _fields_ = [("steps", _i64), ("betas", _dp), ("gammas", _dp), ("trotter_slices", _i64),
                 ("replicas", _i64), ("sweeps_per_beta", _i64), ("worldline_sweeps", _i64),
                 ("cluster_sweeps", _i64), ("mode", _i32), ("seed", _u64), ("init_spins", _bp),
-                ("replica_seeds", _up)]
+                ("replica_seeds", _up), ("refresh_interval", _i64)]
 
 
 class SqaResult(C.Structure):
     _fields_ = [("best_state", _bp), ("best_energy", _dp), ("energy_trace", _dp),
-                ("kernel_ms", _dp), ("accept_rate", _dp)]
+                ("kernel_ms", _dp), ("accept_rate", _dp), ("field_drift", _dp)]
 
 
 class SqaptParams(C.Structure):
@@ -51,13 +51,13 @@ class SqaptParams(C.Structure):
                 ("trotter_slices", _i64), ("sweeps_per_step", _i64), ("worldline_sweeps", _i64),
                 ("cluster_sweeps", _i64), ("steps", _i64), ("swap_interval", _i64), ("mode", _i32),
                 ("seed", _u64), ("init_spins", _bp), ("rung_seeds", _up), ("swap_uniforms", _dp),
-                ("swap_uniforms_len", _i64)]
+                ("swap_uniforms_len", _i64), ("refresh_interval", _i64)]
 
 
 class SqaptResult(C.Structure):
     _fields_ = [("final_states", _bp), ("final_energies", _dp), ("best_state", _bp),
                 ("best_energy", _dp), ("average_energy_trace", _dp), ("swap_acceptance_trace", _dp),
-                ("swap_uniforms_used", _lp), ("kernel_ms", _dp)]
+                ("swap_uniforms_used", _lp), ("kernel_ms", _dp), ("field_drift", _dp)]
 
 
 class ChiParams(C.Structure):
@@ -230,12 +230,13 @@ def sa_run(prob: DeviceProblem, betas, sweeps_per_beta: int, reads: int = 1, mod
 
 def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: int, sweeps_per_beta: int,
             worldline_sweeps: int = 0, cluster_sweeps: int = 0, mode: int = MODE_FP32, seed: int = 0,
-            init_spins=None, replica_seeds=None):
+            init_spins=None, replica_seeds=None, refresh_interval: int = 0):
     betas = np.ascontiguousarray(betas, np.float64)
     gammas = np.ascontiguousarray(gammas, np.float64)
     steps, n = betas.shape[0], prob.n
     p = SqaParams(steps, _d(betas), _d(gammas), trotter_slices, replicas, sweeps_per_beta,
                   worldline_sweeps, cluster_sweeps, mode, seed & (2**64 - 1))
+    p.refresh_interval = refresh_interval
     keep = []
     if init_spins is not None:
         init_spins = np.ascontiguousarray(init_spins, np.int8)
@@ -246,21 +247,23 @@ def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: i
         keep.append(replica_seeds)
         p.replica_seeds = replica_seeds.ctypes.data_as(_up)
     bs, be, tr, ms, ar = np.zeros(n, np.int8), np.zeros(1), np.zeros(steps), np.zeros(1), np.zeros(1)
-    r = SqaResult(_b(bs), _d(be), _d(tr), _d(ms), _d(ar))
+    dr = np.zeros(2)
+    r = SqaResult(_b(bs), _d(be), _d(tr), _d(ms), _d(ar), _d(dr))
     check(lib().qa_sqa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, kernel_ms=float(ms[0]),
-                accept_rate=float(ar[0]))
+                accept_rate=float(ar[0]), field_drift=dr)
 
 
 def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_per_step: int,
               worldline_sweeps: int, steps: int, swap_interval: int = 1, cluster_sweeps: int = 0,
               ladders: int = 1, mode: int = MODE_FP32, seed: int = 0, init_spins=None, rung_seeds=None,
-              swap_uniforms=None):
+              swap_uniforms=None, refresh_interval: int = 0):
     betas = np.ascontiguousarray(betas, np.float64)
     gammas = np.ascontiguousarray(gammas, np.float64)
     R, n = betas.shape[0], prob.n
     p = SqaptParams(R, _d(betas), _d(gammas), ladders, trotter_slices, sweeps_per_step, worldline_sweeps,
                     cluster_sweeps, steps, swap_interval, mode, seed & (2**64 - 1))
+    p.refresh_interval = refresh_interval
     keep = []
     if init_spins is not None:
         init_spins = np.ascontiguousarray(init_spins, np.int8)
@@ -279,11 +282,12 @@ def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_pe
     fs, fe = np.zeros((L, R, n), np.int8), np.zeros((L, R))
     bs, be = np.zeros((L, n), np.int8), np.zeros(L)
     at, st = np.zeros((L, steps)), np.zeros((L, steps))
-    used, ms = np.zeros(1, np.int64), np.zeros(1)
-    r = SqaptResult(_b(fs), _d(fe), _b(bs), _d(be), _d(at), _d(st), used.ctypes.data_as(_lp), _d(ms))
+    used, ms, dr = np.zeros(1, np.int64), np.zeros(1), np.zeros(2)
+    r = SqaptResult(_b(fs), _d(fe), _b(bs), _d(be), _d(at), _d(st), used.ctypes.data_as(_lp), _d(ms), _d(dr))
     check(lib().qa_sqapt_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(final_states=fs, final_energies=fe, best_state=bs, best_energy=be, average_energy_trace=at,
-                swap_acceptance_trace=st, swap_uniforms_used=int(used[0]), kernel_ms=float(ms[0]))
+                swap_acceptance_trace=st, swap_uniforms_used=int(used[0]), kernel_ms=float(ms[0]),
+                field_drift=dr)
 
 
 def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: float, gamma_start: float,

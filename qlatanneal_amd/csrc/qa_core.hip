@@ -327,8 +327,9 @@ qa_status qa_local_fields(const qa_problem *p, int64_t count, const int8_t *spin
         hipStream_t s = p->ctx->stream;
         DevBuf<int8_t> ds(count * p->n);
         DevBuf<float> df(count * p->n);
+        DevBuf<double> scratch(p->kind == QA_KIND_DENSE && p->symmetric ? 0 : count * p->n);
         ds.upload(spins, count * p->n, s);
-        fields_fp32_launch(p, ds.ptr, count, df.ptr, p->n, s);
+        fields_fp32_launch(p, ds.ptr, count, df.ptr, p->n, scratch.ptr, s);
         df.download(fields, count * p->n, s);
         QA_HIP(hipStreamSynchronize(s));
     });

@@ -134,7 +134,7 @@ void build_bf16_split(qa_problem *p, const float *J32, hipStream_t s) {
 }
 
 void fields_fp32_launch(const qa_problem *p, const int8_t *d_spins, int64_t count, float *d_fields, int64_t ldf,
-                        hipStream_t s) {
+                        double *d_scratch, hipStream_t s) {
     const int64_t n = p->n;
     if (count == 0) return;
     if (p->kind == QA_KIND_DENSE && p->symmetric) {
@@ -145,29 +145,24 @@ void fields_fp32_launch(const qa_problem *p, const int8_t *d_spins, int64_t coun
         QA_HIP(hipGetLastError());
     } else {
         // sparse / HUBO / asymmetric dense: FP64 fields in the reference order, rounded once
-        QA_REQUIRE(ldf == n, "fields of non-dense problems are written densely");
-        DevBuf<double> f64(count * n);
-        fields_fp64_launch(p, d_spins, f64.ptr, count, s);
-        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(256), dim3(256), 0, s, f64.ptr, d_fields, count * n);
+        QA_REQUIRE(ldf == n && d_scratch, "fields of non-dense problems are written densely via FP64 scratch");
+        fields_fp64_launch(p, d_spins, d_scratch, count, s);
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(256), dim3(256), 0, s, d_scratch, d_fields, count * n);
         QA_HIP(hipGetLastError());
-        QA_HIP(hipStreamSynchronize(s));
     }
 }
 
-double field_drift(const qa_problem *p, const int8_t *d_spins, const float *d_fields, int64_t ldf, int64_t count,
-                   hipStream_t s) {
+void field_drift_launch(const qa_problem *p, const int8_t *d_spins, const float *d_fields, int64_t ldf, int64_t count,
+                        double *d_scratch, unsigned long long *d_max, hipStream_t s) {
     const int64_t n = p->n;
-    if (count <= 0) return 0.0;
-    DevBuf<double> f64(count * n);
-    DevBuf<unsigned long long> mx(1);
-    QA_HIP(hipMemsetAsync(mx.ptr, 0, sizeof(unsigned long long), s));
-    fields_fp64_launch(p, d_spins, f64.ptr, count, s);
+    if (count <= 0) return;
+    fields_fp64_launch(p, d_spins, d_scratch, count, s);
     hipLaunchKernelGGL(drift_kernel, dim3(static_cast<unsigned>(std::min<int64_t>((count * n + 255) / 256, 1024))),
-                       dim3(256), 0, s, d_fields, ldf, f64.ptr, static_cast<int>(n), count, mx.ptr);
+                       dim3(256), 0, s, d_fields, ldf, d_scratch, static_cast<int>(n), count, d_max);
     QA_HIP(hipGetLastError());
-    unsigned long long bits = 0;
-    mx.download(&bits, 1, s);
-    QA_HIP(hipStreamSynchronize(s));
+}
+
+double drift_value(unsigned long long bits) {
     double d;
     std::memcpy(&d, &bits, 8);
     return d;

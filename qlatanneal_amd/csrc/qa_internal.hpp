@@ -170,14 +170,17 @@ double trotter_coupling(double beta, double gamma, int64_t slices);
 void device_energies(const qa_problem *p, const int8_t *d_spins, int64_t count, double *d_out,
                      hipStream_t s);
 
-// Matrix-core field refresh (qa_fields.hip): bf16 split of the FP32 couplings (dense, symmetric),
-// FP32 fields of `count` resident states (row stride ldf), and the max |FP32 - FP64| field error of
-// resident states against an FP64 recompute in the reference order.
+// Matrix-core field refresh (qa_fields.hip): bf16 split of the FP32 couplings (dense, symmetric);
+// FP32 fields of `count` resident states (row stride ldf; models other than symmetric dense go
+// through an FP64 scratch of count*n doubles); and the max |FP32 - FP64| field error of resident
+// states against an FP64 recompute in the reference order, folded into *d_max as the bits of a
+// non-negative double (read with drift_value). All launches are asynchronous on stream s.
 void build_bf16_split(qa_problem *p, const float *J32, hipStream_t s);
 void fields_fp32_launch(const qa_problem *p, const int8_t *d_spins, int64_t count, float *d_fields, int64_t ldf,
-                        hipStream_t s);
-double field_drift(const qa_problem *p, const int8_t *d_spins, const float *d_fields, int64_t ldf, int64_t count,
-                   hipStream_t s);
+                        double *d_scratch, hipStream_t s);
+void field_drift_launch(const qa_problem *p, const int8_t *d_spins, const float *d_fields, int64_t ldf, int64_t count,
+                        double *d_scratch, unsigned long long *d_max, hipStream_t s);
+double drift_value(unsigned long long bits);
 void fields_fp64_launch(const qa_problem *p, const int8_t *d_spins, double *d_fields, int64_t states,
                         hipStream_t s);
 

@@ -87,6 +87,7 @@ struct SqaArgs {
     double *chain_e;       // tail_mode 1: [R*P] slice energies at the end of the launch
     int *chain_b;          // tail_mode 1: [R*P] sum_i s_t,i s_t+1,i
     uint32_t sweep_base;   // sweep counter offset of this launch (random-stream continuity)
+    int step_base;         // schedule step of this launch's step 0 (recorded in best_step)
     int sweep_kind;        // 0: SQA slice sweeps (wavefront); 1: sqa_chi parity checkerboard
     int mag_burn;          // checkerboard: sweeps before magnetisation is accumulated
     double *mag_m2;        // [R] sum over measured sweeps of m^2 (null: no measurement)
@@ -269,7 +270,7 @@ __device__ void group_tail(const SqaArgs &a, const float *F, const int8_t *S, in
         if (arg >= 0) {
             gbest = mn;
             a.best_e[g] = mn;
-            a.best_step[g] = step;
+            a.best_step[g] = a.step_base + step;
             a.best_chain[g] = r0 * P + arg;
         }
     }
@@ -1149,11 +1150,21 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     dbs.upload(hbs.data(), prm->steps, s);
     djp.upload(hjp.data(), prm->steps, s);
     djoin.upload(hjoin.data(), prm->steps, s);
+    // launches of at most `chunk` schedule steps; between them the FP32 fields are refreshed
+    const int64_t per_step = prm->sweeps_per_beta + prm->worldline_sweeps + prm->cluster_sweeps;
+    const int64_t ri = prm->refresh_interval;
+    QA_REQUIRE(ri >= 0, "refresh_interval must be >= 0");
+    const int64_t chunk = ri > 0 ? std::max<int64_t>(1, ri / per_step) : prm->steps;
+    const int64_t nchunks = (prm->steps + chunk - 1) / chunk;
+    const int64_t nchain = R * P, nsample = std::min<int64_t>(nchain, 64);
     DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
-    DevBuf<float> gF(use_lds ? 1 : R * P * n);
+    DevBuf<float> gF((use_lds && nchunks == 1) ? 1 : R * P * n);
     DevBuf<double> tpart(prm->steps * ngroups), be(ngroups);
     DevBuf<int> bstep(ngroups), bchain(ngroups);
-    DevBuf<unsigned long long> accd(ngroups);
+    DevBuf<unsigned long long> accd(ngroups * nchunks), drift(2);
+    const bool mfma_refresh = p->kind == QA_KIND_DENSE && p->symmetric;
+    DevBuf<double> scratch(nchunks > 1 ? (mfma_refresh ? nsample : nchain) * n : 0);
+    QA_HIP(hipMemsetAsync(drift.ptr, 0, 2 * sizeof(unsigned long long), s));
     std::vector<double> inf(ngroups, std::numeric_limits<double>::infinity());
     be.upload(inf.data(), ngroups, s);
     QA_HIP(hipMemsetAsync(bchain.ptr, 0xff, ngroups * sizeof(int), s));
@@ -1182,18 +1193,42 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     a.accepted = accd.ptr;
 
     QA_HIP(hipEventRecord(ctx->ev0, s));
-    launch_sqa(ctx, pl, a);
+    for (int64_t ci = 0; ci < nchunks; ++ci) {
+        const int64_t start = ci * chunk, len = std::min(chunk, prm->steps - start);
+        a.steps = static_cast<int>(len);
+        a.bs = dbs.ptr + start;
+        a.jp = djp.ptr + start;
+        a.join = djoin.ptr + start;
+        a.trace_part = tpart.ptr + start * ngroups;
+        a.accepted = accd.ptr + ci * ngroups;
+        a.resume = ci > 0 ? 1 : 0;
+        a.save_fields = ci + 1 < nchunks ? 1 : 0;
+        a.sweep_base = static_cast<uint32_t>(start * per_step);
+        a.step_base = static_cast<int>(start);
+        launch_sqa(ctx, pl, a);
+        if (ci + 1 < nchunks) {  // drift of the running fields, matrix-core refresh, drift after it
+            field_drift_launch(p, gS.ptr, gF.ptr, n, nsample, scratch.ptr, drift.ptr, s);
+            fields_fp32_launch(p, gS.ptr, nchain, gF.ptr, n, scratch.ptr, s);
+            field_drift_launch(p, gS.ptr, gF.ptr, n, nsample, scratch.ptr, drift.ptr + 1, s);
+        }
+    }
     QA_HIP(hipEventRecord(ctx->ev1, s));
 
     std::vector<double> hpart(prm->steps * ngroups), hbe(ngroups);
     std::vector<int> hstep(ngroups), hchain(ngroups);
-    std::vector<unsigned long long> hacc(ngroups);
+    std::vector<unsigned long long> hacc(ngroups * nchunks), hdrift(2);
     tpart.download(hpart.data(), hpart.size(), s);
     be.download(hbe.data(), ngroups, s);
     bstep.download(hstep.data(), ngroups, s);
     bchain.download(hchain.data(), ngroups, s);
-    accd.download(hacc.data(), ngroups, s);
+    accd.download(hacc.data(), ngroups * nchunks, s);
+    drift.download(hdrift.data(), 2, s);
     QA_HIP(hipStreamSynchronize(s));
+    if (res->field_drift) {
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        res->field_drift[0] = nchunks > 1 ? drift_value(hdrift[0]) : nan;
+        res->field_drift[1] = nchunks > 1 ? drift_value(hdrift[1]) : nan;
+    }
     float ms = 0;
     QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     for (int64_t k = 0; k < prm->steps; ++k) {
@@ -1249,7 +1284,13 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     DevBuf<double> dbp(R), djpd(R), chain_e(NS * P), be(L), at(L * steps), st(L * steps);
     DevBuf<int> chain_b(NS * P), dsor(NS), dros(NS);
     DevBuf<int8_t> gS(NS * P * n), bstate(L * n);
-    DevBuf<unsigned long long> accd(pl.ngroups);
+    DevBuf<unsigned long long> accd(pl.ngroups), drift(2);
+    const int64_t ri = prm->refresh_interval;
+    QA_REQUIRE(ri >= 0, "refresh_interval must be >= 0");
+    const int64_t nchain = NS * P, nsample = std::min<int64_t>(nchain, 64);
+    const bool mfma_refresh = p->kind == QA_KIND_DENSE && p->symmetric;
+    DevBuf<double> scratch(ri > 0 ? (mfma_refresh ? nsample : nchain) * n : 0);
+    QA_HIP(hipMemsetAsync(drift.ptr, 0, 2 * sizeof(unsigned long long), s));
     dbs.upload(hbs.data(), R, s);
     djp.upload(hjp.data(), R, s);
     djoin.upload(hjoin.data(), R, s);
@@ -1302,17 +1343,34 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     w.swap_trace = st.ptr;
     const size_t wsh = static_cast<size_t>(R) * 24 + 16;
     const uint32_t per_step = static_cast<uint32_t>(prm->sweeps_per_step + prm->worldline_sweeps + prm->cluster_sweeps);
+    int64_t refreshes = 0;
     QA_HIP(hipEventRecord(ctx->ev0, s));
     for (int64_t step = 0; step < steps; ++step) {
         a.resume = step > 0 ? 1 : 0;
         a.sweep_base = static_cast<uint32_t>(step) * per_step;
+        a.step_base = static_cast<int>(step);
         launch_sqa(ctx, pl, a);
         w.step = static_cast<int>(step);
         w.do_swap = ((step + 1) % prm->swap_interval == 0) ? 1 : 0;
         hipLaunchKernelGGL(pt_swap_kernel, dim3(static_cast<unsigned>(L)), dim3(64), wsh, s, w);
         QA_HIP(hipGetLastError());
+        // every refresh_interval sweeps: drift of the running fields, matrix-core refresh, drift after
+        if (ri > 0 && step + 1 < steps && ((step + 1) * static_cast<int64_t>(per_step)) / ri > refreshes) {
+            refreshes = ((step + 1) * static_cast<int64_t>(per_step)) / ri;
+            field_drift_launch(p, gS.ptr, gF.ptr, n, nsample, scratch.ptr, drift.ptr, s);
+            fields_fp32_launch(p, gS.ptr, nchain, gF.ptr, n, scratch.ptr, s);
+            field_drift_launch(p, gS.ptr, gF.ptr, n, nsample, scratch.ptr, drift.ptr + 1, s);
+        }
     }
     QA_HIP(hipEventRecord(ctx->ev1, s));
+    if (res->field_drift) {
+        std::vector<unsigned long long> hdrift(2);
+        drift.download(hdrift.data(), 2, s);
+        QA_HIP(hipStreamSynchronize(s));
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        res->field_drift[0] = refreshes > 0 ? drift_value(hdrift[0]) : nan;
+        res->field_drift[1] = refreshes > 0 ? drift_value(hdrift[1]) : nan;
+    }
     std::vector<double> he(NS * P);
     chain_e.download(he.data(), NS * P, s);
     dsor.download(sor.data(), NS, s);

@@ -94,3 +94,58 @@ def test_local_fields_sparse_hubo_reference_order(ctx):
         for m in range(6):
             want = orc.local_fields(p, S[m]).astype(np.float32)
             assert np.array_equal(F[m], want)
+
+
+def test_refresh_is_identity_for_exact_fields(ctx):
+    """With half-integer couplings every FP32 field is exact, so a matrix-core refresh reproduces
+    the running fields bit for bit: runs with and without refreshes must be identical, and the
+    drift check must read 0 before and after each refresh."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(13)
+    n, P, R = 80, 8, 6
+    Q = rng.integers(-20, 21, (n, n)).astype(np.float64)
+    J = np.triu(Q, 1) / 2.0
+    J = J + J.T
+    h = rng.integers(-40, 41, n) / 4.0
+    dp = abi.DeviceProblem.dense(ctx, h, J, 0.0)
+    betas, gammas = np.full(10, 0.5), np.geomspace(3.0, 0.05, 10)
+    a = abi.sqa_run(dp, betas, gammas, P, R, 2, 0, 0, abi.MODE_FP32, 21)
+    b = abi.sqa_run(dp, betas, gammas, P, R, 2, 0, 0, abi.MODE_FP32, 21, refresh_interval=4)
+    assert np.isnan(a["field_drift"]).all()
+    assert np.array_equal(b["field_drift"], [0.0, 0.0])
+    assert a["accept_rate"] == b["accept_rate"]
+    assert np.array_equal(a["best_state"], b["best_state"])
+    assert np.array_equal(a["energy_trace"], b["energy_trace"])
+    lad = dict(ladders=4, mode=abi.MODE_FP32, seed=8)
+    pa = abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, 2, 0, 8, 1, 0, **lad)
+    pb = abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, 2, 0, 8, 1, 0,
+                       refresh_interval=3, **lad)
+    assert np.array_equal(pb["field_drift"], [0.0, 0.0])
+    for key in ("final_states", "best_state", "final_energies", "average_energy_trace", "swap_acceptance_trace"):
+        assert np.array_equal(pa[key], pb[key]), key
+
+
+def test_refresh_bounds_drift_and_keeps_samples_exact(ctx):
+    """Gaussian couplings: the running FP32 fields drift from the FP64 fields; after each refresh
+    the error is back to the split bound, and every returned sample still carries its exact FP64
+    energy."""
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    orc = Oracle(ORACLE_SO)
+    rng = np.random.default_rng(17)
+    n, P, R = 256, 16, 8
+    J = _sk(rng, n)
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J, 0.0)
+    betas, gammas = np.full(40, 2.0), np.linspace(2.0, 0.05, 40)
+    r = abi.sqa_run(dp, betas, gammas, P, R, 1, 0, 0, abi.MODE_FP32, 3, refresh_interval=10)
+    before, after = r["field_drift"]
+    print("drift before/after refresh", before, after, flush=True)
+    assert np.isfinite(before) and np.isfinite(after)
+    assert after <= 2e-4
+    assert r["best_energy"] == orc.energy(Problem.dense(np.zeros(n), J), r["best_state"])[0]
+    p = abi.sqapt_run(dp, np.geomspace(0.5, 3.0, 4), np.geomspace(2.0, 0.05, 4), P, 2, 0, 12, 1, 0,
+                      ladders=4, mode=abi.MODE_FP32, seed=4, refresh_interval=5)
+    assert np.isfinite(p["field_drift"]).all() and p["field_drift"][1] <= 2e-4
+    prob = Problem.dense(np.zeros(n), J)
+    for l in range(4):
+        assert p["best_energy"][l] == orc.energy(prob, p["best_state"][l])[0]
