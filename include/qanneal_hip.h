@@ -131,6 +131,40 @@ typedef struct qa_sqa_result {
 qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
                      qa_sqa_result *res);
 
+/* ---- Batched SQA: many problems of uneven size in one persistent launch ---------------------
+ * No reference counterpart (the reference runs instances one by one, solve() in solver.py:455-789).
+ * Instance k runs what qa_sqa_run runs in FP32 mode with replicas = reads[k] and seed = seeds[k],
+ * all instances sharing the schedule. Work items (instance, replica group) are dealt to persistent
+ * workgroups through an atomic counter, largest first; one kernel per sweep variant. With
+ * autotune = 0 every instance's replica groups are those of its own qa_sqa_run call and its result
+ * equals that call's bit for bit; autotune = 1 picks replicas per group by timed trials, cached per
+ * (model, size class, P) in the context. use_graph = 1 captures the launch sequence in a hipGraph.
+ * Status: switched off. The launch did not complete on its first MI355X run and no cause has been
+ * shown, so the call validates its arguments and returns QA_EUNSUPPORTED without launching. */
+typedef struct qa_batch_params {
+    int64_t steps;
+    const double *betas;         /* [steps], shared by all instances                         */
+    const double *gammas;        /* [steps]                                                  */
+    int64_t trotter_slices;
+    int64_t sweeps_per_beta;
+    int64_t worldline_sweeps;
+    int64_t cluster_sweeps;
+    const int64_t *reads;        /* [count] replicas per instance                            */
+    const uint64_t *seeds;       /* [count] Philox keys                                      */
+    int32_t autotune;
+    int32_t use_graph;
+} qa_batch_params;
+
+typedef struct qa_batch_result {
+    int8_t *best_states;         /* [sum_k n_k]: instance k at offset n_0 + ... + n_{k-1}    */
+    double *best_energies;       /* [count] FP64 energies of the best states                 */
+    double *kernel_ms;           /* optional: device time of the batch launch sequence       */
+    int64_t *attempts;           /* optional: single-spin attempts of the whole batch        */
+} qa_batch_result;
+
+qa_status qa_sqa_batch_run(qa_ctx *ctx, int64_t count, const qa_problem *const *problems,
+                           const qa_batch_params *prm, qa_batch_result *res);
+
 /* ---- SQA parallel tempering: replaces SQAParallelTemperingAnnealer::run
  *      (sqa_parallel_tempering.cpp:81-346) ------------------------------------------------- */
 typedef struct qa_sqapt_params {

@@ -76,6 +76,16 @@ class ChiResult(C.Structure):
                 ("kernel_ms", _dp)]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("steps", _i64), ("betas", _dp), ("gammas", _dp), ("trotter_slices", _i64),
+                ("sweeps_per_beta", _i64), ("worldline_sweeps", _i64), ("cluster_sweeps", _i64),
+                ("reads", _lp), ("seeds", _up), ("autotune", _i32), ("use_graph", _i32)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("best_states", _bp), ("best_energies", _dp), ("kernel_ms", _dp), ("attempts", _lp)]
+
+
 class QaError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(msg)
@@ -99,7 +109,7 @@ def lib() -> C.CDLL:
         L.qa_last_error.restype = C.c_char_p
         for name in ("qa_ctx_create", "qa_ctx_destroy", "qa_problem_dense", "qa_problem_sparse",
                      "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_local_fields", "qa_sa_run", "qa_sqa_run",
-                     "qa_sqapt_run", "qa_sqa_chi_run"):
+                     "qa_sqapt_run", "qa_sqa_chi_run", "qa_sqa_batch_run"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -321,3 +331,28 @@ def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: f
     return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, beta_schedule=bsch, gamma_schedule=gsch,
                 s_schedule=ssch, scan_s=ss, scan_gamma=sg, scan_chi_B=sc, s_star=sc5[0], gamma_star=sc5[1],
                 j_perp_star=sc5[2], chi_floor=sc5[3], driver_A0=sc5[4], kernel_ms=float(ms[0]))
+
+
+def sqa_batch_run(problems, betas, gammas, trotter_slices: int, sweeps_per_beta: int, reads, seeds,
+                  worldline_sweeps: int = 0, cluster_sweeps: int = 0, autotune: bool = False,
+                  use_graph: bool = False):
+    """Many problems (DeviceProblem, same context) in one persistent launch; per instance the
+    FP32 SQA run of sqa_run(problem, ..., replicas=reads[k], seed=seeds[k])."""
+    betas = np.ascontiguousarray(betas, np.float64)
+    gammas = np.ascontiguousarray(gammas, np.float64)
+    count = len(problems)
+    reads = np.ascontiguousarray(reads, np.int64)
+    seeds = np.ascontiguousarray([int(x) & (2**64 - 1) for x in seeds], np.uint64)
+    assert reads.shape == (count,) and seeds.shape == (count,)
+    handles = (C.c_void_p * count)(*[p.h.value for p in problems])
+    prm = BatchParams(betas.shape[0], _d(betas), _d(gammas), trotter_slices, sweeps_per_beta, worldline_sweeps,
+                      cluster_sweeps, reads.ctypes.data_as(_lp), seeds.ctypes.data_as(_up), int(autotune),
+                      int(use_graph))
+    ns = [p.n for p in problems]
+    bs = np.zeros(sum(ns), np.int8)
+    be, ms, att = np.zeros(count), np.zeros(1), np.zeros(1, np.int64)
+    res = BatchResult(_b(bs), _d(be), _d(ms), att.ctypes.data_as(_lp))
+    check(lib().qa_sqa_batch_run(problems[0].ctx.h, _i64(count), handles, C.byref(prm), C.byref(res)))
+    offs = np.concatenate([[0], np.cumsum(ns)])
+    states = [bs[offs[k]:offs[k + 1]].copy() for k in range(count)]
+    return dict(best_states=states, best_energies=be, kernel_ms=float(ms[0]), attempts=int(att[0]))

@@ -3,6 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <tuple>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -73,6 +76,8 @@ struct qa_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int num_cus = 256;
     size_t lds_per_block = 65536;
+    // batch autotuner: replicas per group chosen for (model kind, n rounded up to 64, P)
+    std::map<std::tuple<int, int64_t, int64_t>, int64_t> tuned_rg;
 };
 
 enum { QA_KIND_DENSE = 0, QA_KIND_SPARSE = 1, QA_KIND_HUBO = 2 };
@@ -193,5 +198,7 @@ void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *p
 void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
 void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
 void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
+void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *problems, const qa_batch_params *prm,
+               qa_batch_result *res);
 
 }  // namespace qa

@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -88,6 +89,7 @@ struct SqaArgs {
     int *chain_b;          // tail_mode 1: [R*P] sum_i s_t,i s_t+1,i
     uint32_t sweep_base;   // sweep counter offset of this launch (random-stream continuity)
     int step_base;         // schedule step of this launch's step 0 (recorded in best_step)
+    int ngroups;           // replica groups of the run (row stride of trace_part)
     int sweep_kind;        // 0: SQA slice sweeps (wavefront); 1: sqa_chi parity checkerboard
     int mag_burn;          // checkerboard: sweeps before magnetisation is accumulated
     double *mag_m2;        // [R] sum over measured sweeps of m^2 (null: no measurement)
@@ -223,9 +225,9 @@ __device__ void group_init(const SqaArgs &a, float *F, int8_t *S, int ldf, int G
 // sum for HUBO). tail_mode 0 keeps the group's first strict minimum (best_e/best_step/best_chain/
 // best_state; gbest lives in thread 0) and writes the group's partial of the energy trace;
 // tail_mode 1 writes per-chain energies and bond sums sum_i s_t,i s_t+1,i.
-__device__ void group_tail(const SqaArgs &a, const float *F, const int8_t *S, int ldf, int G, int r0, int step,
-                           double *ered, int *scal, double &gbest) {
-    const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, P = a.P, g = blockIdx.x;
+__device__ void group_tail(const SqaArgs &a, const float *F, const int8_t *S, int ldf, int G, int r0, int g,
+                           int step, double *ered, int *scal, double &gbest) {
+    const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, P = a.P;
     const int wave = tid >> 6, lane = tid & 63;
     for (int c = wave; c < G; c += nt / 64) {
         const int8_t *sc = S + static_cast<int64_t>(c) * n;
@@ -265,7 +267,7 @@ __device__ void group_tail(const SqaArgs &a, const float *F, const int8_t *S, in
             sum += ered[c];
             if (ered[c] < mn) { mn = ered[c]; arg = c; }
         }
-        a.trace_part[static_cast<int64_t>(step) * gridDim.x + g] = sum;
+        a.trace_part[static_cast<int64_t>(step) * a.ngroups + g] = sum;
         scal[0] = arg;
         if (arg >= 0) {
             gbest = mn;
@@ -293,10 +295,10 @@ __device__ void group_save(const SqaArgs &a, const float *F, const int8_t *S, in
     }
 }
 
+// One replica group of the two-barrier sweep (any model, LDS or global state).
 template <bool kLds>
-__global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
+__device__ void general_group(const SqaArgs &a, const int g) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int g = blockIdx.x;
     const int tid = threadIdx.x;
     const int n = a.pv.n, P = a.P;
     const int r0 = g * a.rg;
@@ -662,7 +664,7 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
             }
         }
         // ---------------- step tail ---------------------------------------------------------
-        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, step, ered, scal, gbest);
+        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, g, step, ered, scal, gbest);
     }
     // final spins out (for the caller's state readback / diagnostics)
     if constexpr (kLds) group_save(a, F, S, ldf, G, r0);
@@ -672,6 +674,11 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     }
 }
 
+
+template <bool kLds>
+__global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
+    general_group<kLds>(a, static_cast<int>(blockIdx.x));
+}
 
 // ---- pipelined slice sweeps (dense couplings, LDS-resident state) ---------------------------
 // The same visit orders, random words and FP32 arithmetic as the two-barrier slice sweep of
@@ -688,10 +695,10 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
 // wave 0 applies the same fmaf to the group itself, reading J[order[k-1]] from the ring, so no
 // LDS word is written by two waves in one step. NQ: 4-float groups per lane in a row (ldf <= 256 NQ).
 template <int NQ>
-__global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
+__device__ void pipe_group(const SqaArgs &a, const int g) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int kB = NQ >= 8 ? 1 : 8 / NQ;  // flipped chains per update batch
-    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: role branches stay scalar
     const int n = a.pv.n, P = a.P;
     const int r0 = g * a.rg;
@@ -941,7 +948,7 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
                 cur ^= 1;
             }
         }
-        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, step, L.ered, L.scal, gbest);
+        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, g, step, L.ered, L.scal, gbest);
     }
     group_save(a, F, S, ldf, G, r0);
     if (tid < kMaxChains) {
@@ -950,6 +957,42 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
     }
 }
 
+
+template <int NQ>
+__global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
+    pipe_group<NQ>(a, static_cast<int>(blockIdx.x));
+}
+
+// ---- batched runs: persistent workgroups over (instance, replica group) items -----------------
+struct BatchItem {
+    int inst;   // index into the per-instance SqaArgs
+    int group;  // replica group of that instance
+};
+
+__global__ void reset_counters_kernel(int *c, int count) {
+    if (static_cast<int>(threadIdx.x) < count) c[threadIdx.x] = 0;
+}
+
+// V > 0: pipelined body with V 4-float row groups per lane; V == 0: two-barrier body with the
+// state in LDS; V < 0: two-barrier body with the state in global memory. Every workgroup takes
+// items from *counter until none is left (the exit every wave reaches).
+template <int V>
+__global__ void __launch_bounds__(kThreads) sqa_batch_kernel(const SqaArgs *__restrict__ inst,
+                                                             const BatchItem *__restrict__ items, int nitems,
+                                                             int *counter) {
+    __shared__ int s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const BatchItem b = items[it];
+        if constexpr (V > 0) pipe_group<V>(inst[b.inst], b.group);
+        else if constexpr (V == 0) general_group<true>(inst[b.inst], b.group);
+        else general_group<false>(inst[b.inst], b.group);
+    }
+}
 
 // ---- SQA-PT exchange (sqa_parallel_tempering.cpp:299-335), one workgroup per ladder ----------
 // States stay where they are; the rung <-> state-slot map moves. For adjacent rungs a, b holding
@@ -1060,7 +1103,10 @@ struct SqaPlan {
     size_t shmem = 0;
 };
 
-SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P) {
+// rg_force > 0 fixes the replicas per group (batch autotuner); lds_cap leaves room for a
+// kernel's static LDS.
+SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, int64_t rg_force = 0,
+                 size_t lds_cap = 160 * 1024) {
     const int64_t n = p->n;
     if (P > kMaxChains)
         throw Error{QA_EUNSUPPORTED, "FP32 mode runs at most 64 Trotter slices per replica in this build"};
@@ -1073,7 +1119,7 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P) {
     // replicas per group: fill a 64-lane wave when there are enough replicas for every CU
     pl.rg = std::max<int64_t>(1, kMaxChains / P);
     while (pl.rg > 1 && (R + pl.rg - 1) / pl.rg < ctx->num_cus) --pl.rg;
-    const size_t lds_cap = 160 * 1024;
+    if (rg_force > 0) pl.rg = std::min<int64_t>(rg_force, std::max<int64_t>(1, kMaxChains / P));
     // LDS without the row ring: two visit orders, per-chain scratch, and (LDS layout) the state
     auto fixed = [&](int64_t r, bool lds) {
         size_t b = 2 * lds_align16(static_cast<size_t>(n) * 4) + kMaxChains * (4 + 4 + 4 + 8 + 4 + 8 + 4) + 32 + 16 +
@@ -1106,6 +1152,7 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P) {
 
 void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     SqaArgs a = args;
+    a.ngroups = static_cast<int>(pl.ngroups);
     a.ldf = pl.ldf;
     a.ldr = pl.ldr;
     a.ring = pl.ring;
@@ -1549,6 +1596,283 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
     device_energies(p, bstate.ptr + bg * n, 1, ex.ptr, s);
     ex.download(best_energy, 1, s);
     QA_HIP(hipStreamSynchronize(s));
+}
+
+// ---- batched SQA (many instances, one persistent launch per sweep variant) --------------------
+namespace {
+
+constexpr size_t kBatchLdsCap = 160 * 1024 - 64;  // room for the batch kernel's static LDS
+
+// Sweep variant a plan runs as: pipelined with 1/2/4/8 row groups per lane, or two-barrier with
+// the state in LDS (0) or global memory (-1).
+int batch_variant(const qa_problem *p, const SqaPlan &pl, const qa_batch_params *prm) {
+    const bool pipe = pl.pipe && prm->worldline_sweeps == 0 && prm->cluster_sweeps == 0 && p->kind == QA_KIND_DENSE;
+    if (pipe) {
+        const int ngr = pl.ldf / 4;
+        return ngr <= 64 ? 1 : ngr <= 128 ? 2 : ngr <= 256 ? 4 : 8;
+    }
+    return pl.use_lds ? 0 : -1;
+}
+
+const void *batch_kernel_of(int v) {
+    switch (v) {
+        case 1: return reinterpret_cast<const void *>(sqa_batch_kernel<1>);
+        case 2: return reinterpret_cast<const void *>(sqa_batch_kernel<2>);
+        case 4: return reinterpret_cast<const void *>(sqa_batch_kernel<4>);
+        case 8: return reinterpret_cast<const void *>(sqa_batch_kernel<8>);
+        case 0: return reinterpret_cast<const void *>(sqa_batch_kernel<0>);
+        default: return reinterpret_cast<const void *>(sqa_batch_kernel<-1>);
+    }
+}
+
+// Autotuner: replicas per group for (model, n rounded up to 64, P). Each candidate group size is
+// timed on a trial that fills every CU with one group of this problem for two sweeps; the fastest
+// attempts/s wins and is cached in the context.
+int64_t tuned_rg(qa_ctx *ctx, const qa_problem *p, const qa_batch_params *prm) {
+    const int64_t P = prm->trotter_slices;
+    const auto key = std::make_tuple(p->kind, (p->n + 63) / 64 * 64, P);
+    const auto hit = ctx->tuned_rg.find(key);
+    if (hit != ctx->tuned_rg.end()) return hit->second;
+    hipStream_t s = ctx->stream;
+    const int64_t n = p->n;
+    int64_t best_rg = 1;
+    double best_rate = -1.0;
+    const float bs = static_cast<float>(prm->betas[0] / static_cast<double>(P));
+    const double jpd = trotter_coupling(prm->betas[0], prm->gammas[0], P);
+    const float jp = static_cast<float>(jpd), join = static_cast<float>(1.0 - std::exp(-2.0 * jpd));
+    DevBuf<float> dbs(1), djp(1), djoin(1);
+    dbs.upload(&bs, 1, s);
+    djp.upload(&jp, 1, s);
+    djoin.upload(&join, 1, s);
+    for (int64_t rg = 1; rg * P <= kMaxChains; rg *= 2) {
+        const SqaPlan pl = plan_sqa(ctx, p, rg * ctx->num_cus, P, rg, kBatchLdsCap);
+        if (pl.rg != rg) continue;
+        const int64_t R = rg * ctx->num_cus, ng = pl.ngroups;
+        DevBuf<int8_t> gS(R * P * n), bstate(ng * n);
+        DevBuf<float> gF(pl.use_lds ? 1 : R * P * n);
+        DevBuf<double> tpart(ng), be(ng);
+        DevBuf<int> bstep(ng), bchain(ng);
+        DevBuf<unsigned long long> accd(ng);
+        std::vector<double> inf(ng, std::numeric_limits<double>::infinity());
+        be.upload(inf.data(), ng, s);
+        SqaArgs a{};
+        a.pv = view_of(p);
+        a.P = static_cast<int>(P);
+        a.R = static_cast<int>(R);
+        a.rg = static_cast<int>(rg);
+        a.steps = 1;
+        a.spb = 2;
+        a.wl = static_cast<int>(std::min<int64_t>(prm->worldline_sweeps, 1));
+        a.cl = static_cast<int>(std::min<int64_t>(prm->cluster_sweeps, 1));
+        a.bs = dbs.ptr;
+        a.jp = djp.ptr;
+        a.join = djoin.ptr;
+        a.gS = gS.ptr;
+        a.gF = gF.ptr;
+        a.trace_part = tpart.ptr;
+        a.best_e = be.ptr;
+        a.best_step = bstep.ptr;
+        a.best_chain = bchain.ptr;
+        a.best_state = bstate.ptr;
+        a.accepted = accd.ptr;
+        QA_HIP(hipEventRecord(ctx->ev0, s));
+        launch_sqa(ctx, pl, a);
+        QA_HIP(hipEventRecord(ctx->ev1, s));
+        QA_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        const double rate = static_cast<double>(R * P * n * (a.spb + a.wl + a.cl)) / std::max(1e-3, double(ms));
+        if (rate > best_rate) {
+            best_rate = rate;
+            best_rg = rg;
+        }
+    }
+    ctx->tuned_rg[key] = best_rg;
+    return best_rg;
+}
+
+}  // namespace
+
+void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const qa_batch_params *prm,
+               qa_batch_result *res) {
+    hipStream_t s = ctx->stream;
+    const int64_t P = prm->trotter_slices, steps = prm->steps;
+    const int64_t per_step = prm->sweeps_per_beta + prm->worldline_sweeps + prm->cluster_sweeps;
+    // shared schedule
+    std::vector<float> hbs(steps), hjp(steps), hjoin(steps);
+    for (int64_t k = 0; k < steps; ++k) {
+        const double jp = trotter_coupling(prm->betas[k], prm->gammas[k], P);
+        hbs[k] = static_cast<float>(prm->betas[k] / static_cast<double>(P));
+        hjp[k] = static_cast<float>(jp);
+        hjoin[k] = static_cast<float>(1.0 - std::exp(-2.0 * jp));
+    }
+    DevBuf<float> dbs(steps), djp(steps), djoin(steps);
+    dbs.upload(hbs.data(), steps, s);
+    djp.upload(hjp.data(), steps, s);
+    djoin.upload(hjoin.data(), steps, s);
+
+    // plans and arena offsets
+    struct Inst {
+        SqaPlan pl;
+        int variant;
+        int64_t R, n, oS, oF, oB, oT, oG;  // offsets: spins, fields, best states, trace, per-group
+    };
+    std::vector<Inst> in(count);
+    int64_t nS = 0, nF = 0, nB = 0, nT = 0, nG = 0, attempts = 0;
+    for (int64_t k = 0; k < count; ++k) {
+        const qa_problem *p = probs[k];
+        Inst &x = in[k];
+        x.R = prm->reads[k];
+        x.n = p->n;
+        const int64_t rg = prm->autotune ? tuned_rg(ctx, p, prm) : 0;
+        x.pl = plan_sqa(ctx, p, x.R, P, rg, kBatchLdsCap);
+        x.variant = batch_variant(p, x.pl, prm);
+        x.oS = nS;
+        x.oF = nF;
+        x.oB = nB;
+        x.oT = nT;
+        x.oG = nG;
+        nS += x.R * P * x.n;
+        if (!x.pl.use_lds) nF += x.R * P * x.n;
+        nB += x.pl.ngroups * x.n;
+        nT += steps * x.pl.ngroups;
+        nG += x.pl.ngroups;
+        attempts += steps * per_step * x.R * P * x.n;
+    }
+    DevBuf<int8_t> gS(nS), bstate(nB);
+    DevBuf<float> gF(std::max<int64_t>(nF, 1));
+    DevBuf<double> tpart(nT), be(nG);
+    DevBuf<int> bstep(nG), bchain(nG);
+    DevBuf<unsigned long long> accd(nG);
+    std::vector<double> inf(nG, std::numeric_limits<double>::infinity());
+    std::vector<int> minus1(nG, -1);
+    be.upload(inf.data(), nG, s);
+    bchain.upload(minus1.data(), nG, s);
+
+    // per-instance arguments and work items per variant, largest first
+    std::vector<SqaArgs> args(count);
+    std::map<int, std::vector<std::pair<double, BatchItem>>> work;
+    std::map<int, size_t> shmem;
+    for (int64_t k = 0; k < count; ++k) {
+        const Inst &x = in[k];
+        SqaArgs &a = args[k];
+        a = SqaArgs{};
+        a.pv = view_of(probs[k]);
+        a.P = static_cast<int>(P);
+        a.R = static_cast<int>(x.R);
+        a.rg = static_cast<int>(x.pl.rg);
+        a.steps = static_cast<int>(steps);
+        a.spb = static_cast<int>(prm->sweeps_per_beta);
+        a.wl = static_cast<int>(prm->worldline_sweeps);
+        a.cl = static_cast<int>(prm->cluster_sweeps);
+        a.bs = dbs.ptr;
+        a.jp = djp.ptr;
+        a.join = djoin.ptr;
+        a.k0 = static_cast<uint32_t>(prm->seeds[k]);
+        a.k1 = static_cast<uint32_t>(prm->seeds[k] >> 32);
+        a.gS = gS.ptr + x.oS;
+        a.gF = gF.ptr + x.oF;
+        a.trace_part = tpart.ptr + x.oT;
+        a.best_e = be.ptr + x.oG;
+        a.best_step = bstep.ptr + x.oG;
+        a.best_chain = bchain.ptr + x.oG;
+        a.best_state = bstate.ptr + x.oB;
+        a.accepted = accd.ptr + x.oG;
+        a.ngroups = static_cast<int>(x.pl.ngroups);
+        a.ldf = x.pl.ldf;
+        a.ldr = x.pl.ldr;
+        a.ring = x.pl.ring;
+        a.pfd = x.pl.pfd;
+        const double cost = static_cast<double>(x.pl.rg * P * x.n) * (probs[k]->kind == QA_KIND_DENSE ? x.n : 16);
+        for (int64_t gi = 0; gi < x.pl.ngroups; ++gi)
+            work[x.variant].push_back({cost, BatchItem{static_cast<int>(k), static_cast<int>(gi)}});
+        shmem[x.variant] = std::max(shmem[x.variant], x.pl.shmem);
+    }
+    DevBuf<SqaArgs> dargs(count);
+    dargs.upload(args.data(), count, s);
+    std::vector<BatchItem> items;
+    struct Launch {
+        const void *kern;
+        int64_t first, n;
+        size_t shmem;
+        unsigned grid;
+    };
+    std::vector<Launch> launches;
+    for (auto &kv : work) {
+        auto &v = kv.second;
+        std::stable_sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+        Launch L{batch_kernel_of(kv.first), static_cast<int64_t>(items.size()), static_cast<int64_t>(v.size()),
+                 shmem[kv.first], 0};
+        for (auto &w : v) items.push_back(w.second);
+        const int64_t per_cu = std::max<int64_t>(1, static_cast<int64_t>((160 * 1024) / (L.shmem + 64)));
+        L.grid = static_cast<unsigned>(std::min<int64_t>(L.n, per_cu * ctx->num_cus));
+        QA_HIP(hipFuncSetAttribute(L.kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(L.shmem)));
+        launches.push_back(L);
+    }
+    DevBuf<BatchItem> ditems(items.size());
+    ditems.upload(items.data(), items.size(), s);
+    DevBuf<int> counters(launches.size());
+
+    auto enqueue = [&]() {
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters.ptr, static_cast<int>(launches.size()));
+        QA_HIP(hipGetLastError());
+        for (size_t li = 0; li < launches.size(); ++li) {
+            const SqaArgs *pa = dargs.ptr;
+            const BatchItem *pi = ditems.ptr + launches[li].first;
+            int ni = static_cast<int>(launches[li].n);
+            int *pc = counters.ptr + li;
+            void *params[] = {&pa, &pi, &ni, &pc};
+            QA_HIP(hipLaunchKernel(launches[li].kern, dim3(launches[li].grid), dim3(kThreads), params,
+                                   launches[li].shmem, s));
+        }
+    };
+    QA_HIP(hipStreamSynchronize(s));
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    if (prm->use_graph) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        QA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        QA_HIP(hipStreamEndCapture(s, &graph));
+        QA_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        QA_HIP(hipGraphLaunch(exec, s));
+        QA_HIP(hipStreamSynchronize(s));
+        QA_HIP(hipGraphExecDestroy(exec));
+        QA_HIP(hipGraphDestroy(graph));
+    } else {
+        enqueue();
+    }
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+
+    // per instance: best group (first strict minimum in (step, replica, slice) order), FP64 energy
+    std::vector<double> hbe(nG);
+    std::vector<int> hstep(nG), hchain(nG);
+    be.download(hbe.data(), nG, s);
+    bstep.download(hstep.data(), nG, s);
+    bchain.download(hchain.data(), nG, s);
+    QA_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    DevBuf<double> ex(count);
+    int64_t off = 0;
+    for (int64_t k = 0; k < count; ++k) {
+        const Inst &x = in[k];
+        int64_t bg = -1;
+        for (int64_t gi = 0; gi < x.pl.ngroups; ++gi) {
+            const int64_t q = x.oG + gi;
+            if (hchain[q] < 0) continue;
+            if (bg < 0 || hbe[q] < hbe[x.oG + bg] || (hbe[q] == hbe[x.oG + bg] && hstep[q] < hstep[x.oG + bg])) bg = gi;
+        }
+        QA_REQUIRE(bg >= 0, "no finite slice energy was produced");
+        const int8_t *src = bstate.ptr + x.oB + bg * x.n;
+        QA_HIP(hipMemcpyAsync(res->best_states + off, src, x.n, hipMemcpyDeviceToHost, s));
+        device_energies(probs[k], src, 1, ex.ptr + k, s);
+        off += x.n;
+    }
+    ex.download(res->best_energies, count, s);
+    QA_HIP(hipStreamSynchronize(s));
+    if (res->kernel_ms) *res->kernel_ms = ms;
+    if (res->attempts) *res->attempts = attempts;
 }
 
 }  // namespace qa

@@ -97,7 +97,7 @@ def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
     rng = np.random.default_rng(41)
     cases = [(200, 8, 16, 1, 0, 0), (97, 5, 7, 3, 1, 1), (512, 32, 4, 2, 0, 0), (33, 1, 3, 2, 0, 0),
              (64, 64, 2, 1, 0, 1), (130, 16, 9, 2, 1, 0), (128, 12, 5, 2, 0, 0), (64, 4, 1024, 1, 0, 0),
-             (301, 20, 3, 1, 0, 0)]
+             (301, 20, 3, 1, 0, 0), (600, 8, 2, 1, 0, 0), (1100, 4, 1, 1, 0, 0)]
     for n, P, R, spb, wl, cl in cases:
         print("sweep-path case", n, P, R, spb, wl, cl, flush=True)
         J = _sk(rng, n)
@@ -152,6 +152,66 @@ def test_sqa_fp32_distribution_vs_oracle(ctx):
     p = ks_2samp(gpu, cpu).pvalue
     print("KS p", p, "gpu mean", gpu.mean(), "cpu mean", cpu.mean(), flush=True)
     assert p > 0.01
+
+
+def _sparse_case(rng, n, density):
+    ei, ej = np.triu_indices(n, 1)
+    keep = rng.random(ei.size) < density
+    return rng.normal(0, 0.3, n), [(int(a), int(b), float(v)) for a, b, v in
+                                   zip(ei[keep], ej[keep], rng.normal(0, 1, int(keep.sum())))]
+
+
+def _hubo_case(rng, n, nterms):
+    """Raw HigherOrderIsing terms: distinct 3- and 4-body supports with N(0,1) coefficients plus
+    N(0, 0.3) linear terms."""
+    seen, terms = set(), []
+    while len(terms) < nterms:
+        t = tuple(sorted(rng.choice(n, int(rng.integers(3, 5)), replace=False).tolist()))
+        if t not in seen:
+            seen.add(t)
+            terms.append((t, float(rng.normal())))
+    return terms + [((i,), float(rng.normal(0, 0.3))) for i in range(n)]
+
+
+def test_sqa_fp32_sparse_hubo(ctx):
+    """Sparse and higher-order couplings through the FP32 two-barrier sweep (slice, worldline and
+    cluster moves): every returned best energy is the oracle's FP64 energy of the returned state,
+    and the best energies of independent reads follow the oracle SQA's distribution (two-sample
+    KS, p > 0.01)."""
+    from scipy.stats import ks_2samp
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    from tests.golden_io import consolidate_hubo
+    orc = Oracle(ORACLE_SO)
+    orc.set_threads(8)
+    rng = np.random.default_rng(23)
+    h, edges = _sparse_case(rng, 160, 0.04)
+    sp_o, sp_d = Problem.sparse(h, edges, 0.5), abi.DeviceProblem.sparse(ctx, h, edges, 0.5)
+    terms = _hubo_case(rng, 50, 150)
+    hb_o = Problem.hubo(50, terms, -0.25)
+    hh, off, var, co, dc = consolidate_hubo(50, terms)
+    hb_d = abi.DeviceProblem.hubo(ctx, 50, hh, off, var, co, -0.25 + dc)
+    for name, po, pd in (("sparse", sp_o, sp_d), ("hubo", hb_o, hb_d)):
+        for P, R, wl, cl in ((8, 3, 0, 0), (6, 2, 1, 1), (1, 4, 0, 1)):
+            print("exact-energy case", name, P, R, wl, cl, flush=True)
+            r = abi.sqa_run(pd, np.full(12, 2.0), np.geomspace(2.0, 0.05, 12), P, R, 1, wl, cl, abi.MODE_FP32, 7)
+            assert r["best_energy"] == orc.energy(po, r["best_state"])[0], (name, P, wl, cl)
+    # distributions: small instances, 128 independent reads per path
+    h, edges = _sparse_case(rng, 24, 0.3)
+    terms = _hubo_case(rng, 16, 30)
+    hh, off, var, co, dc = consolidate_hubo(16, terms)
+    cases = (("sparse", Problem.sparse(h, edges), abi.DeviceProblem.sparse(ctx, h, edges)),
+             ("hubo", Problem.hubo(16, terms), abi.DeviceProblem.hubo(ctx, 16, hh, off, var, co, dc)))
+    steps, P = 40, 8
+    betas, gammas = np.full(steps, 1.5), np.linspace(2.0, 0.05, steps)
+    for name, po, pd in cases:
+        gpu = np.array([abi.sqa_run(pd, betas, gammas, P, 1, 1, 1, 1, abi.MODE_FP32, 300 + k)["best_energy"]
+                        for k in range(128)])
+        cpu = np.array([orc.sqa_run(po, betas, gammas, P, 1, 1, 1, 1, 0, 900 + k)["best_energy"]
+                        for k in range(128)])
+        p = ks_2samp(gpu, cpu).pvalue
+        print("KS", name, p, "gpu mean", gpu.mean(), "cpu mean", cpu.mean(), flush=True)
+        assert p > 0.01, name
 
 
 def test_sqa_fp32_stationary_mean(ctx):
