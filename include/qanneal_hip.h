@@ -218,6 +218,12 @@ typedef struct qa_chi_params {
     const uint64_t *cell_seeds;  /* [(scan_points+1)*R*P]                                     */
     /* FP32 mode: when non-NULL the pilot scan is skipped and these chi_B values are used. */
     const double *chi_override;  /* [scan_points]                                            */
+    /* FP32 mode, chi_override NULL: when non-NULL the chi_B profile is predicted by this trained
+     * surrogate (n_out == scan_points) instead of scanned; the pilot scan still runs when the
+     * ensemble spread exceeds surrogate_max_spread or the instance lies outside the training
+     * range of the surrogate's features. */
+    const struct qa_mlp *surrogate;
+    double surrogate_max_spread;
 } qa_chi_params;
 
 typedef struct qa_chi_result {
@@ -227,10 +233,58 @@ typedef struct qa_chi_result {
     double *scan_s, *scan_gamma, *scan_chi_B;                             /* [scan_points]  */
     double *scalars;             /* [s_star, gamma_star, j_perp_star, chi_floor, driver_A0]  */
     double *kernel_ms;
+    int32_t *surrogate_used;     /* optional: 1 if the surrogate's profile was used, 0 if the
+                                    pilot ran (no surrogate, or the fallback was taken)        */
+    double *surrogate_spread;    /* optional: ensemble spread of the prediction (+inf when the
+                                    features were outside the training range)                */
 } qa_chi_result;
 
 qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm,
                          qa_chi_result *res);
+
+/* Stage 1 of run_chi alone (the pilot chi_B scan, sqa_chi_annealer.cpp:232-281), FP32 mode: the
+ * profile the production schedule is built from, and the training label of the surrogate.
+ * Uses beta, gamma_start, gamma_end, driver_A0, trotter_slices, replicas, scan_points,
+ * scan_sweeps, scan_burn and seed of prm. */
+qa_status qa_chi_pilot(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, double *scan_s,
+                       double *scan_gamma, double *scan_chi_B, double *kernel_ms);
+
+/* ---- chi_B surrogate: no counterpart in the reference ----------------------------------------
+ * For callers of run_chi that want the schedule without the pilot scan. Features of an instance
+ * (QA_CHI_NFEATURES doubles): log2 n, log2(1 + couplings per spin), log field scale, field-scale
+ * heterogeneity, log coupling kurtosis, negative-coupling fraction, mean |h| / field scale, mean
+ * interaction order, log(beta * field scale), log P, log(A0 / field scale),
+ * log(gamma_end / gamma_start). The network is an ensemble of `members` MLPs
+ * n_in -> width (tanh) x depth -> n_out (sigmoid) trained on normalized pilot profiles
+ * chi_B / max chi_B; its ensemble mean is the predicted profile and the largest per-point
+ * standard deviation across members is its spread. */
+#define QA_CHI_NFEATURES 12
+qa_status qa_chi_features(const qa_problem *p, double beta, int64_t trotter_slices, double driver_A0,
+                          double gamma_start, double gamma_end, double *features);
+
+typedef struct qa_mlp qa_mlp;
+typedef struct qa_mlp_train_params {
+    int64_t epochs, batch;
+    double lr, beta1, beta2, eps, weight_decay;  /* Adam; 0 picks 0.9 / 0.999 / 1e-8         */
+    uint64_t seed;                               /* minibatch order                           */
+} qa_mlp_train_params;
+
+qa_status qa_mlp_create(qa_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32_t n_out,
+                        int32_t members, uint64_t seed, qa_mlp **out);
+qa_status qa_mlp_destroy(qa_mlp *m);
+/* parameters per member; layout of one member: for each layer W [out][in] then b [out] */
+qa_status qa_mlp_info(const qa_mlp *m, int64_t *params_per_member, int32_t *trained);
+/* params [members * params_per_member]; norm [4 * n_in]: mean, scale, training low, high */
+qa_status qa_mlp_get_state(const qa_mlp *m, float *params, double *norm);
+qa_status qa_mlp_set_state(qa_mlp *m, const float *params, const double *norm);
+/* y [members][batch][n_out] from raw features x [batch][n_in] */
+qa_status qa_mlp_forward(qa_mlp *m, int64_t batch, const double *x, float *y);
+/* mean squared error per member against t [batch][n_out] and its gradient [members][params] */
+qa_status qa_mlp_loss_grad(qa_mlp *m, int64_t batch, const double *x, const float *t, double *loss,
+                           float *grad);
+/* Adam on minibatches; sets the input normalization from x; mean loss per epoch */
+qa_status qa_mlp_train(qa_mlp *m, int64_t samples, const double *x, const float *t,
+                       const qa_mlp_train_params *tp, double *loss_history);
 
 #ifdef __cplusplus
 }

@@ -66,14 +66,20 @@ class ChiParams(C.Structure):
                 ("scan_points", _i64), ("scan_sweeps", _i64), ("scan_burn", _i64),
                 ("chi_floor_fraction", _dbl), ("driver_A0", _dbl), ("beta_ramp_fraction", _dbl),
                 ("beta_ramp_start", _dbl), ("mode", _i32), ("seed", _u64), ("init_spins", _bp),
-                ("cell_seeds", _up), ("chi_override", _dp)]
+                ("cell_seeds", _up), ("chi_override", _dp), ("surrogate", C.c_void_p),
+                ("surrogate_max_spread", _dbl)]
 
 
 class ChiResult(C.Structure):
     _fields_ = [("best_state", _bp), ("best_energy", _dp), ("energy_trace", _dp),
                 ("beta_schedule", _dp), ("gamma_schedule", _dp), ("s_schedule", _dp),
                 ("scan_s", _dp), ("scan_gamma", _dp), ("scan_chi_B", _dp), ("scalars", _dp),
-                ("kernel_ms", _dp)]
+                ("kernel_ms", _dp), ("surrogate_used", C.POINTER(_i32)), ("surrogate_spread", _dp)]
+
+
+class MlpTrainParams(C.Structure):
+    _fields_ = [("epochs", _i64), ("batch", _i64), ("lr", _dbl), ("beta1", _dbl), ("beta2", _dbl), ("eps", _dbl),
+                ("weight_decay", _dbl), ("seed", _u64)]
 
 
 class BatchParams(C.Structure):
@@ -109,7 +115,9 @@ def lib() -> C.CDLL:
         L.qa_last_error.restype = C.c_char_p
         for name in ("qa_ctx_create", "qa_ctx_destroy", "qa_problem_dense", "qa_problem_sparse",
                      "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_local_fields", "qa_sa_run", "qa_sqa_run",
-                     "qa_sqapt_run", "qa_sqa_chi_run", "qa_sqa_batch_run"):
+                     "qa_sqapt_run", "qa_sqa_chi_run", "qa_sqa_batch_run", "qa_chi_pilot", "qa_chi_features",
+                     "qa_mlp_create", "qa_mlp_destroy", "qa_mlp_info", "qa_mlp_get_state", "qa_mlp_set_state",
+                     "qa_mlp_forward", "qa_mlp_loss_grad", "qa_mlp_train"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -304,7 +312,8 @@ def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: f
                 gamma_end: float, num_steps: int = 100, sweeps_per_step: int = 20, scan_points: int = 16,
                 scan_sweeps: int = 30, scan_burn: int = 10, chi_floor_fraction: float = 1e-6,
                 driver_A0: float = 0.0, beta_ramp_fraction: float = 0.3, beta_ramp_start: float = 0.1,
-                mode: int = MODE_FP32, seed: int = 0, init_spins=None, cell_seeds=None, chi_override=None):
+                mode: int = MODE_FP32, seed: int = 0, init_spins=None, cell_seeds=None, chi_override=None,
+                surrogate: "Optional[Mlp]" = None, surrogate_max_spread: float = 0.05):
     p = ChiParams(trotter_slices, replicas, beta, gamma_start, gamma_end, num_steps, sweeps_per_step, scan_points,
                   scan_sweeps, scan_burn, chi_floor_fraction, driver_A0, beta_ramp_fraction, beta_ramp_start,
                   mode, seed & (2**64 - 1))
@@ -321,16 +330,115 @@ def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: f
         chi_override = np.ascontiguousarray(chi_override, np.float64)
         keep.append(chi_override)
         p.chi_override = _d(chi_override)
+    if surrogate is not None:
+        p.surrogate = surrogate.h
+        p.surrogate_max_spread = float(surrogate_max_spread)
     n = prob.n
     bs, be = np.zeros(n, np.int8), np.zeros(1)
     tr, bsch, gsch, ssch = (np.zeros(num_steps) for _ in range(4))
     ss, sg, sc = (np.zeros(scan_points) for _ in range(3))
     sc5, ms = np.zeros(5), np.zeros(1)
-    r = ChiResult(_b(bs), _d(be), _d(tr), _d(bsch), _d(gsch), _d(ssch), _d(ss), _d(sg), _d(sc), _d(sc5), _d(ms))
+    used, spread = np.zeros(1, np.int32), np.zeros(1)
+    r = ChiResult(_b(bs), _d(be), _d(tr), _d(bsch), _d(gsch), _d(ssch), _d(ss), _d(sg), _d(sc), _d(sc5), _d(ms),
+                  used.ctypes.data_as(C.POINTER(_i32)), _d(spread))
     check(lib().qa_sqa_chi_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, beta_schedule=bsch, gamma_schedule=gsch,
                 s_schedule=ssch, scan_s=ss, scan_gamma=sg, scan_chi_B=sc, s_star=sc5[0], gamma_star=sc5[1],
-                j_perp_star=sc5[2], chi_floor=sc5[3], driver_A0=sc5[4], kernel_ms=float(ms[0]))
+                j_perp_star=sc5[2], chi_floor=sc5[3], driver_A0=sc5[4], kernel_ms=float(ms[0]),
+                surrogate_used=bool(used[0]), surrogate_spread=float(spread[0]))
+
+
+def chi_pilot(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: float, gamma_start: float,
+              gamma_end: float, scan_points: int = 16, scan_sweeps: int = 30, scan_burn: int = 10,
+              driver_A0: float = 0.0, seed: int = 0):
+    """The pilot chi_B scan of run_chi alone (FP32): scan_s, scan_gamma, scan_chi_B, kernel_ms."""
+    p = ChiParams(trotter_slices, replicas, beta, gamma_start, gamma_end, 2, 1, scan_points, scan_sweeps, scan_burn,
+                  1e-6, driver_A0, 0.0, 0.0, MODE_FP32, seed & (2**64 - 1))
+    ss, sg, sc, ms = np.zeros(scan_points), np.zeros(scan_points), np.zeros(scan_points), np.zeros(1)
+    check(lib().qa_chi_pilot(prob.ctx.h, prob.h, C.byref(p), _d(ss), _d(sg), _d(sc), _d(ms)))
+    return dict(scan_s=ss, scan_gamma=sg, scan_chi_B=sc, kernel_ms=float(ms[0]))
+
+
+CHI_NFEATURES = 12
+
+
+def chi_features(prob: DeviceProblem, beta: float, trotter_slices: int, driver_A0: float, gamma_start: float,
+                 gamma_end: float) -> np.ndarray:
+    f = np.zeros(CHI_NFEATURES)
+    check(lib().qa_chi_features(prob.h, _dbl(beta), _i64(trotter_slices), _dbl(driver_A0), _dbl(gamma_start),
+                                _dbl(gamma_end), _d(f)))
+    return f
+
+
+class Mlp:
+    """Device MLP ensemble of the chi_B surrogate (qa_mlp_*): members x (n_in -> width tanh x depth
+    -> n_out sigmoid)."""
+
+    def __init__(self, ctx: Context, n_in: int, width: int, depth: int, n_out: int, members: int = 1,
+                 seed: int = 0):
+        self.ctx = ctx
+        self.n_in, self.width, self.depth, self.n_out, self.members = n_in, width, depth, n_out, members
+        self.h = C.c_void_p()
+        check(lib().qa_mlp_create(ctx.h, C.c_int32(n_in), C.c_int32(width), C.c_int32(depth), C.c_int32(n_out),
+                                  C.c_int32(members), C.c_uint64(seed & (2**64 - 1)), C.byref(self.h)))
+        n, t = C.c_int64(), C.c_int32()
+        check(lib().qa_mlp_info(self.h, C.byref(n), C.byref(t)))
+        self.params_per_member = int(n.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.qa_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    @property
+    def trained(self) -> bool:
+        n, t = C.c_int64(), C.c_int32()
+        check(lib().qa_mlp_info(self.h, C.byref(n), C.byref(t)))
+        return bool(t.value)
+
+    def get_state(self):
+        params = np.zeros(self.members * self.params_per_member, np.float32)
+        norm = np.zeros(4 * self.n_in)
+        check(lib().qa_mlp_get_state(self.h, params.ctypes.data_as(C.POINTER(C.c_float)), _d(norm)))
+        return params.reshape(self.members, self.params_per_member), norm.reshape(4, self.n_in)
+
+    def set_state(self, params=None, norm=None):
+        pp = C.POINTER(C.c_float)()
+        nn = C.POINTER(C.c_double)()
+        if params is not None:
+            params = np.ascontiguousarray(params, np.float32).reshape(-1)
+            assert params.size == self.members * self.params_per_member
+            pp = params.ctypes.data_as(C.POINTER(C.c_float))
+        if norm is not None:
+            norm = np.ascontiguousarray(norm, np.float64).reshape(-1)
+            assert norm.size == 4 * self.n_in
+            nn = _d(norm)
+        check(lib().qa_mlp_set_state(self.h, pp, nn))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, self.n_in)
+        y = np.zeros((self.members, x.shape[0], self.n_out), np.float32)
+        check(lib().qa_mlp_forward(self.h, _i64(x.shape[0]), _d(x), y.ctypes.data_as(C.POINTER(C.c_float))))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, self.n_in)
+        t = np.ascontiguousarray(t, np.float32).reshape(x.shape[0], self.n_out)
+        loss = np.zeros(self.members)
+        grad = np.zeros((self.members, self.params_per_member), np.float32)
+        check(lib().qa_mlp_loss_grad(self.h, _i64(x.shape[0]), _d(x), t.ctypes.data_as(C.POINTER(C.c_float)),
+                                     _d(loss), grad.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss, grad
+
+    def train(self, x, t, epochs: int, batch: int, lr: float = 1e-3, weight_decay: float = 0.0, seed: int = 0,
+              beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, self.n_in)
+        t = np.ascontiguousarray(t, np.float32).reshape(x.shape[0], self.n_out)
+        hist = np.zeros(epochs)
+        tp = MlpTrainParams(epochs, batch, lr, beta1, beta2, eps, weight_decay, seed & (2**64 - 1))
+        check(lib().qa_mlp_train(self.h, _i64(x.shape[0]), _d(x), t.ctypes.data_as(C.POINTER(C.c_float)),
+                                 C.byref(tp), _d(hist)))
+        return hist
 
 
 def sqa_batch_run(problems, betas, gammas, trotter_slices: int, sweeps_per_beta: int, reads, seeds,

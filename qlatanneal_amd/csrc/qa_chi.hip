@@ -224,7 +224,7 @@ __global__ void replica_sums(const int8_t *spins, int P, int n, long long *out) 
 
 void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
     QA_REQUIRE(prm->init_spins && prm->cell_seeds, "validation mode needs init_spins and cell_seeds");
-    QA_REQUIRE(!prm->chi_override, "a supplied chi_B profile is an FP32-mode path");
+    QA_REQUIRE(!prm->chi_override && !prm->surrogate, "a supplied or predicted chi_B profile is an FP32-mode path");
     const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices, SP = prm->scan_points, NS = prm->num_steps;
     const int64_t cells = R * P;
     for (int64_t k = 0; k < (SP + 1) * cells * n; ++k)
@@ -345,37 +345,120 @@ void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, 
     if (res->kernel_ms) *res->kernel_ms = total_ms;
 }
 
+namespace {
+
+// chi_B of each scan point from the FP32 pilot (:232-281): chi = max(n P (<m^2> - <|m|>^2), 0)
+void pilot_chi(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, const std::vector<double> &sjp,
+               std::vector<double> &chi, float *ms) {
+    const int64_t R = prm->replicas, P = prm->trotter_slices, SP = prm->scan_points;
+    std::vector<double> bsv(SP, prm->beta / static_cast<double>(P)), m2, ma;
+    sqa_fp32_pilot(ctx, p, P, R, SP, bsv, sjp, prm->scan_burn, prm->scan_sweeps, prm->seed, m2, ma, ms);
+    const double cnt = static_cast<double>(prm->scan_sweeps * R);
+    chi.assign(SP, 0.0);
+    for (int64_t q = 0; q < SP; ++q) {
+        double s2 = 0.0, sa = 0.0;
+        for (int64_t r = 0; r < R; ++r) {
+            s2 += m2[q * R + r];
+            sa += ma[q * R + r];
+        }
+        if (cnt >= 2.0) {
+            const double m2m = s2 / cnt, mam = sa / cnt;
+            chi[q] = std::max(static_cast<double>(p->n * P) * (m2m - mam * mam), 0.0);
+        }
+    }
+}
+
+// Surrogate profile: ensemble mean of the normalized chi_B prediction. Returns false (pilot
+// fallback) when the features leave the training range by more than a quarter of its width or
+// the largest per-point standard deviation across members exceeds the caller's bound.
+bool surrogate_chi(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, std::vector<double> &chi,
+                   double *spread) {
+    qa_mlp *m = const_cast<qa_mlp *>(prm->surrogate);
+    const int64_t SP = prm->scan_points;
+    QA_REQUIRE(m->ctx == ctx, "the surrogate belongs to another context");
+    QA_REQUIRE(m->trained, "the surrogate has not been trained or loaded");
+    QA_REQUIRE(m->n_in == QA_CHI_NFEATURES && m->n_out == SP,
+               "the surrogate's input/output sizes do not match the features and scan_points");
+    double f[QA_CHI_NFEATURES];
+    chi_features(p, prm->beta, prm->trotter_slices, prm->driver_A0, prm->gamma_start, prm->gamma_end, f);
+    *spread = std::numeric_limits<double>::infinity();
+    for (int k = 0; k < QA_CHI_NFEATURES; ++k) {
+        const double v = (f[k] - m->x_mean[k]) / m->x_scale[k];
+        const double w = 0.25 * (m->x_hi[k] - m->x_lo[k]);
+        if (!(v >= m->x_lo[k] - w && v <= m->x_hi[k] + w)) return false;
+    }
+    std::vector<float> y(static_cast<size_t>(m->members) * SP);
+    mlp_forward(m, 1, f, y.data());
+    chi.assign(SP, 0.0);
+    double worst = 0.0;
+    for (int64_t q = 0; q < SP; ++q) {
+        double mu = 0.0, sq = 0.0;
+        for (int z = 0; z < m->members; ++z) mu += y[z * SP + q];
+        mu /= m->members;
+        for (int z = 0; z < m->members; ++z) sq += (y[z * SP + q] - mu) * (y[z * SP + q] - mu);
+        chi[q] = mu;
+        worst = std::max(worst, std::sqrt(sq / m->members));
+    }
+    *spread = worst;
+    return worst <= prm->surrogate_max_spread;
+}
+
+}  // namespace
+
+void chi_features(const qa_problem *p, double beta, int64_t P, double A0, double gamma_start, double gamma_end,
+                  double *f) {
+    const double F = std::max(p->st_field_scale, 1e-12);
+    const double A = A0 > 0.0 ? A0 : gamma_start;
+    f[0] = std::log2(static_cast<double>(p->n));
+    f[1] = std::log2(1.0 + p->st_mean_degree);
+    f[2] = std::log(F);
+    f[3] = p->st_field_het;
+    f[4] = std::log(std::max(p->st_kurtosis, 1e-12));
+    f[5] = p->st_neg_frac;
+    f[6] = p->st_h_ratio;
+    f[7] = p->st_order;
+    f[8] = std::log(beta * F);
+    f[9] = std::log(static_cast<double>(P));
+    f[10] = std::log(A / F);
+    f[11] = std::log(gamma_end / gamma_start);
+}
+
+void chi_pilot_profile(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, double *scan_s, double *scan_gamma,
+                       double *scan_chi, float *ms) {
+    const ChiGrid g = make_grid(prm);
+    std::vector<double> ss, sg, sjp, chi;
+    scan_grid(prm, g, prm->trotter_slices, ss, sg, sjp);
+    pilot_chi(ctx, p, prm, sjp, chi, ms);
+    for (int64_t q = 0; q < prm->scan_points; ++q) {
+        scan_s[q] = ss[q];
+        scan_gamma[q] = sg[q];
+        scan_chi[q] = chi[q];
+    }
+}
+
 void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
-    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices, SP = prm->scan_points;
-    (void)n;
+    const int64_t P = prm->trotter_slices, SP = prm->scan_points, R = prm->replicas;
     const ChiGrid g = make_grid(prm);
     std::vector<double> ss, sg, sjp;
     scan_grid(prm, g, P, ss, sg, sjp);
     std::vector<double> chi(SP, 0.0);
     float ms_pilot = 0.0f, ms_prod = 0.0f;
+    int32_t used = 0;
+    double spread = std::numeric_limits<double>::quiet_NaN();
     if (prm->chi_override) {
         for (int64_t q = 0; q < SP; ++q) chi[q] = prm->chi_override[q];
+    } else if (prm->surrogate && surrogate_chi(ctx, p, prm, chi, &spread)) {
+        used = 1;
     } else {
-        std::vector<double> bsv(SP, prm->beta / static_cast<double>(P)), m2, ma;
-        sqa_fp32_pilot(ctx, p, P, R, SP, bsv, sjp, prm->scan_burn, prm->scan_sweeps, prm->seed, m2, ma, &ms_pilot);
-        const double cnt = static_cast<double>(prm->scan_sweeps * R);
-        for (int64_t q = 0; q < SP; ++q) {
-            double s2 = 0.0, sa = 0.0;
-            for (int64_t r = 0; r < R; ++r) {
-                s2 += m2[q * R + r];
-                sa += ma[q * R + r];
-            }
-            if (cnt >= 2.0) {
-                const double m2m = s2 / cnt, mam = sa / cnt;
-                chi[q] = std::max(static_cast<double>(p->n * P) * (m2m - mam * mam), 0.0);
-            }
-        }
+        pilot_chi(ctx, p, prm, sjp, chi, &ms_pilot);
     }
     build_schedule(prm, g, P, ss, sg, chi, res);
     sqa_fp32_chi_production(ctx, p, P, R, prm->num_steps, res->beta_schedule, res->gamma_schedule,
                             prm->sweeps_per_step, prm->seed ^ 0x9E3779B97F4A7C15ull, res->best_state,
                             res->best_energy, res->energy_trace, &ms_prod);
     if (res->kernel_ms) *res->kernel_ms = ms_pilot + ms_prod;
+    if (res->surrogate_used) *res->surrogate_used = used;
+    if (res->surrogate_spread) *res->surrogate_spread = spread;
 }
 
 }  // namespace qa

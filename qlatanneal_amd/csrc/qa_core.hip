@@ -93,6 +93,26 @@ static void check_finite(const double *v, int64_t len, const char *what) {
         if (!std::isfinite(v[i])) throw Error{QA_EINVAL, std::string(what) + " must be finite."};
 }
 
+void StatsAcc::finish(qa_problem *p, const double *h) const {
+    const int64_t n = p->n;
+    double dsum = 0.0, fs = 0.0, fs2 = 0.0, hs = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        dsum += deg[i];
+        const double r = std::sqrt(r2[i]);
+        fs += r;
+        fs2 += r * r;
+        hs += std::fabs(h[i]);
+    }
+    const double F = fs / static_cast<double>(n);
+    p->st_mean_degree = dsum / static_cast<double>(n);
+    p->st_field_scale = F;
+    p->st_field_het = F > 0.0 ? std::sqrt(std::max(fs2 / static_cast<double>(n) - F * F, 0.0)) / F : 0.0;
+    p->st_kurtosis = (cnt > 0.0 && s2 > 0.0) ? (s4 / cnt) / ((s2 / cnt) * (s2 / cnt)) : 1.0;
+    p->st_neg_frac = cnt > 0.0 ? neg / cnt : 0.0;
+    p->st_h_ratio = F > 0.0 ? (hs / static_cast<double>(n)) / F : 0.0;
+    p->st_order = cnt > 0.0 ? order_sum / cnt : 2.0;
+}
+
 }  // namespace qa
 
 using namespace qa;
@@ -150,13 +170,22 @@ qa_status qa_problem_dense(qa_ctx *ctx, int64_t n, const double *h, const double
         std::vector<float> hf(n), Jf(static_cast<size_t>(n * n));
         for (int64_t i = 0; i < n; ++i) hf[i] = static_cast<float>(h[i]);
         bool sym = true;
+        StatsAcc st(n);
         for (int64_t i = 0; i < n; ++i)
             for (int64_t j = 0; j < n; ++j) {
                 const double v = J[i * n + j];
                 Jf[i * n + j] = (i == j) ? 0.0f : static_cast<float>(v);
                 if (v != J[j * n + i]) sym = false;
+                if (j > i && v != 0.0) {  // the energy's couplings: J_ij, i < j (dense_ising.cpp:29-44)
+                    st.deg[i] += 1.0;
+                    st.deg[j] += 1.0;
+                    st.r2[i] += v * v;
+                    st.r2[j] += v * v;
+                    st.coupling(v, 2);
+                }
             }
         p->symmetric = sym;
+        st.finish(p.get(), h);
         p->h64.alloc(n);
         p->h32.alloc(n);
         p->J64.alloc(n * n);
@@ -218,6 +247,20 @@ qa_status qa_problem_sparse(qa_ctx *ctx, int64_t n, const double *h, int64_t ned
         for (int64_t i = 0; i < n; ++i) hf[i] = static_cast<float>(h[i]);
         QA_HIP(hipSetDevice(ctx->device));
         auto p = std::make_unique<qa_problem>();
+        {
+            StatsAcc st(n);
+            for (int64_t e = 0; e < nedges; ++e) {
+                const double v = ev[e];
+                if (v == 0.0) continue;
+                st.deg[ei[e]] += 1.0;
+                st.deg[ej[e]] += 1.0;
+                st.r2[ei[e]] += v * v;
+                st.r2[ej[e]] += v * v;
+                st.coupling(v, 2);
+            }
+            p->n = n;
+            st.finish(p.get(), h);
+        }
         p->ctx = ctx;
         p->kind = QA_KIND_SPARSE;
         p->n = n;
@@ -276,6 +319,21 @@ qa_status qa_problem_hubo(qa_ctx *ctx, int64_t n, const double *h, int64_t nterm
         for (int64_t i = 0; i < n; ++i) hf[i] = static_cast<float>(h[i]);
         QA_HIP(hipSetDevice(ctx->device));
         auto p = std::make_unique<qa_problem>();
+        {
+            StatsAcc st(n);
+            for (int64_t t = 0; t < nterms; ++t) {
+                const double v = term_coeff[t];
+                if (v == 0.0) continue;
+                const int k = static_cast<int>(term_offset[t + 1] - term_offset[t]);
+                for (int64_t q = term_offset[t]; q < term_offset[t + 1]; ++q) {
+                    st.deg[term_vars[q]] += k - 1;
+                    st.r2[term_vars[q]] += v * v;
+                }
+                st.coupling(v, k);
+            }
+            p->n = n;
+            st.finish(p.get(), h);
+        }
         p->ctx = ctx;
         p->kind = QA_KIND_HUBO;
         p->n = n;
@@ -359,6 +417,140 @@ qa_status qa_sqa_batch_run(qa_ctx *ctx, int64_t count, const qa_problem *const *
                                      "(it did not complete on the GPU and the cause is not yet known); "
                                      "run the instances with qa_sqa_run"};
         sqa_batch(ctx, count, problems, prm, res);
+    });
+}
+
+
+qa_status qa_chi_features(const qa_problem *p, double beta, int64_t trotter_slices, double driver_A0,
+                          double gamma_start, double gamma_end, double *features) {
+    return guarded([&] {
+        QA_REQUIRE(p && features, "qa_chi_features: NULL argument");
+        QA_REQUIRE(beta > 0.0 && trotter_slices > 0 && gamma_start > 0.0 && gamma_end > 0.0,
+                   "qa_chi_features: beta, trotter_slices and the gamma range must be positive");
+        chi_features(p, beta, trotter_slices, driver_A0, gamma_start, gamma_end, features);
+    });
+}
+
+qa_status qa_chi_pilot(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, double *scan_s,
+                       double *scan_gamma, double *scan_chi_B, double *kernel_ms) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && scan_s && scan_gamma && scan_chi_B, "qa_chi_pilot: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_chi_pilot: problem belongs to another context");
+        QA_REQUIRE(prm->trotter_slices > 0 && prm->replicas > 0, "trotter_slices and replicas must be > 0.");
+        QA_REQUIRE(prm->scan_points >= 2, "scan_points must be >= 2.");
+        QA_REQUIRE(prm->scan_sweeps > 0 && prm->scan_burn >= 0, "scan_sweeps must be > 0, scan_burn >= 0.");
+        QA_REQUIRE(prm->beta > 0.0 && prm->gamma_start > 0.0 && prm->gamma_end > 0.0,
+                   "beta and the gamma range must be positive.");
+        QA_HIP(hipSetDevice(ctx->device));
+        float ms = 0.0f;
+        chi_pilot_profile(ctx, p, prm, scan_s, scan_gamma, scan_chi_B, &ms);
+        if (kernel_ms) *kernel_ms = ms;
+    });
+}
+
+qa_status qa_mlp_create(qa_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32_t n_out, int32_t members,
+                        uint64_t seed, qa_mlp **out) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && out, "qa_mlp_create: NULL argument");
+        QA_REQUIRE(n_in > 0 && width > 0 && depth > 0 && n_out > 0 && members > 0,
+                   "qa_mlp_create: sizes must be > 0");
+        QA_REQUIRE(n_in <= 4096 && width <= 4096 && depth <= 16 && n_out <= 4096 && members <= 64,
+                   "qa_mlp_create: network too large");
+        QA_HIP(hipSetDevice(ctx->device));
+        auto m = std::make_unique<qa_mlp>();
+        m->ctx = ctx;
+        m->n_in = n_in;
+        m->width = width;
+        m->depth = depth;
+        m->n_out = n_out;
+        m->members = members;
+        mlp_init(m.get(), seed);
+        *out = m.release();
+    });
+}
+
+qa_status qa_mlp_destroy(qa_mlp *m) {
+    return guarded([&] { delete m; });
+}
+
+qa_status qa_mlp_info(const qa_mlp *m, int64_t *params_per_member, int32_t *trained) {
+    return guarded([&] {
+        QA_REQUIRE(m, "qa_mlp_info: NULL model");
+        if (params_per_member) *params_per_member = m->nparams;
+        if (trained) *trained = m->trained ? 1 : 0;
+    });
+}
+
+qa_status qa_mlp_get_state(const qa_mlp *m, float *params, double *norm) {
+    return guarded([&] {
+        QA_REQUIRE(m, "qa_mlp_get_state: NULL model");
+        QA_HIP(hipSetDevice(m->ctx->device));
+        if (params) {
+            m->params.download(params, static_cast<size_t>(m->members) * m->nparams, m->ctx->stream);
+            QA_HIP(hipStreamSynchronize(m->ctx->stream));
+        }
+        if (norm)
+            for (int k = 0; k < m->n_in; ++k) {
+                norm[k] = m->x_mean[k];
+                norm[m->n_in + k] = m->x_scale[k];
+                norm[2 * m->n_in + k] = m->x_lo[k];
+                norm[3 * m->n_in + k] = m->x_hi[k];
+            }
+    });
+}
+
+qa_status qa_mlp_set_state(qa_mlp *m, const float *params, const double *norm) {
+    return guarded([&] {
+        QA_REQUIRE(m, "qa_mlp_set_state: NULL model");
+        QA_HIP(hipSetDevice(m->ctx->device));
+        if (params) {
+            for (int64_t e = 0; e < static_cast<int64_t>(m->members) * m->nparams; ++e)
+                QA_REQUIRE(std::isfinite(params[e]), "qa_mlp_set_state: parameters must be finite");
+            m->params.upload(params, static_cast<size_t>(m->members) * m->nparams, m->ctx->stream);
+            mlp_sync_bf16(m);
+            QA_HIP(hipStreamSynchronize(m->ctx->stream));
+        }
+        if (norm) {
+            for (int k = 0; k < m->n_in; ++k) {
+                QA_REQUIRE(std::isfinite(norm[k]) && norm[m->n_in + k] > 0.0, "qa_mlp_set_state: bad normalization");
+                m->x_mean[k] = norm[k];
+                m->x_scale[k] = norm[m->n_in + k];
+                m->x_lo[k] = norm[2 * m->n_in + k];
+                m->x_hi[k] = norm[3 * m->n_in + k];
+            }
+            m->trained = true;
+        }
+    });
+}
+
+qa_status qa_mlp_forward(qa_mlp *m, int64_t batch, const double *x, float *y) {
+    return guarded([&] {
+        QA_REQUIRE(m && x && y && batch >= 0, "qa_mlp_forward: bad argument");
+        check_finite(x, batch * m->n_in, "features");
+        QA_HIP(hipSetDevice(m->ctx->device));
+        mlp_forward(m, batch, x, y);
+    });
+}
+
+qa_status qa_mlp_loss_grad(qa_mlp *m, int64_t batch, const double *x, const float *t, double *loss, float *grad) {
+    return guarded([&] {
+        QA_REQUIRE(m && x && t && batch > 0, "qa_mlp_loss_grad: bad argument");
+        check_finite(x, batch * m->n_in, "features");
+        QA_HIP(hipSetDevice(m->ctx->device));
+        mlp_loss_grad(m, batch, x, t, loss, grad);
+    });
+}
+
+qa_status qa_mlp_train(qa_mlp *m, int64_t samples, const double *x, const float *t, const qa_mlp_train_params *tp,
+                       double *loss_history) {
+    return guarded([&] {
+        QA_REQUIRE(m && x && t && tp, "qa_mlp_train: NULL argument");
+        QA_REQUIRE(tp->lr > 0.0 && tp->weight_decay >= 0.0, "qa_mlp_train: lr must be > 0, weight_decay >= 0");
+        check_finite(x, samples * m->n_in, "features");
+        for (int64_t e = 0; e < samples * m->n_out; ++e)
+            QA_REQUIRE(std::isfinite(t[e]), "qa_mlp_train: targets must be finite");
+        QA_HIP(hipSetDevice(m->ctx->device));
+        mlp_train(m, samples, x, t, tp, loss_history);
     });
 }
 

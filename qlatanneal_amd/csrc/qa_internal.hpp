@@ -112,6 +112,33 @@ struct qa_problem {
     qa::DevBuf<float> t_coef32;
     qa::DevBuf<int32_t> inc_ptr, inc_idx;
     int max_degree = 0;          // sparse: max adjacency length; hubo: max incident terms
+    // instance statistics for the chi_B surrogate's features (accumulated at upload)
+    double st_mean_degree = 0.0;  // couplings per spin (hubo: sum over incident terms of order - 1)
+    double st_field_scale = 0.0;  // mean over spins of sqrt(sum of squared incident couplings)
+    double st_field_het = 0.0;    // std / mean of that per-spin scale
+    double st_kurtosis = 1.0;     // E[J^4] / E[J^2]^2 over the nonzero couplings
+    double st_neg_frac = 0.0;     // fraction of negative couplings
+    double st_h_ratio = 0.0;      // mean |h| / field scale
+    double st_order = 2.0;        // mean interaction order
+};
+
+// chi_B surrogate network (an ensemble of `members` MLPs sharing one launch per layer)
+struct qa_mlp {
+    qa_ctx *ctx = nullptr;
+    int n_in = 0, width = 0, depth = 0, n_out = 0, members = 0, maxdim = 0;
+    int64_t nparams = 0;                 // per member
+    std::vector<int64_t> w_off, b_off;   // per layer, within a member
+    qa::DevBuf<float> params, grads, m1, m2;  // [members][nparams] (FP32 master copy, Adam state)
+    qa::DevBuf<uint16_t> wbf;            // [members][nparams] bf16 copy the products read
+    qa::DevBuf<double> loss;             // [members]
+    std::vector<double> x_mean, x_scale; // input normalization
+    std::vector<double> x_lo, x_hi;      // training range of the normalized inputs
+    bool trained = false;
+    int64_t adam_t = 0;
+    int64_t cap = 0;                     // batch capacity of the work buffers
+    qa::DevBuf<float> acts;              // [members][depth+1][cap][maxdim] layer outputs
+    qa::DevBuf<float> dA, dB;            // [members][cap][maxdim] gradients
+    qa::DevBuf<float> xin, tgt;          // [cap][n_in], [cap][n_out]
 };
 
 // Views passed to kernels (by value).
@@ -198,6 +225,29 @@ void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *p
 void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
 void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
 void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
+// instance statistics from per-spin coupling sums (see qa_problem::st_*)
+struct StatsAcc {
+    std::vector<double> deg, r2;
+    double s2 = 0.0, s4 = 0.0, cnt = 0.0, neg = 0.0, order_sum = 0.0;
+    explicit StatsAcc(int64_t n) : deg(static_cast<size_t>(n), 0.0), r2(static_cast<size_t>(n), 0.0) {}
+    void coupling(double v, int order) {
+        s2 += v * v;
+        s4 += v * v * v * v;
+        cnt += 1.0;
+        if (v < 0.0) neg += 1.0;
+        order_sum += order;
+    }
+    void finish(qa_problem *p, const double *h) const;
+};
+void chi_features(const qa_problem *p, double beta, int64_t P, double A0, double gamma_start, double gamma_end,
+                  double *f);
+void chi_pilot_profile(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, double *scan_s, double *scan_gamma,
+                       double *scan_chi, float *ms);
+void mlp_init(qa_mlp *m, uint64_t seed);
+void mlp_sync_bf16(qa_mlp *m);
+void mlp_forward(qa_mlp *m, int64_t B, const double *x, float *y);
+void mlp_loss_grad(qa_mlp *m, int64_t B, const double *x, const float *t, double *loss, float *grad);
+void mlp_train(qa_mlp *m, int64_t S, const double *x, const float *t, const qa_mlp_train_params *tp, double *history);
 void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *problems, const qa_batch_params *prm,
                qa_batch_result *res);
 

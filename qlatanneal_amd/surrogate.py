@@ -1,0 +1,170 @@
+"""Learned chi_B surrogate for SQAChiAnnealer.run_chi (no counterpart in the reference).
+
+run_chi (sqa_chi_annealer.cpp:232-281) spends a pilot scan -- scan_points x (burn + sweeps)
+checkerboard sweeps -- to measure the bond susceptibility chi_B on a uniform s grid, and builds the
+production Gamma(t) schedule from the normalized profile chi_B / max chi_B alone (peak, relative
+floor, cumulative weight; :282-340). This module learns that profile from instance statistics:
+
+  features  qa_chi_features: size, couplings per spin, field scale and heterogeneity, coupling
+            kurtosis and sign balance, |h| ratio, interaction order, beta, P, A0, Gamma range;
+  labels    normalized pilot profiles measured by the library's own FP32 pilot kernel (qa_chi_pilot);
+  network   an ensemble of MLPs (default 4 members of 12 -> 256 x 3 -> 64) trained and evaluated by
+            the library's hand-written kernels (qa_mlp_*: bf16 weights, FP32 accumulation);
+  use       qa_sqa_chi_run(surrogate=...) takes the ensemble mean as the profile, or runs the real
+            pilot when the ensemble spread is above a bound or the instance lies outside the
+            training range of the features.
+
+Training families follow the sqa_chi workload: Sherrington-Kirkpatrick (J_ij ~ N(0, 1/N)) and
+3-regular MAX-CUT with unit antiferromagnetic couplings (J = +1 on edges, which the energy
+E = sum J s_i s_j minimizes by cutting edges).
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import abi
+
+NFEAT = abi.CHI_NFEATURES
+
+
+@dataclass
+class PilotConfig:
+    """The run_chi arguments a surrogate is trained for (the pilot grid is part of its output)."""
+    trotter_slices: int = 32
+    replicas: int = 4
+    gamma_start: float = 3.0
+    gamma_end: float = 1e-3
+    driver_A0: float = 3.0
+    scan_points: int = 64
+    scan_sweeps: int = 30
+    scan_burn: int = 10
+
+
+def sk_problem(ctx: abi.Context, rng: np.random.Generator, n: int) -> abi.DeviceProblem:
+    J = rng.normal(0.0, 1.0 / np.sqrt(n), (n, n))
+    J = np.triu(J, 1)
+    return abi.DeviceProblem.dense(ctx, np.zeros(n), J + J.T, 0.0)
+
+
+def regular3_edges(rng: np.random.Generator, n: int) -> List[Tuple[int, int]]:
+    """A uniformly paired 3-regular simple graph on n (even) vertices (pairing model with restarts)."""
+    if n % 2 or n < 4:
+        raise ValueError("3-regular graphs need an even n >= 4")
+    for _ in range(1000):
+        pts = rng.permutation(np.repeat(np.arange(n), 3))
+        a, b = pts[0::2], pts[1::2]
+        if np.any(a == b):
+            continue
+        lo, hi = np.minimum(a, b), np.maximum(a, b)
+        if np.unique(lo * n + hi).size != lo.size:
+            continue
+        return list(zip(lo.tolist(), hi.tolist()))
+    raise RuntimeError("could not draw a simple 3-regular graph")
+
+
+def maxcut3_problem(ctx: abi.Context, rng: np.random.Generator, n: int) -> abi.DeviceProblem:
+    edges = [(i, j, 1.0) for i, j in regular3_edges(rng, n)]
+    return abi.DeviceProblem.sparse(ctx, np.zeros(n), edges, 0.0)
+
+
+FAMILIES = {"sk": sk_problem, "maxcut3": maxcut3_problem}
+
+
+@dataclass
+class InstanceSpec:
+    family: str
+    n: int
+    beta: float
+    seed: int
+
+
+def random_specs(count: int, rng: np.random.Generator, sk_n=(256, 2048), maxcut_n=(256, 4096),
+                 beta=(1.0, 16.0)) -> List[InstanceSpec]:
+    """Training/validation instances: half SK, half 3-regular MAX-CUT, log-uniform N and beta."""
+    out = []
+    for k in range(count):
+        fam = "sk" if k % 2 == 0 else "maxcut3"
+        lo, hi = sk_n if fam == "sk" else maxcut_n
+        n = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        n += n % 2
+        b = float(np.exp(rng.uniform(np.log(beta[0]), np.log(beta[1]))))
+        out.append(InstanceSpec(fam, n, b, int(rng.integers(1, 2**62))))
+    return out
+
+
+def build(ctx: abi.Context, spec: InstanceSpec) -> abi.DeviceProblem:
+    return FAMILIES[spec.family](ctx, np.random.default_rng(spec.seed), spec.n)
+
+
+def features(prob: abi.DeviceProblem, beta: float, cfg: PilotConfig) -> np.ndarray:
+    return abi.chi_features(prob, beta, cfg.trotter_slices, cfg.driver_A0, cfg.gamma_start, cfg.gamma_end)
+
+
+def pilot_profile(prob: abi.DeviceProblem, beta: float, cfg: PilotConfig, seed: int):
+    """(normalized profile chi / max chi, raw pilot dict); the profile is None for a flat scan."""
+    r = abi.chi_pilot(prob, cfg.trotter_slices, cfg.replicas, beta, cfg.gamma_start, cfg.gamma_end,
+                      cfg.scan_points, cfg.scan_sweeps, cfg.scan_burn, cfg.driver_A0, seed)
+    mx = float(np.max(r["scan_chi_B"]))
+    prof = r["scan_chi_B"] / mx if mx > 0 and np.isfinite(mx) else None
+    return prof, r
+
+
+def make_dataset(ctx: abi.Context, specs: Sequence[InstanceSpec], cfg: PilotConfig, log=None):
+    """Features X [S][NFEAT] and labels Y [S][scan_points] from GPU pilot scans (flat scans dropped),
+    plus the pilot device time in ms."""
+    X, Y, ms = [], [], 0.0
+    for k, sp in enumerate(specs):
+        prob = build(ctx, sp)
+        prof, r = pilot_profile(prob, sp.beta, cfg, sp.seed ^ 0x5DEECE66D)
+        ms += r["kernel_ms"]
+        if prof is not None:
+            X.append(features(prob, sp.beta, cfg))
+            Y.append(prof)
+        if log and (k + 1) % max(1, len(specs) // 10) == 0:
+            log(f"pilot scans {k + 1}/{len(specs)}")
+        del prob
+    return np.array(X).reshape(-1, NFEAT), np.array(Y).reshape(-1, cfg.scan_points), ms
+
+
+class ChiSurrogate:
+    """Ensemble MLP predicting normalized pilot chi_B profiles (device training and inference)."""
+
+    def __init__(self, ctx: abi.Context, cfg: Optional[PilotConfig] = None, width: int = 256, depth: int = 3,
+                 members: int = 4, seed: int = 0):
+        self.ctx = ctx
+        self.cfg = cfg or PilotConfig()
+        self.mlp = abi.Mlp(ctx, NFEAT, width, depth, self.cfg.scan_points, members, seed)
+
+    def fit(self, X, Y, epochs: int = 400, batch: int = 64, lr: float = 2e-3, weight_decay: float = 1e-5,
+            seed: int = 1) -> np.ndarray:
+        return self.mlp.train(X, Y, epochs, batch, lr, weight_decay, seed)
+
+    def predict(self, X) -> Tuple[np.ndarray, np.ndarray]:
+        """(ensemble mean [S][scan_points], spread [S]: largest per-point std across members)."""
+        y = self.mlp.forward(X).astype(np.float64)
+        return y.mean(axis=0), y.std(axis=0).max(axis=1)
+
+    def save(self, path: str) -> None:
+        params, norm = self.mlp.get_state()
+        c = self.cfg
+        np.savez(path, params=params, norm=norm,
+                 dims=np.array([self.mlp.n_in, self.mlp.width, self.mlp.depth, self.mlp.n_out, self.mlp.members]),
+                 cfg=np.array([c.trotter_slices, c.replicas, c.gamma_start, c.gamma_end, c.driver_A0, c.scan_points,
+                               c.scan_sweeps, c.scan_burn], np.float64))
+
+    @classmethod
+    def load(cls, ctx: abi.Context, path: str) -> "ChiSurrogate":
+        with np.load(path, allow_pickle=False) as z:
+            n_in, width, depth, n_out, members = (int(v) for v in z["dims"])
+            c = z["cfg"]
+            cfg = PilotConfig(int(c[0]), int(c[1]), float(c[2]), float(c[3]), float(c[4]), int(c[5]), int(c[6]),
+                              int(c[7]))
+            if n_in != NFEAT or n_out != cfg.scan_points:
+                raise ValueError("surrogate file does not match the feature/scan layout")
+            s = cls(ctx, cfg, width, depth, members)
+            s.mlp.set_state(z["params"], z["norm"])
+        return s
