@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -63,13 +64,13 @@ def cpu_baseline(J, betas, gammas, sample_steps: int):
 
 
 def traffic_from_profile():
-    f = os.path.join(ROOT, "profiles", "sqa_fp32_c2_traffic.json")
-    if os.path.exists(f):
-        try:
-            return json.load(open(f)).get("hbm_bytes_per_launch")
-        except Exception:
-            return None
-    return None
+    """HBM bytes per sweep launch from the committed counter summary (FETCH_SIZE + WRITE_SIZE
+    passes, each calibrated on a copy kernel of known size), or None."""
+    f = os.path.join(ROOT, "profiles", "sqa_pipe_kernel_c2_traffic.md")
+    if not os.path.exists(f):
+        return None
+    m = re.search(r"^hbm_bytes_per_launch:\s*(\d+)", open(f).read(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def main():
