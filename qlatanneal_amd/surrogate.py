@@ -168,3 +168,106 @@ class ChiSurrogate:
             s = cls(ctx, cfg, width, depth, members)
             s.mlp.set_state(z["params"], z["norm"])
         return s
+
+
+def evaluate(ctx: abi.Context, sur: ChiSurrogate, specs: Sequence[InstanceSpec], num_steps: int = 100,
+             sweeps_per_step: int = 10, max_spread: float = 0.05, seed: int = 12345, log=None) -> dict:
+    """Production run_chi with the pilot schedule and with the surrogate schedule on each instance
+    (same production seed): residual-energy gap, fallback rate, profile error and the pilot time
+    the surrogate saves."""
+    cfg = sur.cfg
+    rows = []
+    for k, sp in enumerate(specs):
+        prob = build(ctx, sp)
+        kw = dict(num_steps=num_steps, sweeps_per_step=sweeps_per_step, scan_points=cfg.scan_points,
+                  scan_sweeps=cfg.scan_sweeps, scan_burn=cfg.scan_burn, driver_A0=cfg.driver_A0, seed=seed + k)
+        args = (prob, cfg.trotter_slices, cfg.replicas, sp.beta, cfg.gamma_start, cfg.gamma_end)
+        pil = abi.sqa_chi_run(*args, **kw)
+        sur_r = abi.sqa_chi_run(*args, surrogate=sur.mlp, surrogate_max_spread=max_spread, **kw)
+        f = features(prob, sp.beta, cfg)
+        pred, spread = sur.predict(f[None, :])
+        prof = pil["scan_chi_B"] / max(np.max(pil["scan_chi_B"]), 1e-300)
+        e_p, e_s = pil["best_energy"], sur_r["best_energy"]
+        rows.append(dict(family=sp.family, n=sp.n, beta=sp.beta, e_pilot=e_p, e_surrogate=e_s,
+                         gap=(e_s - e_p) / max(abs(e_p), 1e-300), used=bool(sur_r["surrogate_used"]),
+                         spread=float(spread[0]), ms_pilot_run=pil["kernel_ms"], ms_surrogate_run=sur_r["kernel_ms"],
+                         profile_mae=float(np.mean(np.abs(pred[0] - prof))),
+                         peak_s_err=float(abs(pil["scan_s"][np.argmax(pred[0])] - pil["s_star"]))))
+        if log:
+            r = rows[-1]
+            log(f"eval {k + 1}/{len(specs)} {sp.family} n={sp.n} beta={sp.beta:.2f} gap={r['gap']:+.4%} "
+                f"used={r['used']} spread={r['spread']:.3f} mae={r['profile_mae']:.3f} "
+                f"ms pilot-run={r['ms_pilot_run']:.0f} surrogate-run={r['ms_surrogate_run']:.0f}")
+        del prob
+    gaps = np.array([r["gap"] for r in rows])
+    used = np.array([r["used"] for r in rows])
+    ms_p = sum(r["ms_pilot_run"] for r in rows)
+    ms_s = sum(r["ms_surrogate_run"] for r in rows)
+    return dict(instances=len(rows), surrogate_used_fraction=float(used.mean()) if rows else 0.0,
+                gap_mean=float(gaps.mean()) if rows else 0.0, gap_max=float(gaps.max()) if rows else 0.0,
+                within_1pct=float(np.mean(gaps <= 0.01)) if rows else 0.0,
+                gap_mean_when_used=float(gaps[used].mean()) if used.any() else None,
+                profile_mae_mean=float(np.mean([r["profile_mae"] for r in rows])) if rows else 0.0,
+                ms_pilot_scheduled=ms_p, ms_surrogate_scheduled=ms_s,
+                wall_saving=(1.0 - ms_s / ms_p) if ms_p > 0 else 0.0, rows=rows)
+
+
+def _main(argv=None) -> int:
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(description="Train or evaluate the chi_B surrogate on GPU pilot scans.")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--count", type=int, default=1200, help="pilot scans (instances) to generate")
+    t.add_argument("--out", required=True, help="output .npz")
+    t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--batch", type=int, default=64)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--members", type=int, default=4)
+    t.add_argument("--seed", type=int, default=2024)
+    t.add_argument("--data", default=None, help="also save the (X, Y) dataset here (.npz)")
+    e = sub.add_parser("evaluate")
+    e.add_argument("--model", required=True)
+    e.add_argument("--count", type=int, default=16)
+    e.add_argument("--seed", type=int, default=99)
+    e.add_argument("--max-n", type=int, default=1024)
+    e.add_argument("--num-steps", type=int, default=100)
+    e.add_argument("--sweeps-per-step", type=int, default=10)
+    e.add_argument("--max-spread", type=float, default=0.05)
+    e.add_argument("--out", default=None, help="write the summary JSON here")
+    a = ap.parse_args(argv)
+    ctx = abi.Context(0)
+    t0 = time.time()
+
+    def log(msg):
+        print(f"[{time.time() - t0:7.1f}s] {msg}", flush=True)
+
+    if a.cmd == "train":
+        cfg = PilotConfig()
+        specs = random_specs(a.count, np.random.default_rng(a.seed))
+        X, Y, ms = make_dataset(ctx, specs, cfg, log=log)
+        log(f"dataset {X.shape[0]} scans, pilot device time {ms / 1e3:.1f} s")
+        if a.data:
+            np.savez(a.data, X=X, Y=Y)
+        sur = ChiSurrogate(ctx, cfg, members=a.members, seed=a.seed)
+        hist = sur.fit(X, Y, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed + 1)
+        log(f"training loss first {hist[0]:.5f} last {hist[-1]:.5f}")
+        sur.save(a.out)
+        print(json.dumps(dict(scans=int(X.shape[0]), pilot_ms=ms, loss_first=float(hist[0]),
+                              loss_last=float(hist[-1]), epochs=a.epochs)))
+    else:
+        sur = ChiSurrogate.load(ctx, a.model)
+        rng = np.random.default_rng(a.seed)
+        specs = random_specs(a.count, rng, sk_n=(256, a.max_n), maxcut_n=(256, a.max_n))
+        res = evaluate(ctx, sur, specs, a.num_steps, a.sweeps_per_step, a.max_spread, log=log)
+        summary = {k: v for k, v in res.items() if k != "rows"}
+        print(json.dumps(summary))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
