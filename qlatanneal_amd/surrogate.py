@@ -172,44 +172,57 @@ class ChiSurrogate:
 
 def evaluate(ctx: abi.Context, sur: ChiSurrogate, specs: Sequence[InstanceSpec], num_steps: int = 100,
              sweeps_per_step: int = 10, max_spread: float = 0.05, seed: int = 12345, log=None) -> dict:
-    """Production run_chi with the pilot schedule and with the surrogate schedule on each instance
-    (same production seed): residual-energy gap, fallback rate, profile error and the pilot time
-    the surrogate saves."""
+    """Production run_chi on each instance under four schedules, same production seed unless noted:
+      pilot      the reference method (pilot scan, then production);
+      surrogate  the surrogate's profile (pilot only on fallback);
+      rerun      pilot-scheduled again with another seed (pilot and production): the run-to-run
+                 spread a gap has to be read against;
+      flat       no pilot and no surrogate: a constant profile, i.e. a schedule linear in s.
+    Reports best-energy gaps relative to the pilot run, the fallback rate, profile errors
+    (surrogate vs pilot, rerun pilot vs pilot) and device times."""
     cfg = sur.cfg
     rows = []
     for k, sp in enumerate(specs):
         prob = build(ctx, sp)
         kw = dict(num_steps=num_steps, sweeps_per_step=sweeps_per_step, scan_points=cfg.scan_points,
-                  scan_sweeps=cfg.scan_sweeps, scan_burn=cfg.scan_burn, driver_A0=cfg.driver_A0, seed=seed + k)
+                  scan_sweeps=cfg.scan_sweeps, scan_burn=cfg.scan_burn, driver_A0=cfg.driver_A0)
         args = (prob, cfg.trotter_slices, cfg.replicas, sp.beta, cfg.gamma_start, cfg.gamma_end)
-        pil = abi.sqa_chi_run(*args, **kw)
-        sur_r = abi.sqa_chi_run(*args, surrogate=sur.mlp, surrogate_max_spread=max_spread, **kw)
+        pil = abi.sqa_chi_run(*args, seed=seed + k, **kw)
+        sur_r = abi.sqa_chi_run(*args, surrogate=sur.mlp, surrogate_max_spread=max_spread, seed=seed + k, **kw)
+        rer = abi.sqa_chi_run(*args, seed=seed + k + 7919, **kw)
+        flat = abi.sqa_chi_run(*args, chi_override=np.ones(cfg.scan_points), seed=seed + k, **kw)
         f = features(prob, sp.beta, cfg)
         pred, spread = sur.predict(f[None, :])
-        prof = pil["scan_chi_B"] / max(np.max(pil["scan_chi_B"]), 1e-300)
-        e_p, e_s = pil["best_energy"], sur_r["best_energy"]
-        rows.append(dict(family=sp.family, n=sp.n, beta=sp.beta, e_pilot=e_p, e_surrogate=e_s,
-                         gap=(e_s - e_p) / max(abs(e_p), 1e-300), used=bool(sur_r["surrogate_used"]),
-                         spread=float(spread[0]), ms_pilot_run=pil["kernel_ms"], ms_surrogate_run=sur_r["kernel_ms"],
+        norm = lambda c: c / max(np.max(c), 1e-300)
+        prof = norm(pil["scan_chi_B"])
+        e_p = pil["best_energy"]
+        gap = lambda e: (e - e_p) / max(abs(e_p), 1e-300)
+        rows.append(dict(family=sp.family, n=sp.n, beta=sp.beta, e_pilot=e_p, e_surrogate=sur_r["best_energy"],
+                         e_rerun=rer["best_energy"], e_flat=flat["best_energy"], gap=gap(sur_r["best_energy"]),
+                         gap_rerun=gap(rer["best_energy"]), gap_flat=gap(flat["best_energy"]),
+                         used=bool(sur_r["surrogate_used"]), spread=float(spread[0]),
+                         ms_pilot_run=pil["kernel_ms"], ms_surrogate_run=sur_r["kernel_ms"],
                          profile_mae=float(np.mean(np.abs(pred[0] - prof))),
+                         profile_mae_rerun=float(np.mean(np.abs(norm(rer["scan_chi_B"]) - prof))),
                          peak_s_err=float(abs(pil["scan_s"][np.argmax(pred[0])] - pil["s_star"]))))
         if log:
             r = rows[-1]
-            log(f"eval {k + 1}/{len(specs)} {sp.family} n={sp.n} beta={sp.beta:.2f} gap={r['gap']:+.4%} "
-                f"used={r['used']} spread={r['spread']:.3f} mae={r['profile_mae']:.3f} "
+            log(f"eval {k + 1}/{len(specs)} {sp.family} n={sp.n} beta={sp.beta:.2f} gap={r['gap']:+.3%} "
+                f"rerun={r['gap_rerun']:+.3%} flat={r['gap_flat']:+.3%} used={r['used']} "
+                f"spread={r['spread']:.3f} mae={r['profile_mae']:.3f}/{r['profile_mae_rerun']:.3f} "
                 f"ms pilot-run={r['ms_pilot_run']:.0f} surrogate-run={r['ms_surrogate_run']:.0f}")
         del prob
-    gaps = np.array([r["gap"] for r in rows])
-    used = np.array([r["used"] for r in rows])
-    ms_p = sum(r["ms_pilot_run"] for r in rows)
-    ms_s = sum(r["ms_surrogate_run"] for r in rows)
-    return dict(instances=len(rows), surrogate_used_fraction=float(used.mean()) if rows else 0.0,
-                gap_mean=float(gaps.mean()) if rows else 0.0, gap_max=float(gaps.max()) if rows else 0.0,
-                within_1pct=float(np.mean(gaps <= 0.01)) if rows else 0.0,
-                gap_mean_when_used=float(gaps[used].mean()) if used.any() else None,
-                profile_mae_mean=float(np.mean([r["profile_mae"] for r in rows])) if rows else 0.0,
-                ms_pilot_scheduled=ms_p, ms_surrogate_scheduled=ms_s,
-                wall_saving=(1.0 - ms_s / ms_p) if ms_p > 0 else 0.0, rows=rows)
+    col = lambda key: np.array([r[key] for r in rows], dtype=float)
+    used = np.array([r["used"] for r in rows], dtype=bool)
+    ms_p, ms_s = col("ms_pilot_run").sum(), col("ms_surrogate_run").sum()
+    stats = lambda g: dict(mean=float(g.mean()), median=float(np.median(g)), max=float(g.max()),
+                           within_1pct=float(np.mean(g <= 0.01)))
+    return dict(instances=len(rows), surrogate_used_fraction=float(used.mean()),
+                gap_surrogate=stats(col("gap")), gap_rerun=stats(col("gap_rerun")), gap_flat=stats(col("gap_flat")),
+                profile_mae_surrogate=float(col("profile_mae").mean()),
+                profile_mae_rerun=float(col("profile_mae_rerun").mean()),
+                ms_pilot_scheduled=float(ms_p), ms_surrogate_scheduled=float(ms_s),
+                wall_saving=float(1.0 - ms_s / ms_p) if ms_p > 0 else 0.0, rows=rows)
 
 
 def _main(argv=None) -> int:
