@@ -48,8 +48,10 @@ struct PhiloxRun {
         constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const uint32_t hi0 = mulhi32(M0, c0), lo0 = M0 * c0;
-            const uint32_t hi1 = mulhi32(M1, c2), lo1 = M1 * c2;
+            // one 32 x 32 -> 64-bit product per multiplier (a single v_mad_u64_u32 on the device)
+            const uint64_t p0 = static_cast<uint64_t>(M0) * c0, p1 = static_cast<uint64_t>(M1) * c2;
+            const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+            const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
             const uint32_t n0 = hi1 ^ c1 ^ k0;
             const uint32_t n2 = hi0 ^ c3 ^ k1;
             c0 = n0;
