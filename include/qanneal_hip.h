@@ -86,6 +86,13 @@ typedef struct qa_sa_params {
     const double *uniforms;      /* validation: [reads*uniforms_per_read], the engine's
                                     uniform_real(0,1) stream after the init draws            */
     int64_t uniforms_per_read;
+    /* Optional, both modes: chained runs (ParallelTemperingAnnealer::run, parallel_tempering.cpp:
+     * 36-122, runs each replica at its own beta and continues its state and running energy). */
+    const double *read_betas;    /* [reads]: read r uses read_betas[r] at every step (betas unused) */
+    const int8_t *init_states;   /* FP32: [reads*n] start states instead of random ones        */
+    const double *init_energies; /* [reads]: running energy at the start (default: the FP64
+                                    energy of the start state)                               */
+    uint64_t counter_base;       /* FP32: Philox stream offset of this run (chained runs)      */
 } qa_sa_params;
 
 typedef struct qa_sa_result {
@@ -94,6 +101,10 @@ typedef struct qa_sa_result {
     double *energy_trace;        /* [reads*steps] energy at the end of each step             */
     int64_t *uniforms_used;      /* validation: [reads] (may be NULL)                        */
     double *kernel_ms;           /* optional: device time of the sweep kernel                */
+    int8_t *final_states;        /* optional [reads*n]: states at the end of the run          */
+    double *final_energies;      /* optional [reads]: running energies at the end of the run  */
+    double *magnetization_trace; /* optional [reads*steps]: mean spin at each step end
+                                    (metrics.hpp:11-20; ReplicaAnnealer::run's trace)        */
 } qa_sa_result;
 
 qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);

@@ -167,6 +167,36 @@ class Oracle:
                                       C.c_uint64(seed), _b(best), C.byref(be), _d(tr)))
         return dict(best_state=best, best_energy=be.value, energy_trace=tr)
 
+    def replica_run(self, p: Problem, betas, sweeps_per_beta: int, replicas: int, seed: int):
+        """ReplicaAnnealer::run (replica_annealer.cpp:46-142) after set_seed(seed)."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        S, R, n = b.shape[0], replicas, p.n
+        bs, be = np.zeros((R, n), np.int8), np.zeros(R)
+        et, mt = np.zeros((R, S)), np.zeros((R, S))
+        gb, gbe = np.zeros(n, np.int8), C.c_double()
+        ae, am = np.zeros(S), np.zeros(S)
+        st, keep = p._struct()
+        self._chk(self.lib.orc_replica_run(C.byref(st), _i64(S), _d(b), _i64(sweeps_per_beta), _i64(R),
+                                           C.c_uint64(seed), _b(bs), _d(be), _d(et), _d(mt), _b(gb),
+                                           C.byref(gbe), _d(ae), _d(am)))
+        return dict(best_states=bs, best_energies=be, energy_traces=et, magnetization_traces=mt,
+                    global_best_state=gb, global_best_energy=gbe.value, average_energy_trace=ae,
+                    average_magnetization_trace=am)
+
+    def pt_run(self, p: Problem, betas, sweeps_per_step: int, steps: int, swap_interval: int, seed: int):
+        """ParallelTemperingAnnealer::run (parallel_tempering.cpp:36-122) after set_seed(seed)."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        R, n = b.shape[0], p.n
+        fs, fe = np.zeros((R, n), np.int8), np.zeros(R)
+        best, be = np.zeros(n, np.int8), C.c_double()
+        at, sw = np.zeros(steps), np.zeros(steps)
+        st, keep = p._struct()
+        self._chk(self.lib.orc_pt_run(C.byref(st), _i64(R), _d(b), _i64(sweeps_per_step), _i64(steps),
+                                      _i64(swap_interval), C.c_uint64(seed), _b(fs), _d(fe), _b(best),
+                                      C.byref(be), _d(at), _d(sw)))
+        return dict(final_states=fs, final_energies=fe, best_state=best, best_energy=be.value,
+                    average_energy_trace=at, swap_acceptance_trace=sw)
+
     def sqa_run(self, p: Problem, betas, gammas, trotter_slices: int, replicas: int,
                 sweeps_per_beta: int, worldline_sweeps: int, cluster_sweeps: int = 0,
                 continuous_time_slices: int = 0, seed: int = 0):

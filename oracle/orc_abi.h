@@ -53,6 +53,21 @@ int orc_qubo_to_ising(int64_t n, const double *Q, double *h, double *J, double *
 int orc_sa_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
                uint64_t seed, int8_t *best_state, double *best_energy, double *trace);
 
+/* ReplicaAnnealer::run (replica_annealer.cpp:46-142) after set_seed(seed). Per replica: best
+ * state/energy, energy and magnetization traces [replicas*steps]; global best; average traces
+ * [steps]. */
+int orc_replica_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
+                    int64_t replicas, uint64_t seed, int8_t *best_states, double *best_energies,
+                    double *energy_traces, double *mag_traces, int8_t *global_best_state,
+                    double *global_best_energy, double *avg_energy_trace, double *avg_mag_trace);
+
+/* ParallelTemperingAnnealer::run (parallel_tempering.cpp:36-122) after set_seed(seed).
+ * final_states [replicas*n], final_energies [replicas], avg_trace/swap_trace [steps]. */
+int orc_pt_run(const orc_problem *p, int64_t replicas, const double *betas, int64_t sweeps_per_step,
+               int64_t steps, int64_t swap_interval, uint64_t seed, int8_t *final_states,
+               double *final_energies, int8_t *best_state, double *best_energy, double *avg_trace,
+               double *swap_trace);
+
 /* SQAAnnealer::run (sqa_annealer.cpp:87-477) after set_seed(seed). trace: [steps] */
 int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
                 int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,

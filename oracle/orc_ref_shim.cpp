@@ -15,6 +15,8 @@
 #endif
 
 #include "qanneal/annealer.hpp"
+#include "qanneal/parallel_tempering.hpp"
+#include "qanneal/replica_annealer.hpp"
 #include "qanneal/dense_ising.hpp"
 #include "qanneal/higher_order_ising.hpp"
 #include "qanneal/qubo.hpp"
@@ -131,6 +133,52 @@ int orc_sa_run(const orc_problem *p, int64_t steps, const double *betas, int64_t
         put_state(res.best_state, best_state);
         *best_energy = res.best_energy;
         std::memcpy(trace, res.energy_trace.data(), res.energy_trace.size() * sizeof(double));
+    });
+}
+
+int orc_replica_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
+                    int64_t replicas, uint64_t seed, int8_t *best_states, double *best_energies,
+                    double *energy_traces, double *mag_traces, int8_t *global_best_state,
+                    double *global_best_energy, double *avg_energy_trace, double *avg_mag_trace) {
+    return guard([&] {
+        auto ham = build(p);
+        qanneal::AnnealSchedule sched;
+        sched.betas.assign(betas, betas + steps);
+        qanneal::ReplicaAnnealer ann(*ham, sched, static_cast<std::size_t>(replicas));
+        ann.set_seed(seed);
+        auto res = ann.run(static_cast<std::size_t>(sweeps_per_beta));
+        for (int64_t r = 0; r < replicas; ++r) {
+            const auto &rr = res.replicas[r];
+            put_state(rr.best_state, best_states + r * p->n);
+            best_energies[r] = rr.best_energy;
+            std::memcpy(energy_traces + r * steps, rr.energy_trace.data(), steps * sizeof(double));
+            std::memcpy(mag_traces + r * steps, rr.magnetization_trace.data(), steps * sizeof(double));
+        }
+        put_state(res.global_best_state, global_best_state);
+        *global_best_energy = res.global_best_energy;
+        std::memcpy(avg_energy_trace, res.average_energy_trace.data(), steps * sizeof(double));
+        std::memcpy(avg_mag_trace, res.average_magnetization_trace.data(), steps * sizeof(double));
+    });
+}
+
+int orc_pt_run(const orc_problem *p, int64_t replicas, const double *betas, int64_t sweeps_per_step,
+               int64_t steps, int64_t swap_interval, uint64_t seed, int8_t *final_states,
+               double *final_energies, int8_t *best_state, double *best_energy, double *avg_trace,
+               double *swap_trace) {
+    return guard([&] {
+        auto ham = build(p);
+        qanneal::ParallelTemperingAnnealer ann(*ham, std::vector<double>(betas, betas + replicas));
+        ann.set_seed(seed);
+        auto res = ann.run(static_cast<std::size_t>(sweeps_per_step), static_cast<std::size_t>(steps),
+                           static_cast<std::size_t>(swap_interval));
+        for (int64_t r = 0; r < replicas; ++r) {
+            put_state(res.final_states[r], final_states + r * p->n);
+            final_energies[r] = res.final_energies[r];
+        }
+        put_state(res.best_state, best_state);
+        *best_energy = res.best_energy;
+        std::memcpy(avg_trace, res.average_energy_trace.data(), steps * sizeof(double));
+        std::memcpy(swap_trace, res.swap_acceptance_trace.data(), steps * sizeof(double));
     });
 }
 

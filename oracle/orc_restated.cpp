@@ -437,6 +437,137 @@ int orc_sa_run(const orc_problem *p, int64_t steps, const double *betas, int64_t
     });
 }
 
+// replica_annealer.cpp:46-142: all replicas' random states first (:55-59), then one engine per
+// replica seeded from the master (:60-63); sites in index order; best after every improving
+// accepted flip of each replica; global best and averages at step ends (:113-137).
+int orc_replica_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
+                    int64_t replicas, uint64_t seed, int8_t *best_states, double *best_energies,
+                    double *energy_traces, double *mag_traces, int8_t *global_best_state,
+                    double *global_best_energy, double *avg_energy_trace, double *avg_mag_trace) {
+    return guard([&] {
+        if (sweeps_per_beta <= 0) throw std::invalid_argument("sweeps_per_beta must be > 0.");
+        if (replicas <= 0) throw std::invalid_argument("replicas must be > 0.");
+        const Model m = build(p);
+        const std::size_t n = m.n, R = static_cast<std::size_t>(replicas);
+        Rng rng(seed);
+        std::vector<std::vector<int8_t>> st(R, std::vector<int8_t>(n));
+        std::vector<double> e(R);
+        for (std::size_t r = 0; r < R; ++r) {
+            random_spins(st[r], rng);
+            e[r] = m.energy(st[r].data());
+        }
+        std::vector<Rng> rr(R);
+        for (std::size_t r = 0; r < R; ++r) rr[r].seed(rng());
+        std::vector<std::vector<int8_t>> best = st;
+        std::vector<double> best_e = e;
+        std::vector<int8_t> gbest = st[0];
+        double gbest_e = std::numeric_limits<double>::infinity();
+        for (std::size_t r = 0; r < R; ++r)
+            if (e[r] < gbest_e) { gbest_e = e[r]; gbest = st[r]; }
+        auto mag = [&](const std::vector<int8_t> &s) {
+            double sum = 0.0;
+            for (int8_t v : s) sum += static_cast<double>(v);
+            return n ? sum / static_cast<double>(n) : 0.0;
+        };
+        for (int64_t step = 0; step < steps; ++step) {
+            const double beta = betas[step];
+            for (std::size_t r = 0; r < R; ++r) {
+                std::uniform_real_distribution<double> uni(0.0, 1.0);
+                for (int64_t sw = 0; sw < sweeps_per_beta; ++sw)
+                    for (std::size_t i = 0; i < n; ++i) {
+                        const double d = m.delta(st[r].data(), i);
+                        if (d <= 0.0 || uni(rr[r]) < std::exp(-beta * d)) {
+                            st[r][i] = static_cast<int8_t>(-st[r][i]);
+                            e[r] += d;
+                            if (e[r] < best_e[r]) { best_e[r] = e[r]; best[r] = st[r]; }
+                        }
+                    }
+            }
+            double ae = 0.0, am = 0.0;
+            for (std::size_t r = 0; r < R; ++r) {
+                if (best_e[r] < gbest_e) { gbest_e = best_e[r]; gbest = best[r]; }
+            }
+            for (std::size_t r = 0; r < R; ++r) {
+                const double mg = mag(st[r]);
+                energy_traces[r * steps + step] = e[r];
+                mag_traces[r * steps + step] = mg;
+                ae += e[r];
+                am += mg;
+            }
+            avg_energy_trace[step] = ae / static_cast<double>(R);
+            avg_mag_trace[step] = am / static_cast<double>(R);
+        }
+        for (std::size_t r = 0; r < R; ++r) {
+            std::memcpy(best_states + r * n, best[r].data(), n);
+            best_energies[r] = best_e[r];
+        }
+        std::memcpy(global_best_state, gbest.data(), n);
+        *global_best_energy = gbest_e;
+    });
+}
+
+// parallel_tempering.cpp:36-122: one master engine for everything -- the random states in replica
+// order (:56-59), then every replica's sweeps in replica order and the swap chain, all drawing
+// uniforms from the same engine; best checked after each replica's sweeps of a step (:87-90).
+int orc_pt_run(const orc_problem *p, int64_t replicas, const double *betas, int64_t sweeps_per_step,
+               int64_t steps, int64_t swap_interval, uint64_t seed, int8_t *final_states,
+               double *final_energies, int8_t *best_state, double *best_energy, double *avg_trace,
+               double *swap_trace) {
+    return guard([&] {
+        if (sweeps_per_step <= 0) throw std::invalid_argument("sweeps_per_step must be > 0.");
+        if (steps <= 0) throw std::invalid_argument("steps must be > 0.");
+        if (swap_interval <= 0) throw std::invalid_argument("swap_interval must be > 0.");
+        const Model m = build(p);
+        const std::size_t n = m.n, R = static_cast<std::size_t>(replicas);
+        Rng rng(seed);
+        std::vector<std::vector<int8_t>> st(R, std::vector<int8_t>(n));
+        std::vector<double> e(R);
+        for (std::size_t r = 0; r < R; ++r) {
+            random_spins(st[r], rng);
+            e[r] = m.energy(st[r].data());
+        }
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        std::vector<int8_t> best(n);
+        double best_e = std::numeric_limits<double>::infinity();
+        for (int64_t step = 0; step < steps; ++step) {
+            for (std::size_t r = 0; r < R; ++r) {
+                const double beta = betas[r];
+                for (int64_t sw = 0; sw < sweeps_per_step; ++sw)
+                    for (std::size_t i = 0; i < n; ++i) {
+                        const double d = m.delta(st[r].data(), i);
+                        if (d <= 0.0 || uni(rng) < std::exp(-beta * d)) {
+                            st[r][i] = static_cast<int8_t>(-st[r][i]);
+                            e[r] += d;
+                        }
+                    }
+                if (e[r] < best_e) { best_e = e[r]; best = st[r]; }
+            }
+            double acc = 0.0, att = 0.0;
+            if ((step + 1) % swap_interval == 0) {
+                for (std::size_t r = 0; r + 1 < R; ++r) {
+                    const double delta = (betas[r] - betas[r + 1]) * (e[r + 1] - e[r]);
+                    att += 1.0;
+                    if (delta <= 0.0 || uni(rng) < std::exp(-delta)) {
+                        std::swap(st[r], st[r + 1]);
+                        std::swap(e[r], e[r + 1]);
+                        acc += 1.0;
+                    }
+                }
+            }
+            double avg = 0.0;
+            for (double v : e) avg += v;
+            avg_trace[step] = avg / static_cast<double>(R);
+            swap_trace[step] = att > 0.0 ? acc / att : 0.0;
+        }
+        for (std::size_t r = 0; r < R; ++r) {
+            std::memcpy(final_states + r * n, st[r].data(), n);
+            final_energies[r] = e[r];
+        }
+        std::memcpy(best_state, best.data(), n);
+        *best_energy = best_e;
+    });
+}
+
 // sqa_annealer.cpp:87-477 (the cached-field, observer-free path)
 int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
                 int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,

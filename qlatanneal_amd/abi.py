@@ -26,12 +26,14 @@ _dp, _bp, _lp, _up, _u32p = (C.POINTER(C.c_double), C.POINTER(C.c_int8), C.POINT
 class SaParams(C.Structure):
     _fields_ = [("steps", _i64), ("betas", _dp), ("sweeps_per_beta", _i64), ("reads", _i64),
                 ("mode", _i32), ("seed", _u64), ("init_spins", _bp), ("uniforms", _dp),
-                ("uniforms_per_read", _i64)]
+                ("uniforms_per_read", _i64), ("read_betas", _dp), ("init_states", _bp), ("init_energies", _dp),
+                ("counter_base", _u64)]
 
 
 class SaResult(C.Structure):
     _fields_ = [("best_states", _bp), ("best_energies", _dp), ("energy_trace", _dp),
-                ("uniforms_used", _lp), ("kernel_ms", _dp)]
+                ("uniforms_used", _lp), ("kernel_ms", _dp), ("final_states", _bp), ("final_energies", _dp),
+                ("magnetization_trace", _dp)]
 
 
 class SqaParams(C.Structure):
@@ -225,11 +227,19 @@ class DeviceProblem:
 
 
 def sa_run(prob: DeviceProblem, betas, sweeps_per_beta: int, reads: int = 1, mode: int = MODE_FP32,
-           seed: int = 0, init_spins=None, uniforms=None):
+           seed: int = 0, init_spins=None, uniforms=None, read_betas=None, init_states=None, init_energies=None,
+           counter_base: int = 0):
     betas = np.ascontiguousarray(betas, np.float64)
     steps, n = betas.shape[0], prob.n
     p = SaParams(steps, _d(betas), sweeps_per_beta, reads, mode, seed & (2**64 - 1))
+    p.counter_base = counter_base & (2**64 - 1)
     keep = []
+    for name, arr, dt in (("read_betas", read_betas, np.float64), ("init_states", init_states, np.int8),
+                          ("init_energies", init_energies, np.float64)):
+        if arr is not None:
+            arr = np.ascontiguousarray(arr, dt)
+            keep.append(arr)
+            setattr(p, name, _b(arr) if dt == np.int8 else _d(arr))
     if init_spins is not None:
         init_spins = np.ascontiguousarray(init_spins, np.int8)
         keep.append(init_spins)
@@ -241,9 +251,11 @@ def sa_run(prob: DeviceProblem, betas, sweeps_per_beta: int, reads: int = 1, mod
         p.uniforms_per_read = uniforms.shape[1]
     bs, be, tr = np.zeros((reads, n), np.int8), np.zeros(reads), np.zeros((reads, steps))
     used, ms = np.zeros(reads, np.int64), np.zeros(1)
-    r = SaResult(_b(bs), _d(be), _d(tr), used.ctypes.data_as(_lp), _d(ms))
+    fs, fe, mg = np.zeros((reads, n), np.int8), np.zeros(reads), np.zeros((reads, steps))
+    r = SaResult(_b(bs), _d(be), _d(tr), used.ctypes.data_as(_lp), _d(ms), _b(fs), _d(fe), _d(mg))
     check(lib().qa_sa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
-    return dict(best_states=bs, best_energies=be, energy_traces=tr, uniforms_used=used, kernel_ms=ms[0])
+    return dict(best_states=bs, best_energies=be, energy_traces=tr, uniforms_used=used, kernel_ms=ms[0],
+                final_states=fs, final_energies=fe, magnetization_traces=mg)
 
 
 def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: int, sweeps_per_beta: int,

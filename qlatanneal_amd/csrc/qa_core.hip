@@ -563,6 +563,9 @@ qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, q
         QA_REQUIRE(prm->reads > 0, "reads must be > 0.");
         QA_REQUIRE(res->best_states && res->best_energies && res->energy_trace, "qa_sa_run: NULL result array");
         check_finite(prm->betas, prm->steps, "betas");
+        if (prm->read_betas) check_finite(prm->read_betas, prm->reads, "read_betas");
+        if (prm->init_energies) check_finite(prm->init_energies, prm->reads, "init_energies");
+        if (prm->init_states) check_spins(prm->init_states, prm->reads * p->n);
         QA_HIP(hipSetDevice(ctx->device));
         if (prm->mode == QA_MODE_FP64_VALIDATION) sa_validation(ctx, p, prm, res);
         else if (prm->mode == QA_MODE_FP32) sa_fp32(ctx, p, prm, res);

@@ -35,6 +35,14 @@ struct SaArgs {
     double *best_e;         // [reads]  (running-energy value at the best state)
     double *trace;          // [reads][steps]
     int ldf;
+    // chained runs (optional)
+    const float *read_betas;     // [reads]: per-read beta at every step (null: betas[step])
+    const int8_t *init_states;   // [reads][n]: start states (null: Philox random)
+    const double *init_e;        // [reads]: running energy at the start (null: FP64 energy)
+    uint32_t ctr_base;           // Philox counter word 2 (distinct streams for chained runs)
+    int8_t *final_states;        // [reads][n] (null: not written)
+    double *final_e;             // [reads]
+    double *mag;                 // [reads][steps]: mean spin at each step end
 };
 
 template <bool kLds>
@@ -65,9 +73,13 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
 
     for (int e = tid; e < G * n; e += kThreads) {
         const int c = e / n, i = e - c * n;
-        const Philox4 r = philox4x32_10(static_cast<uint32_t>(r0 + c), static_cast<uint32_t>(i), 0u, TAG_SA_INIT,
-                                        a.k0, a.k1);
-        S[static_cast<int64_t>(c) * n + i] = (r.x & 1u) ? 1 : -1;
+        if (a.init_states) {
+            S[static_cast<int64_t>(c) * n + i] = a.init_states[static_cast<int64_t>(r0) * n + e];
+        } else {
+            const Philox4 r = philox4x32_10(static_cast<uint32_t>(r0 + c), static_cast<uint32_t>(i), 0u, TAG_SA_INIT,
+                                            a.k0, a.k1);
+            S[static_cast<int64_t>(c) * n + i] = (r.x & 1u) ? 1 : -1;
+        }
     }
     if (tid < kMaxReads) E[tid] = 0.0;
     __syncthreads();
@@ -109,7 +121,7 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
     __syncthreads();
     double ecur = 0.0, ebest = 0.0;
     if (tid < G) {
-        ecur = a.pv.c + E[tid];
+        ecur = a.init_e ? a.init_e[r0 + tid] : a.pv.c + E[tid];
         ebest = ecur;
     }
     for (int i = tid; i < G * n; i += kThreads) a.best[static_cast<int64_t>(r0) * n + i] = S[(i / n) * static_cast<int64_t>(n) + (i % n)];
@@ -117,14 +129,14 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
 
     uint32_t ctr = 0;
     for (int step = 0; step < a.steps; ++step) {
-        const float beta = a.betas[step];
+        const float beta = (a.read_betas && tid < G) ? a.read_betas[r0 + tid] : a.betas[step];
         for (int sw = 0; sw < a.sweeps; ++sw) {
             Philox4 rnd{0, 0, 0, 0};
             for (int i = 0; i < n; ++i, ++ctr) {
                 if (tid < kMaxReads) {
                     bool acc = false, imp = false;
                     if ((ctr & 3) == 0)
-                        rnd = philox4x32_10(static_cast<uint32_t>(r0 + tid), ctr >> 2, 0u, TAG_SA_STEP, a.k0, a.k1);
+                        rnd = philox4x32_10(static_cast<uint32_t>(r0 + tid), ctr >> 2, a.ctr_base, TAG_SA_STEP, a.k0, a.k1);
                     if (tid < G) {
                         int8_t *sc = S + static_cast<int64_t>(tid) * n;
                         const float fs = static_cast<float>(sc[i]);
@@ -192,9 +204,22 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
                 __syncthreads();
             }
         }
-        if (tid < G) a.trace[static_cast<int64_t>(r0 + tid) * a.steps + step] = ecur;
+        if (tid < G) {
+            a.trace[static_cast<int64_t>(r0 + tid) * a.steps + step] = ecur;
+            if (a.mag) {  // magnetization (metrics.hpp:11-20): integer sum, one division
+                const int8_t *sc = S + static_cast<int64_t>(tid) * n;
+                int sum = 0;
+                for (int i = 0; i < n; ++i) sum += sc[i];
+                a.mag[static_cast<int64_t>(r0 + tid) * a.steps + step] = static_cast<double>(sum) / static_cast<double>(n);
+            }
+        }
     }
-    if (tid < G) a.best_e[r0 + tid] = ebest;
+    if (tid < G) {
+        a.best_e[r0 + tid] = ebest;
+        if (a.final_e) a.final_e[r0 + tid] = ecur;
+    }
+    if (a.final_states)
+        for (int e = tid; e < G * n; e += kThreads) a.final_states[static_cast<int64_t>(r0) * n + e] = S[e];
 }
 
 }  // namespace
@@ -212,6 +237,18 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     std::vector<float> hb(prm->steps);
     for (int64_t k = 0; k < prm->steps; ++k) hb[k] = static_cast<float>(prm->betas[k]);
     DevBuf<float> betas(prm->steps), gF(use_lds ? 1 : R * n);
+    DevBuf<float> rbeta(prm->read_betas ? R : 0);
+    DevBuf<int8_t> istates(prm->init_states ? R * n : 0), fstates(res->final_states ? R * n : 0);
+    DevBuf<double> ienergy(prm->init_energies ? R : 0), fenergy(res->final_energies ? R : 0),
+        mag(res->magnetization_trace ? R * prm->steps : 0);
+    if (prm->read_betas) {
+        std::vector<float> rb(R);
+        for (int64_t r = 0; r < R; ++r) rb[r] = static_cast<float>(prm->read_betas[r]);
+        rbeta.upload(rb.data(), R, s);
+        QA_HIP(hipStreamSynchronize(s));
+    }
+    istates.upload(prm->init_states, prm->init_states ? R * n : 0, s);
+    ienergy.upload(prm->init_energies, prm->init_energies ? R : 0, s);
     DevBuf<int8_t> gS(use_lds ? 1 : R * n), best(R * n);
     DevBuf<double> be(R), tr(R * prm->steps), ex(R);
     betas.upload(hb.data(), prm->steps, s);
@@ -229,6 +266,13 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     a.best_e = be.ptr;
     a.trace = tr.ptr;
     a.ldf = use_lds ? ldf : static_cast<int>(n);
+    a.read_betas = rbeta.ptr;
+    a.init_states = istates.ptr;
+    a.init_e = ienergy.ptr;
+    a.ctr_base = static_cast<uint32_t>(prm->counter_base);
+    a.final_states = fstates.ptr;
+    a.final_e = fenergy.ptr;
+    a.mag = mag.ptr;
     auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
     QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                static_cast<int>(shmem)));
@@ -241,6 +285,9 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     best.download(res->best_states, R * n, s);
     ex.download(res->best_energies, R, s);
     tr.download(res->energy_trace, R * prm->steps, s);
+    if (res->final_states) fstates.download(res->final_states, R * n, s);
+    if (res->final_energies) fenergy.download(res->final_energies, R, s);
+    if (res->magnetization_trace) mag.download(res->magnetization_trace, R * prm->steps, s);
     QA_HIP(hipStreamSynchronize(s));
     if (res->uniforms_used)
         for (int64_t r = 0; r < R; ++r) res->uniforms_used[r] = 0;

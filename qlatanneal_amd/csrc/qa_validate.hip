@@ -128,22 +128,26 @@ __device__ double energy_fp64(const ProblemView &pv, const int8_t *s) {
 }
 
 // ---- SA (annealer.cpp:33-75): one thread per read ----------------------------------------
+// Chained runs (optional): read_betas[r] replaces betas[st] for read r; init_e[r] is the running
+// energy at the start (the reference carries it between calls, parallel_tempering.cpp:73-85);
+// final_e[r] and the spins buffer hold the end state; mag[r][st] the magnetization.
 __global__ void sa_validation_kernel(ProblemView pv, int reads, int steps, const double *__restrict__ betas,
                                      int sweeps, int8_t *spins, int8_t *best, double *best_e,
                                      double *trace, const double *__restrict__ uni, int64_t ucap,
-                                     int64_t *used, int *status) {
+                                     int64_t *used, int *status, const double *__restrict__ read_betas,
+                                     const double *__restrict__ init_e, double *final_e, double *mag) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= reads) return;
     const int n = pv.n;
     int8_t *s = spins + static_cast<int64_t>(r) * n;
     int8_t *b = best + static_cast<int64_t>(r) * n;
     const double *u = uni + static_cast<int64_t>(r) * ucap;
-    double e = energy_fp64(pv, s);
+    double e = init_e ? init_e[r] : energy_fp64(pv, s);
     double be = e;
     for (int i = 0; i < n; ++i) b[i] = s[i];
     int64_t cur = 0;
     for (int st = 0; st < steps; ++st) {
-        const double beta = betas[st];
+        const double beta = read_betas ? read_betas[r] : betas[st];
         for (int sw = 0; sw < sweeps; ++sw)
             for (int i = 0; i < n; ++i) {
                 const double d = delta_fp64(pv, s, i);
@@ -162,9 +166,15 @@ __global__ void sa_validation_kernel(ProblemView pv, int reads, int steps, const
                 }
             }
         trace[static_cast<int64_t>(r) * steps + st] = e;
+        if (mag) {
+            int sum = 0;
+            for (int i = 0; i < n; ++i) sum += s[i];
+            mag[static_cast<int64_t>(r) * steps + st] = static_cast<double>(sum) / static_cast<double>(n);
+        }
     }
     best_e[r] = be;
     used[r] = cur;
+    if (final_e) final_e[r] = e;
 }
 
 // ---- SQA sweep phases (sqa_annealer.cpp:176-315), one wave per replica --------------------
@@ -498,6 +508,10 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
     DevBuf<double> betas(prm->steps), be(R), tr(R * prm->steps), uni(std::max<int64_t>(R * prm->uniforms_per_read, 1));
     DevBuf<int64_t> used(R);
     DevBuf<int> status(R);
+    DevBuf<double> rbeta(prm->read_betas ? R : 0), ienergy(prm->init_energies ? R : 0),
+        fenergy(res->final_energies ? R : 0), mag(res->magnetization_trace ? R * prm->steps : 0);
+    rbeta.upload(prm->read_betas, prm->read_betas ? R : 0, s);
+    ienergy.upload(prm->init_energies, prm->init_energies ? R : 0, s);
     spins.upload(prm->init_spins, R * n, s);
     betas.upload(prm->betas, prm->steps, s);
     uni.upload(prm->uniforms, R * prm->uniforms_per_read, s);
@@ -507,7 +521,7 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
     hipLaunchKernelGGL(sa_validation_kernel, dim3(static_cast<unsigned>((R + block - 1) / block)), dim3(block), 0, s,
                        view_of(p), static_cast<int>(R), static_cast<int>(prm->steps), betas.ptr,
                        static_cast<int>(prm->sweeps_per_beta), spins.ptr, best.ptr, be.ptr, tr.ptr, uni.ptr,
-                       prm->uniforms_per_read, used.ptr, status.ptr);
+                       prm->uniforms_per_read, used.ptr, status.ptr, rbeta.ptr, ienergy.ptr, fenergy.ptr, mag.ptr);
     QA_HIP(hipGetLastError());
     QA_HIP(hipEventRecord(ctx->ev1, s));
     std::vector<int> st(R);
@@ -517,6 +531,9 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
     best.download(res->best_states, R * n, s);
     be.download(res->best_energies, R, s);
     tr.download(res->energy_trace, R * prm->steps, s);
+    if (res->final_states) spins.download(res->final_states, R * n, s);
+    if (res->final_energies) fenergy.download(res->final_energies, R, s);
+    if (res->magnetization_trace) mag.download(res->magnetization_trace, R * prm->steps, s);
     QA_HIP(hipStreamSynchronize(s));
     for (int64_t r = 0; r < R; ++r)
         if (st[r]) throw Error{QA_EINVAL, "validation uniform stream exhausted (uniforms_per_read too small)"};

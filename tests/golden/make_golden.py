@@ -211,6 +211,27 @@ def main():
                     "scan_burn": c["sb"], "seed": c["seed"], "out": out_json(res)})
     dump("sqa_chi.json", {"cases": out})
 
+    # ---- classical ReplicaAnnealer / ParallelTemperingAnnealer (own stream: the fixtures above
+    #      do not move) ---------------------------------------------------------------------------
+    rng2 = np.random.default_rng(20261204)
+    out = []
+    for p, betas, sw, R, seed in [(sk(rng2, 24), np.geomspace(0.1, 3.0, 20), 3, 4, 3),
+                                  (sparse_problem(rng2, 18), np.linspace(0.1, 2.5, 15), 2, 3, 9),
+                                  (hubo_problem(rng2, 12, 18), np.linspace(0.1, 3.0, 12), 2, 5, 4),
+                                  (qubo_problem(ref, rng2, 10)[0], np.linspace(0.01, 0.3, 10), 2, 2, 8)]:
+        res = ref.replica_run(p, betas, sw, R, seed)
+        out.append({"problem": problem_json(p), "betas": enc(betas), "sweeps_per_beta": sw, "replicas": R,
+                    "seed": seed, "out": out_json(res)})
+    dump("replica.json", {"cases": out})
+    out = []
+    for p, betas, sps, steps, si, seed in [(sk(rng2, 20), np.geomspace(0.2, 4.0, 6), 2, 25, 1, 5),
+                                           (sparse_problem(rng2, 16), np.geomspace(0.3, 3.0, 4), 3, 20, 2, 6),
+                                           (hubo_problem(rng2, 10, 14), np.geomspace(0.2, 3.0, 5), 2, 15, 3, 7)]:
+        res = ref.pt_run(p, betas, sps, steps, si, seed)
+        out.append({"problem": problem_json(p), "betas": enc(betas), "sweeps_per_step": sps, "steps": steps,
+                    "swap_interval": si, "seed": seed, "out": out_json(res)})
+    dump("pt.json", {"cases": out})
+
 
 def dump(name, obj):
     obj["source"] = ("reference sources compiled by oracle/Makefile (Release flags), "

@@ -345,3 +345,31 @@ def test_sqa_chi_fp32(ctx):
     p = ks_2samp(gpu, cpu).pvalue
     print("chi KS p", p, gpu.mean(), cpu.mean(), flush=True)
     assert p > 0.01
+
+
+def test_sqapt_fp32_c3_shape_with_refresh(ctx):
+    """SQA-PT at the C3 shape — dense SK N=2048, P=64, 32 rungs x 4 ladders, 10 slice sweeps per
+    step, field refresh on the matrix cores — for a short schedule: every ladder's best and final
+    energies are the exact FP64 energies of the returned states, the drift after a refresh stays at
+    the split bound, and swap acceptances are probabilities."""
+    import time
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(31)
+    n, P, rungs, L, steps = 2048, 64, 32, 4, 6
+    J = _sk(rng, n)
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+    betas = np.geomspace(0.5, 4.0, rungs)
+    gammas = np.geomspace(3.0, 0.05, rungs)
+    t = time.time()
+    r = abi.sqapt_run(dp, betas, gammas, P, 10, 0, steps, 1, 0, ladders=L, mode=abi.MODE_FP32, seed=3,
+                      refresh_interval=20)
+    wall = time.time() - t
+    attempts = steps * 10 * rungs * L * P * n
+    print(f"C3 shape: {steps} steps, kernel {r['kernel_ms']:.0f} ms, wall {wall:.1f} s, "
+          f"{attempts / (r['kernel_ms'] / 1e3):.3e} attempts/s, drift {r['field_drift']}", flush=True)
+    for l in range(L):
+        assert r["best_energy"][l] == dp.energy(r["best_state"][l])[0]
+        assert np.array_equal(r["final_energies"][l], dp.energy(r["final_states"][l]))
+    assert np.all((r["swap_acceptance_trace"] >= 0) & (r["swap_acceptance_trace"] <= 1))
+    before, after = r["field_drift"]
+    assert np.isfinite(before) and after <= 5e-4

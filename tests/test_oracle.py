@@ -57,6 +57,47 @@ def test_golden_sa(orc):
             assert bits_equal(res[k], exp[k]), k
 
 
+def test_golden_replica(orc):
+    for c in load("replica.json")["cases"]:
+        res = orc.replica_run(problem(c["problem"]), dec(c["betas"]), c["sweeps_per_beta"], c["replicas"], c["seed"])
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(np.ravel(res[k]), np.ravel(exp[k])), k
+
+
+def test_golden_pt(orc):
+    for c in load("pt.json")["cases"]:
+        res = orc.pt_run(problem(c["problem"]), dec(c["betas"]), c["sweeps_per_step"], c["steps"],
+                         c["swap_interval"], c["seed"])
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(np.ravel(res[k]), np.ravel(exp[k])), k
+
+
+def test_replica_pt_closed_forms(orc):
+    """ReplicaAnnealer: the average traces are the sequential means of the per-replica traces and the
+    global best is the smallest replica best. ParallelTempering: with every beta equal a swap has
+    dE-weight 0 and is always accepted (trace exactly 1), and with swaps every step the best energy
+    is the exact energy of the best state."""
+    rng = np.random.default_rng(12)
+    n = 14
+    J = rng.normal(size=(n, n))
+    J = np.triu(J, 1) + np.triu(J, 1).T
+    p = Problem.dense(rng.normal(size=n) * 0.2, J, 0.3)
+    r = orc.replica_run(p, np.linspace(0.2, 2.0, 9), 2, 5, 44)
+    for k, key in (("energy_traces", "average_energy_trace"), ("magnetization_traces", "average_magnetization_trace")):
+        acc = np.zeros(9)
+        for row in r[k]:
+            acc = acc + row
+        assert bits_equal(acc / 5.0, r[key])
+    assert r["global_best_energy"] == r["best_energies"].min()
+    assert np.all(np.abs(r["magnetization_traces"]) <= 1.0)
+    t = orc.pt_run(p, np.full(4, 1.3), 1, 12, 1, 9)
+    assert np.all(t["swap_acceptance_trace"] == 1.0)
+    assert abs(orc.energy(p, t["best_state"])[0] - t["best_energy"]) < 1e-9
+    assert np.allclose(orc.energy(p, t["final_states"]), t["final_energies"], atol=1e-9)
+
+
 def test_golden_sqa(orc):
     for c in load("sqa.json")["cases"]:
         res = orc.sqa_run(problem(c["problem"]), dec(c["betas"]), dec(c["gammas"]), c["trotter_slices"],
@@ -245,6 +286,9 @@ def test_random_against_reference(orc):
                      (orc.sqapt_run(p, b[:3], g[:3], 2 + case % 3, 1, 1, 4, 1, case % 2, 0, case),
                       ref.sqapt_run(p, b[:3], g[:3], 2 + case % 3, 1, 1, 4, 1, case % 2, 0, case)),
                      (orc.sqa_chi_run(p, 2 + case % 4, 2, 2.0, 3.0, 0.05, 8, 2, 5, 3, 1, seed=case),
-                      ref.sqa_chi_run(p, 2 + case % 4, 2, 2.0, 3.0, 0.05, 8, 2, 5, 3, 1, seed=case))]:
+                      ref.sqa_chi_run(p, 2 + case % 4, 2, 2.0, 3.0, 0.05, 8, 2, 5, 3, 1, seed=case)),
+                     (orc.replica_run(p, b, 2, 1 + case % 4, case), ref.replica_run(p, b, 2, 1 + case % 4, case)),
+                     (orc.pt_run(p, b[:2 + case % 4], 1, 6, 1 + case % 2, case),
+                      ref.pt_run(p, b[:2 + case % 4], 1, 6, 1 + case % 2, case))]:
             for k in r:
                 assert bits_equal(a[k], r[k]), (case, k)
