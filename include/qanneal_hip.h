@@ -102,9 +102,14 @@ typedef struct qa_sa_result {
     int64_t *uniforms_used;      /* validation: [reads] (may be NULL)                        */
     double *kernel_ms;           /* optional: device time of the sweep kernel                */
     int8_t *final_states;        /* optional [reads*n]: states at the end of the run          */
-    double *final_energies;      /* optional [reads]: running energies at the end of the run  */
+    double *final_energies;      /* optional [reads]: energies at the end of the run; FP32 mode:
+                                    the FP64 energy of the final state (recomputed on the
+                                    device), validation: the running energy                  */
     double *magnetization_trace; /* optional [reads*steps]: mean spin at each step end
                                     (metrics.hpp:11-20; ReplicaAnnealer::run's trace)        */
+    double *best_energy_trace;   /* optional [reads*(steps+1)]: best running energy at the start
+                                    and at each step end (replays ReplicaAnnealer::run's
+                                    global-best updates, replica_annealer.cpp:111-116)       */
 } qa_sa_result;
 
 qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);

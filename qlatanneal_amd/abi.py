@@ -33,7 +33,7 @@ class SaParams(C.Structure):
 class SaResult(C.Structure):
     _fields_ = [("best_states", _bp), ("best_energies", _dp), ("energy_trace", _dp),
                 ("uniforms_used", _lp), ("kernel_ms", _dp), ("final_states", _bp), ("final_energies", _dp),
-                ("magnetization_trace", _dp)]
+                ("magnetization_trace", _dp), ("best_energy_trace", _dp)]
 
 
 class SqaParams(C.Structure):
@@ -252,10 +252,11 @@ def sa_run(prob: DeviceProblem, betas, sweeps_per_beta: int, reads: int = 1, mod
     bs, be, tr = np.zeros((reads, n), np.int8), np.zeros(reads), np.zeros((reads, steps))
     used, ms = np.zeros(reads, np.int64), np.zeros(1)
     fs, fe, mg = np.zeros((reads, n), np.int8), np.zeros(reads), np.zeros((reads, steps))
-    r = SaResult(_b(bs), _d(be), _d(tr), used.ctypes.data_as(_lp), _d(ms), _b(fs), _d(fe), _d(mg))
+    bt = np.zeros((reads, steps + 1))
+    r = SaResult(_b(bs), _d(be), _d(tr), used.ctypes.data_as(_lp), _d(ms), _b(fs), _d(fe), _d(mg), _d(bt))
     check(lib().qa_sa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(best_states=bs, best_energies=be, energy_traces=tr, uniforms_used=used, kernel_ms=ms[0],
-                final_states=fs, final_energies=fe, magnetization_traces=mg)
+                final_states=fs, final_energies=fe, magnetization_traces=mg, best_energy_traces=bt)
 
 
 def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: int, sweeps_per_beta: int,

@@ -267,6 +267,46 @@ PYBIND11_MODULE(_qanneal, m) {
         .def("run_reads", &Annealer::run_reads, py::arg("sweeps_per_beta"), py::arg("reads"), py::arg("seed"),
              py::call_guard<py::gil_scoped_release>());
 
+    py::class_<ReplicaResult>(m, "ReplicaResult")
+        .def_readonly("best_state", &ReplicaResult::best_state)
+        .def_readonly("best_energy", &ReplicaResult::best_energy)
+        .def_readonly("energy_trace", &ReplicaResult::energy_trace)
+        .def_readonly("magnetization_trace", &ReplicaResult::magnetization_trace);
+
+    py::class_<MultiAnnealResult>(m, "MultiAnnealResult")
+        .def_readonly("replicas", &MultiAnnealResult::replicas)
+        .def_readonly("global_best_state", &MultiAnnealResult::global_best_state)
+        .def_readonly("global_best_energy", &MultiAnnealResult::global_best_energy)
+        .def_readonly("average_energy_trace", &MultiAnnealResult::average_energy_trace)
+        .def_readonly("average_magnetization_trace", &MultiAnnealResult::average_magnetization_trace);
+
+    py::class_<ReplicaAnnealer>(m, "ReplicaAnnealer")
+        .def(py::init([](std::shared_ptr<Hamiltonian> ham, AnnealSchedule schedule, std::size_t replicas,
+                         const std::string &backend) {
+                 return ReplicaAnnealer(backend_for(std::move(ham), backend), std::move(schedule), replicas);
+             }),
+             py::arg("hamiltonian"), py::arg("schedule"), py::arg("replicas"), py::arg("backend") = "hip",
+             py::keep_alive<1, 2>())
+        .def("set_seed", &ReplicaAnnealer::set_seed)
+        .def("run", &ReplicaAnnealer::run, py::arg("sweeps_per_beta"), py::call_guard<py::gil_scoped_release>());
+
+    py::class_<ParallelTemperingResult>(m, "ParallelTemperingResult")
+        .def_readonly("final_states", &ParallelTemperingResult::final_states)
+        .def_readonly("final_energies", &ParallelTemperingResult::final_energies)
+        .def_readonly("best_state", &ParallelTemperingResult::best_state)
+        .def_readonly("best_energy", &ParallelTemperingResult::best_energy)
+        .def_readonly("average_energy_trace", &ParallelTemperingResult::average_energy_trace)
+        .def_readonly("swap_acceptance_trace", &ParallelTemperingResult::swap_acceptance_trace);
+
+    py::class_<ParallelTemperingAnnealer>(m, "ParallelTemperingAnnealer")
+        .def(py::init([](std::shared_ptr<Hamiltonian> ham, const std::vector<double> &betas, const std::string &backend) {
+                 return ParallelTemperingAnnealer(backend_for(std::move(ham), backend), betas);
+             }),
+             py::arg("hamiltonian"), py::arg("betas"), py::arg("backend") = "hip", py::keep_alive<1, 2>())
+        .def("set_seed", &ParallelTemperingAnnealer::set_seed)
+        .def("run", &ParallelTemperingAnnealer::run, py::arg("sweeps_per_step"), py::arg("steps"),
+             py::arg("swap_interval") = 1, py::call_guard<py::gil_scoped_release>());
+
     py::class_<SQAParallelTemperingResult>(m, "SQAParallelTemperingResult")
         .def_readonly("final_states", &SQAParallelTemperingResult::final_states)
         .def_readonly("final_energies", &SQAParallelTemperingResult::final_energies)

@@ -3,6 +3,7 @@
 // reference's host code draws from the annealer's std::mt19937_64 (initial states, per-replica
 // seeds, uniform streams) and advances the engine by what the device consumed, so consecutive
 // runs and set_seed behave as in the reference.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -11,6 +12,8 @@
 #include <stdexcept>
 
 #include "qanneal/annealer.hpp"
+#include "qanneal/parallel_tempering.hpp"
+#include "qanneal/replica_annealer.hpp"
 #include "qanneal/sqa_annealer.hpp"
 #include "qanneal/sqa_chi_annealer.hpp"
 #include "qanneal/sqa_parallel_tempering.hpp"
@@ -141,6 +144,214 @@ std::vector<AnnealResult> Annealer::run_reads(std::size_t sweeps_per_beta, std::
         all[r].energy_trace.assign(tr.begin() + r * steps, tr.begin() + (r + 1) * steps);
     }
     return all;
+}
+
+// ============================================================================ SA replicas
+ReplicaAnnealer::ReplicaAnnealer(const Hamiltonian &hamiltonian, AnnealSchedule schedule, std::size_t replicas)
+    : backend_(make_backend(BackendKind::HIP, hamiltonian)), schedule_(std::move(schedule)), replicas_(replicas),
+      rng_(std::random_device{}()) {
+    if (schedule_.betas.empty()) throw std::invalid_argument("Schedule must contain at least one beta.");
+    if (replicas_ == 0) throw std::invalid_argument("replicas must be > 0.");
+}
+
+ReplicaAnnealer::ReplicaAnnealer(std::shared_ptr<Backend> backend, AnnealSchedule schedule, std::size_t replicas)
+    : backend_(std::move(backend)), schedule_(std::move(schedule)), replicas_(replicas), rng_(std::random_device{}()) {
+    if (!backend_) throw std::invalid_argument("ReplicaAnnealer requires a backend.");
+    if (schedule_.betas.empty()) throw std::invalid_argument("Schedule must contain at least one beta.");
+    if (replicas_ == 0) throw std::invalid_argument("replicas must be > 0.");
+}
+
+void ReplicaAnnealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
+
+// replica_annealer.cpp:46-140. All replicas run as the reads of one SA launch. Validation mode
+// draws what the reference draws from the annealer's engine: the R start states, then R replica
+// seeds whose engines supply each replica's uniform stream. The global best is replayed on the
+// host from each replica's best-so-far energy at the start and at every step end, with the
+// reference's strict-less updates in replica order, so ties resolve to the same replica.
+MultiAnnealResult ReplicaAnnealer::run(std::size_t sweeps_per_beta) {
+    if (sweeps_per_beta == 0) throw std::invalid_argument("sweeps_per_beta must be > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), steps = schedule_.size(), R = replicas_;
+    qa_sa_params prm{};
+    prm.steps = static_cast<int64_t>(steps);
+    prm.betas = schedule_.betas.data();
+    prm.sweeps_per_beta = static_cast<int64_t>(sweeps_per_beta);
+    prm.reads = static_cast<int64_t>(R);
+    prm.mode = mode_of(hb);
+    std::vector<int8_t> init;
+    std::vector<double> uni;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng_();
+    } else {
+        init.resize(R * n);
+        for (std::size_t r = 0; r < R; ++r) {  // replica_annealer.cpp:56-59
+            State st = State::random(n, rng_);
+            std::memcpy(&init[r * n], st.spins.data(), n);
+        }
+        const std::size_t per = steps * sweeps_per_beta * n;
+        uni.resize(R * per);
+        for (std::size_t r = 0; r < R; ++r) {  // replica_annealer.cpp:60-63
+            std::mt19937_64 eng(rng_());
+            auto u = uniform_stream(eng, per);
+            std::memcpy(&uni[r * per], u.data(), per * sizeof(double));
+        }
+        prm.init_spins = init.data();
+        prm.uniforms = uni.data();
+        prm.uniforms_per_read = static_cast<int64_t>(per);
+    }
+    std::vector<int8_t> best(R * n);
+    std::vector<double> be(R), tr(R * steps), mg(R * steps), bt(R * (steps + 1));
+    qa_sa_result out{};
+    out.best_states = best.data();
+    out.best_energies = be.data();
+    out.energy_trace = tr.data();
+    out.magnetization_trace = mg.data();
+    out.best_energy_trace = bt.data();
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    MultiAnnealResult res;
+    res.replicas.resize(R);
+    for (std::size_t r = 0; r < R; ++r) {
+        res.replicas[r].best_state = state_from(&best[r * n], n);
+        res.replicas[r].best_energy = be[r];
+        res.replicas[r].energy_trace.assign(tr.begin() + r * steps, tr.begin() + (r + 1) * steps);
+        res.replicas[r].magnetization_trace.assign(mg.begin() + r * steps, mg.begin() + (r + 1) * steps);
+    }
+    double g = std::numeric_limits<double>::infinity();
+    std::size_t owner = R;
+    for (std::size_t k = 0; k <= steps; ++k)  // k = 0: start states (replica_annealer.cpp:77-80)
+        for (std::size_t r = 0; r < R; ++r)
+            if (bt[r * (steps + 1) + k] < g) {
+                g = bt[r * (steps + 1) + k];
+                owner = r;
+            }
+    res.global_best_energy = owner < R ? be[owner] : g;
+    if (owner < R) res.global_best_state = res.replicas[owner].best_state;
+    res.average_energy_trace.resize(steps);
+    res.average_magnetization_trace.resize(steps);
+    for (std::size_t k = 0; k < steps; ++k) {  // replica_annealer.cpp:118-133, replica order
+        double ae = 0.0, am = 0.0;
+        for (std::size_t r = 0; r < R; ++r) {
+            ae += tr[r * steps + k];
+            am += mg[r * steps + k];
+        }
+        res.average_energy_trace[k] = ae / static_cast<double>(R);
+        res.average_magnetization_trace[k] = am / static_cast<double>(R);
+    }
+    return res;
+}
+
+// ==================================================================== parallel tempering
+ParallelTemperingAnnealer::ParallelTemperingAnnealer(const Hamiltonian &hamiltonian, std::vector<double> betas)
+    : backend_(make_backend(BackendKind::HIP, hamiltonian)), betas_(std::move(betas)), rng_(std::random_device{}()) {
+    if (betas_.size() < 2) throw std::invalid_argument("Parallel tempering requires at least two betas.");
+}
+
+ParallelTemperingAnnealer::ParallelTemperingAnnealer(std::shared_ptr<Backend> backend, std::vector<double> betas)
+    : backend_(std::move(backend)), betas_(std::move(betas)), rng_(std::random_device{}()) {
+    if (!backend_) throw std::invalid_argument("ParallelTemperingAnnealer requires a backend.");
+    if (betas_.size() < 2) throw std::invalid_argument("Parallel tempering requires at least two betas.");
+}
+
+void ParallelTemperingAnnealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
+
+// parallel_tempering.cpp:36-122. FP32: each step is one SA launch over all rungs (read r at
+// betas[r]) continuing from the current states and exact energies, with the step index as the
+// Philox stream offset; the final energies come back recomputed in FP64, so swaps and the best
+// state see exact energies. Validation: rung r's sweeps of a step consume the annealer's engine
+// right after rung r-1's, so each rung-step is its own launch on the engine's current stream,
+// and the engine is advanced by what the device used. Swaps draw from the same engine.
+ParallelTemperingResult ParallelTemperingAnnealer::run(std::size_t sweeps_per_step, std::size_t steps,
+                                                       std::size_t swap_interval) {
+    if (sweeps_per_step == 0) throw std::invalid_argument("sweeps_per_step must be > 0.");
+    if (steps == 0) throw std::invalid_argument("steps must be > 0.");
+    if (swap_interval == 0) throw std::invalid_argument("swap_interval must be > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), R = betas_.size();
+    const int mode = mode_of(hb);
+    std::vector<int8_t> cur(R * n), nxt(R * n), best(R * n);
+    std::vector<double> energies(R), fe(R), be(R), tr(R);
+    for (std::size_t r = 0; r < R; ++r) {  // parallel_tempering.cpp:54-57
+        State st = State::random(n, rng_);
+        std::memcpy(&cur[r * n], st.spins.data(), n);
+    }
+    const uint64_t key = mode == QA_MODE_FP32 ? rng_() : 0;
+    ParallelTemperingResult res;
+    res.best_energy = std::numeric_limits<double>::infinity();
+    res.average_energy_trace.reserve(steps);
+    res.swap_acceptance_trace.reserve(steps);
+    std::uniform_real_distribution<double> uniform(0.0, 1.0);
+    std::lock_guard<std::mutex> lock(hb.device_mutex());
+    for (std::size_t step = 0; step < steps; ++step) {
+        qa_sa_params prm{};
+        prm.steps = 1;
+        prm.sweeps_per_beta = static_cast<int64_t>(sweeps_per_step);
+        prm.mode = mode;
+        qa_sa_result out{};
+        if (mode == QA_MODE_FP32) {
+            prm.betas = betas_.data();
+            prm.read_betas = betas_.data();
+            prm.reads = static_cast<int64_t>(R);
+            prm.seed = key;
+            prm.init_states = cur.data();
+            prm.init_energies = step == 0 ? nullptr : energies.data();
+            prm.counter_base = step;
+            out.best_states = best.data();
+            out.best_energies = be.data();
+            out.energy_trace = tr.data();
+            out.final_states = nxt.data();
+            out.final_energies = fe.data();
+            throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+        } else {
+            for (std::size_t r = 0; r < R; ++r) {
+                const auto uni = uniform_stream(rng_, sweeps_per_step * n);
+                int64_t used = 0;
+                prm.betas = &betas_[r];
+                prm.reads = 1;
+                prm.init_spins = &cur[r * n];
+                prm.init_energies = step == 0 ? nullptr : &energies[r];
+                prm.uniforms = uni.data();
+                prm.uniforms_per_read = static_cast<int64_t>(uni.size());
+                out.best_states = &best[r * n];
+                out.best_energies = &be[r];
+                out.energy_trace = &tr[r];
+                out.uniforms_used = &used;
+                out.final_states = &nxt[r * n];
+                out.final_energies = &fe[r];
+                throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+                rng_.discard(static_cast<unsigned long long>(used));
+            }
+        }
+        cur.swap(nxt);
+        energies = fe;
+        for (std::size_t r = 0; r < R; ++r)  // parallel_tempering.cpp:84-87
+            if (energies[r] < res.best_energy) {
+                res.best_energy = energies[r];
+                res.best_state = state_from(&cur[r * n], n);
+            }
+        double accepted = 0.0, attempted = 0.0;
+        if ((step + 1) % swap_interval == 0) {  // parallel_tempering.cpp:92-106
+            for (std::size_t r = 0; r + 1 < R; ++r) {
+                const double delta = (betas_[r] - betas_[r + 1]) * (energies[r + 1] - energies[r]);
+                ++attempted;
+                if (delta <= 0.0 || uniform(rng_) < std::exp(-delta)) {
+                    std::swap_ranges(cur.begin() + r * n, cur.begin() + (r + 1) * n, cur.begin() + (r + 1) * n);
+                    std::swap(energies[r], energies[r + 1]);
+                    ++accepted;
+                }
+            }
+        }
+        double avg = 0.0;
+        for (double e : energies) avg += e;
+        res.average_energy_trace.push_back(avg / static_cast<double>(R));
+        res.swap_acceptance_trace.push_back(attempted > 0.0 ? accepted / attempted : 0.0);
+    }
+    res.final_states.reserve(R);
+    for (std::size_t r = 0; r < R; ++r) res.final_states.push_back(state_from(&cur[r * n], n));
+    res.final_energies = energies;
+    return res;
 }
 
 // ====================================================================================== SQA

@@ -41,8 +41,8 @@ struct SaArgs {
     const double *init_e;        // [reads]: running energy at the start (null: FP64 energy)
     uint32_t ctr_base;           // Philox counter word 2 (distinct streams for chained runs)
     int8_t *final_states;        // [reads][n] (null: not written)
-    double *final_e;             // [reads]
     double *mag;                 // [reads][steps]: mean spin at each step end
+    double *btrace;              // [reads][steps+1]: best running energy at the start and each step end
 };
 
 template <bool kLds>
@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
     if (tid < G) {
         ecur = a.init_e ? a.init_e[r0 + tid] : a.pv.c + E[tid];
         ebest = ecur;
+        if (a.btrace) a.btrace[static_cast<int64_t>(r0 + tid) * (a.steps + 1)] = ebest;
     }
     for (int i = tid; i < G * n; i += kThreads) a.best[static_cast<int64_t>(r0) * n + i] = S[(i / n) * static_cast<int64_t>(n) + (i % n)];
     __syncthreads();
@@ -206,6 +207,7 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
         }
         if (tid < G) {
             a.trace[static_cast<int64_t>(r0 + tid) * a.steps + step] = ecur;
+            if (a.btrace) a.btrace[static_cast<int64_t>(r0 + tid) * (a.steps + 1) + step + 1] = ebest;
             if (a.mag) {  // magnetization (metrics.hpp:11-20): integer sum, one division
                 const int8_t *sc = S + static_cast<int64_t>(tid) * n;
                 int sum = 0;
@@ -216,7 +218,6 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
     }
     if (tid < G) {
         a.best_e[r0 + tid] = ebest;
-        if (a.final_e) a.final_e[r0 + tid] = ecur;
     }
     if (a.final_states)
         for (int e = tid; e < G * n; e += kThreads) a.final_states[static_cast<int64_t>(r0) * n + e] = S[e];
@@ -238,9 +239,12 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     for (int64_t k = 0; k < prm->steps; ++k) hb[k] = static_cast<float>(prm->betas[k]);
     DevBuf<float> betas(prm->steps), gF(use_lds ? 1 : R * n);
     DevBuf<float> rbeta(prm->read_betas ? R : 0);
-    DevBuf<int8_t> istates(prm->init_states ? R * n : 0), fstates(res->final_states ? R * n : 0);
+    // Final energies are the FP64 energies of the final states, recomputed on the device, so that
+    // chained runs (parallel tempering) carry exact energies rather than sums of FP32 deltas.
+    const bool want_final = res->final_states || res->final_energies;
+    DevBuf<int8_t> istates(prm->init_states ? R * n : 0), fstates(want_final ? R * n : 0);
     DevBuf<double> ienergy(prm->init_energies ? R : 0), fenergy(res->final_energies ? R : 0),
-        mag(res->magnetization_trace ? R * prm->steps : 0);
+        mag(res->magnetization_trace ? R * prm->steps : 0), btr(res->best_energy_trace ? R * (prm->steps + 1) : 0);
     if (prm->read_betas) {
         std::vector<float> rb(R);
         for (int64_t r = 0; r < R; ++r) rb[r] = static_cast<float>(prm->read_betas[r]);
@@ -271,8 +275,8 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     a.init_e = ienergy.ptr;
     a.ctr_base = static_cast<uint32_t>(prm->counter_base);
     a.final_states = fstates.ptr;
-    a.final_e = fenergy.ptr;
     a.mag = mag.ptr;
+    a.btrace = btr.ptr;
     auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
     QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                static_cast<int>(shmem)));
@@ -282,12 +286,14 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     QA_HIP(hipGetLastError());
     QA_HIP(hipEventRecord(ctx->ev1, s));
     device_energies(p, best.ptr, R, ex.ptr, s);
+    if (res->final_energies) device_energies(p, fstates.ptr, R, fenergy.ptr, s);
     best.download(res->best_states, R * n, s);
     ex.download(res->best_energies, R, s);
     tr.download(res->energy_trace, R * prm->steps, s);
     if (res->final_states) fstates.download(res->final_states, R * n, s);
     if (res->final_energies) fenergy.download(res->final_energies, R, s);
     if (res->magnetization_trace) mag.download(res->magnetization_trace, R * prm->steps, s);
+    if (res->best_energy_trace) btr.download(res->best_energy_trace, R * (prm->steps + 1), s);
     QA_HIP(hipStreamSynchronize(s));
     if (res->uniforms_used)
         for (int64_t r = 0; r < R; ++r) res->uniforms_used[r] = 0;

@@ -135,7 +135,8 @@ __global__ void sa_validation_kernel(ProblemView pv, int reads, int steps, const
                                      int sweeps, int8_t *spins, int8_t *best, double *best_e,
                                      double *trace, const double *__restrict__ uni, int64_t ucap,
                                      int64_t *used, int *status, const double *__restrict__ read_betas,
-                                     const double *__restrict__ init_e, double *final_e, double *mag) {
+                                     const double *__restrict__ init_e, double *final_e, double *mag,
+                                     double *btrace) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= reads) return;
     const int n = pv.n;
@@ -145,6 +146,7 @@ __global__ void sa_validation_kernel(ProblemView pv, int reads, int steps, const
     double e = init_e ? init_e[r] : energy_fp64(pv, s);
     double be = e;
     for (int i = 0; i < n; ++i) b[i] = s[i];
+    if (btrace) btrace[static_cast<int64_t>(r) * (steps + 1)] = be;
     int64_t cur = 0;
     for (int st = 0; st < steps; ++st) {
         const double beta = read_betas ? read_betas[r] : betas[st];
@@ -166,6 +168,7 @@ __global__ void sa_validation_kernel(ProblemView pv, int reads, int steps, const
                 }
             }
         trace[static_cast<int64_t>(r) * steps + st] = e;
+        if (btrace) btrace[static_cast<int64_t>(r) * (steps + 1) + st + 1] = be;
         if (mag) {
             int sum = 0;
             for (int i = 0; i < n; ++i) sum += s[i];
@@ -509,7 +512,8 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
     DevBuf<int64_t> used(R);
     DevBuf<int> status(R);
     DevBuf<double> rbeta(prm->read_betas ? R : 0), ienergy(prm->init_energies ? R : 0),
-        fenergy(res->final_energies ? R : 0), mag(res->magnetization_trace ? R * prm->steps : 0);
+        fenergy(res->final_energies ? R : 0), mag(res->magnetization_trace ? R * prm->steps : 0),
+        btr(res->best_energy_trace ? R * (prm->steps + 1) : 0);
     rbeta.upload(prm->read_betas, prm->read_betas ? R : 0, s);
     ienergy.upload(prm->init_energies, prm->init_energies ? R : 0, s);
     spins.upload(prm->init_spins, R * n, s);
@@ -521,7 +525,8 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
     hipLaunchKernelGGL(sa_validation_kernel, dim3(static_cast<unsigned>((R + block - 1) / block)), dim3(block), 0, s,
                        view_of(p), static_cast<int>(R), static_cast<int>(prm->steps), betas.ptr,
                        static_cast<int>(prm->sweeps_per_beta), spins.ptr, best.ptr, be.ptr, tr.ptr, uni.ptr,
-                       prm->uniforms_per_read, used.ptr, status.ptr, rbeta.ptr, ienergy.ptr, fenergy.ptr, mag.ptr);
+                       prm->uniforms_per_read, used.ptr, status.ptr, rbeta.ptr, ienergy.ptr, fenergy.ptr, mag.ptr,
+                       btr.ptr);
     QA_HIP(hipGetLastError());
     QA_HIP(hipEventRecord(ctx->ev1, s));
     std::vector<int> st(R);
@@ -534,6 +539,7 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
     if (res->final_states) spins.download(res->final_states, R * n, s);
     if (res->final_energies) fenergy.download(res->final_energies, R, s);
     if (res->magnetization_trace) mag.download(res->magnetization_trace, R * prm->steps, s);
+    if (res->best_energy_trace) btr.download(res->best_energy_trace, R * (prm->steps + 1), s);
     QA_HIP(hipStreamSynchronize(s));
     for (int64_t r = 0; r < R; ++r)
         if (st[r]) throw Error{QA_EINVAL, "validation uniform stream exhausted (uniforms_per_read too small)"};

@@ -121,3 +121,27 @@ def test_cpu_backend_refuses_to_run():
         qa.SQAAnnealer(d, qa.auto_schedule_sqa(5), 4, 1, backend="tpu")
     with pytest.raises(RuntimeError):
         qa.Annealer(d, qa.auto_schedule_sa(), backend="cuda")
+    r = qa.ReplicaAnnealer(d, qa.AnnealSchedule.linear(0.1, 1.0, 4), 2, backend="cpu")
+    with pytest.raises(RuntimeError, match="GPU only"):
+        r.run(1)
+    t = qa.ParallelTemperingAnnealer(d, [0.5, 1.0], backend="cpu")
+    with pytest.raises(RuntimeError, match="GPU only"):
+        t.run(1, 2)
+
+
+def test_classical_argument_errors():
+    """The reference's std::invalid_argument cases (replica_annealer.cpp:17-21,47-49,
+    parallel_tempering.cpp:16-18,39-47) raise ValueError before any device work."""
+    d = qa.DenseIsing(np.zeros(3), np.zeros((3, 3)))
+    with pytest.raises(ValueError, match="replicas"):
+        qa.ReplicaAnnealer(d, qa.AnnealSchedule.linear(0.1, 1.0, 4), 0, backend="cpu")
+    with pytest.raises(ValueError, match="at least one beta"):
+        qa.ReplicaAnnealer(d, qa.AnnealSchedule(), 2, backend="cpu")
+    with pytest.raises(ValueError, match="sweeps_per_beta"):
+        qa.ReplicaAnnealer(d, qa.AnnealSchedule.linear(0.1, 1.0, 4), 2, backend="cpu").run(0)
+    with pytest.raises(ValueError, match="two betas"):
+        qa.ParallelTemperingAnnealer(d, [1.0], backend="cpu")
+    t = qa.ParallelTemperingAnnealer(d, [0.5, 1.0], backend="cpu")
+    for args, msg in (((0, 1, 1), "sweeps_per_step"), ((1, 0, 1), "steps"), ((1, 1, 0), "swap_interval")):
+        with pytest.raises(ValueError, match=msg):
+            t.run(*args)
