@@ -40,7 +40,7 @@ public:
     SQAResult run(std::size_t sweeps_per_beta, std::size_t worldline_sweeps, std::size_t cluster_sweeps = 0,
                   std::size_t continuous_time_slices = 0, SQAObserver *observer = nullptr);
 
-    // Bond-susceptibility adaptive schedule (sqa_annealer.cpp:499-952): not in this build.
+    // Bond-susceptibility adaptive J_perp schedule (sqa_annealer.cpp:499-952) on the device.
     SQAResult run_optimal(double beta, double j_perp_start, double j_perp_end, double eps_tilde, double alpha,
                           std::size_t num_steps, std::size_t sweeps_per_step, std::size_t worldline_sweeps = 0,
                           std::size_t cluster_sweeps = 0, std::size_t calib_probes = 12,

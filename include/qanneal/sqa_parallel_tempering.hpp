@@ -47,7 +47,7 @@ public:
                                                         std::size_t swap_interval = 1,
                                                         std::size_t cluster_sweeps = 0);
 
-    // Adaptive shared-J_perp schedule (sqa_parallel_tempering.cpp:348-882): not in this build.
+    // Adaptive shared-J_perp schedule (sqa_parallel_tempering.cpp:348-882) on the device.
     SQAParallelTemperingResult run_optimal(std::size_t num_steps, std::size_t sweeps_per_step,
                                            std::size_t worldline_sweeps, double eps_tilde,
                                            double alpha = 15.0 / 14.0, double j_perp_end = 0.0,

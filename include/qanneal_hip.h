@@ -218,6 +218,81 @@ typedef struct qa_sqapt_result {
 qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm,
                        qa_sqapt_result *res);
 
+/* ---- Adaptive transverse-coupling schedules -------------------------------------------------
+ * The bond susceptibility chi_B = Var(B), B = sum_t sum_i s_t,i s_t+1,i, sampled after every
+ * slice and cluster sweep, sets how fast J_perp advances: a calibration pilot measures chi_B at
+ * calib_probes midpoints of [j_perp_start, j_perp_end], and J_perp(step) inverts the cumulative
+ * integral of chi_B^alpha (eps_tilde <= 0), or J_perp advances online by eps_tilde chi_B^-alpha
+ * (eps_tilde > 0). Both modes: the B sums are exact integers, so chi_B and the schedule follow the
+ * reference's arithmetic; FP32 mode sweeps as qa_sqa_run does. debug_csv_path (NULL or "": none)
+ * writes the reference's per-step diagnostic CSV. */
+
+/* Replaces SQAAnnealer::run_optimal (sqa_annealer.cpp:499-952): a beta ramp at j_perp_start
+ * for beta_ramp_fraction of the steps, then the adaptive J_perp schedule at beta. */
+typedef struct qa_sqa_opt_params {
+    double beta, j_perp_start, j_perp_end, eps_tilde, alpha;
+    int64_t num_steps, sweeps_per_step, worldline_sweeps, cluster_sweeps, calib_probes, calib_sweeps;
+    double beta_ramp_fraction, beta_ramp_start;
+    int64_t trotter_slices, replicas;
+    int32_t mode;
+    uint64_t seed;                 /* FP32: Philox key                                         */
+    const int8_t *init_spins;      /* validation: [replicas*P*n], SQAState::random draws       */
+    const uint64_t *replica_seeds; /* validation: [replicas]                                   */
+    const char *debug_csv_path;
+} qa_sqa_opt_params;
+
+typedef struct qa_opt_schedule {
+    double *j_perp_trace;          /* [num_steps]                                               */
+    double *chi_B_trace;           /* [num_steps] chi_B that drove each step (floored)          */
+    double *calibrated_eps_tilde, *j_perp_start, *resolved_j_perp_end, *final_j_perp;
+} qa_opt_schedule;
+
+typedef struct qa_sqa_opt_result {
+    int8_t *best_state;            /* [n]                                                       */
+    double *best_energy;           /* FP64 energy of best_state                                 */
+    double *energy_trace;          /* [num_steps] mean slice energy at each step end            */
+    qa_opt_schedule sched;
+    double *kernel_ms;             /* optional                                                  */
+} qa_sqa_opt_result;
+
+qa_status qa_sqa_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_opt_params *prm,
+                             qa_sqa_opt_result *res);
+
+/* Replaces SQAParallelTemperingAnnealer::run_optimal (sqa_parallel_tempering.cpp:348-882): every
+ * rung sweeps at its beta/P with one shared J_perp from trotter_coupling(betas[0], gammas[0]) to
+ * j_perp_end (<= 0: the coldest rung's coupling); swaps compare (beta_r/P) sum_t E_t. */
+typedef struct qa_sqapt_opt_params {
+    int64_t rungs;
+    const double *betas, *gammas;  /* [rungs]                                                   */
+    int64_t trotter_slices;        /* P actually used (caller resolves continuous_time_slices)  */
+    int64_t num_steps, sweeps_per_step, worldline_sweeps, cluster_sweeps, swap_interval;
+    int64_t calib_probes, calib_sweeps;
+    double eps_tilde, alpha, j_perp_end;
+    int32_t mode;
+    uint64_t seed;                 /* FP32: Philox key (the pilot uses a key derived from it)   */
+    const int8_t *init_spins;      /* validation: [rungs*P*n]                                   */
+    const uint64_t *rung_seeds;    /* validation: [rungs]                                       */
+    const uint64_t *scratch_seeds; /* validation, eps_tilde <= 0: [rungs] the pilot's engines   */
+    const double *swap_uniforms;   /* validation: master-engine uniform stream for swaps        */
+    int64_t swap_uniforms_len;
+    const char *debug_csv_path;
+} qa_sqapt_opt_params;
+
+typedef struct qa_sqapt_opt_result {
+    int8_t *final_states;          /* [rungs*n] best slice per rung at the end                  */
+    double *final_energies;        /* [rungs]                                                   */
+    int8_t *best_state;            /* [n]                                                       */
+    double *best_energy;
+    double *average_energy_trace;  /* [num_steps]                                               */
+    double *swap_acceptance_trace; /* [num_steps]                                               */
+    qa_opt_schedule sched;
+    int64_t *swap_uniforms_used;   /* validation (may be NULL)                                  */
+    double *kernel_ms;
+} qa_sqapt_opt_result;
+
+qa_status qa_sqapt_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_opt_params *prm,
+                               qa_sqapt_opt_result *res);
+
 /* ---- sqa_chi: replaces SQAChiAnnealer::run_chi (sqa_chi_annealer.cpp:146-438) ------------ */
 typedef struct qa_chi_params {
     int64_t trotter_slices;

@@ -197,6 +197,47 @@ class Oracle:
         return dict(final_states=fs, final_energies=fe, best_state=best, best_energy=be.value,
                     average_energy_trace=at, swap_acceptance_trace=sw)
 
+    def sqa_opt_run(self, p: Problem, trotter_slices: int, replicas: int, beta: float, j_perp_start: float,
+                    j_perp_end: float, eps_tilde: float, alpha: float, num_steps: int, sweeps_per_step: int,
+                    worldline_sweeps: int = 0, cluster_sweeps: int = 0, calib_probes: int = 12,
+                    calib_sweeps: int = 10, beta_ramp_fraction: float = 0.3, beta_ramp_start: float = 0.1,
+                    seed: int = 0):
+        """SQAAnnealer::run_optimal (sqa_annealer.cpp:499-952) after set_seed(seed)."""
+        n, S = p.n, num_steps
+        best, be = np.zeros(n, np.int8), C.c_double()
+        tr, jt, ct, sc = np.zeros(S), np.zeros(S), np.zeros(S), np.zeros(4)
+        st, keep = p._struct()
+        self._chk(self.lib.orc_sqa_opt_run(
+            C.byref(st), _i64(trotter_slices), _i64(replicas), C.c_double(beta), C.c_double(j_perp_start),
+            C.c_double(j_perp_end), C.c_double(eps_tilde), C.c_double(alpha), _i64(num_steps),
+            _i64(sweeps_per_step), _i64(worldline_sweeps), _i64(cluster_sweeps), _i64(calib_probes),
+            _i64(calib_sweeps), C.c_double(beta_ramp_fraction), C.c_double(beta_ramp_start), C.c_uint64(seed),
+            _b(best), C.byref(be), _d(tr), _d(jt), _d(ct), _d(sc)))
+        return dict(best_state=best, best_energy=be.value, energy_trace=tr, j_perp_trace=jt, chi_B_trace=ct,
+                    calibrated_eps_tilde=sc[0], j_perp_start=sc[1], resolved_j_perp_end=sc[2], final_j_perp=sc[3])
+
+    def sqapt_opt_run(self, p: Problem, betas, gammas, trotter_slices: int, num_steps: int, sweeps_per_step: int,
+                      worldline_sweeps: int = 0, eps_tilde: float = 0.0, alpha: float = 15.0 / 14.0,
+                      j_perp_end: float = 0.0, cluster_sweeps: int = 0, swap_interval: int = 1,
+                      continuous_time_slices: int = 0, calib_probes: int = 12, calib_sweeps: int = 10, seed: int = 0):
+        """SQAParallelTemperingAnnealer::run_optimal (sqa_parallel_tempering.cpp:348-882) after set_seed(seed)."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        g = np.ascontiguousarray(gammas, dtype=np.float64)
+        R, n, S = b.shape[0], p.n, num_steps
+        fs, fe = np.zeros((R, n), np.int8), np.zeros(R)
+        best, be = np.zeros(n, np.int8), C.c_double()
+        at, sw, jt, ct, sc = np.zeros(S), np.zeros(S), np.zeros(S), np.zeros(S), np.zeros(4)
+        st, keep = p._struct()
+        self._chk(self.lib.orc_sqapt_opt_run(
+            C.byref(st), _i64(R), _d(b), _d(g), _i64(trotter_slices), _i64(num_steps), _i64(sweeps_per_step),
+            _i64(worldline_sweeps), C.c_double(eps_tilde), C.c_double(alpha), C.c_double(j_perp_end),
+            _i64(cluster_sweeps), _i64(swap_interval), _i64(continuous_time_slices), _i64(calib_probes),
+            _i64(calib_sweeps), C.c_uint64(seed), _b(fs), _d(fe), _b(best), C.byref(be), _d(at), _d(sw), _d(jt),
+            _d(ct), _d(sc)))
+        return dict(final_states=fs, final_energies=fe, best_state=best, best_energy=be.value,
+                    average_energy_trace=at, swap_acceptance_trace=sw, j_perp_trace=jt, chi_B_trace=ct,
+                    calibrated_eps_tilde=sc[0], j_perp_start=sc[1], resolved_j_perp_end=sc[2], final_j_perp=sc[3])
+
     def sqa_run(self, p: Problem, betas, gammas, trotter_slices: int, replicas: int,
                 sweeps_per_beta: int, worldline_sweeps: int, cluster_sweeps: int = 0,
                 continuous_time_slices: int = 0, seed: int = 0):

@@ -83,6 +83,29 @@ int orc_sqapt_run(const orc_problem *p, int64_t rungs, const double *betas, cons
                   double *final_energies, int8_t *best_state, double *best_energy,
                   double *avg_trace, double *swap_trace);
 
+/* SQAAnnealer::run_optimal (sqa_annealer.cpp:499-952) after set_seed(seed). energy_trace,
+ * j_perp_trace, chi_trace: [num_steps]; scalars [4]: calibrated_eps_tilde, j_perp_start,
+ * resolved_j_perp_end, final_j_perp. No debug CSV. */
+int orc_sqa_opt_run(const orc_problem *p, int64_t trotter_slices, int64_t replicas, double beta,
+                    double j_perp_start, double j_perp_end, double eps_tilde, double alpha,
+                    int64_t num_steps, int64_t sweeps_per_step, int64_t worldline_sweeps,
+                    int64_t cluster_sweeps, int64_t calib_probes, int64_t calib_sweeps,
+                    double beta_ramp_fraction, double beta_ramp_start, uint64_t seed,
+                    int8_t *best_state, double *best_energy, double *energy_trace,
+                    double *j_perp_trace, double *chi_trace, double *scalars);
+
+/* SQAParallelTemperingAnnealer::run_optimal (sqa_parallel_tempering.cpp:348-882) after
+ * set_seed(seed). final_states [rungs*n], final_energies [rungs], the traces [num_steps],
+ * scalars [4] as above. No debug CSV. */
+int orc_sqapt_opt_run(const orc_problem *p, int64_t rungs, const double *betas, const double *gammas,
+                      int64_t trotter_slices, int64_t num_steps, int64_t sweeps_per_step,
+                      int64_t worldline_sweeps, double eps_tilde, double alpha, double j_perp_end,
+                      int64_t cluster_sweeps, int64_t swap_interval, int64_t continuous_time_slices,
+                      int64_t calib_probes, int64_t calib_sweeps, uint64_t seed, int8_t *final_states,
+                      double *final_energies, int8_t *best_state, double *best_energy,
+                      double *avg_trace, double *swap_trace, double *j_perp_trace, double *chi_trace,
+                      double *scalars);
+
 /* SQAChiAnnealer::run_chi (sqa_chi_annealer.cpp:146-438) after set_seed(seed).
  * trace/beta_sched/gamma_sched/s_sched: [num_steps]; scan_*: [scan_points];
  * scalars: [s_star, gamma_star, j_perp_star, chi_floor, driver_A0]. */

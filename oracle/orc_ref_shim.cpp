@@ -231,6 +231,72 @@ int orc_sqapt_run(const orc_problem *p, int64_t rungs, const double *betas, cons
     });
 }
 
+int orc_sqa_opt_run(const orc_problem *p, int64_t trotter_slices, int64_t replicas, double beta,
+                    double j_perp_start, double j_perp_end, double eps_tilde, double alpha,
+                    int64_t num_steps, int64_t sweeps_per_step, int64_t worldline_sweeps,
+                    int64_t cluster_sweeps, int64_t calib_probes, int64_t calib_sweeps,
+                    double beta_ramp_fraction, double beta_ramp_start, uint64_t seed,
+                    int8_t *best_state, double *best_energy, double *energy_trace,
+                    double *j_perp_trace, double *chi_trace, double *scalars) {
+    return guard([&] {
+        auto ham = build(p);
+        auto sched = qanneal::SQASchedule::from_vectors({beta}, {1.0});  // unused by run_optimal
+        qanneal::SQAAnnealer ann(*ham, sched, static_cast<std::size_t>(trotter_slices),
+                                 static_cast<std::size_t>(replicas));
+        ann.set_seed(seed);
+        auto res = ann.run_optimal(beta, j_perp_start, j_perp_end, eps_tilde, alpha,
+                                   static_cast<std::size_t>(num_steps), static_cast<std::size_t>(sweeps_per_step),
+                                   static_cast<std::size_t>(worldline_sweeps),
+                                   static_cast<std::size_t>(cluster_sweeps), static_cast<std::size_t>(calib_probes),
+                                   static_cast<std::size_t>(calib_sweeps), "", beta_ramp_fraction, beta_ramp_start);
+        put_state(res.best_state, best_state);
+        *best_energy = res.best_energy;
+        std::memcpy(energy_trace, res.energy_trace.data(), num_steps * sizeof(double));
+        std::memcpy(j_perp_trace, res.j_perp_trace.data(), num_steps * sizeof(double));
+        std::memcpy(chi_trace, res.chi_B_trace.data(), num_steps * sizeof(double));
+        scalars[0] = res.calibrated_eps_tilde;
+        scalars[1] = res.j_perp_start;
+        scalars[2] = res.resolved_j_perp_end;
+        scalars[3] = res.final_j_perp;
+    });
+}
+
+int orc_sqapt_opt_run(const orc_problem *p, int64_t rungs, const double *betas, const double *gammas,
+                      int64_t trotter_slices, int64_t num_steps, int64_t sweeps_per_step,
+                      int64_t worldline_sweeps, double eps_tilde, double alpha, double j_perp_end,
+                      int64_t cluster_sweeps, int64_t swap_interval, int64_t continuous_time_slices,
+                      int64_t calib_probes, int64_t calib_sweeps, uint64_t seed, int8_t *final_states,
+                      double *final_energies, int8_t *best_state, double *best_energy,
+                      double *avg_trace, double *swap_trace, double *j_perp_trace, double *chi_trace,
+                      double *scalars) {
+    return guard([&] {
+        auto ham = build(p);
+        qanneal::SQAParallelTemperingAnnealer ann(*ham, std::vector<double>(betas, betas + rungs),
+                                                  std::vector<double>(gammas, gammas + rungs),
+                                                  static_cast<std::size_t>(trotter_slices));
+        ann.set_seed(seed);
+        auto res = ann.run_optimal(static_cast<std::size_t>(num_steps), static_cast<std::size_t>(sweeps_per_step),
+                                   static_cast<std::size_t>(worldline_sweeps), eps_tilde, alpha, j_perp_end,
+                                   static_cast<std::size_t>(cluster_sweeps), static_cast<std::size_t>(swap_interval),
+                                   static_cast<std::size_t>(continuous_time_slices),
+                                   static_cast<std::size_t>(calib_probes), static_cast<std::size_t>(calib_sweeps), "");
+        for (int64_t r = 0; r < rungs; ++r) {
+            put_state(res.final_states[r], final_states + r * p->n);
+            final_energies[r] = res.final_energies[r];
+        }
+        put_state(res.best_state, best_state);
+        *best_energy = res.best_energy;
+        std::memcpy(avg_trace, res.average_energy_trace.data(), num_steps * sizeof(double));
+        std::memcpy(swap_trace, res.swap_acceptance_trace.data(), num_steps * sizeof(double));
+        std::memcpy(j_perp_trace, res.j_perp_trace.data(), num_steps * sizeof(double));
+        std::memcpy(chi_trace, res.chi_B_trace.data(), num_steps * sizeof(double));
+        scalars[0] = res.calibrated_eps_tilde;
+        scalars[1] = res.j_perp_start;
+        scalars[2] = res.resolved_j_perp_end;
+        scalars[3] = res.final_j_perp;
+    });
+}
+
 int orc_sqa_chi_run(const orc_problem *p, int64_t trotter_slices, int64_t replicas, double beta,
                     double gamma_start, double gamma_end, int64_t num_steps,
                     int64_t sweeps_per_step, int64_t scan_points, int64_t scan_sweeps,

@@ -341,6 +341,157 @@ struct SqaReplica {
     }
 };
 
+// One rung's sweeps in SQA parallel tempering: no field cache, the classical dE of every attempt
+// is the O(n) delta_energy — sqa_parallel_tempering.cpp:208-296 (run) and :468-588 (run_optimal,
+// same arithmetic with one shared J_perp).
+struct PtReplica {
+    const Model *m;
+    std::size_t n, P;
+    int8_t *spins;  // [P][n]
+    std::vector<std::size_t> *order;
+    std::vector<char> *in_cluster;
+    std::vector<std::size_t> *stack;
+
+    int8_t *sl(std::size_t t) const { return spins + t * n; }
+
+    void slice_sweep(Rng &g, double bs, double jp) {
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        std::shuffle(order->begin(), order->end(), g);
+        for (std::size_t t = 0; t < P; ++t) {
+            int8_t *cur = sl(t);
+            const int8_t *prv = sl(t == 0 ? P - 1 : t - 1), *nxt = sl((t + 1) % P);
+            for (std::size_t q = 0; q < n; ++q) {
+                const std::size_t i = (*order)[q];
+                const double dcl = m->delta(cur, i);
+                const double dtr = 2.0 * jp * static_cast<double>(cur[i]) * static_cast<double>(prv[i] + nxt[i]);
+                const double dt = std::fma(bs, dcl, dtr);  // :215
+                if (dt <= 0.0 || uni(g) < std::exp(-dt)) cur[i] = static_cast<int8_t>(-cur[i]);
+            }
+        }
+    }
+
+    void cluster_sweep(Rng &g, double bs, double jp, double join) {
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        std::uniform_int_distribution<std::size_t> pick(0, P - 1);
+        std::vector<char> &inc = *in_cluster;
+        std::vector<std::size_t> &sk = *stack;
+        std::shuffle(order->begin(), order->end(), g);
+        for (std::size_t q = 0; q < n; ++q) {
+            const std::size_t i = (*order)[q];
+            const std::size_t sd = pick(g);
+            std::fill(inc.begin(), inc.end(), 0);
+            sk.clear();
+            sk.push_back(sd);
+            inc[sd] = 1;
+            const int8_t s0 = sl(sd)[i];
+            while (!sk.empty()) {
+                const std::size_t t = sk.back();
+                sk.pop_back();
+                const std::size_t pv = t == 0 ? P - 1 : t - 1, nx = (t + 1) % P;
+                if (!inc[pv] && sl(pv)[i] == s0 && uni(g) < join) { inc[pv] = 1; sk.push_back(pv); }
+                if (!inc[nx] && sl(nx)[i] == s0 && uni(g) < join) { inc[nx] = 1; sk.push_back(nx); }
+            }
+            double dcl = 0.0, dtm = 0.0;
+            for (std::size_t t = 0; t < P; ++t) {
+                if (!inc[t]) continue;
+                dcl += m->delta(sl(t), i);
+                const std::size_t pv = t == 0 ? P - 1 : t - 1, nx = (t + 1) % P;
+                const double st_ = static_cast<double>(sl(t)[i]);
+                if (!inc[pv]) dtm += 2.0 * jp * st_ * static_cast<double>(sl(pv)[i]);
+                if (!inc[nx]) dtm += 2.0 * jp * st_ * static_cast<double>(sl(nx)[i]);
+            }
+            const double dt = std::fma(bs, dcl, dtm);  // :268
+            if (dt <= 0.0 || uni(g) < std::exp(-dt))
+                for (std::size_t t = 0; t < P; ++t)
+                    if (inc[t]) sl(t)[i] = static_cast<int8_t>(-sl(t)[i]);
+        }
+    }
+
+    void worldline_sweep(Rng &g, double bs) {
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        std::shuffle(order->begin(), order->end(), g);
+        for (std::size_t q = 0; q < n; ++q) {
+            const std::size_t i = (*order)[q];
+            double dcl = 0.0;
+            for (std::size_t t = 0; t < P; ++t) dcl += m->delta(sl(t), i);
+            const double dt = bs * dcl;
+            if (dt <= 0.0 || uni(g) < std::exp(-dt))
+                for (std::size_t t = 0; t < P; ++t) sl(t)[i] = static_cast<int8_t>(-sl(t)[i]);
+        }
+    }
+};
+
+// B = sum_t sum_i s_t,i s_t+1,i of one replica (sqa_annealer.cpp:483-497): an exact integer.
+double bond_sum(const int8_t *s, std::size_t P, std::size_t n) {
+    double sum = 0.0;
+    for (std::size_t t = 0; t < P; ++t) {
+        const int8_t *a = s + t * n, *b = s + ((t + 1) % P) * n;
+        for (std::size_t i = 0; i < n; ++i) sum += static_cast<double>(a[i]) * static_cast<double>(b[i]);
+    }
+    return sum;
+}
+
+// Var(B) over every replica's samples, no floor (sqa_annealer.cpp:729-757; parallel tempering
+// :597-615): b2m - bm*bm contracts to one fma in the reference build. fallback_b: B used when
+// fewer than two samples exist.
+double var_b(const std::vector<std::vector<double>> &samples, double nM, double fallback_b) {
+    double sum = 0.0, sum2 = 0.0;
+    std::size_t total = 0;
+    for (const auto &v : samples)
+        for (double b : v) {
+            sum += b;
+            sum2 += b * b;
+            ++total;
+        }
+    if (total >= 2) {
+        const double bm = sum / static_cast<double>(total);
+        const double b2m = sum2 / static_cast<double>(total);
+        return std::fma(-bm, bm, b2m);
+    }
+    const double b_avg = fallback_b / nM;
+    return nM * std::fma(-b_avg, b_avg, 1.0);
+}
+
+// chi_B(J) between calibration probes, linear and clamped; -1 with no probes (:880-891).
+double profile_at(const std::vector<double> &J, const std::vector<double> &chi, double x) {
+    const std::size_t M = J.size();
+    if (M == 0) return -1.0;
+    if (x <= J.front()) return chi.front();
+    if (x >= J.back()) return chi.back();
+    std::size_t hi = 1;
+    while (hi < M && J[hi] < x) ++hi;
+    const std::size_t lo = hi - 1;
+    const double t = (x - J[lo]) / (J[hi] - J[lo]);
+    return std::fma(t, chi[hi] - chi[lo], chi[lo]);
+}
+
+// Inverse-CDF trajectory over a 4096-cell trapezoid grid of max(chi, floor)^alpha (:895-921).
+std::vector<double> trajectory(const std::vector<double> &J, const std::vector<double> &chi, double j0, double j1,
+                               double floor_chi, double alpha, std::size_t count) {
+    const std::size_t GRID = 4096;
+    const double h = (j1 - j0) / static_cast<double>(GRID);
+    std::vector<double> G(GRID + 1, 0.0);
+    double prev = std::pow(std::max(profile_at(J, chi, j0), floor_chi), alpha);
+    for (std::size_t k = 1; k <= GRID; ++k) {
+        const double Jk = std::fma(static_cast<double>(k), h, j0);
+        const double cur = std::pow(std::max(profile_at(J, chi, Jk), floor_chi), alpha);
+        G[k] = std::fma(0.5 * (prev + cur), h, G[k - 1]);
+        prev = cur;
+    }
+    const double total = G[GRID];
+    std::vector<double> out(count);
+    std::size_t k = 0;
+    const double denom = count > 1 ? static_cast<double>(count - 1) : 1.0;
+    for (std::size_t t = 0; t < count; ++t) {
+        const double target = total > 0.0 ? (static_cast<double>(t) / denom) * total : 0.0;
+        while (k < GRID && G[k + 1] < target) ++k;
+        const double g0 = G[k], g1 = G[k + 1];
+        const double frac = g1 > g0 ? (target - g0) / (g1 - g0) : 0.0;
+        out[t] = std::fma(static_cast<double>(k) + frac, h, j0);
+    }
+    return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -693,76 +844,13 @@ int orc_sqapt_run(const orc_problem *p, int64_t rungs, const double *betas, cons
 #pragma omp parallel for if (R > 1) schedule(static)
             for (long long rl = 0; rl < static_cast<long long>(R); ++rl) {
                 const std::size_t r = static_cast<std::size_t>(rl);
-                std::vector<int8_t> &s = st[r];
-                Rng &g = rrng[r];
-                std::uniform_real_distribution<double> uni(0.0, 1.0);
-                std::uniform_int_distribution<std::size_t> pick(0, P - 1);
-                const double beta = betas[r], gamma = gammas[r];
-                const double bs = beta / static_cast<double>(P);
-                const double jp = trotter_coupling(beta, gamma, P);
+                const double bs = betas[r] / static_cast<double>(P);
+                const double jp = trotter_coupling(betas[r], gammas[r], P);
                 const double join = 1.0 - std::exp(-2.0 * jp);
-                std::vector<std::size_t> &ord = order[r];
-                auto sl = [&](std::size_t t) { return &s[t * n]; };
-                for (int64_t k = 0; k < sweeps_per_step; ++k) {  // :208-221, O(N) delta, no cache
-                    std::shuffle(ord.begin(), ord.end(), g);
-                    for (std::size_t t = 0; t < P; ++t) {
-                        int8_t *cur = sl(t);
-                        const int8_t *prv = sl(t == 0 ? P - 1 : t - 1), *nxt = sl((t + 1) % P);
-                        for (std::size_t q = 0; q < n; ++q) {
-                            const std::size_t i = ord[q];
-                            const double dcl = m.delta(cur, i);
-                            const double dtr = 2.0 * jp * static_cast<double>(cur[i]) *
-                                               static_cast<double>(prv[i] + nxt[i]);
-                            const double dt = std::fma(bs, dcl, dtr);  // :215
-                            if (dt <= 0.0 || uni(g) < std::exp(-dt)) cur[i] = static_cast<int8_t>(-cur[i]);
-                        }
-                    }
-                }
-                for (int64_t k = 0; k < cluster_sweeps; ++k) {  // :223-278
-                    std::vector<char> &inc = incl[r];
-                    std::vector<std::size_t> &sk = stk[r];
-                    std::shuffle(ord.begin(), ord.end(), g);
-                    for (std::size_t q = 0; q < n; ++q) {
-                        const std::size_t i = ord[q];
-                        const std::size_t sd = pick(g);
-                        std::fill(inc.begin(), inc.end(), 0);
-                        sk.clear();
-                        sk.push_back(sd);
-                        inc[sd] = 1;
-                        const int8_t s0 = sl(sd)[i];
-                        while (!sk.empty()) {
-                            const std::size_t t = sk.back();
-                            sk.pop_back();
-                            const std::size_t pv = t == 0 ? P - 1 : t - 1, nx = (t + 1) % P;
-                            if (!inc[pv] && sl(pv)[i] == s0 && uni(g) < join) { inc[pv] = 1; sk.push_back(pv); }
-                            if (!inc[nx] && sl(nx)[i] == s0 && uni(g) < join) { inc[nx] = 1; sk.push_back(nx); }
-                        }
-                        double dcl = 0.0, dtm = 0.0;
-                        for (std::size_t t = 0; t < P; ++t) {
-                            if (!inc[t]) continue;
-                            dcl += m.delta(sl(t), i);
-                            const std::size_t pv = t == 0 ? P - 1 : t - 1, nx = (t + 1) % P;
-                            const double st_ = static_cast<double>(sl(t)[i]);
-                            if (!inc[pv]) dtm += 2.0 * jp * st_ * static_cast<double>(sl(pv)[i]);
-                            if (!inc[nx]) dtm += 2.0 * jp * st_ * static_cast<double>(sl(nx)[i]);
-                        }
-                        const double dt = std::fma(bs, dcl, dtm);  // :268
-                        if (dt <= 0.0 || uni(g) < std::exp(-dt))
-                            for (std::size_t t = 0; t < P; ++t)
-                                if (inc[t]) sl(t)[i] = static_cast<int8_t>(-sl(t)[i]);
-                    }
-                }
-                for (int64_t k = 0; k < worldline_sweeps; ++k) {  // :280-296
-                    std::shuffle(ord.begin(), ord.end(), g);
-                    for (std::size_t q = 0; q < n; ++q) {
-                        const std::size_t i = ord[q];
-                        double dcl = 0.0;
-                        for (std::size_t t = 0; t < P; ++t) dcl += m.delta(sl(t), i);
-                        const double dt = bs * dcl;
-                        if (dt <= 0.0 || uni(g) < std::exp(-dt))
-                            for (std::size_t t = 0; t < P; ++t) sl(t)[i] = static_cast<int8_t>(-sl(t)[i]);
-                    }
-                }
+                PtReplica rep{&m, n, P, st[r].data(), &order[r], &incl[r], &stk[r]};
+                for (int64_t k = 0; k < sweeps_per_step; ++k) rep.slice_sweep(rrng[r], bs, jp);      // :208-221
+                for (int64_t k = 0; k < cluster_sweeps; ++k) rep.cluster_sweep(rrng[r], bs, jp, join);  // :223-278
+                for (int64_t k = 0; k < worldline_sweeps; ++k) rep.worldline_sweep(rrng[r], bs);    // :280-296
             }
             for (std::size_t r = 0; r < R; ++r) {  // :299-306
                 std::size_t bt;
@@ -990,6 +1078,314 @@ int orc_sqa_chi_run(const orc_problem *p, int64_t trotter_slices, int64_t replic
         }
         std::memcpy(best_state, best.data(), n);
         *best_energy = best_e;
+    });
+}
+
+// sqa_annealer.cpp:499-952
+int orc_sqa_opt_run(const orc_problem *p, int64_t trotter_slices, int64_t replicas, double beta,
+                    double j_perp_start, double j_perp_end, double eps_tilde, double alpha,
+                    int64_t num_steps, int64_t sweeps_per_step, int64_t worldline_sweeps,
+                    int64_t cluster_sweeps, int64_t calib_probes, int64_t calib_sweeps,
+                    double beta_ramp_fraction, double beta_ramp_start, uint64_t seed,
+                    int8_t *best_state, double *best_energy, double *energy_trace,
+                    double *j_perp_trace, double *chi_trace, double *scalars) {
+    return guard([&] {
+        if (num_steps <= 0) throw std::invalid_argument("num_steps must be > 0.");
+        if (sweeps_per_step == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+            throw std::invalid_argument(
+                "At least one of sweeps_per_step, worldline_sweeps, or cluster_sweeps must be > 0.");
+        if (j_perp_end <= j_perp_start) throw std::invalid_argument("j_perp_end must be > j_perp_start.");
+        const bool calibrate = eps_tilde <= 0.0;
+        const Model m = build(p);
+        const std::size_t n = m.n, P = static_cast<std::size_t>(trotter_slices), R = static_cast<std::size_t>(replicas);
+        const std::size_t steps = static_cast<std::size_t>(num_steps);
+        const double nM = static_cast<double>(n * P);
+        double floor_chi = 1e-4 * nM;
+        Rng rng(seed);
+        std::vector<int8_t> spins(R * P * n);
+        random_spins(spins, rng);  // :534
+        std::vector<Rng> rrng(R);
+        for (auto &g : rrng) g.seed(rng());  // :541-544
+        std::vector<double> fields(R * P * n);
+        for (std::size_t k = 0; k < R * P; ++k) m.fields(&spins[k * n], &fields[k * n]);
+        std::vector<std::vector<std::size_t>> order(R, std::vector<std::size_t>(n));
+        for (auto &o : order) std::iota(o.begin(), o.end(), std::size_t{0});
+        std::vector<std::vector<char>> incl(R, std::vector<char>(P));
+        std::vector<std::vector<std::size_t>> stk(R);
+        std::vector<std::vector<double>> samples(R);
+
+        // one step of every replica's sweeps, one B sample per slice and cluster sweep (:578-726)
+        auto sweep = [&](double jp, double bs, int64_t nm, int64_t nc, int64_t nw) {
+            const double join = 1.0 - std::exp(-2.0 * jp);
+#pragma omp parallel for if (R > 1) schedule(static)
+            for (long long rr = 0; rr < static_cast<long long>(R); ++rr) {
+                const std::size_t r = static_cast<std::size_t>(rr);
+                SqaReplica rep{&m, n, P, &spins[r * P * n], &fields[r * P * n], &order[r], &incl[r], &stk[r]};
+                samples[r].clear();
+                for (int64_t k = 0; k < nm; ++k) {
+                    rep.slice_sweep(rrng[r], bs, jp);
+                    samples[r].push_back(bond_sum(rep.spins, P, n));
+                }
+                for (int64_t k = 0; k < nc; ++k) {
+                    rep.cluster_sweep(rrng[r], bs, jp, join);
+                    samples[r].push_back(bond_sum(rep.spins, P, n));
+                }
+                for (int64_t k = 0; k < nw; ++k) rep.worldline_sweep(rrng[r], bs);
+            }
+        };
+        auto chi_raw = [&]() {  // :729-757 with the single-replica guard
+            double var = var_b(samples, nM, bond_sum(spins.data(), P, n));
+            if (R == 1 && !samples[0].empty()) {
+                const double b_avg = samples[0].back() / nM;
+                var = std::max(var, nM * std::fma(-b_avg, b_avg, 1.0));
+            }
+            return var;
+        };
+        double best_e = std::numeric_limits<double>::infinity();
+        std::vector<int8_t> best(n, 1);
+        std::size_t rec = 0;
+        auto record = [&](double jp, double chi) {  // :761-776
+            double avg = 0.0;
+            for (std::size_t k = 0; k < R * P; ++k) {
+                const double e = m.energy(&spins[k * n]);
+                avg += e;
+                if (e < best_e) {
+                    best_e = e;
+                    std::memcpy(best.data(), &spins[k * n], n);
+                }
+            }
+            energy_trace[rec] = avg / static_cast<double>(R * P);
+            j_perp_trace[rec] = jp;
+            chi_trace[rec] = chi;
+            ++rec;
+        };
+        const double bs_cold = beta / static_cast<double>(P);
+        std::size_t n1 = 0;
+        if (beta_ramp_fraction > 0.0) {  // :784-789
+            n1 = static_cast<std::size_t>(beta_ramp_fraction * static_cast<double>(steps));
+            if (n1 >= steps) n1 = steps - 1;
+        }
+        const std::size_t n2 = steps - n1;
+        for (std::size_t k = 0; k < n1; ++k) {  // :801-813
+            const double frac = n1 > 1 ? static_cast<double>(k) / static_cast<double>(n1 - 1) : 1.0;
+            const double beta_t = std::fma(beta - beta_ramp_start, frac, beta_ramp_start);
+            sweep(j_perp_start, beta_t / static_cast<double>(P), sweeps_per_step, cluster_sweeps, worldline_sweeps);
+            record(j_perp_start, std::max(chi_raw(), 0.0));
+        }
+        std::vector<double> pJ, pchi;
+        double eps = eps_tilde;
+        if (calibrate) {  // :820-876
+            const std::size_t M = std::max<std::size_t>(static_cast<std::size_t>(calib_probes), 2);
+            const int64_t cs = std::max<int64_t>(calib_sweeps, 1);
+            const int64_t pcl = cluster_sweeps > 0 ? std::max<int64_t>(cluster_sweeps, 1) : 0;
+            const std::vector<int8_t> saved_s = spins;
+            const std::vector<double> saved_f = fields;
+            const double dJ = j_perp_end - j_perp_start;
+            const double dJp = dJ / static_cast<double>(M);
+            pJ.resize(M);
+            pchi.assign(M, 0.0);
+            double integral = 0.0;
+            for (std::size_t q = 0; q < M; ++q) {
+                const double Jm = std::fma(static_cast<double>(q) + 0.5, dJp, j_perp_start);
+                sweep(Jm, bs_cold, cs, pcl, 0);
+                double chi = chi_raw();
+                if (!(chi > 0.0)) chi = 0.0;
+                pJ[q] = Jm;
+                pchi[q] = chi;
+                integral = std::fma(std::pow(std::max(chi, 1e-12), alpha), dJp, integral);
+            }
+            spins = saved_s;
+            fields = saved_f;
+            eps = integral / static_cast<double>(n2);
+            if (!(eps > 0.0) || !std::isfinite(eps)) {
+                eps = std::max(dJ, 0.0) / static_cast<double>(steps);
+                if (!(eps > 0.0)) eps = 1e-9;
+            }
+            floor_chi = std::max(1e-4 * *std::max_element(pchi.begin(), pchi.end()), 1e-12);
+        }
+        std::vector<double> traj;
+        if (calibrate && !pJ.empty() && j_perp_end > j_perp_start)
+            traj = trajectory(pJ, pchi, j_perp_start, j_perp_end, floor_chi, alpha, n2);
+        double j = j_perp_start;
+        for (std::size_t k = 0; k < n2; ++k) {  // :925-949
+            if (!traj.empty()) j = traj[k];
+            sweep(j, bs_cold, sweeps_per_step, cluster_sweeps, worldline_sweeps);
+            const double chi_online = chi_raw();
+            const double chi_prof = profile_at(pJ, pchi, j);
+            const double raw = chi_prof >= 0.0 ? chi_prof : chi_online;
+            const double drive = std::max(raw, floor_chi);
+            const double dj = eps * std::pow(drive, -alpha);
+            record(j, drive);
+            if (traj.empty()) j = std::min(j + dj, j_perp_end);
+        }
+        std::memcpy(best_state, best.data(), n);
+        *best_energy = best_e;
+        scalars[0] = eps;
+        scalars[1] = j_perp_start;
+        scalars[2] = j_perp_end;
+        scalars[3] = j;
+    });
+}
+
+// sqa_parallel_tempering.cpp:348-882
+int orc_sqapt_opt_run(const orc_problem *p, int64_t rungs, const double *betas, const double *gammas,
+                      int64_t trotter_slices, int64_t num_steps, int64_t sweeps_per_step,
+                      int64_t worldline_sweeps, double eps_tilde, double alpha, double j_perp_end,
+                      int64_t cluster_sweeps, int64_t swap_interval, int64_t continuous_time_slices,
+                      int64_t calib_probes, int64_t calib_sweeps, uint64_t seed, int8_t *final_states,
+                      double *final_energies, int8_t *best_state, double *best_energy,
+                      double *avg_trace, double *swap_trace, double *j_perp_trace, double *chi_trace,
+                      double *scalars) {
+    return guard([&] {
+        if (rungs < 2) throw std::invalid_argument("SQA parallel tempering requires >=2 beta/gamma pairs.");
+        if (num_steps <= 0) throw std::invalid_argument("num_steps must be > 0.");
+        if (swap_interval <= 0) throw std::invalid_argument("swap_interval must be > 0.");
+        if (sweeps_per_step == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+            throw std::invalid_argument("At least one of sweeps_per_step/worldline_sweeps/cluster_sweeps must be > 0.");
+        const bool calibrate = eps_tilde <= 0.0;
+        const Model m = build(p);
+        const std::size_t n = m.n, R = static_cast<std::size_t>(rungs), steps = static_cast<std::size_t>(num_steps);
+        const std::size_t P = (continuous_time_slices > 0 && continuous_time_slices > trotter_slices)
+                                  ? static_cast<std::size_t>(continuous_time_slices)
+                                  : static_cast<std::size_t>(trotter_slices);
+        const double nM = static_cast<double>(n * P);
+        const double j0 = trotter_coupling(betas[0], gammas[0], P);  // :375-377
+        double j = j0, j1 = j_perp_end;
+        if (j1 <= 0.0) j1 = trotter_coupling(betas[R - 1], gammas[R - 1], P);
+        if (j1 <= j) j1 = j + 1e6;
+        Rng rng(seed);
+        std::vector<std::vector<int8_t>> st(R, std::vector<int8_t>(P * n));
+        for (auto &s : st) random_spins(s, rng);  // :406-409
+        std::vector<Rng> rrng(R);
+        for (auto &g : rrng) g.seed(rng());  // :411-412
+        std::vector<std::vector<std::size_t>> order(R, std::vector<std::size_t>(n));
+        for (auto &o : order) std::iota(o.begin(), o.end(), std::size_t{0});
+        std::vector<std::vector<char>> incl(R, std::vector<char>(P));
+        std::vector<std::vector<std::size_t>> stk(R);
+        std::vector<std::vector<double>> samples(R);
+        std::vector<double> bs(R);
+        for (std::size_t r = 0; r < R; ++r) bs[r] = betas[r] / static_cast<double>(P);
+
+        auto sweep = [&](std::vector<std::vector<int8_t>> &S, std::vector<Rng> &G, double jp, int64_t nm, int64_t nc,
+                         int64_t nw) {  // :454-592
+            const double join = 1.0 - std::exp(-2.0 * jp);
+#pragma omp parallel for if (R > 1) schedule(static)
+            for (long long rl = 0; rl < static_cast<long long>(R); ++rl) {
+                const std::size_t r = static_cast<std::size_t>(rl);
+                PtReplica rep{&m, n, P, S[r].data(), &order[r], &incl[r], &stk[r]};
+                samples[r].clear();
+                for (int64_t k = 0; k < nm; ++k) {
+                    rep.slice_sweep(G[r], bs[r], jp);
+                    samples[r].push_back(bond_sum(rep.spins, P, n));
+                }
+                for (int64_t k = 0; k < nc; ++k) {
+                    rep.cluster_sweep(G[r], bs[r], jp, join);
+                    samples[r].push_back(bond_sum(rep.spins, P, n));
+                }
+                for (int64_t k = 0; k < nw; ++k) rep.worldline_sweep(G[r], bs[r]);
+            }
+        };
+        auto proj = [&](const std::vector<int8_t> &s, std::size_t *bt) {  // :442-451
+            double b = std::numeric_limits<double>::infinity();
+            std::size_t t0 = 0;
+            for (std::size_t t = 0; t < P; ++t) {
+                const double e = m.energy(&s[t * n]);
+                if (e < b) { b = e; t0 = t; }
+            }
+            *bt = t0;
+            return b;
+        };
+        auto csum = [&](const std::vector<int8_t> &s) {  // :434-439
+            double c = 0.0;
+            for (std::size_t t = 0; t < P; ++t) c += m.energy(&s[t * n]);
+            return c;
+        };
+        double best_e = std::numeric_limits<double>::infinity();
+        std::vector<int8_t> best(n, 1);
+        std::vector<double> en(R);
+        auto take_best = [&]() {
+            for (std::size_t r = 0; r < R; ++r) {
+                std::size_t bt;
+                en[r] = proj(st[r], &bt);
+                if (en[r] < best_e) { best_e = en[r]; std::memcpy(best.data(), &st[r][bt * n], n); }
+            }
+        };
+        take_best();  // :624-631
+        double floor_chi = 1e-4 * nM, eps = eps_tilde;
+        std::vector<double> pJ, pchi;
+        if (calibrate) {  // :652-711
+            const std::size_t M = std::max<std::size_t>(static_cast<std::size_t>(calib_probes), 2);
+            const int64_t cs = std::max<int64_t>(calib_sweeps, 1);
+            const int64_t pcl = cluster_sweeps > 0 ? std::max<int64_t>(cluster_sweeps, 1) : 0;
+            // scratch states and engines; the visit orders are the run's own, shuffled in place
+            std::vector<std::vector<int8_t>> scratch = st;
+            std::vector<Rng> srng(R);
+            for (auto &g : srng) g.seed(rng());
+            const double dJ = j1 - j0;
+            const double dJp = dJ / static_cast<double>(M);
+            pJ.resize(M);
+            pchi.assign(M, 0.0);
+            double integral = 0.0;
+            for (std::size_t q = 0; q < M; ++q) {
+                const double Jm = std::fma(static_cast<double>(q) + 0.5, dJp, j0);
+                sweep(scratch, srng, Jm, cs, pcl, 0);
+                double chi = var_b(samples, nM, bond_sum(scratch[0].data(), P, n));
+                if (!(chi > 0.0)) chi = 0.0;
+                pJ[q] = Jm;
+                pchi[q] = chi;
+                integral = std::fma(std::pow(std::max(chi, 1e-12), alpha), dJp, integral);
+            }
+            eps = integral / static_cast<double>(steps);
+            if (!(eps > 0.0) || !std::isfinite(eps)) {
+                eps = std::max(dJ, 0.0) / static_cast<double>(steps);
+                if (!(eps > 0.0)) eps = 1e-9;
+            }
+            floor_chi = std::max(1e-4 * *std::max_element(pchi.begin(), pchi.end()), 1e-12);
+        }
+        std::vector<double> traj;
+        if (calibrate && !pJ.empty() && j1 > j0) traj = trajectory(pJ, pchi, j0, j1, floor_chi, alpha, steps);
+        for (std::size_t step = 0; step < steps; ++step) {  // :784-872
+            if (!traj.empty()) j = traj[step];
+            sweep(st, rrng, j, sweeps_per_step, cluster_sweeps, worldline_sweeps);
+            take_best();
+            const double chi_online = var_b(samples, nM, bond_sum(st[0].data(), P, n));
+            const double chi_prof = profile_at(pJ, pchi, j);
+            const double raw = chi_prof >= 0.0 ? chi_prof : chi_online;
+            const double drive = std::max(raw, floor_chi);
+            const double dj = eps * std::pow(drive, -alpha);
+            j_perp_trace[step] = j;
+            chi_trace[step] = drive;
+            if (traj.empty()) j = std::min(j + dj, j1);
+            double acc = 0.0, att = 0.0;
+            if ((step + 1) % static_cast<std::size_t>(swap_interval) == 0) {
+                std::uniform_real_distribution<double> uni(0.0, 1.0);
+                for (std::size_t r = 0; r + 1 < R; ++r) {
+                    const double c0 = csum(st[r]), c1 = csum(st[r + 1]);
+                    const double delta = (bs[r] - bs[r + 1]) * (c1 - c0);
+                    att += 1.0;
+                    if (delta <= 0.0 || uni(rng) < std::exp(-delta)) {
+                        std::swap(st[r], st[r + 1]);
+                        std::swap(en[r], en[r + 1]);
+                        acc += 1.0;
+                    }
+                }
+            }
+            double avg = 0.0;
+            for (double e : en) avg += e;
+            avg_trace[step] = avg / static_cast<double>(R);
+            swap_trace[step] = att > 0.0 ? acc / att : 0.0;
+        }
+        for (std::size_t r = 0; r < R; ++r) {
+            std::size_t bt;
+            final_energies[r] = proj(st[r], &bt);
+            std::memcpy(final_states + r * n, &st[r][bt * n], n);
+        }
+        std::memcpy(best_state, best.data(), n);
+        *best_energy = best_e;
+        scalars[0] = eps;
+        scalars[1] = j0;
+        scalars[2] = j1;
+        scalars[3] = j;
     });
 }
 

@@ -426,9 +426,69 @@ SQAResult SQAAnnealer::run(std::size_t sweeps_per_beta, std::size_t worldline_sw
     return res;
 }
 
-SQAResult SQAAnnealer::run_optimal(double, double, double, double, double, std::size_t, std::size_t, std::size_t,
-                                   std::size_t, std::size_t, std::size_t, const std::string &, double, double) {
-    throw std::runtime_error("run_optimal (the bond-susceptibility adaptive schedule) is not in this build");
+// sqa_annealer.cpp:499-952. Validation draws the start state and replica seeds as run() does
+// (:534-541); the pilot's random draws come from the replicas' own engines.
+SQAResult SQAAnnealer::run_optimal(double beta, double j_perp_start, double j_perp_end, double eps_tilde, double alpha,
+                                   std::size_t num_steps, std::size_t sweeps_per_step, std::size_t worldline_sweeps,
+                                   std::size_t cluster_sweeps, std::size_t calib_probes, std::size_t calib_sweeps,
+                                   const std::string &debug_csv_path, double beta_ramp_fraction,
+                                   double beta_ramp_start) {
+    if (num_steps == 0) throw std::invalid_argument("num_steps must be > 0.");
+    if (sweeps_per_step == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+        throw std::invalid_argument(
+            "At least one of sweeps_per_step, worldline_sweeps, or cluster_sweeps must be > 0.");
+    if (j_perp_end <= j_perp_start) throw std::invalid_argument("j_perp_end must be > j_perp_start.");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), P = slices_, R = replicas_;
+    qa_sqa_opt_params prm{};
+    prm.beta = beta;
+    prm.j_perp_start = j_perp_start;
+    prm.j_perp_end = j_perp_end;
+    prm.eps_tilde = eps_tilde;
+    prm.alpha = alpha;
+    prm.num_steps = static_cast<int64_t>(num_steps);
+    prm.sweeps_per_step = static_cast<int64_t>(sweeps_per_step);
+    prm.worldline_sweeps = static_cast<int64_t>(worldline_sweeps);
+    prm.cluster_sweeps = static_cast<int64_t>(cluster_sweeps);
+    prm.calib_probes = static_cast<int64_t>(calib_probes);
+    prm.calib_sweeps = static_cast<int64_t>(calib_sweeps);
+    prm.beta_ramp_fraction = beta_ramp_fraction;
+    prm.beta_ramp_start = beta_ramp_start;
+    prm.trotter_slices = static_cast<int64_t>(P);
+    prm.replicas = static_cast<int64_t>(R);
+    prm.mode = mode_of(hb);
+    prm.debug_csv_path = debug_csv_path.c_str();
+    SQAState init;
+    std::vector<uint64_t> seeds;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng_();
+    } else {
+        init = SQAState::random(R, P, n, rng_);
+        seeds.resize(R);
+        for (auto &s : seeds) s = rng_();
+        prm.init_spins = init.data();
+        prm.replica_seeds = seeds.data();
+    }
+    SQAResult res;
+    res.best_state = State(n);
+    res.energy_trace.resize(num_steps);
+    res.j_perp_trace.resize(num_steps);
+    res.chi_B_trace.resize(num_steps);
+    double kms = 0.0;
+    qa_sqa_opt_result out{};
+    out.best_state = res.best_state.spins.data();
+    out.best_energy = &res.best_energy;
+    out.energy_trace = res.energy_trace.data();
+    out.sched = qa_opt_schedule{res.j_perp_trace.data(), res.chi_B_trace.data(), &res.calibrated_eps_tilde,
+                                &res.j_perp_start, &res.resolved_j_perp_end, &res.final_j_perp};
+    out.kernel_ms = &kms;
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sqa_optimal_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    last_kernel_ms_ = kms;
+    last_accept_rate_ = 0.0;
+    return res;
 }
 
 // ================================================================================== SQA-PT
@@ -563,11 +623,88 @@ std::vector<SQAParallelTemperingResult> SQAParallelTemperingAnnealer::run_ladder
     return all;
 }
 
-SQAParallelTemperingResult SQAParallelTemperingAnnealer::run_optimal(std::size_t, std::size_t, std::size_t, double,
-                                                                     double, double, std::size_t, std::size_t,
-                                                                     std::size_t, std::size_t, std::size_t,
-                                                                     const std::string &) {
-    throw std::runtime_error("run_optimal (the adaptive shared-J_perp schedule) is not in this build");
+// sqa_parallel_tempering.cpp:348-882. Validation draws, in the reference's order: the rungs'
+// start states (:406-409), the rung seeds (:411-412), the pilot's scratch seeds when calibrating
+// (:655-656); swaps then consume the engine's uniforms, and the engine is advanced by them.
+SQAParallelTemperingResult SQAParallelTemperingAnnealer::run_optimal(
+    std::size_t num_steps, std::size_t sweeps_per_step, std::size_t worldline_sweeps, double eps_tilde, double alpha,
+    double j_perp_end, std::size_t cluster_sweeps, std::size_t swap_interval, std::size_t continuous_time_slices,
+    std::size_t calib_probes, std::size_t calib_sweeps, const std::string &debug_csv_path) {
+    if (num_steps == 0) throw std::invalid_argument("num_steps must be > 0.");
+    if (swap_interval == 0) throw std::invalid_argument("swap_interval must be > 0.");
+    if (sweeps_per_step == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+        throw std::invalid_argument("At least one of sweeps_per_step/worldline_sweeps/cluster_sweeps must be > 0.");
+    HIPBackend &hb = device_backend(backend_);
+    const std::size_t n = hb.size(), R = betas_.size();
+    const std::size_t P =
+        (continuous_time_slices > 0 && continuous_time_slices > slices_) ? continuous_time_slices : slices_;
+    qa_sqapt_opt_params prm{};
+    prm.rungs = static_cast<int64_t>(R);
+    prm.betas = betas_.data();
+    prm.gammas = gammas_.data();
+    prm.trotter_slices = static_cast<int64_t>(P);
+    prm.num_steps = static_cast<int64_t>(num_steps);
+    prm.sweeps_per_step = static_cast<int64_t>(sweeps_per_step);
+    prm.worldline_sweeps = static_cast<int64_t>(worldline_sweeps);
+    prm.cluster_sweeps = static_cast<int64_t>(cluster_sweeps);
+    prm.swap_interval = static_cast<int64_t>(swap_interval);
+    prm.calib_probes = static_cast<int64_t>(calib_probes);
+    prm.calib_sweeps = static_cast<int64_t>(calib_sweeps);
+    prm.eps_tilde = eps_tilde;
+    prm.alpha = alpha;
+    prm.j_perp_end = j_perp_end;
+    prm.mode = mode_of(hb);
+    prm.debug_csv_path = debug_csv_path.c_str();
+    std::vector<int8_t> init;
+    std::vector<uint64_t> seeds, scratch;
+    std::vector<double> swap_u;
+    if (prm.mode == QA_MODE_FP32) {
+        prm.seed = rng_();
+    } else {
+        init.resize(R * P * n);
+        for (std::size_t r = 0; r < R; ++r) {
+            SQAState s = SQAState::random(1, P, n, rng_);
+            std::memcpy(&init[r * P * n], s.data(), P * n);
+        }
+        seeds.resize(R);
+        for (auto &s : seeds) s = rng_();
+        if (eps_tilde <= 0.0) {
+            scratch.resize(R);
+            for (auto &s : scratch) s = rng_();
+        }
+        swap_u = uniform_stream(rng_, (num_steps / swap_interval) * (R - 1));
+        prm.init_spins = init.data();
+        prm.rung_seeds = seeds.data();
+        prm.scratch_seeds = scratch.empty() ? nullptr : scratch.data();
+        prm.swap_uniforms = swap_u.data();
+        prm.swap_uniforms_len = static_cast<int64_t>(swap_u.size());
+    }
+    SQAParallelTemperingResult res;
+    std::vector<int8_t> fs(R * n), bs(n);
+    res.final_energies.resize(R);
+    res.average_energy_trace.resize(num_steps);
+    res.swap_acceptance_trace.resize(num_steps);
+    res.j_perp_trace.resize(num_steps);
+    res.chi_B_trace.resize(num_steps);
+    int64_t used = 0;
+    qa_sqapt_opt_result out{};
+    out.final_states = fs.data();
+    out.final_energies = res.final_energies.data();
+    out.best_state = bs.data();
+    out.best_energy = &res.best_energy;
+    out.average_energy_trace = res.average_energy_trace.data();
+    out.swap_acceptance_trace = res.swap_acceptance_trace.data();
+    out.sched = qa_opt_schedule{res.j_perp_trace.data(), res.chi_B_trace.data(), &res.calibrated_eps_tilde,
+                                &res.j_perp_start, &res.resolved_j_perp_end, &res.final_j_perp};
+    out.swap_uniforms_used = &used;
+    {
+        std::lock_guard<std::mutex> lock(hb.device_mutex());
+        throw_on_status(qa_sqapt_optimal_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    if (prm.mode != QA_MODE_FP32) rng_.discard(static_cast<unsigned long long>(used));
+    for (std::size_t r = 0; r < R; ++r) res.final_states.push_back(state_from(&fs[r * n], n));
+    res.best_state = state_from(bs.data(), n);
+    return res;
 }
 
 // ================================================================================= sqa_chi

@@ -616,6 +616,56 @@ qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *
     });
 }
 
+qa_status qa_sqa_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_opt_params *prm,
+                             qa_sqa_opt_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sqa_optimal_run: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sqa_optimal_run: problem belongs to another context");
+        QA_REQUIRE(res->best_state && res->best_energy && res->energy_trace && res->sched.j_perp_trace &&
+                       res->sched.chi_B_trace, "qa_sqa_optimal_run: NULL result array");
+        QA_REQUIRE(prm->num_steps > 0, "num_steps must be > 0.");
+        QA_REQUIRE(prm->sweeps_per_step >= 0 && prm->worldline_sweeps >= 0 && prm->cluster_sweeps >= 0 &&
+                       prm->calib_probes >= 0 && prm->calib_sweeps >= 0, "sweep counts must be >= 0.");
+        QA_REQUIRE(prm->sweeps_per_step > 0 || prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0,
+                   "At least one of sweeps_per_step, worldline_sweeps, or cluster_sweeps must be > 0.");
+        QA_REQUIRE(prm->j_perp_end > prm->j_perp_start, "j_perp_end must be > j_perp_start.");
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->replicas > 0, "replicas must be > 0.");
+        const double sc[] = {prm->beta, prm->j_perp_start, prm->j_perp_end, prm->eps_tilde, prm->alpha,
+                             prm->beta_ramp_fraction, prm->beta_ramp_start};
+        check_finite(sc, 7, "run_optimal scalars");
+        QA_HIP(hipSetDevice(ctx->device));
+        if (prm->mode != QA_MODE_FP32 && prm->mode != QA_MODE_FP64_VALIDATION) throw Error{QA_EINVAL, "unknown mode"};
+        sqa_optimal(ctx, p, prm, res);
+    });
+}
+
+qa_status qa_sqapt_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_opt_params *prm,
+                               qa_sqapt_opt_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sqapt_optimal_run: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sqapt_optimal_run: problem belongs to another context");
+        QA_REQUIRE(res->final_states && res->final_energies && res->best_state && res->best_energy &&
+                       res->average_energy_trace && res->swap_acceptance_trace && res->sched.j_perp_trace &&
+                       res->sched.chi_B_trace, "qa_sqapt_optimal_run: NULL result array");
+        QA_REQUIRE(prm->rungs >= 2 && prm->betas && prm->gammas, "SQA parallel tempering requires >=2 beta/gamma pairs.");
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->num_steps > 0, "num_steps must be > 0.");
+        QA_REQUIRE(prm->swap_interval > 0, "swap_interval must be > 0.");
+        QA_REQUIRE(prm->sweeps_per_step >= 0 && prm->worldline_sweeps >= 0 && prm->cluster_sweeps >= 0 &&
+                       prm->calib_probes >= 0 && prm->calib_sweeps >= 0, "sweep counts must be >= 0.");
+        QA_REQUIRE(prm->sweeps_per_step > 0 || prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0,
+                   "At least one of sweeps_per_step/worldline_sweeps/cluster_sweeps must be > 0.");
+        check_finite(prm->betas, prm->rungs, "betas");
+        check_finite(prm->gammas, prm->rungs, "gammas");
+        const double sc[] = {prm->eps_tilde, prm->alpha, prm->j_perp_end};
+        check_finite(sc, 3, "run_optimal scalars");
+        QA_HIP(hipSetDevice(ctx->device));
+        if (prm->mode != QA_MODE_FP32 && prm->mode != QA_MODE_FP64_VALIDATION) throw Error{QA_EINVAL, "unknown mode"};
+        sqapt_optimal(ctx, p, prm, res);
+    });
+}
+
 qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
     return guarded([&] {
         QA_REQUIRE(ctx && p && prm && res, "qa_sqa_chi_run: NULL argument");

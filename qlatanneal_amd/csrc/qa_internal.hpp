@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <tuple>
 
 #include <cstdint>
@@ -217,6 +218,8 @@ void fields_fp64_launch(const qa_problem *p, const int8_t *d_spins, double *d_fi
                         hipStream_t s);
 
 // Engine entry points (implemented per engine file).
+void sqa_optimal(qa_ctx *ctx, const qa_problem *p, const qa_sqa_opt_params *prm, qa_sqa_opt_result *res);
+void sqapt_optimal(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_opt_params *prm, qa_sqapt_opt_result *res);
 void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
 void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
 void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
@@ -250,5 +253,94 @@ void mlp_loss_grad(qa_mlp *m, int64_t B, const double *x, const float *t, double
 void mlp_train(qa_mlp *m, int64_t S, const double *x, const float *t, const qa_mlp_train_params *tp, double *history);
 void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *problems, const qa_batch_params *prm,
                qa_batch_result *res);
+
+// ---- adaptive J_perp schedules (run_optimal) -----------------------------------------------
+// B statistics of one step: sums of B and B^2 over every replica's samples (one sample after
+// each slice and each cluster sweep; B = sum_t sum_i s_t,i s_t+1,i is an integer, so the sums
+// are exact), the sample count, and B of replica 0 (buffer 0) at the end of the step.
+struct BondStats {
+    double sum = 0.0, sum2 = 0.0;
+    int64_t count = 0;
+    double b0 = 0.0;
+};
+
+// FP32 SQA state kept on the device across launches. run() sweeps k steps with per-step
+// (beta/P, J_perp), or one step with per-replica parameters (param_of_rep[r] indexes bs/jp);
+// tail 0 records the step ends (best state, mean slice energy), tail 1 leaves per-chain slice
+// energies in chain_e (FP64 sums over the FP32 fields), tail 2 records nothing. The Philox
+// streams continue from launch to launch.
+class SqaFp32Session {
+public:
+    SqaFp32Session(qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, uint64_t key);
+    void copy_state_from(const SqaFp32Session &o);
+    void run(int64_t k, const double *bs, const double *jp, int64_t nparam, const int *param_of_rep, int64_t spb,
+             int64_t cl, int64_t wl, int tail, BondStats *stats, double *trace);
+    void save();
+    void restore();
+    // best slice state over the tail-0 steps and its FP64 energy recomputed on the device
+    void best(int8_t *state, double *energy);
+    const int8_t *device_spins() const { return gS_.ptr; }
+    std::vector<double> chain_e;  // tail 1: [R*P]
+    std::vector<int> chain_b;     // tail 1: [R*P] sum_i s_t,i s_t+1,i per chain
+    float ms = 0.0f;
+
+private:
+    qa_ctx *ctx_;
+    const qa_problem *p_;
+    int64_t R_, P_, rg_, ngroups_;
+    bool use_lds_;
+    uint64_t key_;
+    uint32_t sweeps_ = 0;
+    int64_t recorded_ = 0;
+    bool started_ = false;
+    DevBuf<int8_t> gS_, sS_, bstate_;
+    DevBuf<float> gF_, sF_;
+    DevBuf<double> be_, ce_;
+    DevBuf<int> bstep_, bchain_, cb_;
+};
+
+// FP64 validation replay of SQAAnnealer's sweeps (sqa_annealer.cpp:176-315) with the caller's
+// per-replica engines; the host draws the visit orders and uniform streams and advances the
+// engines by what the device consumed.
+class SqaValSession {
+public:
+    SqaValSession(qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, const int8_t *init_spins,
+                  const uint64_t *seeds);
+    void phase(int ph, double bs, double jp, double join);
+    void bonds(std::vector<int64_t> &b);  // B of every replica from the current spins
+    double tail();                        // step end: best update (first strict minimum), mean energy
+    void save();
+    void restore();
+    std::vector<int8_t> best;
+    double best_e;
+    float ms = 0.0f;
+    static constexpr int kSlice = 0, kCluster = 1, kWorldline = 2;
+
+private:
+    struct Impl;
+    std::shared_ptr<Impl> d_;
+};
+
+// FP64 validation replay of SQAParallelTemperingAnnealer's per-rung sweeps
+// (sqa_parallel_tempering.cpp:208-296): rung r keeps its engine and visit order while states
+// move between rungs on swaps (slot[r] = buffer holding rung r).
+class PtValSession {
+public:
+    PtValSession(qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, const int8_t *d_or_h_spins, bool device_src,
+                 const uint64_t *seeds);
+    void phase(int ph, const double *bs, const double *jp, const double *join);
+    void bonds(std::vector<int64_t> &b);    // by rung
+    void energies(std::vector<double> &e);  // [rung][slice] FP64 slice energies
+    void swap(int64_t r);                   // rungs r and r + 1 exchange states
+    void adopt_orders(const PtValSession &o);  // take o's per-rung visit orders (shared in the reference)
+    void slice_state(int64_t r, int64_t t, int8_t *out);
+    const int8_t *device_spins() const;
+    const std::vector<int> &slots() const;
+    float ms = 0.0f;
+
+private:
+    struct Impl;
+    std::shared_ptr<Impl> d_;
+};
 
 }  // namespace qa

@@ -94,6 +94,11 @@ struct SqaArgs {
     int mag_burn;          // checkerboard: sweeps before magnetisation is accumulated
     double *mag_m2;        // [R] sum over measured sweeps of m^2 (null: no measurement)
     double *mag_abs;       // [R] sum over measured sweeps of |m|
+    // Imaginary-time bond sums B_r = sum_t sum_i s_t,i s_t+1,i (adaptive J_perp schedules): one
+    // sample of every replica after each slice sweep and each cluster sweep (worldline flips
+    // keep B). Integer-valued, so the double sums below are exact while they stay under 2^53.
+    double *bstat;         // non-null: [steps][ngroups][2] sum of B and of B^2 over the group's samples
+    double *bend;          // [steps]: B of replica 0 at the end of each step (group 0 writes)
 };
 
 
@@ -443,6 +448,32 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 
     const int wave = tid >> 6, lane = tid & 63;
 
+    // B of every replica of the group from the current spins; lanes tid < nrep accumulate their
+    // replica's sample (count = false: only refresh blast, the value at the end of a step).
+    double bacc = 0.0, b2acc = 0.0, blast = 0.0;
+    auto bond_sample = [&](bool count) {
+        for (int c = wave; c < G; c += kThreads / 64) {
+            const int t = c % P;
+            const int8_t *sc = S + static_cast<int64_t>(c) * n;
+            const int8_t *sn = S + static_cast<int64_t>((c / P) * P + (t + 1 == P ? 0 : t + 1)) * n;
+            int b = 0;
+            for (int i = lane; i < n; i += 64) b += sc[i] * sn[i];
+            for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+            if (lane == 0) ered[c] = static_cast<double>(b);
+        }
+        __syncthreads();
+        if (tid < nrep) {
+            double b = 0.0;
+            for (int t = 0; t < P; ++t) b += ered[tid * P + t];
+            if (count) {
+                bacc += b;
+                b2acc += b * b;
+            }
+            blast = b;
+        }
+        __syncthreads();
+    };
+
     uint32_t sweep_id = a.sweep_base;
     for (int step = 0; step < a.steps; ++step) {
         const int pidx = a.param_of_rep ? (decider ? a.param_of_rep[r0 + rr_me] : 0) : step;
@@ -561,6 +592,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 ring_store(tau + a.pfd);
                 __syncthreads();
             }
+            if (a.bstat) bond_sample(true);
         }
         // ---------------- cluster sweeps (sqa_annealer.cpp:215-288) ------------------------
         for (int sw = 0; sw < a.cl; ++sw, ++sweep_id) {
@@ -624,6 +656,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 ring_store(k + a.pfd);
                 __syncthreads();
             }
+            if (a.bstat) bond_sample(true);
         }
         // ---------------- worldline sweeps (sqa_annealer.cpp:294-315) -----------------------
         for (int sw = 0; sw < a.wl; ++sw, ++sweep_id) {
@@ -662,6 +695,24 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 ring_store(k + a.pfd);
                 __syncthreads();
             }
+        }
+        // ---------------- bond statistics of the step ----------------------------------------
+        if (a.bstat) {
+            if (a.sweep_kind != 0 || a.spb + a.cl == 0) bond_sample(false);
+            if (tid < 64) {  // wave 0: nrep <= 64 lanes hold the group's sums
+                double x = bacc, y = b2acc;
+                for (int o = 32; o > 0; o >>= 1) {
+                    x += __shfl_xor(x, o);
+                    y += __shfl_xor(y, o);
+                }
+                if (tid == 0) {
+                    a.bstat[(static_cast<int64_t>(step) * a.ngroups + g) * 2] = x;
+                    a.bstat[(static_cast<int64_t>(step) * a.ngroups + g) * 2 + 1] = y;
+                    if (g == 0 && a.bend) a.bend[step] = blast;
+                }
+            }
+            bacc = 0.0;
+            b2acc = 0.0;
         }
         // ---------------- step tail ---------------------------------------------------------
         if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, g, step, ered, scal, gbest);
@@ -1158,7 +1209,8 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     a.ring = pl.ring;
     a.pfd = pl.pfd;
     // the pipelined kernel runs plain slice sweeps over dense couplings with an LDS state
-    const bool pipe = pl.pipe && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 && a.pv.kind == QA_KIND_DENSE;
+    const bool pipe = pl.pipe && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 && a.pv.kind == QA_KIND_DENSE &&
+                      a.bstat == nullptr;
     const void *kern;
     unsigned threads = kThreads;
     if (pipe) {
@@ -1304,6 +1356,171 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     }
 }
 
+
+// ---- device-resident session for the adaptive J_perp schedules ------------------------------
+SqaFp32Session::SqaFp32Session(qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, uint64_t key)
+    : ctx_(ctx), p_(p), R_(R), P_(P), key_(key) {
+    const SqaPlan pl = plan_sqa(ctx, p, R, P);
+    rg_ = pl.rg;
+    ngroups_ = pl.ngroups;
+    use_lds_ = pl.use_lds;
+    const int64_t n = p->n;
+    gS_.alloc(R * P * n);
+    gF_.alloc(R * P * n);
+    bstate_.alloc(ngroups_ * n);
+    be_.alloc(ngroups_);
+    bstep_.alloc(ngroups_);
+    bchain_.alloc(ngroups_);
+    std::vector<double> inf(ngroups_, std::numeric_limits<double>::infinity());
+    std::vector<int> neg(ngroups_, -1);
+    be_.upload(inf.data(), ngroups_, ctx->stream);
+    bchain_.upload(neg.data(), ngroups_, ctx->stream);
+    QA_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void SqaFp32Session::copy_state_from(const SqaFp32Session &o) {
+    QA_REQUIRE(o.R_ == R_ && o.P_ == P_ && o.p_ == p_ && o.started_, "session shapes differ");
+    const size_t cnt = static_cast<size_t>(R_ * P_ * p_->n);
+    QA_HIP(hipMemcpyAsync(gS_.ptr, o.gS_.ptr, cnt, hipMemcpyDeviceToDevice, ctx_->stream));
+    QA_HIP(hipMemcpyAsync(gF_.ptr, o.gF_.ptr, cnt * sizeof(float), hipMemcpyDeviceToDevice, ctx_->stream));
+    started_ = true;
+}
+
+void SqaFp32Session::save() {
+    QA_REQUIRE(started_, "nothing to save");
+    const size_t cnt = static_cast<size_t>(R_ * P_ * p_->n);
+    if (!sS_.ptr) {
+        sS_.alloc(cnt);
+        sF_.alloc(cnt);
+    }
+    QA_HIP(hipMemcpyAsync(sS_.ptr, gS_.ptr, cnt, hipMemcpyDeviceToDevice, ctx_->stream));
+    QA_HIP(hipMemcpyAsync(sF_.ptr, gF_.ptr, cnt * sizeof(float), hipMemcpyDeviceToDevice, ctx_->stream));
+}
+
+void SqaFp32Session::restore() {
+    QA_REQUIRE(sS_.ptr, "no saved state");
+    const size_t cnt = static_cast<size_t>(R_ * P_ * p_->n);
+    QA_HIP(hipMemcpyAsync(gS_.ptr, sS_.ptr, cnt, hipMemcpyDeviceToDevice, ctx_->stream));
+    QA_HIP(hipMemcpyAsync(gF_.ptr, sF_.ptr, cnt * sizeof(float), hipMemcpyDeviceToDevice, ctx_->stream));
+}
+
+void SqaFp32Session::run(int64_t k, const double *bs, const double *jp, int64_t nparam, const int *param_of_rep,
+                         int64_t spb, int64_t cl, int64_t wl, int tail, BondStats *stats, double *trace) {
+    QA_REQUIRE(k > 0, "run: no steps");
+    QA_REQUIRE(!param_of_rep || k == 1, "per-replica parameters run one step per launch");
+    const SqaPlan pl = plan_sqa(ctx_, p_, R_, P_);
+    hipStream_t s = ctx_->stream;
+    const int64_t np = param_of_rep ? nparam : k;
+    std::vector<float> hbs(np), hjp(np), hjoin(np);
+    for (int64_t q = 0; q < np; ++q) {
+        hbs[q] = static_cast<float>(bs[q]);
+        hjp[q] = static_cast<float>(jp[q]);
+        hjoin[q] = static_cast<float>(1.0 - std::exp(-2.0 * jp[q]));
+    }
+    DevBuf<float> dbs(np), djp(np), djoin(np);
+    DevBuf<int> dpor(param_of_rep ? R_ : 0);
+    dbs.upload(hbs.data(), np, s);
+    djp.upload(hjp.data(), np, s);
+    djoin.upload(hjoin.data(), np, s);
+    dpor.upload(param_of_rep, param_of_rep ? R_ : 0, s);
+    DevBuf<double> bst(stats ? k * ngroups_ * 2 : 0), bend(stats ? k : 0), tpart(tail == 0 ? k * ngroups_ : 0);
+    DevBuf<unsigned long long> acc(ngroups_);
+    if (tail == 1 && !ce_.ptr) {
+        ce_.alloc(R_ * P_);
+        cb_.alloc(R_ * P_);
+    }
+    SqaArgs a{};
+    a.pv = view_of(p_);
+    a.P = static_cast<int>(P_);
+    a.R = static_cast<int>(R_);
+    a.rg = static_cast<int>(rg_);
+    a.steps = static_cast<int>(k);
+    a.spb = static_cast<int>(spb);
+    a.wl = static_cast<int>(wl);
+    a.cl = static_cast<int>(cl);
+    a.bs = dbs.ptr;
+    a.jp = djp.ptr;
+    a.join = djoin.ptr;
+    a.k0 = static_cast<uint32_t>(key_);
+    a.k1 = static_cast<uint32_t>(key_ >> 32);
+    a.gS = gS_.ptr;
+    a.gF = gF_.ptr;
+    a.trace_part = tpart.ptr;
+    a.best_e = be_.ptr;
+    a.best_step = bstep_.ptr;
+    a.best_chain = bchain_.ptr;
+    a.best_state = bstate_.ptr;
+    a.accepted = acc.ptr;
+    a.param_of_rep = dpor.ptr;
+    a.resume = started_ ? 1 : 0;
+    a.save_fields = 1;
+    a.tail_mode = tail;
+    a.chain_e = ce_.ptr;
+    a.chain_b = cb_.ptr;
+    a.sweep_base = sweeps_;
+    a.step_base = static_cast<int>(recorded_);
+    a.bstat = bst.ptr;
+    a.bend = bend.ptr;
+    QA_HIP(hipEventRecord(ctx_->ev0, s));
+    launch_sqa(ctx_, pl, a);
+    QA_HIP(hipEventRecord(ctx_->ev1, s));
+    started_ = true;
+    sweeps_ += static_cast<uint32_t>(k * (spb + cl + wl));
+    if (tail == 0) recorded_ += k;
+    std::vector<double> hb(bst.count), he(bend.count), ht(tpart.count);
+    bst.download(hb.data(), hb.size(), s);
+    bend.download(he.data(), he.size(), s);
+    tpart.download(ht.data(), ht.size(), s);
+    if (tail == 1) {
+        chain_e.resize(R_ * P_);
+        chain_b.resize(R_ * P_);
+        ce_.download(chain_e.data(), R_ * P_, s);
+        cb_.download(chain_b.data(), R_ * P_, s);
+    }
+    QA_HIP(hipStreamSynchronize(s));
+    float t = 0.0f;
+    QA_HIP(hipEventElapsedTime(&t, ctx_->ev0, ctx_->ev1));
+    ms += t;
+    for (int64_t q = 0; q < k; ++q) {
+        if (stats) {
+            BondStats &st = stats[q];
+            st = BondStats{};
+            for (int64_t g = 0; g < ngroups_; ++g) {
+                st.sum += hb[(q * ngroups_ + g) * 2];
+                st.sum2 += hb[(q * ngroups_ + g) * 2 + 1];
+            }
+            st.count = R_ * (spb + cl);
+            st.b0 = he[q];
+        }
+        if (tail == 0 && trace) {
+            double sum = 0.0;
+            for (int64_t g = 0; g < ngroups_; ++g) sum += ht[q * ngroups_ + g];
+            trace[q] = sum / static_cast<double>(R_ * P_);
+        }
+    }
+}
+
+void SqaFp32Session::best(int8_t *state, double *energy) {
+    hipStream_t s = ctx_->stream;
+    std::vector<double> hbe(ngroups_);
+    std::vector<int> hstep(ngroups_), hchain(ngroups_);
+    be_.download(hbe.data(), ngroups_, s);
+    bstep_.download(hstep.data(), ngroups_, s);
+    bchain_.download(hchain.data(), ngroups_, s);
+    QA_HIP(hipStreamSynchronize(s));
+    int64_t bg = -1;  // first strict minimum in (step, replica, slice) order
+    for (int64_t gi = 0; gi < ngroups_; ++gi) {
+        if (hchain[gi] < 0) continue;
+        if (bg < 0 || hbe[gi] < hbe[bg] || (hbe[gi] == hbe[bg] && hstep[gi] < hstep[bg])) bg = gi;
+    }
+    QA_REQUIRE(bg >= 0, "no finite slice energy was produced");
+    const int64_t n = p_->n;
+    QA_HIP(hipMemcpyAsync(state, bstate_.ptr + bg * n, n, hipMemcpyDeviceToHost, s));
+    DevBuf<double> ex(1);
+    device_energies(p_, bstate_.ptr + bg * n, 1, ex.ptr, s);
+    ex.download(energy, 1, s);
+    QA_HIP(hipStreamSynchronize(s));
+}
 
 void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res) {
     QA_REQUIRE(res->final_states && res->final_energies && res->best_state && res->best_energy &&

@@ -555,97 +555,315 @@ void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa
 // std::shuffle on that replica's engine (sqa_annealer.cpp:181,216,295), hands the device a
 // chunk of raw engine outputs drawn from a copy of the engine, and afterwards advances the
 // engine by exactly the number of outputs the device consumed.
-void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
-    QA_REQUIRE(prm->init_spins && prm->replica_seeds, "validation mode needs init_spins and replica_seeds");
-    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
+// ---- SQA validation session (SQAAnnealer's sweeps, replayed phase by phase) ----------------
+struct SqaValSession::Impl {
+    qa_ctx *ctx;
+    const qa_problem *p;
+    int64_t R, P, n, cap_max;
+    DevBuf<int8_t> spins, saved_spins;
+    DevBuf<double> fields, saved_fields, energies;
+    DevBuf<int32_t> order;
+    DevBuf<int64_t> used, bonds;
+    DevBuf<int> status;
+    DevBuf<uint64_t> stream;
+    std::vector<std::mt19937_64> eng;
+    std::vector<std::vector<std::size_t>> ord;
+    std::vector<uint64_t> hstream;
+    std::vector<int32_t> hord;
+    std::vector<int64_t> hused;
+    std::vector<int> hst;
+    std::vector<double> hen;
+};
+
+SqaValSession::SqaValSession(qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, const int8_t *init_spins,
+                             const uint64_t *seeds)
+    : d_(std::make_shared<Impl>()) {
+    QA_REQUIRE(init_spins && seeds, "validation mode needs init_spins and replica_seeds");
     QA_REQUIRE(P <= 4096, "validation mode supports at most 4096 Trotter slices");
+    const int64_t n = p->n;
     for (int64_t k = 0; k < R * P * n; ++k)
-        QA_REQUIRE(prm->init_spins[k] == 1 || prm->init_spins[k] == -1, "State spins must be -1 or +1.");
+        QA_REQUIRE(init_spins[k] == 1 || init_spins[k] == -1, "State spins must be -1 or +1.");
+    Impl &d = *d_;
+    d.ctx = ctx;
+    d.p = p;
+    d.R = R;
+    d.P = P;
+    d.n = n;
+    d.cap_max = std::max<int64_t>({P * n, n * (2 * P + 2) + 64, n});
     hipStream_t s = ctx->stream;
-    DevBuf<int8_t> spins(R * P * n);
-    DevBuf<double> fields(R * P * n), energies(R * P);
-    DevBuf<int32_t> order(R * n);
-    DevBuf<int64_t> used(R);
-    DevBuf<int> status(R);
-    spins.upload(prm->init_spins, R * P * n, s);
-    fields_fp64_launch(p, spins.ptr, fields.ptr, R * P, s);
-    QA_HIP(hipMemsetAsync(status.ptr, 0, R * sizeof(int), s));
+    d.spins.alloc(R * P * n);
+    d.fields.alloc(R * P * n);
+    d.energies.alloc(R * P);
+    d.order.alloc(R * n);
+    d.used.alloc(R);
+    d.bonds.alloc(R);
+    d.status.alloc(R);
+    d.stream.alloc(R * d.cap_max);
+    d.spins.upload(init_spins, R * P * n, s);
+    fields_fp64_launch(p, d.spins.ptr, d.fields.ptr, R * P, s);
+    QA_HIP(hipMemsetAsync(d.status.ptr, 0, R * sizeof(int), s));
+    d.eng.resize(R);
+    for (int64_t r = 0; r < R; ++r) d.eng[r].seed(seeds[r]);
+    d.ord.assign(R, std::vector<std::size_t>(n));
+    for (auto &o : d.ord) std::iota(o.begin(), o.end(), std::size_t{0});
+    d.hstream.resize(R * d.cap_max);
+    d.hord.resize(R * n);
+    d.hused.resize(R);
+    d.hst.resize(R);
+    d.hen.resize(R * P);
+    best.assign(n, 1);
+    best_e = std::numeric_limits<double>::infinity();
+}
 
-    std::vector<std::mt19937_64> eng(R);
-    for (int64_t r = 0; r < R; ++r) eng[r].seed(prm->replica_seeds[r]);
-    std::vector<std::vector<std::size_t>> ord(R, std::vector<std::size_t>(n));
-    for (auto &o : ord) std::iota(o.begin(), o.end(), std::size_t{0});
-    const int64_t cap_max = std::max<int64_t>({P * n, n * (2 * P + 2) + 64, n});
-    DevBuf<uint64_t> stream(R * cap_max);
-    std::vector<uint64_t> hstream(R * cap_max);
-    std::vector<int32_t> hord(R * n);
-    std::vector<int64_t> hused(R);
-    std::vector<int> hst(R);
-    std::vector<double> hen(R * P);
-    double best_e = std::numeric_limits<double>::infinity();
-    std::vector<int8_t> best(n, 1);
-    float total_ms = 0.0f;
+void SqaValSession::phase(int ph, double bs, double jp, double join) {
+    Impl &d = *d_;
+    const int64_t R = d.R, P = d.P, n = d.n;
+    hipStream_t s = d.ctx->stream;
+    const int64_t cap = ph == PH_SLICE ? P * n : (ph == PH_CLUSTER ? n * (2 * P + 2) + 64 : n);
+    for (int64_t r = 0; r < R; ++r) {
+        std::shuffle(d.ord[r].begin(), d.ord[r].end(), d.eng[r]);
+        for (int64_t k = 0; k < n; ++k) d.hord[r * n + k] = static_cast<int32_t>(d.ord[r][k]);
+        std::mt19937_64 copy = d.eng[r];
+        for (int64_t k = 0; k < cap; ++k) d.hstream[r * cap + k] = copy();
+    }
+    d.order.upload(d.hord.data(), R * n, s);
+    d.stream.upload(d.hstream.data(), R * cap, s);
+    SqaPhaseArgs a{view_of(d.p), static_cast<int>(P), ph, bs, jp, join, d.spins.ptr, d.fields.ptr, d.order.ptr,
+                   d.stream.ptr, cap, d.used.ptr, d.status.ptr};
+    const size_t lds = ((P + 15) / 16) * 16 + P * sizeof(int);
+    QA_HIP(hipEventRecord(d.ctx->ev0, s));
+    hipLaunchKernelGGL(sqa_phase_kernel, dim3(static_cast<unsigned>(R)), dim3(64), lds, s, a);
+    QA_HIP(hipGetLastError());
+    QA_HIP(hipEventRecord(d.ctx->ev1, s));
+    d.used.download(d.hused.data(), R, s);
+    d.status.download(d.hst.data(), R, s);
+    QA_HIP(hipStreamSynchronize(s));
+    float t = 0;
+    QA_HIP(hipEventElapsedTime(&t, d.ctx->ev0, d.ctx->ev1));
+    ms += t;
+    for (int64_t r = 0; r < R; ++r) {
+        if (d.hst[r]) throw Error{QA_EDEVICE, "validation stream exhausted inside a sweep phase"};
+        d.eng[r].discard(static_cast<unsigned long long>(d.hused[r]));
+    }
+}
 
-    auto phase = [&](int ph, double bs, double jp, double join) {
-        const int64_t cap = ph == PH_SLICE ? P * n : (ph == PH_CLUSTER ? n * (2 * P + 2) + 64 : n);
-        for (int64_t r = 0; r < R; ++r) {
-            std::shuffle(ord[r].begin(), ord[r].end(), eng[r]);
-            for (int64_t k = 0; k < n; ++k) hord[r * n + k] = static_cast<int32_t>(ord[r][k]);
-            std::mt19937_64 copy = eng[r];
-            for (int64_t k = 0; k < cap; ++k) hstream[r * cap + k] = copy();
+void SqaValSession::bonds(std::vector<int64_t> &b) {
+    Impl &d = *d_;
+    hipStream_t s = d.ctx->stream;
+    hipLaunchKernelGGL(bond_kernel, dim3(static_cast<unsigned>(d.R)), dim3(256), 0, s, d.spins.ptr,
+                       static_cast<int>(d.P), static_cast<int>(d.n), d.bonds.ptr);
+    QA_HIP(hipGetLastError());
+    b.resize(d.R);
+    d.bonds.download(b.data(), d.R, s);
+    QA_HIP(hipStreamSynchronize(s));
+}
+
+// Step tail (sqa_annealer.cpp:455-473): FP64 slice energies on the device, then the reference's
+// ordered scan for the mean and the first strict minimum.
+double SqaValSession::tail() {
+    Impl &d = *d_;
+    hipStream_t s = d.ctx->stream;
+    const int64_t RP = d.R * d.P;
+    device_energies(d.p, d.spins.ptr, RP, d.energies.ptr, s);
+    d.energies.download(d.hen.data(), RP, s);
+    QA_HIP(hipStreamSynchronize(s));
+    double avg = 0.0;
+    int64_t arg = -1;
+    for (int64_t k = 0; k < RP; ++k) {
+        avg += d.hen[k];
+        if (d.hen[k] < best_e) {
+            best_e = d.hen[k];
+            arg = k;
         }
-        order.upload(hord.data(), R * n, s);
-        stream.upload(hstream.data(), R * cap, s);
-        SqaPhaseArgs a{view_of(p), static_cast<int>(P), ph, bs, jp, join, spins.ptr, fields.ptr, order.ptr,
-                       stream.ptr, cap, used.ptr, status.ptr};
-        const size_t lds = ((P + 15) / 16) * 16 + P * sizeof(int);
-        QA_HIP(hipEventRecord(ctx->ev0, s));
-        hipLaunchKernelGGL(sqa_phase_kernel, dim3(static_cast<unsigned>(R)), dim3(64), lds, s, a);
-        QA_HIP(hipGetLastError());
-        QA_HIP(hipEventRecord(ctx->ev1, s));
-        used.download(hused.data(), R, s);
-        status.download(hst.data(), R, s);
+    }
+    if (arg >= 0) {
+        QA_HIP(hipMemcpyAsync(best.data(), d.spins.ptr + arg * d.n, d.n, hipMemcpyDeviceToHost, s));
         QA_HIP(hipStreamSynchronize(s));
-        float ms = 0;
-        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        total_ms += ms;
-        for (int64_t r = 0; r < R; ++r) {
-            if (hst[r]) throw Error{QA_EDEVICE, "validation stream exhausted inside a sweep phase"};
-            eng[r].discard(static_cast<unsigned long long>(hused[r]));
-        }
-    };
+    }
+    return avg / static_cast<double>(RP);
+}
 
+void SqaValSession::save() {
+    Impl &d = *d_;
+    const int64_t cnt = d.R * d.P * d.n;
+    if (!d.saved_spins.ptr) {
+        d.saved_spins.alloc(cnt);
+        d.saved_fields.alloc(cnt);
+    }
+    QA_HIP(hipMemcpyAsync(d.saved_spins.ptr, d.spins.ptr, cnt, hipMemcpyDeviceToDevice, d.ctx->stream));
+    QA_HIP(hipMemcpyAsync(d.saved_fields.ptr, d.fields.ptr, cnt * sizeof(double), hipMemcpyDeviceToDevice,
+                          d.ctx->stream));
+}
+
+void SqaValSession::restore() {
+    Impl &d = *d_;
+    QA_REQUIRE(d.saved_spins.ptr, "no saved state");
+    const int64_t cnt = d.R * d.P * d.n;
+    QA_HIP(hipMemcpyAsync(d.spins.ptr, d.saved_spins.ptr, cnt, hipMemcpyDeviceToDevice, d.ctx->stream));
+    QA_HIP(hipMemcpyAsync(d.fields.ptr, d.saved_fields.ptr, cnt * sizeof(double), hipMemcpyDeviceToDevice,
+                          d.ctx->stream));
+}
+
+void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    const int64_t P = prm->trotter_slices;
+    SqaValSession v(ctx, p, prm->replicas, P, prm->init_spins, prm->replica_seeds);
     for (int64_t step = 0; step < prm->steps; ++step) {
         const double beta = prm->betas[step], gamma = prm->gammas[step];
         const double jp = trotter_coupling(beta, gamma, P);
         const double bs = beta / static_cast<double>(P);
         const double join = 1.0 - std::exp(-2.0 * jp);
-        for (int64_t k = 0; k < prm->sweeps_per_beta; ++k) phase(PH_SLICE, bs, jp, join);
-        for (int64_t k = 0; k < prm->cluster_sweeps; ++k) phase(PH_CLUSTER, bs, jp, join);
-        for (int64_t k = 0; k < prm->worldline_sweeps; ++k) phase(PH_WORLDLINE, bs, jp, join);
-        // step tail (sqa_annealer.cpp:455-473): FP64 slice energies on the device, then the
-        // reference's ordered scan for the mean and the first strict minimum.
-        device_energies(p, spins.ptr, R * P, energies.ptr, s);
-        energies.download(hen.data(), R * P, s);
-        QA_HIP(hipStreamSynchronize(s));
-        double avg = 0.0;
-        int64_t arg = -1;
-        for (int64_t k = 0; k < R * P; ++k) {
-            avg += hen[k];
-            if (hen[k] < best_e) { best_e = hen[k]; arg = k; }
-        }
-        if (arg >= 0) {
-            QA_HIP(hipMemcpyAsync(best.data(), spins.ptr + arg * n, n, hipMemcpyDeviceToHost, s));
-            QA_HIP(hipStreamSynchronize(s));
-        }
-        res->energy_trace[step] = avg / static_cast<double>(R * P);
+        for (int64_t k = 0; k < prm->sweeps_per_beta; ++k) v.phase(PH_SLICE, bs, jp, join);
+        for (int64_t k = 0; k < prm->cluster_sweeps; ++k) v.phase(PH_CLUSTER, bs, jp, join);
+        for (int64_t k = 0; k < prm->worldline_sweeps; ++k) v.phase(PH_WORLDLINE, bs, jp, join);
+        res->energy_trace[step] = v.tail();
     }
-    std::memcpy(res->best_state, best.data(), n);
-    *res->best_energy = best_e;
-    if (res->kernel_ms) *res->kernel_ms = total_ms;
+    std::memcpy(res->best_state, v.best.data(), p->n);
+    *res->best_energy = v.best_e;
+    if (res->kernel_ms) *res->kernel_ms = v.ms;
     if (res->accept_rate) *res->accept_rate = -1.0;
 }
 
+// ---- SQA-PT validation session (per-rung sweeps; states move between rungs on swaps) -------
+struct PtValSession::Impl {
+    qa_ctx *ctx;
+    const qa_problem *p;
+    int64_t R, P, n, cap_max;
+    DevBuf<int8_t> spins;
+    DevBuf<double> energies, dbs, djp, djoin;
+    DevBuf<int32_t> order, dslot;
+    DevBuf<int64_t> used, bonds;
+    DevBuf<int> status;
+    DevBuf<uint64_t> stream;
+    std::vector<int> slot;
+    std::vector<std::mt19937_64> eng;
+    std::vector<std::vector<std::size_t>> ord;
+    std::vector<uint64_t> hstream;
+    std::vector<int32_t> hord;
+    std::vector<int64_t> hused, hbond;
+    std::vector<int> hst;
+};
+
+PtValSession::PtValSession(qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, const int8_t *src,
+                           bool device_src, const uint64_t *seeds)
+    : d_(std::make_shared<Impl>()) {
+    QA_REQUIRE(src && seeds, "validation mode needs init_spins and rung seeds");
+    QA_REQUIRE(P <= 4096, "validation mode supports at most 4096 Trotter slices");
+    const int64_t n = p->n;
+    if (!device_src)
+        for (int64_t k = 0; k < R * P * n; ++k)
+            QA_REQUIRE(src[k] == 1 || src[k] == -1, "State spins must be -1 or +1.");
+    Impl &d = *d_;
+    d.ctx = ctx;
+    d.p = p;
+    d.R = R;
+    d.P = P;
+    d.n = n;
+    d.cap_max = std::max<int64_t>({P * n, n * (2 * P + 2) + 64, n});
+    hipStream_t s = ctx->stream;
+    d.spins.alloc(R * P * n);
+    d.energies.alloc(R * P);
+    d.dbs.alloc(R);
+    d.djp.alloc(R);
+    d.djoin.alloc(R);
+    d.order.alloc(R * n);
+    d.dslot.alloc(R);
+    d.used.alloc(R);
+    d.bonds.alloc(R);
+    d.status.alloc(R);
+    d.stream.alloc(R * d.cap_max);
+    if (device_src)
+        QA_HIP(hipMemcpyAsync(d.spins.ptr, src, R * P * n, hipMemcpyDeviceToDevice, s));
+    else
+        d.spins.upload(src, R * P * n, s);
+    QA_HIP(hipMemsetAsync(d.status.ptr, 0, R * sizeof(int), s));
+    d.slot.resize(R);
+    std::iota(d.slot.begin(), d.slot.end(), 0);
+    d.eng.resize(R);
+    for (int64_t r = 0; r < R; ++r) d.eng[r].seed(seeds[r]);
+    d.ord.assign(R, std::vector<std::size_t>(n));
+    for (auto &o : d.ord) std::iota(o.begin(), o.end(), std::size_t{0});
+    d.hstream.resize(R * d.cap_max);
+    d.hord.resize(R * n);
+    d.hused.resize(R);
+    d.hbond.resize(R);
+    d.hst.resize(R);
+    QA_HIP(hipStreamSynchronize(s));
+}
+
+void PtValSession::phase(int ph, const double *bs, const double *jp, const double *join) {
+    Impl &d = *d_;
+    const int64_t R = d.R, P = d.P, n = d.n;
+    hipStream_t s = d.ctx->stream;
+    const int64_t cap = ph == PH_SLICE ? P * n : (ph == PH_CLUSTER ? n * (2 * P + 2) + 64 : n);
+    for (int64_t r = 0; r < R; ++r) {
+        std::shuffle(d.ord[r].begin(), d.ord[r].end(), d.eng[r]);
+        for (int64_t k = 0; k < n; ++k) d.hord[r * n + k] = static_cast<int32_t>(d.ord[r][k]);
+        std::mt19937_64 copy = d.eng[r];
+        for (int64_t k = 0; k < cap; ++k) d.hstream[r * cap + k] = copy();
+    }
+    d.order.upload(d.hord.data(), R * n, s);
+    d.stream.upload(d.hstream.data(), R * cap, s);
+    d.dslot.upload(d.slot.data(), R, s);
+    d.dbs.upload(bs, R, s);
+    d.djp.upload(jp, R, s);
+    d.djoin.upload(join, R, s);
+    PtPhaseArgs a{view_of(d.p), static_cast<int>(P), ph, d.dbs.ptr, d.djp.ptr, d.djoin.ptr, d.dslot.ptr,
+                  d.spins.ptr, d.order.ptr, d.stream.ptr, cap, d.used.ptr, d.status.ptr};
+    const size_t lds = ((P + 15) / 16) * 16 + P * sizeof(int);
+    QA_HIP(hipEventRecord(d.ctx->ev0, s));
+    hipLaunchKernelGGL(pt_phase_kernel, dim3(static_cast<unsigned>(R)), dim3(64), lds, s, a);
+    QA_HIP(hipGetLastError());
+    QA_HIP(hipEventRecord(d.ctx->ev1, s));
+    d.used.download(d.hused.data(), R, s);
+    d.status.download(d.hst.data(), R, s);
+    QA_HIP(hipStreamSynchronize(s));
+    float t = 0;
+    QA_HIP(hipEventElapsedTime(&t, d.ctx->ev0, d.ctx->ev1));
+    ms += t;
+    for (int64_t r = 0; r < R; ++r) {
+        if (d.hst[r]) throw Error{QA_EDEVICE, "validation stream exhausted inside a sweep phase"};
+        d.eng[r].discard(static_cast<unsigned long long>(d.hused[r]));
+    }
+}
+
+void PtValSession::bonds(std::vector<int64_t> &b) {
+    Impl &d = *d_;
+    hipStream_t s = d.ctx->stream;
+    hipLaunchKernelGGL(bond_kernel, dim3(static_cast<unsigned>(d.R)), dim3(256), 0, s, d.spins.ptr,
+                       static_cast<int>(d.P), static_cast<int>(d.n), d.bonds.ptr);
+    QA_HIP(hipGetLastError());
+    d.bonds.download(d.hbond.data(), d.R, s);
+    QA_HIP(hipStreamSynchronize(s));
+    b.resize(d.R);
+    for (int64_t r = 0; r < d.R; ++r) b[r] = d.hbond[d.slot[r]];
+}
+
+void PtValSession::energies(std::vector<double> &e) {
+    Impl &d = *d_;
+    hipStream_t s = d.ctx->stream;
+    const int64_t R = d.R, P = d.P;
+    std::vector<double> h(R * P);
+    device_energies(d.p, d.spins.ptr, R * P, d.energies.ptr, s);
+    d.energies.download(h.data(), R * P, s);
+    QA_HIP(hipStreamSynchronize(s));
+    e.resize(R * P);
+    for (int64_t r = 0; r < R; ++r)
+        for (int64_t t = 0; t < P; ++t) e[r * P + t] = h[static_cast<int64_t>(d.slot[r]) * P + t];
+}
+
+void PtValSession::swap(int64_t r) { std::swap(d_->slot[r], d_->slot[r + 1]); }
+
+void PtValSession::adopt_orders(const PtValSession &o) { d_->ord = o.d_->ord; }
+
+void PtValSession::slice_state(int64_t r, int64_t t, int8_t *out) {
+    Impl &d = *d_;
+    QA_HIP(hipMemcpyAsync(out, d.spins.ptr + (static_cast<int64_t>(d.slot[r]) * d.P + t) * d.n, d.n,
+                          hipMemcpyDeviceToHost, d.ctx->stream));
+    QA_HIP(hipStreamSynchronize(d.ctx->stream));
+}
+
+const int8_t *PtValSession::device_spins() const { return d_->spins.ptr; }
+const std::vector<int> &PtValSession::slots() const { return d_->slot; }
 
 // SQA-PT validation: rung r keeps its engine (seeded from rung_seeds[r]) and its visit order
 // while states move between rungs on accepted swaps (sqa_parallel_tempering.cpp:320-321); the

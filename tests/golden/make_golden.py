@@ -232,6 +232,36 @@ def main():
                     "swap_interval": si, "seed": seed, "out": out_json(res)})
     dump("pt.json", {"cases": out})
 
+    # ---- run_optimal: adaptive J_perp schedules (own stream) -------------------------------------
+    rng3 = np.random.default_rng(20261205)
+    out = []
+    for p, P, R, beta, j0, j1, eps, steps, sps, wl, cl, probes, cs, frac, seed in [
+            (sk(rng3, 16), 6, 2, 3.0, 0.05, 2.5, 0.0, 14, 2, 1, 0, 5, 2, 0.3, 11),
+            (sk(rng3, 12), 4, 1, 2.0, 0.1, 2.0, 0.0, 10, 1, 0, 1, 4, 3, 0.0, 12),
+            (sparse_problem(rng3, 14), 5, 3, 2.5, 0.05, 3.0, 0.04, 12, 1, 1, 1, 4, 2, 0.25, 13),
+            (hubo_problem(rng3, 10, 14), 4, 2, 3.0, 0.02, 2.0, 0.0, 12, 2, 0, 0, 6, 2, 0.5, 14)]:
+        res = ref.sqa_opt_run(p, P, R, beta, j0, j1, eps, 15 / 14, steps, sps, wl, cl, probes, cs, frac, 0.1, seed)
+        out.append({"problem": problem_json(p), "trotter_slices": P, "replicas": R, "beta": beta,
+                    "j_perp_start": j0, "j_perp_end": j1, "eps_tilde": eps, "alpha": 15 / 14, "num_steps": steps,
+                    "sweeps_per_step": sps, "worldline_sweeps": wl, "cluster_sweeps": cl, "calib_probes": probes,
+                    "calib_sweeps": cs, "beta_ramp_fraction": frac, "beta_ramp_start": 0.1, "seed": seed,
+                    "out": out_json(res)})
+    dump("sqa_opt.json", {"cases": out})
+    out = []
+    for p, nb, P, steps, sps, wl, cl, si, eps, j1, probes, seed in [
+            (sk(rng3, 14), 4, 4, 12, 2, 1, 0, 1, 0.0, 2.5, 4, 21),
+            (sparse_problem(rng3, 12), 3, 5, 10, 1, 0, 1, 2, 0.0, 0.0, 5, 22),
+            (hubo_problem(rng3, 10, 12), 3, 4, 10, 1, 1, 0, 1, 0.05, 2.0, 3, 23),
+            (sk(rng3, 10), 2, 3, 8, 2, 0, 0, 1, 0.0, 0.001, 3, 24)]:
+        betas = np.geomspace(0.5, 3.0, nb)
+        gammas = np.geomspace(2.0, 0.4, nb)
+        res = ref.sqapt_opt_run(p, betas, gammas, P, steps, sps, wl, eps, 15 / 14, j1, cl, si, 0, probes, 2, seed)
+        out.append({"problem": problem_json(p), "betas": enc(betas), "gammas": enc(gammas), "trotter_slices": P,
+                    "num_steps": steps, "sweeps_per_step": sps, "worldline_sweeps": wl, "cluster_sweeps": cl,
+                    "swap_interval": si, "eps_tilde": eps, "alpha": 15 / 14, "j_perp_end": j1,
+                    "calib_probes": probes, "calib_sweeps": 2, "seed": seed, "out": out_json(res)})
+    dump("sqapt_opt.json", {"cases": out})
+
 
 def dump(name, obj):
     obj["source"] = ("reference sources compiled by oracle/Makefile (Release flags), "

@@ -98,6 +98,66 @@ def test_replica_pt_closed_forms(orc):
     assert np.allclose(orc.energy(p, t["final_states"]), t["final_energies"], atol=1e-9)
 
 
+def _opt_args(c, keys):
+    return {k: c[k] for k in keys}
+
+
+SQA_OPT_KEYS = ("trotter_slices", "replicas", "beta", "j_perp_start", "j_perp_end", "eps_tilde", "alpha",
+                "num_steps", "sweeps_per_step", "worldline_sweeps", "cluster_sweeps", "calib_probes", "calib_sweeps",
+                "beta_ramp_fraction", "beta_ramp_start", "seed")
+SQAPT_OPT_KEYS = ("trotter_slices", "num_steps", "sweeps_per_step", "worldline_sweeps", "eps_tilde", "alpha",
+                  "j_perp_end", "cluster_sweeps", "swap_interval", "calib_probes", "calib_sweeps", "seed")
+
+
+def test_golden_sqa_optimal(orc):
+    for c in load("sqa_opt.json")["cases"]:
+        res = orc.sqa_opt_run(problem(c["problem"]), **_opt_args(c, SQA_OPT_KEYS))
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(np.ravel(res[k]), np.ravel(exp[k])), k
+
+
+def test_golden_sqapt_optimal(orc):
+    for c in load("sqapt_opt.json")["cases"]:
+        res = orc.sqapt_opt_run(problem(c["problem"]), dec(c["betas"]), dec(c["gammas"]),
+                                **_opt_args(c, SQAPT_OPT_KEYS))
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(np.ravel(res[k]), np.ravel(exp[k])), k
+
+
+def test_optimal_schedule_invariants(orc):
+    """What the schedule construction fixes (sqa_annealer.cpp:777-949): the beta-ramp steps sit at
+    j_perp_start; the calibrated trajectory starts exactly at j_perp_start, is non-decreasing and ends
+    at j_perp_end to rounding; eps_tilde equals the probe integral over the phase-2 budget; every
+    chi_B entry is at least the floor; the online schedule never passes j_perp_end; best_energy is
+    the FP64 energy of best_state; and the SQA-PT start J_perp is the hottest rung's Trotter coupling."""
+    rng = np.random.default_rng(3)
+    n = 12
+    J = rng.normal(0, 1 / np.sqrt(n), (n, n))
+    J = np.triu(J, 1) + np.triu(J, 1).T
+    p = Problem.dense(np.zeros(n), J)
+    r = orc.sqa_opt_run(p, 6, 2, 3.0, 0.05, 2.5, 0.0, 15 / 14, 20, 2, 0, 0, 6, 3, 0.3, 0.1, 5)
+    jt = r["j_perp_trace"]
+    n1 = 6
+    assert np.all(jt[:n1] == 0.05)
+    assert jt[n1] == 0.05 and np.all(np.diff(jt[n1:]) >= 0) and abs(jt[-1] - 2.5) < 1e-12
+    assert r["final_j_perp"] == jt[-1]
+    assert np.all(r["chi_B_trace"][n1:] > 0)
+    assert abs(orc.energy(p, r["best_state"])[0] - r["best_energy"]) < 1e-12
+    o = orc.sqa_opt_run(p, 6, 2, 3.0, 0.05, 2.5, 0.02, 15 / 14, 20, 2, 0, 0, 6, 3, 0.0, 0.1, 5)
+    assert o["calibrated_eps_tilde"] == 0.02 and np.all(np.diff(o["j_perp_trace"]) >= 0)
+    assert np.all(o["j_perp_trace"] <= 2.5) and o["j_perp_trace"][0] == 0.05
+    betas, gammas = np.array([0.5, 1.5, 3.0]), np.array([2.0, 1.0, 0.5])
+    t = orc.sqapt_opt_run(p, betas, gammas, 4, 10, 1, 0, 0.0, 15 / 14, 2.0, 0, 1, 0, 4, 2, 9)
+    jp0 = 0.5 * np.log(1.0 / np.tanh(max(0.5 * 2.0 / 4, 1e-12)))
+    assert abs(t["j_perp_start"] - jp0) < 1e-15 and t["j_perp_trace"][0] == t["j_perp_start"]
+    assert abs(t["j_perp_trace"][-1] - 2.0) < 1e-12
+    d = orc.sqapt_opt_run(p, betas, gammas, 4, 6, 1, 0, 0.0, 15 / 14, 0.0, 0, 1, 0, 4, 2, 9)
+    cold = 0.5 * np.log(1.0 / np.tanh(max(3.0 * 0.5 / 4, 1e-12)))
+    assert abs(d["resolved_j_perp_end"] - cold) < 1e-15 or d["resolved_j_perp_end"] == d["j_perp_start"] + 1e6
+
+
 def test_golden_sqa(orc):
     for c in load("sqa.json")["cases"]:
         res = orc.sqa_run(problem(c["problem"]), dec(c["betas"]), dec(c["gammas"]), c["trotter_slices"],
@@ -289,6 +349,14 @@ def test_random_against_reference(orc):
                       ref.sqa_chi_run(p, 2 + case % 4, 2, 2.0, 3.0, 0.05, 8, 2, 5, 3, 1, seed=case)),
                      (orc.replica_run(p, b, 2, 1 + case % 4, case), ref.replica_run(p, b, 2, 1 + case % 4, case)),
                      (orc.pt_run(p, b[:2 + case % 4], 1, 6, 1 + case % 2, case),
-                      ref.pt_run(p, b[:2 + case % 4], 1, 6, 1 + case % 2, case))]:
+                      ref.pt_run(p, b[:2 + case % 4], 1, 6, 1 + case % 2, case)),
+                     (orc.sqa_opt_run(p, 3, 1 + case % 2, 2.0, 0.05, 2.0, 0.0 if case % 3 else 0.05, 15 / 14, 8, 1,
+                                      case % 2, (case // 2) % 2, 4, 2, 0.25, 0.1, case),
+                      ref.sqa_opt_run(p, 3, 1 + case % 2, 2.0, 0.05, 2.0, 0.0 if case % 3 else 0.05, 15 / 14, 8, 1,
+                                      case % 2, (case // 2) % 2, 4, 2, 0.25, 0.1, case)),
+                     (orc.sqapt_opt_run(p, b[:3], np.geomspace(2.0, 0.5, 3), 3, 6, 1, case % 2,
+                                        0.0 if case % 3 else 0.05, 15 / 14, 2.0, (case // 2) % 2, 1, 0, 3, 2, case),
+                      ref.sqapt_opt_run(p, b[:3], np.geomspace(2.0, 0.5, 3), 3, 6, 1, case % 2,
+                                        0.0 if case % 3 else 0.05, 15 / 14, 2.0, (case // 2) % 2, 1, 0, 3, 2, case))]:
             for k in r:
                 assert bits_equal(a[k], r[k]), (case, k)
