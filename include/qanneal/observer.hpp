@@ -1,5 +1,6 @@
-// Observer hooks (mirrors observer.hpp). Device runs do not call back per sweep: passing an
-// observer to a run on the HIP backend raises std::invalid_argument.
+// Observer hooks (mirrors observer.hpp:9-29). Device runs call them from the host between launches:
+// a run with an observer goes step by step (a SweepObserver: sweep by sweep) and hands over the
+// state after each, with the trajectory the run has without an observer.
 #pragma once
 
 #include <cstddef>
@@ -14,8 +15,9 @@ public:
     virtual void record(std::size_t step, double beta, double energy, const State &state) = 0;
 };
 
-class SweepObserver : public Observer {
+class SweepObserver {
 public:
+    virtual ~SweepObserver() = default;
     virtual void record_sweep(std::size_t step, std::size_t sweep, double beta, double energy, const State &state) = 0;
 };
 

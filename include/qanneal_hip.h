@@ -115,6 +115,13 @@ typedef struct qa_sa_result {
 qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
 
 /* ---- SQA: replaces SQAAnnealer::run (sqa_annealer.cpp:87-477) ----------------------------- */
+/* Observer callbacks (SQAAnnealer::run's observer calls, sqa_annealer.cpp:340-350, 474-476). They run
+ * on the calling thread between launches; `spins` is a host copy of the whole [R*P*n] lattice,
+ * valid during the call. phase: 0 slice, 1 worldline, 2 cluster (the reference's SQASweepPhase). */
+typedef void (*qa_sqa_step_fn)(void *user, int64_t step, double avg_energy, const int8_t *spins);
+typedef void (*qa_sqa_sweep_fn)(void *user, int64_t step, int64_t replica, int64_t sweep, int32_t phase,
+                                double avg_energy, const double *replica_energies, const int8_t *spins);
+
 typedef struct qa_sqa_params {
     int64_t steps;
     const double *betas;         /* [steps]                                                  */
@@ -131,6 +138,14 @@ typedef struct qa_sqa_params {
     int64_t refresh_interval;    /* FP32: sweeps between recomputations of all FP32 fields on
                                     the matrix cores from the spins (0: never; no reference
                                     counterpart, the reference keeps FP64 fields)             */
+    qa_sqa_step_fn on_step;      /* optional: after every step (the run goes launch by launch;
+                                    same trajectory; not with refresh_interval)               */
+    qa_sqa_sweep_fn on_sweep;    /* optional: selects the reference's sequential tracing path
+                                    (sqa_annealer.cpp:317-453), FP64 on the device in either
+                                    mode; needs init_spins and trace_engine                    */
+    void *callback_user;
+    const char *trace_engine;    /* tracing path: the annealer's std::mt19937_64 after the
+                                    replica seeds, in the standard's text form (operator<<)   */
 } qa_sqa_params;
 
 typedef struct qa_sqa_result {
@@ -142,6 +157,8 @@ typedef struct qa_sqa_result {
     double *field_drift;         /* optional [2], FP32 with refresh_interval > 0: max |f32 -
                                     f64| over up to 64 chains before each refresh, and after
                                     it (f64: the reference-order FP64 fields); NaN if none    */
+    char *trace_engine_out;      /* tracing path: the engine after the run, in text form       */
+    int64_t trace_engine_out_len;/* capacity of trace_engine_out (8192 is enough)             */
 } qa_sqa_result;
 
 qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,

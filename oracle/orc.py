@@ -197,6 +197,38 @@ class Oracle:
         return dict(final_states=fs, final_energies=fe, best_state=best, best_energy=be.value,
                     average_energy_trace=at, swap_acceptance_trace=sw)
 
+    def sa_observe_run(self, p: Problem, betas, sweeps_per_beta: int, seed: int):
+        """Annealer::run with an observer that also records every sweep (annealer.cpp:33-75)."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        S, n = b.shape[0], p.n
+        best, be, tr, mg = np.zeros(n, np.int8), C.c_double(), np.zeros(S), np.zeros(S)
+        se, ss = np.zeros(S * sweeps_per_beta), np.zeros((S * sweeps_per_beta, n), np.int8)
+        st, keep = p._struct()
+        self._chk(self.lib.orc_sa_observe_run(C.byref(st), _i64(S), _d(b), _i64(sweeps_per_beta), C.c_uint64(seed),
+                                              _b(best), C.byref(be), _d(tr), _d(mg), _d(se), _b(ss)))
+        return dict(best_state=best, best_energy=be.value, energy_trace=tr, magnetization_trace=mg,
+                    sweep_energies=se, sweep_states=ss)
+
+    def sqa_observe_run(self, p: Problem, betas, gammas, trotter_slices: int, replicas: int, sweeps_per_beta: int,
+                        worldline_sweeps: int = 0, cluster_sweeps: int = 0, tracing: bool = False, seed: int = 0):
+        """SQAAnnealer::run with an SQAObserver; tracing: also an SQASweepObserver (sqa_annealer.cpp:317-453)."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        g = np.ascontiguousarray(gammas, dtype=np.float64)
+        S, n, R, P = b.shape[0], p.n, replicas, trotter_slices
+        K = S * R * (sweeps_per_beta + worldline_sweeps + cluster_sweeps) if tracing else 0
+        best, be, tr, mg = np.zeros(n, np.int8), C.c_double(), np.zeros(S), np.zeros(S)
+        meta, av, rep = np.zeros((max(K, 1), 4), np.int64), np.zeros(max(K, 1)), np.zeros((max(K, 1), R))
+        states = np.zeros((max(K, 1), R * P * n), np.int8)
+        st, keep = p._struct()
+        self._chk(self.lib.orc_sqa_observe_run(
+            C.byref(st), _i64(S), _d(b), _d(g), _i64(P), _i64(R), _i64(sweeps_per_beta), _i64(worldline_sweeps),
+            _i64(cluster_sweeps), C.c_int32(1 if tracing else 0), C.c_uint64(seed), _b(best), C.byref(be), _d(tr),
+            _d(mg), meta.ctypes.data_as(C.POINTER(C.c_int64)), _d(av), _d(rep), _b(states)))
+        out = dict(best_state=best, best_energy=be.value, energy_trace=tr, magnetization_trace=mg)
+        if tracing:
+            out.update(sweep_meta=meta[:K], sweep_avg=av[:K], sweep_rep=rep[:K], sweep_states=states[:K])
+        return out
+
     def sqa_opt_run(self, p: Problem, trotter_slices: int, replicas: int, beta: float, j_perp_start: float,
                     j_perp_end: float, eps_tilde: float, alpha: float, num_steps: int, sweeps_per_step: int,
                     worldline_sweeps: int = 0, cluster_sweeps: int = 0, calib_probes: int = 12,

@@ -83,6 +83,25 @@ int orc_sqapt_run(const orc_problem *p, int64_t rungs, const double *betas, cons
                   double *final_energies, int8_t *best_state, double *best_energy,
                   double *avg_trace, double *swap_trace);
 
+/* Annealer::run (annealer.cpp:33-75) after set_seed(seed) with one observer that is an Observer
+ * and a SweepObserver: step_mag [steps] the mean spin after each step; sweep_energies
+ * [steps*sweeps_per_beta] and sweep_states [steps*sweeps_per_beta*n] after each sweep. */
+int orc_sa_observe_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
+                       uint64_t seed, int8_t *best_state, double *best_energy, double *trace, double *step_mag,
+                       double *sweep_energies, int8_t *sweep_states);
+
+/* SQAAnnealer::run (sqa_annealer.cpp:87-477) after set_seed(seed) with an SQAObserver
+ * (tracing = 0: the parallel path) or an observer that is also an SQASweepObserver (tracing = 1:
+ * the sequential path, :317-453). step_mag [steps]: mean spin of the lattice after each step
+ * (SQAMetricsObserver). tracing = 1, K = steps*replicas*(sweeps_per_beta+cluster+worldline)
+ * records: sweep_meta [K*4] (step, replica, sweep, phase), sweep_avg [K], sweep_rep [K*replicas],
+ * sweep_states [K*replicas*P*n]. */
+int orc_sqa_observe_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
+                        int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                        int64_t worldline_sweeps, int64_t cluster_sweeps, int32_t tracing, uint64_t seed,
+                        int8_t *best_state, double *best_energy, double *trace, double *step_mag,
+                        int64_t *sweep_meta, double *sweep_avg, double *sweep_rep, int8_t *sweep_states);
+
 /* SQAAnnealer::run_optimal (sqa_annealer.cpp:499-952) after set_seed(seed). energy_trace,
  * j_perp_trace, chi_trace: [num_steps]; scalars [4]: calibrated_eps_tilde, j_perp_start,
  * resolved_j_perp_end, final_j_perp. No debug CSV. */

@@ -19,6 +19,9 @@
 #include "qanneal/replica_annealer.hpp"
 #include "qanneal/dense_ising.hpp"
 #include "qanneal/higher_order_ising.hpp"
+#include "qanneal/metrics.hpp"
+#include "qanneal/observer.hpp"
+#include "qanneal/sqa_observer.hpp"
 #include "qanneal/qubo.hpp"
 #include "qanneal/schedule.hpp"
 #include "qanneal/sparse_ising.hpp"
@@ -130,6 +133,107 @@ int orc_sa_run(const orc_problem *p, int64_t steps, const double *betas, int64_t
         qanneal::Annealer ann(*ham, sched);
         ann.set_seed(seed);
         auto res = ann.run(static_cast<std::size_t>(sweeps_per_beta));
+        put_state(res.best_state, best_state);
+        *best_energy = res.best_energy;
+        std::memcpy(trace, res.energy_trace.data(), res.energy_trace.size() * sizeof(double));
+    });
+}
+
+namespace {
+struct SaBoth : qanneal::Observer, qanneal::SweepObserver {
+    double *mag, *se;
+    int8_t *ss;
+    std::size_t n, k = 0;
+    void record(std::size_t step, double, double, const qanneal::State &s) override {
+        mag[step] = qanneal::magnetization(s);
+    }
+    void record_sweep(std::size_t, std::size_t, double, double e, const qanneal::State &s) override {
+        se[k] = e;
+        std::memcpy(ss + k * n, s.spins.data(), n);
+        ++k;
+    }
+};
+
+struct SqaStep : qanneal::SQAObserver {
+    double *mag;
+    void record(std::size_t step, double, double, double, const qanneal::SQAState &st) override {
+        double sum = 0.0;
+        for (std::size_t r = 0; r < st.replicas(); ++r)
+            for (std::size_t t = 0; t < st.slices(); ++t) {
+                const int8_t *q = st.slice_ptr(r, t);
+                for (std::size_t i = 0; i < st.spins(); ++i) sum += static_cast<double>(q[i]);
+            }
+        mag[step] = sum / static_cast<double>(st.replicas() * st.slices() * st.spins());
+    }
+};
+
+struct SqaBoth : SqaStep, qanneal::SQASweepObserver {
+    int64_t *meta;
+    double *avg, *rep;
+    int8_t *states;
+    std::size_t k = 0;
+    void record_sweep(std::size_t step, std::size_t replica, std::size_t sweep, qanneal::SQASweepPhase phase, double,
+                      double, double a, const std::vector<double> &re, const qanneal::SQAState &st) override {
+        meta[k * 4] = static_cast<int64_t>(step);
+        meta[k * 4 + 1] = static_cast<int64_t>(replica);
+        meta[k * 4 + 2] = static_cast<int64_t>(sweep);
+        meta[k * 4 + 3] = static_cast<int64_t>(phase);
+        avg[k] = a;
+        std::memcpy(rep + k * re.size(), re.data(), re.size() * sizeof(double));
+        const std::size_t cnt = st.replicas() * st.slices() * st.spins();
+        std::memcpy(states + k * cnt, st.slice_ptr(0, 0), cnt);
+        ++k;
+    }
+};
+}  // namespace
+
+int orc_sa_observe_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
+                       uint64_t seed, int8_t *best_state, double *best_energy, double *trace, double *step_mag,
+                       double *sweep_energies, int8_t *sweep_states) {
+    return guard([&] {
+        auto ham = build(p);
+        qanneal::AnnealSchedule sched;
+        sched.betas.assign(betas, betas + steps);
+        qanneal::Annealer ann(*ham, sched);
+        ann.set_seed(seed);
+        SaBoth obs;
+        obs.mag = step_mag;
+        obs.se = sweep_energies;
+        obs.ss = sweep_states;
+        obs.n = static_cast<std::size_t>(p->n);
+        auto res = ann.run(static_cast<std::size_t>(sweeps_per_beta), &obs);
+        put_state(res.best_state, best_state);
+        *best_energy = res.best_energy;
+        std::memcpy(trace, res.energy_trace.data(), res.energy_trace.size() * sizeof(double));
+    });
+}
+
+int orc_sqa_observe_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
+                        int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                        int64_t worldline_sweeps, int64_t cluster_sweeps, int32_t tracing, uint64_t seed,
+                        int8_t *best_state, double *best_energy, double *trace, double *step_mag,
+                        int64_t *sweep_meta, double *sweep_avg, double *sweep_rep, int8_t *sweep_states) {
+    return guard([&] {
+        auto ham = build(p);
+        auto sched = qanneal::SQASchedule::from_vectors(std::vector<double>(betas, betas + steps),
+                                                        std::vector<double>(gammas, gammas + steps));
+        qanneal::SQAAnnealer ann(*ham, sched, static_cast<std::size_t>(trotter_slices),
+                                 static_cast<std::size_t>(replicas));
+        ann.set_seed(seed);
+        SqaBoth both;
+        SqaStep plain;
+        qanneal::SQAObserver *obs = &plain;
+        plain.mag = step_mag;
+        if (tracing) {
+            both.mag = step_mag;
+            both.meta = sweep_meta;
+            both.avg = sweep_avg;
+            both.rep = sweep_rep;
+            both.states = sweep_states;
+            obs = &both;
+        }
+        auto res = ann.run(static_cast<std::size_t>(sweeps_per_beta), static_cast<std::size_t>(worldline_sweeps),
+                           static_cast<std::size_t>(cluster_sweeps), 0, obs);
         put_state(res.best_state, best_state);
         *best_energy = res.best_energy;
         std::memcpy(trace, res.energy_trace.data(), res.energy_trace.size() * sizeof(double));

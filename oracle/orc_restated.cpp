@@ -351,12 +351,13 @@ struct PtReplica {
     std::vector<std::size_t> *order;
     std::vector<char> *in_cluster;
     std::vector<std::size_t> *stack;
+    bool shuffle = true;  // false: sites in index order (SQAAnnealer's tracing path, :317-453)
 
     int8_t *sl(std::size_t t) const { return spins + t * n; }
 
     void slice_sweep(Rng &g, double bs, double jp) {
         std::uniform_real_distribution<double> uni(0.0, 1.0);
-        std::shuffle(order->begin(), order->end(), g);
+        if (shuffle) std::shuffle(order->begin(), order->end(), g);
         for (std::size_t t = 0; t < P; ++t) {
             int8_t *cur = sl(t);
             const int8_t *prv = sl(t == 0 ? P - 1 : t - 1), *nxt = sl((t + 1) % P);
@@ -375,7 +376,7 @@ struct PtReplica {
         std::uniform_int_distribution<std::size_t> pick(0, P - 1);
         std::vector<char> &inc = *in_cluster;
         std::vector<std::size_t> &sk = *stack;
-        std::shuffle(order->begin(), order->end(), g);
+        if (shuffle) std::shuffle(order->begin(), order->end(), g);
         for (std::size_t q = 0; q < n; ++q) {
             const std::size_t i = (*order)[q];
             const std::size_t sd = pick(g);
@@ -409,7 +410,7 @@ struct PtReplica {
 
     void worldline_sweep(Rng &g, double bs) {
         std::uniform_real_distribution<double> uni(0.0, 1.0);
-        std::shuffle(order->begin(), order->end(), g);
+        if (shuffle) std::shuffle(order->begin(), order->end(), g);
         for (std::size_t q = 0; q < n; ++q) {
             const std::size_t i = (*order)[q];
             double dcl = 0.0;
@@ -1075,6 +1076,148 @@ int orc_sqa_chi_run(const orc_problem *p, int64_t trotter_slices, int64_t replic
                 if (e < best_e) { best_e = e; std::memcpy(best.data(), &spins[c * n], n); }
             }
             trace[k] = avg / static_cast<double>(R * P);
+        }
+        std::memcpy(best_state, best.data(), n);
+        *best_energy = best_e;
+    });
+}
+
+// annealer.cpp:33-75 with an observer that is also a SweepObserver (the trajectory is unchanged)
+int orc_sa_observe_run(const orc_problem *p, int64_t steps, const double *betas, int64_t sweeps_per_beta,
+                       uint64_t seed, int8_t *best_state, double *best_energy, double *trace, double *step_mag,
+                       double *sweep_energies, int8_t *sweep_states) {
+    return guard([&] {
+        if (steps <= 0) throw std::invalid_argument("Schedule must contain at least one beta.");
+        if (sweeps_per_beta <= 0) throw std::invalid_argument("sweeps_per_beta must be > 0.");
+        const Model m = build(p);
+        const std::size_t n = m.n;
+        Rng rng(seed);
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        std::vector<int8_t> s(n);
+        random_spins(s, rng);
+        double e = m.energy(s.data());
+        std::vector<int8_t> best = s;
+        double best_e = e;
+        std::size_t k = 0;
+        for (int64_t step = 0; step < steps; ++step) {
+            const double beta = betas[step];
+            for (int64_t sw = 0; sw < sweeps_per_beta; ++sw) {
+                for (std::size_t i = 0; i < n; ++i) {
+                    const double d = m.delta(s.data(), i);
+                    if (d <= 0.0 || uni(rng) < std::exp(-beta * d)) {
+                        s[i] = static_cast<int8_t>(-s[i]);
+                        e += d;
+                        if (e < best_e) {
+                            best_e = e;
+                            best = s;
+                        }
+                    }
+                }
+                sweep_energies[k] = e;  // :63-65
+                std::memcpy(sweep_states + k * n, s.data(), n);
+                ++k;
+            }
+            trace[step] = e;
+            double sum = 0.0;  // metrics.hpp magnetization: double sum, one division
+            for (std::size_t i = 0; i < n; ++i) sum += static_cast<double>(s[i]);
+            step_mag[step] = sum / static_cast<double>(n);
+        }
+        std::memcpy(best_state, best.data(), n);
+        *best_energy = best_e;
+    });
+}
+
+// sqa_annealer.cpp:87-477 with an observer; tracing selects the sequential path (:317-453)
+int orc_sqa_observe_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
+                        int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                        int64_t worldline_sweeps, int64_t cluster_sweeps, int32_t tracing, uint64_t seed,
+                        int8_t *best_state, double *best_energy, double *trace, double *step_mag,
+                        int64_t *sweep_meta, double *sweep_avg, double *sweep_rep, int8_t *sweep_states) {
+    return guard([&] {
+        if (steps <= 0) throw std::invalid_argument("SQA schedule must contain betas.");
+        if (sweeps_per_beta == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
+            throw std::invalid_argument(
+                "At least one of sweeps_per_beta, worldline_sweeps, or cluster_sweeps must be > 0.");
+        const Model m = build(p);
+        const std::size_t n = m.n, R = static_cast<std::size_t>(replicas), P = static_cast<std::size_t>(trotter_slices);
+        Rng rng(seed);
+        std::vector<int8_t> spins(R * P * n);
+        random_spins(spins, rng);
+        std::vector<Rng> rrng(R);
+        for (auto &g : rrng) g.seed(rng());
+        std::vector<double> fields(R * P * n);
+        for (std::size_t k = 0; k < R * P; ++k) m.fields(&spins[k * n], &fields[k * n]);
+        std::vector<std::vector<std::size_t>> order(R, std::vector<std::size_t>(n));
+        for (auto &o : order) std::iota(o.begin(), o.end(), std::size_t{0});
+        std::vector<std::vector<char>> incl(R, std::vector<char>(P));
+        std::vector<std::vector<std::size_t>> stk(R);
+        std::vector<std::size_t> ident(n);
+        std::iota(ident.begin(), ident.end(), std::size_t{0});
+        std::vector<double> rep(R);
+        auto avg_energy = [&]() {  // compute_avg_energy (:112-125)
+            double total = 0.0;
+            for (std::size_t r = 0; r < R; ++r) {
+                double sum = 0.0;
+                for (std::size_t t = 0; t < P; ++t) sum += m.energy(&spins[(r * P + t) * n]);
+                rep[r] = sum / static_cast<double>(P);
+                total += sum;
+            }
+            return total / static_cast<double>(R * P);
+        };
+        double best_e = std::numeric_limits<double>::infinity();
+        std::vector<int8_t> best(n, 1);
+        std::size_t k = 0;
+        for (int64_t step = 0; step < steps; ++step) {
+            const double beta = betas[step], gamma = gammas[step];
+            const double jp = trotter_coupling(beta, gamma, P);
+            const double bs = beta / static_cast<double>(P);
+            const double join = 1.0 - std::exp(-2.0 * jp);
+            if (!tracing) {
+#pragma omp parallel for if (R > 1) schedule(static)
+                for (long long rr = 0; rr < static_cast<long long>(R); ++rr) {
+                    const std::size_t r = static_cast<std::size_t>(rr);
+                    SqaReplica rep_{&m, n, P, &spins[r * P * n], &fields[r * P * n], &order[r], &incl[r], &stk[r]};
+                    for (int64_t q = 0; q < sweeps_per_beta; ++q) rep_.slice_sweep(rrng[r], bs, jp);
+                    for (int64_t q = 0; q < cluster_sweeps; ++q) rep_.cluster_sweep(rrng[r], bs, jp, join);
+                    for (int64_t q = 0; q < worldline_sweeps; ++q) rep_.worldline_sweep(rrng[r], bs);
+                }
+            } else {
+                std::vector<std::size_t> ord = ident;
+                for (std::size_t r = 0; r < R; ++r) {
+                    PtReplica tr{&m, n, P, &spins[r * P * n], &ord, &incl[0], &stk[0]};
+                    tr.shuffle = false;
+                    const struct { int kind, code; int64_t count; } phases[3] = {
+                        {0, 0, sweeps_per_beta}, {2, 2, cluster_sweeps}, {1, 1, worldline_sweeps}};
+                    for (const auto &ph : phases)
+                        for (int64_t q = 0; q < ph.count; ++q) {
+                            if (ph.kind == 0) tr.slice_sweep(rng, bs, jp);
+                            else if (ph.kind == 2) tr.cluster_sweep(rng, bs, jp, join);
+                            else tr.worldline_sweep(rng, bs);
+                            const double a = avg_energy();
+                            sweep_meta[k * 4] = step;
+                            sweep_meta[k * 4 + 1] = static_cast<int64_t>(r);
+                            sweep_meta[k * 4 + 2] = q;
+                            sweep_meta[k * 4 + 3] = ph.code;
+                            sweep_avg[k] = a;
+                            std::memcpy(sweep_rep + k * R, rep.data(), R * sizeof(double));
+                            std::memcpy(sweep_states + k * R * P * n, spins.data(), R * P * n);
+                            ++k;
+                        }
+                }
+            }
+            double avg = 0.0;
+            for (std::size_t q = 0; q < R * P; ++q) {
+                const double e = m.energy(&spins[q * n]);
+                avg += e;
+                if (e < best_e) {
+                    best_e = e;
+                    std::memcpy(best.data(), &spins[q * n], n);
+                }
+            }
+            trace[step] = avg / static_cast<double>(R * P);
+            double sum = 0.0;
+            for (std::size_t q = 0; q < R * P * n; ++q) sum += static_cast<double>(spins[q]);
+            step_mag[step] = sum / static_cast<double>(R * P * n);
         }
         std::memcpy(best_state, best.data(), n);
         *best_energy = best_e;

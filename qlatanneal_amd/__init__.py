@@ -30,8 +30,9 @@ __version__ = version_string()
 __all__ = [
     "__version__", "version_string", "version_major", "version_minor", "version_patch",
     "State", "Hamiltonian", "DenseIsing", "SparseEdge", "SparseIsing", "QUBO", "HigherOrderIsing",
-    "AnnealSchedule", "Observer", "AnnealResult", "Annealer", "ReplicaResult", "MultiAnnealResult",
-    "ReplicaAnnealer", "ParallelTemperingAnnealer", "ParallelTemperingResult", "SQASchedule", "SQAObserver", "SQAResult",
+    "AnnealSchedule", "Observer", "MetricsObserver", "StateTraceObserver", "AnnealResult", "Annealer", "ReplicaResult", "MultiAnnealResult",
+    "ReplicaAnnealer", "ParallelTemperingAnnealer", "ParallelTemperingResult", "SQASchedule", "SQAObserver", "SQASweepPhase",
+    "SQAMetricsObserver", "SQAStateTraceObserver", "SQAResult",
     "SQAAnnealer", "SQAChiResult", "SQAChiAnnealer", "SQAParallelTemperingAnnealer",
     "SQAParallelTemperingResult", "magnetization", "overlap", "SolveResult", "solve", "auto_schedule_sa",
     "auto_schedule_sqa", "auto_schedule_sa_tuned", "auto_schedule_sqa_tuned", "auto_ladder_sqa_tuned",

@@ -40,12 +40,14 @@ class SqaParams(C.Structure):
     _fields_ = [("steps", _i64), ("betas", _dp), ("gammas", _dp), ("trotter_slices", _i64),
                 ("replicas", _i64), ("sweeps_per_beta", _i64), ("worldline_sweeps", _i64),
                 ("cluster_sweeps", _i64), ("mode", _i32), ("seed", _u64), ("init_spins", _bp),
-                ("replica_seeds", _up), ("refresh_interval", _i64)]
+                ("replica_seeds", _up), ("refresh_interval", _i64), ("on_step", C.c_void_p),
+                ("on_sweep", C.c_void_p), ("callback_user", C.c_void_p), ("trace_engine", C.c_char_p)]
 
 
 class SqaResult(C.Structure):
     _fields_ = [("best_state", _bp), ("best_energy", _dp), ("energy_trace", _dp),
-                ("kernel_ms", _dp), ("accept_rate", _dp), ("field_drift", _dp)]
+                ("kernel_ms", _dp), ("accept_rate", _dp), ("field_drift", _dp),
+                ("trace_engine_out", C.c_char_p), ("trace_engine_out_len", _i64)]
 
 
 class SqaptParams(C.Structure):

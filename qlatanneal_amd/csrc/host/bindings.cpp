@@ -249,7 +249,46 @@ PYBIND11_MODULE(_qanneal, m) {
         });
 
     py::class_<Observer, std::shared_ptr<Observer>>(m, "Observer");
+    py::class_<MetricsObserver, Observer, std::shared_ptr<MetricsObserver>>(m, "MetricsObserver")
+        .def(py::init<>())
+        .def_readonly("energy_trace", &MetricsObserver::energy_trace)
+        .def_readonly("magnetization_trace", &MetricsObserver::magnetization_trace)
+        .def("clear", &MetricsObserver::clear);
+    py::class_<StateTraceObserver, Observer, std::shared_ptr<StateTraceObserver>>(m, "StateTraceObserver")
+        .def(py::init<>())
+        .def_readwrite("stride", &StateTraceObserver::stride)
+        .def_readonly("step_trace", &StateTraceObserver::step_trace)
+        .def_readonly("sweep_trace", &StateTraceObserver::sweep_trace)
+        .def_readonly("beta_trace", &StateTraceObserver::beta_trace)
+        .def_readonly("energy_trace", &StateTraceObserver::energy_trace)
+        .def_readonly("state_trace", &StateTraceObserver::state_trace)
+        .def("clear", &StateTraceObserver::clear);
     py::class_<SQAObserver, std::shared_ptr<SQAObserver>>(m, "SQAObserver");
+    py::enum_<SQASweepPhase>(m, "SQASweepPhase")
+        .value("SLICE", SQASweepPhase::Slice)
+        .value("WORLDLINE", SQASweepPhase::Worldline)
+        .value("CLUSTER", SQASweepPhase::Cluster);
+    py::class_<SQAMetricsObserver, SQAObserver, std::shared_ptr<SQAMetricsObserver>>(m, "SQAMetricsObserver")
+        .def(py::init<>())
+        .def_readonly("energy_trace", &SQAMetricsObserver::energy_trace)
+        .def_readonly("magnetization_trace", &SQAMetricsObserver::magnetization_trace)
+        .def("clear", &SQAMetricsObserver::clear);
+    py::class_<SQAStateTraceObserver, SQAObserver, std::shared_ptr<SQAStateTraceObserver>>(m, "SQAStateTraceObserver")
+        .def(py::init<>())
+        .def_readwrite("stride", &SQAStateTraceObserver::stride)
+        .def_readonly("replicas", &SQAStateTraceObserver::replicas)
+        .def_readonly("slices", &SQAStateTraceObserver::slices)
+        .def_readonly("spins", &SQAStateTraceObserver::spins)
+        .def_readonly("step_trace", &SQAStateTraceObserver::step_trace)
+        .def_readonly("replica_trace", &SQAStateTraceObserver::replica_trace)
+        .def_readonly("sweep_trace", &SQAStateTraceObserver::sweep_trace)
+        .def_readonly("phase_trace", &SQAStateTraceObserver::phase_trace)
+        .def_readonly("beta_trace", &SQAStateTraceObserver::beta_trace)
+        .def_readonly("gamma_trace", &SQAStateTraceObserver::gamma_trace)
+        .def_readonly("avg_energy_trace", &SQAStateTraceObserver::avg_energy_trace)
+        .def_readonly("replica_energy_trace", &SQAStateTraceObserver::replica_energy_trace)
+        .def_readonly("state_trace", &SQAStateTraceObserver::state_trace)
+        .def("clear", &SQAStateTraceObserver::clear);
 
     py::class_<AnnealResult>(m, "AnnealResult")
         .def_readonly("best_state", &AnnealResult::best_state)

@@ -9,9 +9,12 @@
 #include <limits>
 #include <mutex>
 #include <random>
+#include <sstream>
+#include <string>
 #include <stdexcept>
 
 #include "qanneal/annealer.hpp"
+#include "qanneal/metrics_observer.hpp"
 #include "qanneal/parallel_tempering.hpp"
 #include "qanneal/replica_annealer.hpp"
 #include "qanneal/sqa_annealer.hpp"
@@ -49,6 +52,98 @@ State state_from(const int8_t *p, std::size_t n) {
     return s;
 }
 
+// Annealer::run with an observer (annealer.cpp:33-75): one launch per step (per sweep for a
+// SweepObserver), each continuing from the previous launch's state and running energy, so the
+// host can hand the state to the observer in between. In validation mode the chained launches
+// consume the annealer's engine exactly as the single launch does; a launch's best replaces the
+// run's best only when strictly lower, which is the reference's rule across the boundary too.
+AnnealResult sa_observed(HIPBackend &hb, const AnnealSchedule &sched, std::mt19937_64 &rng, std::size_t spb,
+                         Observer *observer) {
+    auto *sweep_obs = dynamic_cast<SweepObserver *>(observer);
+    const std::size_t n = hb.size(), steps = sched.size();
+    const int mode = mode_of(hb);
+    State state = State::random(n, rng);              // annealer.cpp:40
+    double energy = hb.energy(state.spins.data(), n);  // :41
+    const uint64_t key = mode == QA_MODE_FP32 ? rng() : 0;
+    AnnealResult res;
+    res.best_state = state;
+    res.best_energy = energy;
+    res.energy_trace.reserve(steps);
+    std::vector<int8_t> bs(n), fs(n);
+    double be = 0.0, tr = 0.0, fe = 0.0;
+    int64_t used = 0;
+    uint64_t launch = 0;
+    std::lock_guard<std::mutex> lock(hb.device_mutex());
+    for (std::size_t step = 0; step < steps; ++step) {
+        const double beta = sched.betas[step];
+        const std::size_t per = sweep_obs ? 1 : spb;
+        for (std::size_t sw = 0; sw < spb; sw += per) {
+            qa_sa_params prm{};
+            prm.steps = 1;
+            prm.betas = &sched.betas[step];
+            prm.sweeps_per_beta = static_cast<int64_t>(per);
+            prm.reads = 1;
+            prm.mode = mode;
+            prm.init_energies = &energy;
+            std::vector<double> uni;
+            if (mode == QA_MODE_FP32) {
+                prm.seed = key;
+                prm.init_states = state.spins.data();
+                prm.counter_base = launch++;
+            } else {
+                uni = uniform_stream(rng, per * n);
+                prm.init_spins = state.spins.data();
+                prm.uniforms = uni.data();
+                prm.uniforms_per_read = static_cast<int64_t>(uni.size());
+            }
+            qa_sa_result out{};
+            out.best_states = bs.data();
+            out.best_energies = &be;
+            out.energy_trace = &tr;
+            out.uniforms_used = &used;
+            out.final_states = fs.data();
+            out.final_energies = &fe;
+            throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+            if (mode != QA_MODE_FP32) rng.discard(static_cast<unsigned long long>(used));
+            if (be < res.best_energy) {
+                res.best_energy = be;
+                res.best_state = state_from(bs.data(), n);
+            }
+            std::memcpy(state.spins.data(), fs.data(), n);
+            energy = fe;
+            if (sweep_obs) sweep_obs->record_sweep(step, sw, beta, energy, state);
+        }
+        res.energy_trace.push_back(energy);
+        observer->record(step, beta, energy, state);
+    }
+    return res;
+}
+
+// SQAAnnealer::run's observer calls through the C-ABI callbacks.
+struct SqaObserverCtx {
+    SQAObserver *obs;
+    SQASweepObserver *sweep;
+    const SQASchedule *sched;
+    SQAState state;
+    std::vector<double> rep;
+};
+
+void sqa_on_step(void *user, int64_t step, double avg, const int8_t *spins) {
+    auto &c = *static_cast<SqaObserverCtx *>(user);
+    std::memcpy(c.state.data(), spins, c.state.replicas() * c.state.slices() * c.state.spins());
+    c.obs->record(static_cast<std::size_t>(step), c.sched->betas[step], c.sched->gammas[step], avg, c.state);
+}
+
+void sqa_on_sweep(void *user, int64_t step, int64_t replica, int64_t sweep, int32_t phase, double avg,
+                  const double *rep_e, const int8_t *spins) {
+    auto &c = *static_cast<SqaObserverCtx *>(user);
+    std::memcpy(c.state.data(), spins, c.state.replicas() * c.state.slices() * c.state.spins());
+    c.rep.assign(rep_e, rep_e + c.state.replicas());
+    c.sweep->record_sweep(static_cast<std::size_t>(step), static_cast<std::size_t>(replica),
+                          static_cast<std::size_t>(sweep), static_cast<SQASweepPhase>(phase), c.sched->betas[step],
+                          c.sched->gammas[step], avg, c.rep, c.state);
+}
+
 }  // namespace
 
 // ======================================================================================= SA
@@ -68,8 +163,8 @@ void Annealer::set_seed(std::uint64_t seed) { rng_.seed(seed); }
 
 AnnealResult Annealer::run(std::size_t sweeps_per_beta, Observer *observer) {
     if (sweeps_per_beta == 0) throw std::invalid_argument("sweeps_per_beta must be > 0.");
-    if (observer) throw std::invalid_argument("observers are not called by device runs");
     HIPBackend &hb = device_backend(backend_);
+    if (observer) return sa_observed(hb, schedule_, rng_, sweeps_per_beta, observer);
     const std::size_t n = hb.size(), steps = schedule_.size();
     qa_sa_params prm{};
     prm.steps = static_cast<int64_t>(steps);
@@ -386,7 +481,6 @@ SQAResult SQAAnnealer::run(std::size_t sweeps_per_beta, std::size_t worldline_sw
     if (sweeps_per_beta == 0 && worldline_sweeps == 0 && cluster_sweeps == 0)
         throw std::invalid_argument(
             "At least one of sweeps_per_beta, worldline_sweeps, or cluster_sweeps must be > 0.");
-    if (observer) throw std::invalid_argument("observers are not called by device runs");
     HIPBackend &hb = device_backend(backend_);
     const std::size_t n = hb.size();
     const std::size_t P = continuous_time_slices > slices_ ? continuous_time_slices : slices_;  // :97-99
@@ -403,7 +497,9 @@ SQAResult SQAAnnealer::run(std::size_t sweeps_per_beta, std::size_t worldline_sw
     prm.mode = mode_of(hb);
     SQAState init;
     std::vector<uint64_t> seeds;
-    if (prm.mode == QA_MODE_FP32) {
+    auto *sweep_obs = dynamic_cast<SQASweepObserver *>(observer);
+    std::string engine_text;
+    if (prm.mode == QA_MODE_FP32 && !sweep_obs) {
         prm.seed = rng_();
     } else {
         init = SQAState::random(R, P, n, rng_);  // :100
@@ -412,14 +508,36 @@ SQAResult SQAAnnealer::run(std::size_t sweeps_per_beta, std::size_t worldline_sw
         prm.init_spins = init.data();
         prm.replica_seeds = seeds.data();
     }
+    SqaObserverCtx octx{observer, sweep_obs, &schedule_, observer ? SQAState(R, P, n) : SQAState(), {}};
+    if (observer) {
+        prm.on_step = sqa_on_step;
+        prm.callback_user = &octx;
+    }
+    std::vector<char> engine_out;
+    if (sweep_obs) {  // the tracing path continues the annealer's engine (:317-453)
+        std::ostringstream o;
+        o << rng_;
+        engine_text = o.str();
+        prm.on_sweep = sqa_on_sweep;
+        prm.trace_engine = engine_text.c_str();
+        engine_out.assign(engine_text.size() + 4096, '\0');
+    }
     SQAResult res;
     res.best_state = State(n);
     res.energy_trace.resize(steps);
     double kms = 0.0, acc = 0.0;
     qa_sqa_result out{res.best_state.spins.data(), &res.best_energy, res.energy_trace.data(), &kms, &acc};
+    if (sweep_obs) {
+        out.trace_engine_out = engine_out.data();
+        out.trace_engine_out_len = static_cast<int64_t>(engine_out.size());
+    }
     {
         std::lock_guard<std::mutex> lock(hb.device_mutex());
         throw_on_status(qa_sqa_run(hb.context(), hb.device_problem(), &prm, &out));
+    }
+    if (sweep_obs) {
+        std::istringstream in(engine_out.data());
+        in >> rng_;
     }
     last_kernel_ms_ = kms;
     last_accept_rate_ = acc;

@@ -588,9 +588,10 @@ qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
         check_finite(prm->betas, prm->steps, "betas");
         check_finite(prm->gammas, prm->steps, "gammas");
         QA_HIP(hipSetDevice(ctx->device));
-        if (prm->mode == QA_MODE_FP64_VALIDATION) sqa_validation(ctx, p, prm, res);
-        else if (prm->mode == QA_MODE_FP32) sqa_fp32(ctx, p, prm, res);
-        else throw Error{QA_EINVAL, "unknown mode"};
+        if (prm->mode != QA_MODE_FP64_VALIDATION && prm->mode != QA_MODE_FP32) throw Error{QA_EINVAL, "unknown mode"};
+        if (prm->on_sweep) sqa_trace(ctx, p, prm, res);
+        else if (prm->mode == QA_MODE_FP64_VALIDATION) sqa_validation(ctx, p, prm, res);
+        else sqa_fp32(ctx, p, prm, res);
     });
 }
 

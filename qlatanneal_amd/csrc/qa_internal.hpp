@@ -223,6 +223,7 @@ void sqapt_optimal(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_opt_params *
 void sa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
 void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res);
 void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
+void sqa_trace(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
 void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
 void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
 void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
@@ -309,6 +310,7 @@ public:
     void phase(int ph, double bs, double jp, double join);
     void bonds(std::vector<int64_t> &b);  // B of every replica from the current spins
     double tail();                        // step end: best update (first strict minimum), mean energy
+    void spins(int8_t *out);              // host copy of the lattice [R*P*n]
     void save();
     void restore();
     std::vector<int8_t> best;

@@ -1232,7 +1232,32 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     QA_HIP(hipGetLastError());
 }
 
+// A per-step observer: one launch per step on a device-resident state, the lattice copied back
+// after each (SQAAnnealer::run's observer call, sqa_annealer.cpp:474-476).
+void sqa_fp32_observed(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    if (prm->refresh_interval > 0)
+        throw Error{QA_EUNSUPPORTED, "a per-step observer does not combine with refresh_interval"};
+    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
+    SqaFp32Session f(ctx, p, R, P, prm->seed);
+    std::vector<int8_t> h(R * P * n);
+    for (int64_t step = 0; step < prm->steps; ++step) {
+        const double bs = prm->betas[step] / static_cast<double>(P);
+        const double jp = trotter_coupling(prm->betas[step], prm->gammas[step], P);
+        f.run(1, &bs, &jp, 0, nullptr, prm->sweeps_per_beta, prm->cluster_sweeps, prm->worldline_sweeps, 0, nullptr,
+              res->energy_trace + step);
+        QA_HIP(hipMemcpy(h.data(), f.device_spins(), R * P * n, hipMemcpyDeviceToHost));
+        prm->on_step(prm->callback_user, step, res->energy_trace[step], h.data());
+    }
+    f.best(res->best_state, res->best_energy);
+    if (res->kernel_ms) *res->kernel_ms = f.ms;
+    if (res->accept_rate) *res->accept_rate = -1.0;
+}
+
 void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    if (prm->on_step) {
+        sqa_fp32_observed(ctx, p, prm, res);
+        return;
+    }
     const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
     const SqaPlan pl = plan_sqa(ctx, p, R, P);
     const int64_t rg = pl.rg, ngroups = pl.ngroups;

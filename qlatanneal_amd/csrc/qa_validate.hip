@@ -18,6 +18,8 @@
 #include <memory>
 #include <numeric>
 #include <random>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "qa_internal.hpp"
@@ -683,6 +685,12 @@ double SqaValSession::tail() {
     return avg / static_cast<double>(RP);
 }
 
+void SqaValSession::spins(int8_t *out) {
+    Impl &d = *d_;
+    QA_HIP(hipMemcpyAsync(out, d.spins.ptr, d.R * d.P * d.n, hipMemcpyDeviceToHost, d.ctx->stream));
+    QA_HIP(hipStreamSynchronize(d.ctx->stream));
+}
+
 void SqaValSession::save() {
     Impl &d = *d_;
     const int64_t cnt = d.R * d.P * d.n;
@@ -716,11 +724,140 @@ void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, 
         for (int64_t k = 0; k < prm->cluster_sweeps; ++k) v.phase(PH_CLUSTER, bs, jp, join);
         for (int64_t k = 0; k < prm->worldline_sweeps; ++k) v.phase(PH_WORLDLINE, bs, jp, join);
         res->energy_trace[step] = v.tail();
+        if (prm->on_step) {
+            std::vector<int8_t> h(prm->replicas * P * p->n);
+            v.spins(h.data());
+            prm->on_step(prm->callback_user, step, res->energy_trace[step], h.data());
+        }
     }
     std::memcpy(res->best_state, v.best.data(), p->n);
     *res->best_energy = v.best_e;
     if (res->kernel_ms) *res->kernel_ms = v.ms;
     if (res->accept_rate) *res->accept_rate = -1.0;
+}
+
+// ---- SQA tracing path (sqa_annealer.cpp:317-453) -------------------------------------------
+// With a sweep observer the reference runs the replicas one after another from the annealer's own
+// engine: slices in order, sites 0..n-1 without a shuffle, the classical dE from delta_energy (no
+// field cache), then cluster and worldline sweeps, recording after every sweep. That is the SQA-PT
+// rung sweep (pt_phase_kernel) on one replica with the identity visit order, fed from one engine.
+void sqa_trace(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    QA_REQUIRE(prm->init_spins && prm->trace_engine, "the tracing path needs init_spins and trace_engine");
+    QA_REQUIRE(res->trace_engine_out && res->trace_engine_out_len > 0, "the tracing path needs trace_engine_out");
+    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices;
+    QA_REQUIRE(P <= 4096, "the tracing path supports at most 4096 Trotter slices");
+    for (int64_t k = 0; k < R * P * n; ++k)
+        QA_REQUIRE(prm->init_spins[k] == 1 || prm->init_spins[k] == -1, "State spins must be -1 or +1.");
+    std::mt19937_64 eng;
+    {
+        std::istringstream in(prm->trace_engine);
+        in >> eng;
+        QA_REQUIRE(!in.fail(), "trace_engine is not a std::mt19937_64 state");
+    }
+    hipStream_t s = ctx->stream;
+    const int64_t cap_max = std::max<int64_t>({P * n, n * (2 * P + 2) + 64, n});
+    DevBuf<int8_t> spins(R * P * n);
+    DevBuf<double> energies(R * P), dbs(1), djp(1), djoin(1);
+    DevBuf<int32_t> order(n), dslot(1);
+    DevBuf<int64_t> used(1);
+    DevBuf<int> status(1);
+    DevBuf<uint64_t> stream(cap_max);
+    spins.upload(prm->init_spins, R * P * n, s);
+    std::vector<int32_t> ident(n);
+    std::iota(ident.begin(), ident.end(), 0);
+    order.upload(ident.data(), n, s);
+    QA_HIP(hipMemsetAsync(status.ptr, 0, sizeof(int), s));
+    std::vector<uint64_t> hstream(cap_max);
+    std::vector<double> hen(R * P), rep_e(R);
+    std::vector<int8_t> host_spins(R * P * n), best(n, 1);
+    double best_e = std::numeric_limits<double>::infinity();
+    float total_ms = 0.0f;
+    const size_t lds = ((P + 15) / 16) * 16 + P * sizeof(int);
+
+    auto run_phase = [&](int ph, int64_t r) {
+        const int64_t cap = ph == PH_SLICE ? P * n : (ph == PH_CLUSTER ? n * (2 * P + 2) + 64 : n);
+        std::mt19937_64 copy = eng;
+        for (int64_t k = 0; k < cap; ++k) hstream[k] = copy();
+        stream.upload(hstream.data(), cap, s);
+        const int32_t slot = static_cast<int32_t>(r);
+        dslot.upload(&slot, 1, s);
+        PtPhaseArgs a{view_of(p), static_cast<int>(P), ph, dbs.ptr, djp.ptr, djoin.ptr, dslot.ptr, spins.ptr,
+                      order.ptr, stream.ptr, cap, used.ptr, status.ptr};
+        QA_HIP(hipEventRecord(ctx->ev0, s));
+        hipLaunchKernelGGL(pt_phase_kernel, dim3(1), dim3(64), lds, s, a);
+        QA_HIP(hipGetLastError());
+        QA_HIP(hipEventRecord(ctx->ev1, s));
+        int64_t u = 0;
+        int st = 0;
+        used.download(&u, 1, s);
+        status.download(&st, 1, s);
+        QA_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        total_ms += ms;
+        if (st) throw Error{QA_EDEVICE, "tracing stream exhausted inside a sweep"};
+        eng.discard(static_cast<unsigned long long>(u));
+    };
+    auto energies_now = [&]() {
+        device_energies(p, spins.ptr, R * P, energies.ptr, s);
+        energies.download(hen.data(), R * P, s);
+        spins.download(host_spins.data(), R * P * n, s);
+        QA_HIP(hipStreamSynchronize(s));
+    };
+    // compute_avg_energy (sqa_annealer.cpp:112-125): per-replica slice sums in slice order
+    auto avg_energy = [&]() {
+        double total = 0.0;
+        for (int64_t r = 0; r < R; ++r) {
+            double sum = 0.0;
+            for (int64_t t = 0; t < P; ++t) sum += hen[r * P + t];
+            rep_e[r] = sum / static_cast<double>(P);
+            total += sum;
+        }
+        return total / static_cast<double>(R * P);
+    };
+    for (int64_t step = 0; step < prm->steps; ++step) {
+        const double beta = prm->betas[step], gamma = prm->gammas[step];
+        const double jp = trotter_coupling(beta, gamma, P);
+        const double bs = beta / static_cast<double>(P);
+        const double join = 1.0 - std::exp(-2.0 * jp);
+        dbs.upload(&bs, 1, s);
+        djp.upload(&jp, 1, s);
+        djoin.upload(&join, 1, s);
+        for (int64_t r = 0; r < R; ++r) {
+            const struct { int ph, code; int64_t count; } phases[3] = {
+                {PH_SLICE, 0, prm->sweeps_per_beta}, {PH_CLUSTER, 2, prm->cluster_sweeps},
+                {PH_WORLDLINE, 1, prm->worldline_sweeps}};
+            for (const auto &ph : phases)
+                for (int64_t k = 0; k < ph.count; ++k) {
+                    run_phase(ph.ph, r);
+                    energies_now();
+                    const double avg = avg_energy();
+                    prm->on_sweep(prm->callback_user, step, r, k, ph.code, avg, rep_e.data(), host_spins.data());
+                }
+        }
+        energies_now();  // step tail (:455-473)
+        double avg = 0.0;
+        int64_t arg = -1;
+        for (int64_t k = 0; k < R * P; ++k) {
+            avg += hen[k];
+            if (hen[k] < best_e) {
+                best_e = hen[k];
+                arg = k;
+            }
+        }
+        if (arg >= 0) std::memcpy(best.data(), host_spins.data() + arg * n, n);
+        res->energy_trace[step] = avg / static_cast<double>(R * P);
+        if (prm->on_step) prm->on_step(prm->callback_user, step, res->energy_trace[step], host_spins.data());
+    }
+    std::memcpy(res->best_state, best.data(), n);
+    *res->best_energy = best_e;
+    if (res->kernel_ms) *res->kernel_ms = total_ms;
+    if (res->accept_rate) *res->accept_rate = -1.0;
+    std::ostringstream o;
+    o << eng;
+    const std::string text = o.str();
+    QA_REQUIRE(static_cast<int64_t>(text.size()) < res->trace_engine_out_len, "trace_engine_out is too small");
+    std::memcpy(res->trace_engine_out, text.c_str(), text.size() + 1);
 }
 
 // ---- SQA-PT validation session (per-rung sweeps; states move between rungs on swaps) -------

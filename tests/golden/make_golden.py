@@ -262,6 +262,24 @@ def main():
                     "calib_probes": probes, "calib_sweeps": 2, "seed": seed, "out": out_json(res)})
     dump("sqapt_opt.json", {"cases": out})
 
+    # ---- observers: per-step / per-sweep records (own stream) ------------------------------------
+    rng4 = np.random.default_rng(20261206)
+    sa_cases, sqa_cases = [], []
+    for p, betas, sw, seed in [(sk(rng4, 12), np.linspace(0.2, 2.5, 8), 3, 31),
+                               (hubo_problem(rng4, 9, 12), np.linspace(0.1, 2.0, 6), 2, 32)]:
+        res = ref.sa_observe_run(p, betas, sw, seed)
+        sa_cases.append({"problem": problem_json(p), "betas": enc(betas), "sweeps_per_beta": sw, "seed": seed,
+                         "out": out_json(res)})
+    for p, P, R, spb, wl, cl, tracing, seed in [(sk(rng4, 10), 4, 2, 2, 1, 1, True, 41),
+                                                (sparse_problem(rng4, 9), 3, 2, 1, 1, 0, True, 42),
+                                                (sk(rng4, 12), 4, 3, 2, 1, 1, False, 43)]:
+        betas, gammas = np.linspace(0.5, 3.0, 5), np.geomspace(2.5, 0.1, 5)
+        res = ref.sqa_observe_run(p, betas, gammas, P, R, spb, wl, cl, tracing, seed)
+        sqa_cases.append({"problem": problem_json(p), "betas": enc(betas), "gammas": enc(gammas),
+                          "trotter_slices": P, "replicas": R, "sweeps_per_beta": spb, "worldline_sweeps": wl,
+                          "cluster_sweeps": cl, "tracing": tracing, "seed": seed, "out": out_json(res)})
+    dump("observers.json", {"sa": sa_cases, "sqa": sqa_cases})
+
 
 def dump(name, obj):
     obj["source"] = ("reference sources compiled by oracle/Makefile (Release flags), "

@@ -36,6 +36,37 @@ def test_library_exports_declared_symbols():
         assert hasattr(lib, name)
 
 
+STRUCTS = {"SaParams": "qa_sa_params", "SaResult": "qa_sa_result", "SqaParams": "qa_sqa_params",
+           "SqaResult": "qa_sqa_result", "SqaptParams": "qa_sqapt_params", "SqaptResult": "qa_sqapt_result",
+           "ChiParams": "qa_chi_params", "ChiResult": "qa_chi_result", "MlpTrainParams": "qa_mlp_train_params",
+           "BatchParams": "qa_batch_params", "BatchResult": "qa_batch_result"}
+
+
+def test_ctypes_structs_match_header(tmp_path):
+    """Every ctypes mirror in qlatanneal_amd/abi.py has the C header's size and field offsets
+    (a field missing on the Python side would let the library read past the caller's struct)."""
+    from qlatanneal_amd import abi
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "qanneal_hip.h"', 'int main(void) {']
+    for py, cname in STRUCTS.items():
+        lines.append(f'printf("{py} size %zu\\n", sizeof({cname}));')
+        for fname, _ in getattr(abi, py)._fields_:
+            lines.append(f'printf("{py} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        py, field, val = line.split()
+        got[(py, field)] = int(val)
+    for py in STRUCTS:
+        cls = getattr(abi, py)
+        assert C.sizeof(cls) == got[(py, "size")], py
+        for fname, _ in cls._fields_:
+            assert getattr(cls, fname).offset == got[(py, fname)], (py, fname)
+
+
 def test_mt64_matches_standard_and_libstdcxx():
     g = MT64()
     for _ in range(9999):

@@ -158,6 +158,47 @@ def test_optimal_schedule_invariants(orc):
     assert abs(d["resolved_j_perp_end"] - cold) < 1e-15 or d["resolved_j_perp_end"] == d["j_perp_start"] + 1e6
 
 
+def test_golden_observers(orc):
+    obs = load("observers.json")
+    for c in obs["sa"]:
+        res = orc.sa_observe_run(problem(c["problem"]), dec(c["betas"]), c["sweeps_per_beta"], c["seed"])
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(np.ravel(res[k]).astype(exp[k].dtype), np.ravel(exp[k])), k
+    for c in obs["sqa"]:
+        res = orc.sqa_observe_run(problem(c["problem"]), dec(c["betas"]), dec(c["gammas"]), c["trotter_slices"],
+                                  c["replicas"], c["sweeps_per_beta"], c["worldline_sweeps"], c["cluster_sweeps"],
+                                  c["tracing"], c["seed"])
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(np.ravel(res[k]).astype(exp[k].dtype), np.ravel(exp[k])), k
+
+
+def test_observer_records_consistent(orc):
+    """An observer does not change the SA trajectory (annealer.cpp:51-70): the result equals the
+    unobserved run, the last sweep of each step is the step-end state, and the per-step energy
+    is the running energy; the SQA tracing path's step tail is the mean over the final lattice."""
+    rng = np.random.default_rng(8)
+    n = 10
+    J = rng.normal(size=(n, n))
+    J = np.triu(J, 1) + np.triu(J, 1).T
+    p = Problem.dense(rng.normal(size=n) * 0.1, J)
+    b = np.linspace(0.1, 2.0, 7)
+    o = orc.sa_observe_run(p, b, 3, 5)
+    r = orc.sa_run(p, b, 3, 5)
+    assert bits_equal(o["energy_trace"], r["energy_trace"]) and o["best_energy"] == r["best_energy"]
+    assert bits_equal(o["sweep_energies"][2::3], o["energy_trace"])
+    last = o["sweep_states"][2::3]
+    assert np.allclose(last.mean(axis=1), o["magnetization_trace"])
+    t = orc.sqa_observe_run(p, b, np.geomspace(2, 0.1, 7), 3, 2, 1, 1, 1, True, 6)
+    R, P = 2, 3
+    per_step = R * 3
+    final = t["sweep_states"][per_step - 1::per_step].reshape(7, R * P, n)
+    e = np.array([[orc.energy(p, f[k:k + 1])[0] for k in range(R * P)] for f in final])
+    assert np.allclose(e.mean(axis=1), t["energy_trace"], atol=1e-12)
+    assert np.all(t["sweep_meta"][:, 3] == np.tile([0, 2, 1], 7 * R))
+
+
 def test_golden_sqa(orc):
     for c in load("sqa.json")["cases"]:
         res = orc.sqa_run(problem(c["problem"]), dec(c["betas"]), dec(c["gammas"]), c["trotter_slices"],
