@@ -73,6 +73,119 @@ def traffic_from_profile():
     return int(m.group(1)) if m else None
 
 
+def _line(metric_value, steps, warmup, wall, workload_cfg, roofline, dtype="f32"):
+    return {
+        "metric": METRIC, "value": metric_value, "unit": "attempts/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded SK instance, J_ij ~ N(0, 1/N))",
+        "config": workload_cfg, "roofline": roofline,
+    }
+
+
+def sk_instance(n: int, seed: int):
+    rng = np.random.default_rng(seed)
+    J = rng.normal(0.0, 1.0 / np.sqrt(n), (n, n))
+    J = np.triu(J, 1)
+    return J + J.T
+
+
+def bench_c1(args):
+    """Plain SA on a dense SK Ising: N=64, 32 reads, geometric beta 0.1 -> 3.0 over 100 steps x 10
+    sweeps; all 32 reads in one launch (Annealer.run_reads). Not HBM-bound (a 16 KB FP32 J): the
+    roofline entry states the algorithmic bytes (J row 256 B read per sweep-site step of the 32-read
+    group, fields r+w 8 B and spins r+w 2 B per attempt) against HBM peak for completeness."""
+    import qlatanneal_amd as qa
+    n, reads, steps, spb = 64, 32, 100, 10
+    J = sk_instance(n, 20261203)
+    betas = (0.1 * (30.0 ** (np.arange(steps) / (steps - 1)))).tolist()
+    ann = qa.Annealer(qa.DenseIsing(np.zeros(n), J), qa.AnnealSchedule.from_betas(betas), backend="hip")
+    attempts = reads * steps * spb * n
+    for w in range(args.warmup):
+        ann.run_reads(spb, reads, 10 + w)
+    t0 = time.perf_counter()
+    best = []
+    for k in range(args.steps):
+        best.append(min(r.best_energy for r in ann.run_reads(spb, reads, 1000 + 32 * k)))
+    wall = time.perf_counter() - t0
+    bytes_per_launch = steps * spb * n * (n * 4 + reads * 10)
+    sec = wall / args.steps
+    achieved = bytes_per_launch / sec / 1e9
+    line = _line(args.steps * attempts / wall, args.steps, args.warmup, wall,
+                 {"workload": "sa_dense_sk_n64_32reads_100x10_geometric_fp32", "N": n, "reads": reads,
+                  "steps": steps, "sweeps_per_beta": spb, "attempts_per_step": attempts,
+                  "best_energy_mean": float(np.mean(best)), "timing": "wall clock of run_reads (launch + copies)"},
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None})
+    if not args.no_cpu:
+        from oracle.orc import Oracle, Problem, REF_PORTABLE_SO, ORACLE_SO
+        path, kind = (REF_PORTABLE_SO, "reference") if os.path.exists(REF_PORTABLE_SO) else (ORACLE_SO, "port")
+        lib = Oracle(path)
+        lib.set_threads(1)
+        p = Problem.dense(np.zeros(n), J)
+        t = time.perf_counter()
+        for r in range(reads):  # solve() runs reads one after another (solver.py:679)
+            lib.sa_run(p, np.array(betas), spb, 1000 + r)
+        dt = time.perf_counter() - t
+        line["cpu_baseline"] = {"value": attempts / dt, "unit": "attempts/s", "cores": 1, "kind": kind,
+                                "sample": f"the whole workload (32 sequential reads), {dt:.2f} s"}
+    print(json.dumps(line), flush=True)
+
+
+def bench_c3(args):
+    """SQA-PT with matrix-core field refresh: dense SK N=2048, P=64, 32 rungs x 4 ladders, 10 sweeps
+    per step, 500 steps (5000 sweeps), swaps every step, FP32 fields refreshed every 100 sweeps (bf16
+    hi+lo S.J, FP32 accumulate) with the FP64 drift check. Algorithmic bytes per sweep (128
+    replicas): fields r+w 134.2 MB + spins r+w 33.6 MB + J once 16.8 MB = 11.0 B/attempt; the
+    timed device span includes the swap and refresh kernels."""
+    from qlatanneal_amd import abi
+    n, P, rungs, L, spb, steps = 2048, 64, 32, 4, 10, 500
+    J = sk_instance(n, 20261204)
+    betas = np.geomspace(0.3, 5.0, rungs)
+    gammas = np.geomspace(3.0, 0.05, rungs)
+    ctx = abi.Context(0)
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+    attempts = L * rungs * P * n * spb * steps
+    for w in range(args.warmup):
+        abi.sqapt_run(dp, betas, gammas, P, spb, 0, 5, 1, 0, L, abi.MODE_FP32, 10 + w, refresh_interval=100)
+    kms, drift, best = [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r = abi.sqapt_run(dp, betas, gammas, P, spb, 0, steps, 1, 0, L, abi.MODE_FP32, 1000 + k,
+                          refresh_interval=100)
+        kms.append(r["kernel_ms"])
+        drift.append(r["field_drift"].tolist())
+        best.append(float(np.min(r["best_energy"])))
+        print(f"step {k}: device {kms[-1]:.1f} ms best {best[-1]:.4f} drift {drift[-1]}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    sweeps = spb * steps
+    bytes_per_launch = (2 * L * rungs * P * n * 4 + 2 * L * rungs * P * n + n * n * 4) * spb
+    launch_s = float(np.mean(kms)) / 1e3 / steps
+    achieved = bytes_per_launch / launch_s / 1e9
+    line = _line(args.steps * attempts / wall, args.steps, args.warmup, wall,
+                 {"workload": "sqapt_dense_sk_n2048_p64_32rungs_x4_5000sweeps_refresh100_fp32", "N": n,
+                  "trotter_slices": P, "rungs": rungs, "ladders": L, "sweeps": sweeps,
+                  "attempts_per_step": attempts, "device_ms_mean": float(np.mean(kms)),
+                  "field_drift_before_after_refresh": drift[-1], "best_energy_per_spin": float(np.min(best)) / n},
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None})
+    if not args.no_cpu:
+        from oracle.orc import Oracle, Problem, REF_PORTABLE_SO, ORACLE_SO
+        path, kind = (REF_PORTABLE_SO, "reference") if os.path.exists(REF_PORTABLE_SO) else (ORACLE_SO, "port")
+        cores = min(16, os.cpu_count() or 1)
+        lib = Oracle(path)
+        lib.set_threads(cores)
+        p = Problem.dense(np.zeros(n), J)
+        ss = max(1, args.cpu_sample_steps // 8)
+        t = time.perf_counter()
+        lib.sqapt_run(p, betas, gammas, P, spb, 0, ss, 1, 0, 0, 7)  # one ladder
+        dt = time.perf_counter() - t
+        att = rungs * P * n * spb * ss
+        line["cpu_baseline"] = {"value": att / dt, "unit": "attempts/s", "cores": cores, "kind": kind,
+                                "sample": (f"one ladder (32 rungs), {ss} of {steps} steps of the same instance "
+                                           f"({dt:.1f} s); the reference's cost per step is acceptance-independent")}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +193,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("c2", "c1", "c3"), default="c2",
+                    help="c2 (default): the headline SQA sweep; c1: SA reads; c3: SQA-PT with field refresh")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run here has a single MI355X, and the sweep shards by "
               "independent replicas/instances (see DESIGN.md); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c1":
+        bench_c1(args)
+        return
+    if args.workload == "c3":
+        bench_c3(args)
+        return
 
     import qlatanneal_amd as qa
 

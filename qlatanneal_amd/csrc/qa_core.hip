@@ -46,40 +46,81 @@ qa_status guarded(F &&f) {
 // Every product is an FP64 value times +-1, so contraction cannot change a rounding; the sum is
 // sequential, hence bit-identical to the host.
 // ---------------------------------------------------------------------------------------------
-__global__ void energy_kernel(ProblemView pv, const int8_t *__restrict__ spins, int64_t count,
-                              double *__restrict__ out) {
-    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (k >= count) return;
+// Exact FP64 energy of resident states in the reference's order (dense_ising.cpp:29-44,
+// sparse_ising.cpp:43-56, higher_order_ising.cpp:92-111): c, then h_i s_i for i ascending, then the
+// couplings (dense: i < j row by row; sparse: edge order; HUBO: term order). One wavefront per
+// state: the lanes form 64 consecutive terms at once (each term is a coefficient times +-1, so
+// exact), and every lane adds them to its running sum one by one in order through v_readlane --
+// the same additions in the same order as one thread walking the terms, without a memory access
+// on the dependent chain.
+__device__ inline double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+__device__ inline double add_terms(double e, double term, int cnt) {
+    if (cnt == 64) {
+#pragma unroll
+        for (int q = 0; q < 64; ++q) e += readlane_f64(term, q);
+    } else {
+        for (int q = 0; q < cnt; ++q) e += readlane_f64(term, q);
+    }
+    return e;
+}
+
+__global__ void __launch_bounds__(256) energy_kernel(ProblemView pv, const int8_t *__restrict__ spins, int64_t count,
+                                                     double *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (k >= count) return;  // wave-uniform
     const int n = pv.n;
     const int8_t *s = spins + k * n;
     double e = pv.c;
-    for (int i = 0; i < n; ++i) e += pv.h64[i] * static_cast<double>(s[i]);
+    for (int b = 0; b < n; b += 64) {
+        const int i = b + lane;
+        const double t = i < n ? pv.h64[i] * static_cast<double>(s[i]) : 0.0;
+        e = add_terms(e, t, min(64, n - b));
+    }
     if (pv.kind == QA_KIND_DENSE) {
         for (int i = 0; i < n; ++i) {
             const double si = static_cast<double>(s[i]);
             const double *row = pv.J64 + static_cast<int64_t>(i) * n;
-            for (int j = i + 1; j < n; ++j) e += row[j] * si * static_cast<double>(s[j]);
+            for (int b = i + 1; b < n; b += 64) {
+                const int j = b + lane;
+                const double t = j < n ? row[j] * si * static_cast<double>(s[j]) : 0.0;
+                e = add_terms(e, t, min(64, n - b));
+            }
         }
     } else if (pv.kind == QA_KIND_SPARSE) {
-        for (int q = 0; q < pv.nedges; ++q)
-            e += pv.ev64[q] * static_cast<double>(s[pv.ei[q]]) * static_cast<double>(s[pv.ej[q]]);
+        for (int b = 0; b < pv.nedges; b += 64) {
+            const int q = b + lane;
+            const double t = q < pv.nedges
+                                 ? pv.ev64[q] * static_cast<double>(s[pv.ei[q]]) * static_cast<double>(s[pv.ej[q]])
+                                 : 0.0;
+            e = add_terms(e, t, min(64, pv.nedges - b));
+        }
     } else {
-        for (int t = 0; t < pv.nterms; ++t) {
-            double prod = pv.t_coef64[t];
-            for (int q = pv.t_off[t]; q < pv.t_off[t + 1]; ++q) prod *= static_cast<double>(s[pv.t_vars[q]]);
-            e += prod;
+        for (int b = 0; b < pv.nterms; b += 64) {
+            const int t = b + lane;
+            double prod = 0.0;
+            if (t < pv.nterms) {
+                prod = pv.t_coef64[t];
+                for (int q = pv.t_off[t]; q < pv.t_off[t + 1]; ++q) prod *= static_cast<double>(s[pv.t_vars[q]]);
+            }
+            e = add_terms(e, prod, min(64, pv.nterms - b));
         }
     }
-    out[k] = e;
+    if (lane == 0) out[k] = e;
 }
 
 void device_energies(const qa_problem *p, const int8_t *d_spins, int64_t count, double *d_out,
                      hipStream_t s) {
     if (count <= 0) return;
-    const int block = 64;
-    const int64_t grid = (count + block - 1) / block;
-    hipLaunchKernelGGL(energy_kernel, dim3(static_cast<unsigned>(grid)), dim3(block), 0, s, view_of(p),
-                       d_spins, count, d_out);
+    const int64_t grid = (count + 3) / 4;
+    hipLaunchKernelGGL(energy_kernel, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, view_of(p), d_spins, count,
+                       d_out);
     QA_HIP(hipGetLastError());
 }
 
