@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "qa_internal.hpp"
@@ -136,7 +138,8 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
             for (int i = 0; i < n; ++i, ++ctr) {
                 if (tid < kMaxReads) {
                     bool acc = false, imp = false;
-                    if ((ctr & 3) == 0)
+                    // a sweep may start inside a four-word block (n % 4 != 0): draw the block then too
+                    if ((ctr & 3) == 0 || i == 0)
                         rnd = philox4x32_10(static_cast<uint32_t>(r0 + tid), ctr >> 2, a.ctr_base, TAG_SA_STEP, a.k0, a.k1);
                     if (tid < G) {
                         int8_t *sc = S + static_cast<int64_t>(tid) * n;
@@ -223,6 +226,124 @@ __global__ void __launch_bounds__(kThreads) sa_fp32_kernel(SaArgs a) {
         for (int e = tid; e < G * n; e += kThreads) a.final_states[static_cast<int64_t>(r0) * n + e] = S[e];
 }
 
+// ---- small dense problems: one wavefront per read -------------------------------------------
+// For n <= 128 the block kernel above spends a site step on two workgroup barriers for one
+// attempt per read. Here a wavefront owns one read: site i's spin and FP32 field live in lane
+// i % 64 (register slot i / 64), the Metropolis decision is taken uniformly by every lane from the
+// value read through v_readlane, and an accepted flip updates every lane's field with one fmaf
+// from the J row staged in LDS (the row of the next site is loaded before its decision). Random
+// words, dE, the acceptance test and the field update are those of sa_fp32_kernel (same Philox
+// counters), so both kernels produce the same spin trajectory for a given key.
+constexpr int kWaveMaxN = 128;
+
+__device__ inline float readlane_f32(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+__global__ void __launch_bounds__(256) sa_wave_kernel(SaArgs a) {
+    extern __shared__ __align__(16) float Jl[];  // [n][n] FP32 couplings, zero diagonal
+    const int n = a.pv.n, lane = threadIdx.x & 63;
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) Jl[e] = a.pv.J32[e];
+    __syncthreads();
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= a.reads) return;  // wave-uniform; no barrier follows
+    const int64_t rn = static_cast<int64_t>(r) * n;
+    float sp[2], fl[2], bsp[2];
+    for (int q = 0; q < 2; ++q) {
+        const int i = lane + 64 * q;
+        sp[q] = 1.0f;
+        if (i < n) {
+            if (a.init_states) {
+                sp[q] = static_cast<float>(a.init_states[rn + i]);
+            } else {
+                const Philox4 w = philox4x32_10(static_cast<uint32_t>(r), static_cast<uint32_t>(i), 0u, TAG_SA_INIT,
+                                                a.k0, a.k1);
+                sp[q] = (w.x & 1u) ? 1.0f : -1.0f;
+            }
+        }
+    }
+    // FP64 fields in the block kernel's order (j ascending, j != i), rounded once; the starting
+    // energy c + sum_i s_i (h_i + f_i) / 2 as a wave reduction
+    double part = 0.0;
+    for (int q = 0; q < 2; ++q) {
+        const int i = lane + 64 * q;
+        double acc = i < n ? a.pv.h64[i] : 0.0;
+        for (int j = 0; j < n; ++j) {
+            const float sj = readlane_f32((j >> 6) ? sp[1] : sp[0], j & 63);
+            if (i < n && j != i) acc += a.pv.J64[static_cast<int64_t>(i) * n + j] * static_cast<double>(sj);
+        }
+        fl[q] = static_cast<float>(acc);
+        if (i < n) part += 0.5 * static_cast<double>(sp[q]) * (a.pv.h64[i] + acc);
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    double ecur = a.init_e ? a.init_e[r] : a.pv.c + part;
+    double ebest = ecur;
+    bsp[0] = sp[0];
+    bsp[1] = sp[1];
+    if (a.btrace) a.btrace[static_cast<int64_t>(r) * (a.steps + 1)] = ebest;
+    const int nq = n > 64 ? 2 : 1;
+    uint32_t ctr = 0;
+    for (int step = 0; step < a.steps; ++step) {
+        const float beta = a.read_betas ? a.read_betas[r] : a.betas[step];
+        for (int sw = 0; sw < a.sweeps; ++sw) {
+            // this sweep's random words: Philox blocks (ctr >> 2) .. ((ctr + n - 1) >> 2), one per lane
+            const uint32_t b0 = ctr >> 2;
+            const uint32_t nb = ((ctr + static_cast<uint32_t>(n) - 1u) >> 2) - b0 + 1u;
+            Philox4 rw{0, 0, 0, 0};
+            if (static_cast<uint32_t>(lane) < nb)
+                rw = philox4x32_10(static_cast<uint32_t>(r), b0 + static_cast<uint32_t>(lane), a.ctr_base, TAG_SA_STEP,
+                                   a.k0, a.k1);
+            float jr0 = Jl[min(lane, n - 1)], jr1 = nq > 1 ? Jl[min(64 + lane, n - 1)] : 0.0f;  // row 0
+            for (int i = 0; i < n; ++i, ++ctr) {
+                const int qi = i >> 6, li = i & 63;
+                const float fi = readlane_f32(qi ? fl[1] : fl[0], li);
+                const float si = readlane_f32(qi ? sp[1] : sp[0], li);
+                const int wl = static_cast<int>((ctr >> 2) - b0);
+                const uint32_t c = ctr & 3u;
+                const uint32_t w = __builtin_amdgcn_readlane(
+                    static_cast<int>(c == 0 ? rw.x : c == 1 ? rw.y : c == 2 ? rw.z : rw.w), wl);
+                const float de = -2.0f * si * fi;
+                const bool acc = de <= 0.0f || u01(w) < __expf(-beta * de);
+                const float rj0 = jr0, rj1 = jr1;
+                if (i + 1 < n) {  // next site's row, loaded before this decision resolves
+                    jr0 = Jl[(i + 1) * n + min(lane, n - 1)];
+                    if (nq > 1) jr1 = Jl[(i + 1) * n + min(64 + lane, n - 1)];
+                }
+                if (acc) {
+                    const float ds = -2.0f * si;
+                    if (lane < n) fl[0] = fmaf(ds, rj0, fl[0]);
+                    if (nq > 1 && 64 + lane < n) fl[1] = fmaf(ds, rj1, fl[1]);
+                    if (lane == li) {
+                        if (qi) sp[1] = -sp[1];
+                        else sp[0] = -sp[0];
+                    }
+                    ecur += static_cast<double>(de);
+                    if (ecur < ebest) {
+                        ebest = ecur;
+                        bsp[0] = sp[0];
+                        bsp[1] = sp[1];
+                    }
+                }
+            }
+        }
+        if (lane == 0) a.trace[static_cast<int64_t>(r) * a.steps + step] = ecur;
+        if (a.btrace && lane == 0) a.btrace[static_cast<int64_t>(r) * (a.steps + 1) + step + 1] = ebest;
+        if (a.mag) {  // magnetization (metrics.hpp:11-20): integer sum, one division
+            int sum = (lane < n ? static_cast<int>(sp[0]) : 0) + (64 + lane < n ? static_cast<int>(sp[1]) : 0);
+            for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+            if (lane == 0) a.mag[static_cast<int64_t>(r) * a.steps + step] = static_cast<double>(sum) / static_cast<double>(n);
+        }
+    }
+    for (int q = 0; q < 2; ++q) {
+        const int i = lane + 64 * q;
+        if (i < n) {
+            a.best[rn + i] = static_cast<int8_t>(bsp[q]);
+            if (a.final_states) a.final_states[rn + i] = static_cast<int8_t>(sp[q]);
+        }
+    }
+    if (lane == 0) a.best_e[r] = ebest;
+}
+
 }  // namespace
 
 void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res) {
@@ -277,12 +398,22 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     a.final_states = fstates.ptr;
     a.mag = mag.ptr;
     a.btrace = btr.ptr;
-    auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
-    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>(shmem)));
-    const unsigned grid = static_cast<unsigned>((R + kMaxReads - 1) / kMaxReads);
+    // small dense problems: one wavefront per read (QANNEAL_SA_KERNEL=block forces the block kernel)
+    const char *env = std::getenv("QANNEAL_SA_KERNEL");
+    const bool wave = p->kind == QA_KIND_DENSE && n <= kWaveMaxN && !(env && std::string(env) == "block");
     QA_HIP(hipEventRecord(ctx->ev0, s));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), shmem, s, a);
+    if (wave) {
+        const size_t jsh = static_cast<size_t>(n) * n * sizeof(float);
+        QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(sa_wave_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(jsh)));
+        hipLaunchKernelGGL(sa_wave_kernel, dim3(static_cast<unsigned>((R + 3) / 4)), dim3(256), jsh, s, a);
+    } else {
+        auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
+        QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(shmem)));
+        const unsigned grid = static_cast<unsigned>((R + kMaxReads - 1) / kMaxReads);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), shmem, s, a);
+    }
     QA_HIP(hipGetLastError());
     QA_HIP(hipEventRecord(ctx->ev1, s));
     device_energies(p, best.ptr, R, ex.ptr, s);
