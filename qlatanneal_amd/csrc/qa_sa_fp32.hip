@@ -240,6 +240,10 @@ __device__ inline float readlane_f32(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// NQ: register slots per lane (site i lives in lane i % 64, slot i / 64); n <= 64 NQ. The site
+// loop is straight-line: every update is a select on the uniform acceptance, so no branch is
+// taken per site.
+template <int NQ>
 __global__ void __launch_bounds__(256) sa_wave_kernel(SaArgs a) {
     extern __shared__ __align__(16) float Jl[];  // [n][n] FP32 couplings, zero diagonal
     const int n = a.pv.n, lane = threadIdx.x & 63;
@@ -248,11 +252,14 @@ __global__ void __launch_bounds__(256) sa_wave_kernel(SaArgs a) {
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= a.reads) return;  // wave-uniform; no barrier follows
     const int64_t rn = static_cast<int64_t>(r) * n;
-    float sp[2], fl[2], bsp[2];
-    for (int q = 0; q < 2; ++q) {
+    float sp[NQ], fl[NQ], bsp[NQ], u[NQ];
+    bool valid[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
         const int i = lane + 64 * q;
+        valid[q] = i < n;
         sp[q] = 1.0f;
-        if (i < n) {
+        if (valid[q]) {
             if (a.init_states) {
                 sp[q] = static_cast<float>(a.init_states[rn + i]);
             } else {
@@ -265,78 +272,88 @@ __global__ void __launch_bounds__(256) sa_wave_kernel(SaArgs a) {
     // FP64 fields in the block kernel's order (j ascending, j != i), rounded once; the starting
     // energy c + sum_i s_i (h_i + f_i) / 2 as a wave reduction
     double part = 0.0;
-    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
         const int i = lane + 64 * q;
-        double acc = i < n ? a.pv.h64[i] : 0.0;
+        double acc = valid[q] ? a.pv.h64[i] : 0.0;
         for (int j = 0; j < n; ++j) {
-            const float sj = readlane_f32((j >> 6) ? sp[1] : sp[0], j & 63);
-            if (i < n && j != i) acc += a.pv.J64[static_cast<int64_t>(i) * n + j] * static_cast<double>(sj);
+            float sj = 0.0f;
+#pragma unroll
+            for (int qq = 0; qq < NQ; ++qq)
+                if ((j >> 6) == qq) sj = readlane_f32(sp[qq], j & 63);
+            if (valid[q] && j != i) acc += a.pv.J64[static_cast<int64_t>(i) * n + j] * static_cast<double>(sj);
         }
         fl[q] = static_cast<float>(acc);
-        if (i < n) part += 0.5 * static_cast<double>(sp[q]) * (a.pv.h64[i] + acc);
+        if (valid[q]) part += 0.5 * static_cast<double>(sp[q]) * (a.pv.h64[i] + acc);
     }
     for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
     double ecur = a.init_e ? a.init_e[r] : a.pv.c + part;
     double ebest = ecur;
-    bsp[0] = sp[0];
-    bsp[1] = sp[1];
-    if (a.btrace) a.btrace[static_cast<int64_t>(r) * (a.steps + 1)] = ebest;
-    const int nq = n > 64 ? 2 : 1;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) bsp[q] = sp[q];
+    if (a.btrace && lane == 0) a.btrace[static_cast<int64_t>(r) * (a.steps + 1)] = ebest;
     uint32_t ctr = 0;
     for (int step = 0; step < a.steps; ++step) {
         const float beta = a.read_betas ? a.read_betas[r] : a.betas[step];
-        for (int sw = 0; sw < a.sweeps; ++sw) {
-            // this sweep's random words: Philox blocks (ctr >> 2) .. ((ctr + n - 1) >> 2), one per lane
-            const uint32_t b0 = ctr >> 2;
-            const uint32_t nb = ((ctr + static_cast<uint32_t>(n) - 1u) >> 2) - b0 + 1u;
-            Philox4 rw{0, 0, 0, 0};
-            if (static_cast<uint32_t>(lane) < nb)
-                rw = philox4x32_10(static_cast<uint32_t>(r), b0 + static_cast<uint32_t>(lane), a.ctr_base, TAG_SA_STEP,
-                                   a.k0, a.k1);
-            float jr0 = Jl[min(lane, n - 1)], jr1 = nq > 1 ? Jl[min(64 + lane, n - 1)] : 0.0f;  // row 0
-            for (int i = 0; i < n; ++i, ++ctr) {
-                const int qi = i >> 6, li = i & 63;
-                const float fi = readlane_f32(qi ? fl[1] : fl[0], li);
-                const float si = readlane_f32(qi ? sp[1] : sp[0], li);
-                const int wl = static_cast<int>((ctr >> 2) - b0);
-                const uint32_t c = ctr & 3u;
-                const uint32_t w = __builtin_amdgcn_readlane(
-                    static_cast<int>(c == 0 ? rw.x : c == 1 ? rw.y : c == 2 ? rw.z : rw.w), wl);
+        for (int sw = 0; sw < a.sweeps; ++sw, ctr += static_cast<uint32_t>(n)) {
+            // this sweep's uniforms, one per site, from the same Philox words as the block kernel
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const uint32_t c = ctr + static_cast<uint32_t>(lane + 64 * q);
+                const Philox4 w = philox4x32_10(static_cast<uint32_t>(r), c >> 2, a.ctr_base, TAG_SA_STEP, a.k0, a.k1);
+                const uint32_t k = c & 3u;
+                u[q] = u01(k == 0 ? w.x : k == 1 ? w.y : k == 2 ? w.z : w.w);
+            }
+            float jr[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) jr[q] = Jl[min(lane + 64 * q, n - 1)];  // row 0
+            for (int i = 0; i < n; ++i) {
+                const int li = i & 63;
+                float fi = 0.0f, si = 0.0f, ui = 0.0f;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    if (NQ == 1 || (i >> 6) == q) {
+                        fi = readlane_f32(fl[q], li);
+                        si = readlane_f32(sp[q], li);
+                        ui = readlane_f32(u[q], li);
+                    }
+                float rj[NQ];
+                const int nxt = min(i + 1, n - 1);  // next site's row, loaded before this decision resolves
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    rj[q] = jr[q];
+                    jr[q] = Jl[nxt * n + min(lane + 64 * q, n - 1)];
+                }
                 const float de = -2.0f * si * fi;
-                const bool acc = de <= 0.0f || u01(w) < __expf(-beta * de);
-                const float rj0 = jr0, rj1 = jr1;
-                if (i + 1 < n) {  // next site's row, loaded before this decision resolves
-                    jr0 = Jl[(i + 1) * n + min(lane, n - 1)];
-                    if (nq > 1) jr1 = Jl[(i + 1) * n + min(64 + lane, n - 1)];
+                const bool acc = de <= 0.0f || ui < __expf(-beta * de);
+                const float ds = -2.0f * si;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const float upd = fmaf(ds, rj[q], fl[q]);
+                    fl[q] = (acc && valid[q]) ? upd : fl[q];
+                    sp[q] = (acc && (NQ == 1 || (i >> 6) == q) && lane == li) ? -sp[q] : sp[q];
                 }
-                if (acc) {
-                    const float ds = -2.0f * si;
-                    if (lane < n) fl[0] = fmaf(ds, rj0, fl[0]);
-                    if (nq > 1 && 64 + lane < n) fl[1] = fmaf(ds, rj1, fl[1]);
-                    if (lane == li) {
-                        if (qi) sp[1] = -sp[1];
-                        else sp[0] = -sp[0];
-                    }
-                    ecur += static_cast<double>(de);
-                    if (ecur < ebest) {
-                        ebest = ecur;
-                        bsp[0] = sp[0];
-                        bsp[1] = sp[1];
-                    }
-                }
+                ecur = acc ? ecur + static_cast<double>(de) : ecur;
+                const bool imp = acc && ecur < ebest;
+                ebest = imp ? ecur : ebest;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) bsp[q] = imp ? sp[q] : bsp[q];
             }
         }
         if (lane == 0) a.trace[static_cast<int64_t>(r) * a.steps + step] = ecur;
         if (a.btrace && lane == 0) a.btrace[static_cast<int64_t>(r) * (a.steps + 1) + step + 1] = ebest;
         if (a.mag) {  // magnetization (metrics.hpp:11-20): integer sum, one division
-            int sum = (lane < n ? static_cast<int>(sp[0]) : 0) + (64 + lane < n ? static_cast<int>(sp[1]) : 0);
+            int sum = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) sum += valid[q] ? static_cast<int>(sp[q]) : 0;
             for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
             if (lane == 0) a.mag[static_cast<int64_t>(r) * a.steps + step] = static_cast<double>(sum) / static_cast<double>(n);
         }
     }
-    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
         const int i = lane + 64 * q;
-        if (i < n) {
+        if (valid[q]) {
             a.best[rn + i] = static_cast<int8_t>(bsp[q]);
             if (a.final_states) a.final_states[rn + i] = static_cast<int8_t>(sp[q]);
         }
@@ -404,9 +421,10 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     QA_HIP(hipEventRecord(ctx->ev0, s));
     if (wave) {
         const size_t jsh = static_cast<size_t>(n) * n * sizeof(float);
-        QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(sa_wave_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(jsh)));
-        hipLaunchKernelGGL(sa_wave_kernel, dim3(static_cast<unsigned>((R + 3) / 4)), dim3(256), jsh, s, a);
+        auto kern = n <= 64 ? sa_wave_kernel<1> : sa_wave_kernel<2>;
+        QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   static_cast<int>(jsh)));
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((R + 3) / 4)), dim3(256), jsh, s, a);
     } else {
         auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
         QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
