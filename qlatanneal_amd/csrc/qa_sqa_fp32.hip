@@ -301,7 +301,10 @@ __device__ void group_save(const SqaArgs &a, const float *F, const int8_t *S, in
 }
 
 // One replica group of the two-barrier sweep (any model, LDS or global state).
-template <bool kLds>
+// NJ > 0 (global state, dense, n % 4 == 0, n <= 1024 NJ): field updates in float4 groups, NJ per
+// thread, with every load of a batch of flipped chains issued before any of its stores, so a batch
+// costs one round trip to L2/MALL instead of one per column stride (same fmaf per element).
+template <bool kLds, int NJ = 0>
 __device__ void general_group(const SqaArgs &a, const int g) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
@@ -345,6 +348,56 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     auto apply_updates = [&]() {
         const unsigned long long m_all = *flipmask;
         if (m_all == 0ull) return;
+        if constexpr (!kLds && NJ > 0) {
+            constexpr int B = NJ >= 8 ? 1 : 8 / NJ;
+            uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
+            uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
+            unsigned long long m = (static_cast<unsigned long long>(mhi) << 32) | mlo;
+            const int n4 = n >> 2;
+            f4 *F4g = reinterpret_cast<f4 *>(F);
+            while (m) {
+                int cs[B];
+                float dsv[B];
+                const f4 *rows[B];
+#pragma unroll
+                for (int q = 0; q < B; ++q) {
+                    cs[q] = -1;
+                    dsv[q] = 0.0f;
+                    rows[q] = nullptr;
+                    if (m) {
+                        const int c = __ffsll(static_cast<long long>(m)) - 1;
+                        m &= m - 1ull;
+                        cs[q] = c;
+                        dsv[q] = DS[c];
+                        rows[q] = reinterpret_cast<const f4 *>(a.pv.J32 + static_cast<int64_t>(ROW[c]) * n);
+                    }
+                }
+                f4 rv[B][NJ], fv[B][NJ];
+#pragma unroll
+                for (int q = 0; q < B; ++q)
+#pragma unroll
+                    for (int u = 0; u < NJ; ++u) {
+                        const int gi = tid + kThreads * u;
+                        if (cs[q] >= 0 && gi < n4) {
+                            rv[q][u] = rows[q][gi];
+                            fv[q][u] = F4g[static_cast<int64_t>(cs[q]) * n4 + gi];
+                        }
+                    }
+#pragma unroll
+                for (int q = 0; q < B; ++q)
+#pragma unroll
+                    for (int u = 0; u < NJ; ++u) {
+                        const int gi = tid + kThreads * u;
+                        if (cs[q] >= 0 && gi < n4) {
+                            const f4 x = rv[q][u], y = fv[q][u];
+                            F4g[static_cast<int64_t>(cs[q]) * n4 + gi] =
+                                f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                                   fmaf(dsv[q], x.w, y.w)};
+                        }
+                    }
+            }
+            return;
+        }
         if (a.pv.kind == QA_KIND_DENSE) {
             uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
             uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
@@ -726,9 +779,9 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 }
 
 
-template <bool kLds>
+template <bool kLds, int NJ = 0>
 __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
-    general_group<kLds>(a, static_cast<int>(blockIdx.x));
+    general_group<kLds, NJ>(a, static_cast<int>(blockIdx.x));
 }
 
 // ---- pipelined slice sweeps (dense couplings, LDS-resident state) ---------------------------
@@ -1222,8 +1275,22 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
              : ngr <= 256 ? reinterpret_cast<const void *>(sqa_pipe_kernel<4>)
                           : reinterpret_cast<const void *>(sqa_pipe_kernel<8>);
     } else {
-        kern = pl.use_lds ? reinterpret_cast<const void *>(sqa_fp32_kernel<true>)
-                          : reinterpret_cast<const void *>(sqa_fp32_kernel<false>);
+        const int n = a.pv.n;
+        const char *env = std::getenv("QANNEAL_SQA_GLOBAL_F4");
+        const bool f4ok = !pl.use_lds && a.pv.kind == QA_KIND_DENSE && n % 4 == 0 && n <= 8192 &&
+                          !(env && env[0] == '0');
+        if (pl.use_lds)
+            kern = reinterpret_cast<const void *>(sqa_fp32_kernel<true>);
+        else if (!f4ok)
+            kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false>);
+        else if (n <= 1024)
+            kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false, 1>);
+        else if (n <= 2048)
+            kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false, 2>);
+        else if (n <= 4096)
+            kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false, 4>);
+        else
+            kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false, 8>);
     }
     QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pl.shmem)));
     void *params[] = {&a};

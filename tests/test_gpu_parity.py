@@ -89,6 +89,46 @@ _SWEEP_PATHS = {"lockstep": {"QANNEAL_SQA_PIPELINE": "0"},
                 "pipelined": {"QANNEAL_SQA_PIPELINE": "1"}}
 
 
+def test_sqa_fp32_global_state_update_paths_bit_identical(ctx, monkeypatch):
+    """Global-memory state (fields too large for LDS): the float4 batched field update (all loads of
+    a batch of flipped chains issued before its stores) must reproduce the column-loop update bit
+    for bit -- slice, cluster and worldline sweeps, SQA-PT, and the sqa_chi checkerboard."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(43)
+    for n, P, R, spb, wl, cl in [(1100, 64, 2, 1, 1, 1), (2048, 64, 1, 2, 0, 0), (700, 48, 3, 1, 0, 1)]:
+        print("global-state case", n, P, R, spb, wl, cl, flush=True)
+        J = _sk(rng, n)
+        dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), J, 0.5)
+        betas, gammas = np.full(4, 2.0), np.geomspace(3.0, 0.05, 4)
+        res = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", v)
+            res.append(abi.sqa_run(dp, betas, gammas, P, R, spb, wl, cl, abi.MODE_FP32, 17))
+        a, b = res
+        assert a["accept_rate"] == b["accept_rate"]
+        assert np.array_equal(a["best_state"], b["best_state"]) and bits_equal([a["best_energy"]], [b["best_energy"]])
+        assert bits_equal(a["energy_trace"], b["energy_trace"])
+    n, P = 1024, 64
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), _sk(rng, n))
+    res = []
+    for v in ("1", "0"):
+        monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", v)
+        res.append(abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 3), np.geomspace(3.0, 0.05, 3), P, 2, 0, 3, 1, 0,
+                                 ladders=2, mode=abi.MODE_FP32, seed=5))
+    for key in ("final_states", "best_state"):
+        assert np.array_equal(res[0][key], res[1][key]), key
+    for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
+        assert bits_equal(res[0][key].ravel(), res[1][key].ravel()), key
+    res = []
+    for v in ("1", "0"):
+        monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", v)
+        res.append(abi.sqa_chi_run(dp, 64, 1, 2.0, 3.0, 0.05, num_steps=6, sweeps_per_step=2, scan_points=4,
+                                   scan_sweeps=2, scan_burn=1, seed=9))
+    assert np.array_equal(res[0]["best_state"], res[1]["best_state"])
+    assert bits_equal(res[0]["energy_trace"], res[1]["energy_trace"])
+    monkeypatch.delenv("QANNEAL_SQA_GLOBAL_F4", raising=False)
+
+
 def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
     """The one-barrier pipelined sweep (decisions of step t overlapped with the field updates of
     step t-1) must reproduce the two-barrier lockstep sweep bit for bit: same spins, same FP32
