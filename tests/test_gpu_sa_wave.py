@@ -52,3 +52,27 @@ def test_sa_wave_kernel_equals_block_kernel(monkeypatch):
             assert np.array_equal(a2["final_states"], b2["final_states"])
             assert np.array_equal(a2["energy_traces"], b2["energy_traces"])  # same start energy: exact
     monkeypatch.delenv("QANNEAL_SA_KERNEL", raising=False)
+
+
+def test_sa_fp32_distribution_vs_oracle_256_reads():
+    """Residual-energy distributions of FP32 SA against the oracle (the reference's CPU
+    algorithm), two-sample KS p > 0.01 over 256 reads: the C1 configuration (N=64 SK, geometric
+    beta 0.1 -> 3.0, 100 x 10, wave kernel), a short anneal whose energies are broad (wave kernel)
+    and a larger instance (N=200, block kernel). Energies are rounded to 1e-9 so exact ties stay
+    ties."""
+    from scipy.stats import ks_2samp
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    ctx = abi.Context(0)
+    orc = Oracle(ORACLE_SO)
+    rng = np.random.default_rng(71)
+    geo = 0.1 * 30.0 ** (np.arange(100) / 99.0)
+    for n, betas, spb in [(64, geo, 10), (64, np.linspace(0.1, 1.5, 12), 1), (200, np.linspace(0.1, 2.0, 20), 2)]:
+        J = _sk(rng, n)
+        dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+        gpu = abi.sa_run(dp, betas, spb, 256, abi.MODE_FP32, seed=5 + n)["best_energies"]
+        p = Problem.dense(np.zeros(n), J)
+        cpu = np.array([orc.sa_run(p, betas, spb, 1000 + k)["best_energy"] for k in range(256)])
+        pv = ks_2samp(np.round(gpu, 9), np.round(cpu, 9)).pvalue
+        print("SA KS", n, len(betas), spb, "p", pv, "gpu mean", gpu.mean(), "cpu mean", cpu.mean(), flush=True)
+        assert pv > 0.01
