@@ -415,6 +415,20 @@ def solve(problem: Any,
     samples: List[np.ndarray] = []
     energies: List[float] = []
     trace: Optional[List[float]] = None
+    if method == "sa" and seed is not None and reads > 1:
+        # Seeded SA reads are independent chains: run them in one launch. Read r starts from the
+        # engine state set_seed(seed + r) gives (Annealer.run_reads), as the loop below would.
+        batch = annealer.run_reads(sweeps_per_beta, reads, seed)
+        for r in _iter_reads(reads, progress, logger):
+            res = batch[r]
+            spins = np.array(res.best_state.spins, dtype=int)
+            samples.append(((spins + 1) // 2).astype(int) if (return_bits and is_qubo) else spins)
+            energies.append(float(res.best_energy))
+            if trace is None:
+                trace = list(res.energy_trace)
+        best = int(np.argmin(energies))
+        return SolveResult(method=method, samples=samples, energies=energies, best_sample=samples[best],
+                           best_energy=energies[best], trace=trace, var_order=var_order, return_bits=return_bits)
     for r in _iter_reads(reads, progress, logger):
         if seed is not None:
             annealer.set_seed(seed + int(r))

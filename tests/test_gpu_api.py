@@ -129,6 +129,28 @@ def test_solve_seeded_reads_reproducible():
     assert a.energies == b.energies
 
 
+def test_solve_sa_reads_batched_equal_sequential():
+    """solve(method="sa", seed=s) runs its reads in one launch; in validation mode every read equals
+    set_seed(s + r) followed by Annealer.run, as the reference's read loop (solver.py:679) does."""
+    rng = np.random.default_rng(15)
+    n = 20
+    J = _sk(rng, n)
+    d = qa.DenseIsing(rng.normal(0, 0.2, n), J)
+    sched = qa.auto_schedule_sa()
+    res = qa.solve(d, method="sa", reads=5, seed=11, backend="hip_fp64", progress=False, sweeps_per_beta=10,
+                   schedule=sched)
+    a = qa.Annealer(d, sched, backend="hip_fp64")
+    for r in range(5):
+        a.set_seed(11 + r)
+        one = a.run(10)
+        assert res.energies[r] == one.best_energy
+        assert list(res.samples[r]) == list(one.best_state.spins)
+    fast = qa.solve(d, method="sa", reads=8, seed=3, progress=False)
+    assert len(fast.energies) == 8 and fast.best_energy == min(fast.energies)
+    for s_, e in zip(fast.samples, fast.energies):
+        assert abs(d.energy(np.asarray(s_).astype(int).tolist()) - e) < 1e-9
+
+
 def test_sa_fp32_reads():
     rng = np.random.default_rng(14)
     n = 64
