@@ -73,12 +73,12 @@ def traffic_from_profile():
     return int(m.group(1)) if m else None
 
 
-def _line(metric_value, steps, warmup, wall, workload_cfg, roofline, dtype="f32"):
+def _line(metric_value, steps, warmup, wall, workload_cfg, roofline, dtype="f32",
+          data="synthetic (seeded SK instance, J_ij ~ N(0, 1/N))"):
     return {
         "metric": METRIC, "value": metric_value, "unit": "attempts/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded SK instance, J_ij ~ N(0, 1/N))",
-        "config": workload_cfg, "roofline": roofline,
+        "vs_baseline": None, "dtype": dtype, "data": data, "config": workload_cfg, "roofline": roofline,
     }
 
 
@@ -186,6 +186,92 @@ def bench_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def c4_sample(count: int, seed: int = 20261207):
+    """A seeded sample of the C4 instance mix: 80 % dense QUBOs (N ~ U[50, 1500], edge density
+    ~ U[0.05, 1], integer Q_ij ~ U{-100..100} symmetric on the kept pairs, converted with
+    QUBO.to_ising) and 20 % 3-body HUBOs (N = 256, 4N distinct triplets, J ~ N(0, 1))."""
+    import qlatanneal_amd as qa
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        if k % 5 == 4:
+            n = 256
+            trip = set()
+            while len(trip) < 4 * n:
+                trip.add(tuple(sorted(rng.choice(n, 3, replace=False).tolist())))
+            terms = [(list(t), float(rng.normal())) for t in sorted(trip)]
+            out.append(("hubo", n, qa.HigherOrderIsing(terms, n)))
+        else:
+            n = int(rng.integers(50, 1501))
+            dens = rng.uniform(0.05, 1.0)
+            Q = rng.integers(-100, 101, (n, n)).astype(float)
+            keep = rng.random((n, n)) < dens
+            Q = np.triu(Q * keep)
+            Q = Q + np.triu(Q, 1).T
+            out.append(("qubo", n, qa.QUBO(Q).to_ising()))
+    return out
+
+
+def bench_c4(args):
+    """C4's instances through the plain per-instance GPU path (the batched persistent launch is
+    switched off): SQA P=16, 64 replicas per instance (the 64 reads of an instance run together),
+    auto_schedule_sqa (50 steps x 20 slice sweeps + 5 worldline sweeps). One timed step = a
+    seeded sample of `--c4-instances` instances of the C4 mix, run one after another; algorithmic
+    bytes ~10 B per attempt (fields and spins read and written) plus J once per instance sweep."""
+    import qlatanneal_amd as qa
+    P, R, spb, wl = 16, 64, 20, 5
+    sched = qa.auto_schedule_sqa()
+    steps = len(sched.betas)
+    inst = c4_sample(args.c4_instances)
+    attempts = sum(steps * R * (spb * P * n + wl * n) for _, n, _ in inst)
+    nbytes = sum(steps * (spb + wl) * (R * P * n * 10 + n * n * 4) for _, n, _ in inst)
+    anns = [qa.SQAAnnealer(h, sched, P, R, backend="hip") for _, _, h in inst]
+    for w in range(args.warmup):
+        for a in anns[:4]:
+            a.set_seed(10 + w)
+            a.run(spb, wl)
+    kms, best = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        tot = 0.0
+        for i, a in enumerate(anns):
+            a.set_seed(1000 + 97 * k + i)
+            best.append(a.run(spb, wl).best_energy)
+            tot += a.last_kernel_ms()
+        kms.append(tot)
+        print(f"step {k}: sweep kernels {tot:.1f} ms over {len(anns)} instances", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    kern_s = float(np.mean(kms)) / 1e3
+    achieved = nbytes / kern_s / 1e9
+    sizes = sorted(n for _, n, _ in inst)
+    line = _line(args.steps * attempts / wall, args.steps, args.warmup, wall,
+                 {"workload": f"c4_mix_sample{len(inst)}_sqa_p16_r64_auto50x(20+5)_fp32_per_instance",
+                  "instances": len(inst), "hubo": sum(1 for k, _, _ in inst if k == "hubo"),
+                  "n_min": sizes[0], "n_median": sizes[len(sizes) // 2], "n_max": sizes[-1],
+                  "attempts_per_step": attempts, "sweep_kernel_ms_mean": kern_s * 1e3,
+                  "path": "plain per-instance qa_sqa_run (the batched launch is switched off)"},
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                 data="synthetic (seeded C4 mix: integer QUBOs of random density and 3-body HUBOs)")
+    if not args.no_cpu:
+        from oracle.orc import Oracle, Problem, REF_PORTABLE_SO, ORACLE_SO
+        path, kind = (REF_PORTABLE_SO, "reference") if os.path.exists(REF_PORTABLE_SO) else (ORACLE_SO, "port")
+        lib = Oracle(path)
+        lib.set_threads(1)  # one read of one replica: the reference's OpenMP fork is over replicas
+        _, n, h = next(x for x in inst if x[0] == "qubo")
+        p = Problem.dense(np.asarray(h.fields()), np.asarray(h.couplings()), h.constant())
+        nreads = 4
+        t = time.perf_counter()
+        for r in range(nreads):  # the reference runs an instance's reads one after another
+            lib.sqa_run(p, np.array(sched.betas), np.array(sched.gammas), P, 1, spb, wl, 0, 0, 3 + r)
+        dt = time.perf_counter() - t
+        att = nreads * steps * (spb * P * n + wl * n)
+        line["cpu_baseline"] = {"value": att / dt, "unit": "attempts/s", "cores": 1, "kind": kind,
+                                "sample": (f"{nreads} of the 64 reads (1 replica each, the full schedule) of the "
+                                           f"first QUBO instance (N={n}), {dt:.1f} s")}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,8 +279,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c1", "c3"), default="c2",
-                    help="c2 (default): the headline SQA sweep; c1: SA reads; c3: SQA-PT with field refresh")
+    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c4"), default="c2",
+                    help="c2 (default): the headline SQA sweep; c1: SA reads; c3: SQA-PT with field refresh; "
+                         "c4: a sample of the batched-QUBO mix through the per-instance path")
+    ap.add_argument("--c4-instances", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run here has a single MI355X, and the sweep shards by "
@@ -205,6 +293,9 @@ def main():
         return
     if args.workload == "c3":
         bench_c3(args)
+        return
+    if args.workload == "c4":
+        bench_c4(args)
         return
 
     import qlatanneal_amd as qa
