@@ -186,6 +186,13 @@ def bench_c3(args):
     print(json.dumps(line), flush=True)
 
 
+class _Hubo:
+    """A HigherOrderIsing plus its (already consolidated: distinct sorted triplets) terms."""
+
+    def __init__(self, ham, trips, coeffs):
+        self.ham, self.trips, self.coeffs = ham, trips, coeffs
+
+
 def c4_sample(count: int, seed: int = 20261207):
     """A seeded sample of the C4 instance mix: 80 % dense QUBOs (N ~ U[50, 1500], edge density
     ~ U[0.05, 1], integer Q_ij ~ U{-100..100} symmetric on the kept pairs, converted with
@@ -199,8 +206,10 @@ def c4_sample(count: int, seed: int = 20261207):
             trip = set()
             while len(trip) < 4 * n:
                 trip.add(tuple(sorted(rng.choice(n, 3, replace=False).tolist())))
-            terms = [(list(t), float(rng.normal())) for t in sorted(trip)]
-            out.append(("hubo", n, qa.HigherOrderIsing(terms, n)))
+            trips = sorted(trip)
+            coeffs = [float(rng.normal()) for _ in trips]
+            hobj = qa.HigherOrderIsing([(list(t), c) for t, c in zip(trips, coeffs)], n)
+            out.append(("hubo", n, _Hubo(hobj, trips, coeffs)))
         else:
             n = int(rng.integers(50, 1501))
             dens = rng.uniform(0.05, 1.0)
@@ -216,8 +225,9 @@ def bench_c4(args):
     """C4's instances through the plain per-instance GPU path (the batched persistent launch is
     switched off): SQA P=16, 64 replicas per instance (the 64 reads of an instance run together),
     auto_schedule_sqa (50 steps x 20 slice sweeps + 5 worldline sweeps). One timed step = a
-    seeded sample of `--c4-instances` instances of the C4 mix, run one after another; algorithmic
-    bytes ~10 B per attempt (fields and spins read and written) plus J once per instance sweep."""
+    seeded sample of `--c4-instances` instances of the C4 mix, driven through the SQAAnnealer
+    class by `--c4-streams` host threads (1: one after another); algorithmic bytes ~10 B per
+    attempt (fields and spins read and written) plus J once per instance sweep."""
     import qlatanneal_amd as qa
     P, R, spb, wl = 16, 64, 20, 5
     sched = qa.auto_schedule_sqa()
@@ -225,23 +235,33 @@ def bench_c4(args):
     inst = c4_sample(args.c4_instances)
     attempts = sum(steps * R * (spb * P * n + wl * n) for _, n, _ in inst)
     nbytes = sum(steps * (spb + wl) * (R * P * n * 10 + n * n * 4) for _, n, _ in inst)
-    anns = [qa.SQAAnnealer(h, sched, P, R, backend="hip") for _, _, h in inst]
-    for w in range(args.warmup):
-        for a in anns[:4]:
-            a.set_seed(10 + w)
-            a.run(spb, wl)
     kms, best = [], []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        tot = 0.0
-        for i, a in enumerate(anns):
-            a.set_seed(1000 + 97 * k + i)
-            best.append(a.run(spb, wl).best_energy)
-            tot += a.last_kernel_ms()
-        kms.append(tot)
-        print(f"step {k}: sweep kernels {tot:.1f} ms over {len(anns)} instances", file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    kern_s = float(np.mean(kms)) / 1e3
+    # Each annealer's backend is bound round-robin to one of the library's HIP contexts (one stream
+    # each); K host threads each drive every K-th instance, so with K > 1 the runs of different
+    # streams execute concurrently on the GPU's hardware queues (the bindings release the GIL).
+    anns = [qa.SQAAnnealer(h.ham if kind == "hubo" else h, sched, P, R, backend="hip") for kind, _, h in inst]
+    K = max(1, args.c4_streams)
+
+    def lane(j, seed0, limit=None):
+        out = []
+        for i in list(range(j, len(anns), K))[:limit]:
+            anns[i].set_seed(seed0 + i)
+            out.append((anns[i].run(spb, wl).best_energy, anns[i].last_kernel_ms()))
+        return out
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(K) as pool:
+        for w in range(args.warmup):
+            list(pool.map(lambda j: lane(j, 10 + w, 1), range(K)))
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            res = [x for part in pool.map(lambda j: lane(j, 1000 + 97 * k), range(K)) for x in part]
+            best.extend(e for e, _ in res)
+            kms.append(sum(m for _, m in res))
+            print(f"step {k}: {K} stream(s), sweep kernels {kms[-1]:.1f} ms summed over {len(anns)} instances",
+                  file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+    kern_s = float(np.mean(kms)) / 1e3 if K == 1 else wall / args.steps
     achieved = nbytes / kern_s / 1e9
     sizes = sorted(n for _, n, _ in inst)
     line = _line(args.steps * attempts / wall, args.steps, args.warmup, wall,
@@ -249,7 +269,9 @@ def bench_c4(args):
                   "instances": len(inst), "hubo": sum(1 for k, _, _ in inst if k == "hubo"),
                   "n_min": sizes[0], "n_median": sizes[len(sizes) // 2], "n_max": sizes[-1],
                   "attempts_per_step": attempts, "sweep_kernel_ms_mean": kern_s * 1e3,
-                  "path": "plain per-instance qa_sqa_run (the batched launch is switched off)"},
+                  "streams": K,
+                  "path": ("SQAAnnealer per instance (the batched launch is switched off)"
+                           + ("" if K == 1 else f", {K} host threads over the library's HIP streams"))},
                  {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK_GBS, "traffic": None},
                  data="synthetic (seeded C4 mix: integer QUBOs of random density and 3-body HUBOs)")
@@ -283,6 +305,9 @@ def main():
                     help="c2 (default): the headline SQA sweep; c1: SA reads; c3: SQA-PT with field refresh; "
                          "c4: a sample of the batched-QUBO mix through the per-instance path")
     ap.add_argument("--c4-instances", type=int, default=20)
+    ap.add_argument("--c4-streams", type=int, default=4,
+                    help="c4: host threads driving the instances (each annealer runs on one of the "
+                         "library's HIP streams; 1 runs the instances one after another)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run here has a single MI355X, and the sweep shards by "

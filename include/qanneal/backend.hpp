@@ -81,7 +81,9 @@ protected:
 };
 
 // HIP backend: the per-spin host methods of CPUBackend plus a device copy of the problem,
-// uploaded on first use into the process-wide context of the selected GPU and reused by every run.
+// uploaded on first use and reused by every run. Each backend is bound at construction to one of
+// a small pool of contexts (one HIP stream each) on the selected GPU, assigned round-robin, so
+// runs of different backends from different host threads execute concurrently on the GPU.
 class HIPBackend final : public CPUBackend {
 public:
     HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision);
@@ -93,15 +95,20 @@ public:
 
     qa_ctx *context() const;
     qa_problem *device_problem() const;  // uploads on first call
-    std::mutex &device_mutex() const;    // serializes runs on the shared context
+    std::mutex &device_mutex() const;    // serializes runs on this backend's context
 
 private:
     DevicePrecision precision_;
+    int slot_;                           // context pool slot
     mutable qa_problem *problem_ = nullptr;
     mutable std::mutex upload_mu_;
 };
 
 std::shared_ptr<Backend> make_backend(BackendSpec spec, std::shared_ptr<const Hamiltonian> ham);
+
+// Number of HIP contexts (streams) HIP backends are spread over: QANNEAL_HIP_STREAMS (1..16),
+// default 4. No GPU call is made.
+int hip_context_count();
 
 inline std::shared_ptr<Backend> make_backend(BackendKind kind, std::shared_ptr<const Hamiltonian> ham) {
     return make_backend(BackendSpec{kind, DevicePrecision::FP32}, std::move(ham));

@@ -28,7 +28,7 @@ from .solver import (  # noqa: F401
 __version__ = version_string()
 
 __all__ = [
-    "__version__", "version_string", "version_major", "version_minor", "version_patch",
+    "__version__", "version_string", "version_major", "version_minor", "version_patch", "hip_context_count",
     "State", "Hamiltonian", "DenseIsing", "SparseEdge", "SparseIsing", "QUBO", "HigherOrderIsing",
     "AnnealSchedule", "Observer", "MetricsObserver", "StateTraceObserver", "AnnealResult", "Annealer", "ReplicaResult", "MultiAnnealResult",
     "ReplicaAnnealer", "ParallelTemperingAnnealer", "ParallelTemperingResult", "SQASchedule", "SQAObserver", "SQASweepPhase",

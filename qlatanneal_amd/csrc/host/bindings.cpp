@@ -79,6 +79,8 @@ std::shared_ptr<Backend> backend_for(std::shared_ptr<Hamiltonian> ham, const std
 PYBIND11_MODULE(_qanneal, m) {
     m.doc() = "qanneal annealing-sweep API on the gfx950 HIP backend";
     m.def("version_string", &version_string);
+    m.def("hip_context_count", &hip_context_count,
+          "HIP contexts (one stream each) that device engines are spread over, round-robin");
     m.attr("version_major") = version_major;
     m.attr("version_minor") = version_minor;
     m.attr("version_patch") = version_patch;

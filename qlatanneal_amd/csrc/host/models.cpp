@@ -1,12 +1,15 @@
 // Host FP64 methods of the problem models, QUBO conversion, and the HIP backend's device plumbing.
 // Semantics follow dense_ising.cpp, sparse_ising.cpp, higher_order_ising.cpp and qubo.cpp.
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "qanneal/backend.hpp"
 #include "qanneal/dense_ising.hpp"
@@ -282,28 +285,64 @@ struct DeviceContext {
     std::mutex run_mu;
 };
 
-DeviceContext &device_context() {
-    static DeviceContext dc;
-    static std::once_flag once;
-    std::call_once(once, [] {
+// Contexts of the selected GPU, one HIP stream each. A backend is bound to one slot for its
+// lifetime (its device problem lives in that slot's context); backends on different slots run
+// concurrently when driven from different host threads. The pool size defaults to the GPU's
+// hardware queues per process (4) and can be set with QANNEAL_HIP_STREAMS (1..16).
+struct ContextPool {
+    int device = 0;
+    std::vector<std::unique_ptr<DeviceContext>> slots;
+    std::once_flag once;
+};
+
+ContextPool &context_pool() {
+    static ContextPool pool;
+    return pool;
+}
+
+int pool_size() {
+    static const int k = [] {
+        const char *env = std::getenv("QANNEAL_HIP_STREAMS");
+        const int v = env ? std::atoi(env) : 4;
+        return std::clamp(v, 1, 16);
+    }();
+    return k;
+}
+
+int next_slot() {
+    static std::atomic<unsigned> counter{0};
+    return static_cast<int>(counter.fetch_add(1, std::memory_order_relaxed) % static_cast<unsigned>(pool_size()));
+}
+
+DeviceContext &device_context(int slot) {
+    ContextPool &pool = context_pool();
+    std::call_once(pool.once, [&pool] {
         const char *env = std::getenv("QANNEAL_HIP_DEVICE");
-        const int dev = env ? std::atoi(env) : 0;
-        throw_on_status(qa_ctx_create(dev, &dc.ctx));
+        pool.device = env ? std::atoi(env) : 0;
+        std::vector<std::unique_ptr<DeviceContext>> slots;
+        for (int k = 0; k < pool_size(); ++k) {
+            auto dc = std::make_unique<DeviceContext>();
+            throw_on_status(qa_ctx_create(pool.device, &dc->ctx));
+            slots.push_back(std::move(dc));
+        }
+        pool.slots = std::move(slots);
     });
-    return dc;
+    return *pool.slots[static_cast<std::size_t>(slot)];
 }
 }  // namespace
 
+int hip_context_count() { return pool_size(); }
+
 HIPBackend::HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision)
-    : CPUBackend(std::move(ham)), precision_(precision) {}
+    : CPUBackend(std::move(ham)), precision_(precision), slot_(next_slot()) {}
 
 HIPBackend::~HIPBackend() {
     if (problem_) qa_problem_destroy(problem_);
 }
 
-qa_ctx *HIPBackend::context() const { return device_context().ctx; }
+qa_ctx *HIPBackend::context() const { return device_context(slot_).ctx; }
 
-std::mutex &HIPBackend::device_mutex() const { return device_context().run_mu; }
+std::mutex &HIPBackend::device_mutex() const { return device_context(slot_).run_mu; }
 
 qa_problem *HIPBackend::device_problem() const {
     std::lock_guard<std::mutex> lock(upload_mu_);

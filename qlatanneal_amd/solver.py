@@ -9,6 +9,8 @@ from __future__ import annotations
 
 import logging
 import math
+import queue
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Any, Iterable, List, Optional, Sequence, Tuple
 
@@ -25,6 +27,7 @@ from ._qanneal import (
     SQAChiAnnealer,
     SQAParallelTemperingAnnealer,
     SQASchedule,
+    hip_context_count,
 )
 
 try:  # optional input formats, as in the reference
@@ -339,8 +342,11 @@ def solve(problem: Any,
           chi_driver_A0: float = 0.0) -> SolveResult:
     """Solve with "sa", "sqa", "sqapt" or "sqa_chi" (reference solve(), solver.py:455-789).
 
-    Reads run one after another with set_seed(seed + r) per read, as in the reference, so a
-    seeded call is reproducible; every run executes on the GPU.
+    Read r of a seeded call is set_seed(seed + r) and one run, as in the reference's read loop, so
+    the call is reproducible. Seeded reads run concurrently, on one engine per HIP context
+    (hip_context_count()), unless a debug CSV is requested; seeded SA reads run in one launch.
+    Unseeded reads run one after another on one engine, continuing its random stream. Every run
+    executes on the GPU.
     """
     method = method.lower().strip()
     schedule_type = schedule_type.lower().strip()
@@ -370,11 +376,7 @@ def solve(problem: Any,
         else:
             schedule = auto_schedule_sa()
 
-    if method == "sqa":
-        annealer = SQAAnnealer(ising, schedule, trotter_slices, replicas, backend=backend)
-    elif method == "sqa_chi":
-        annealer = SQAChiAnnealer(ising, trotter_slices, replicas, backend=backend)
-    elif method == "sqapt":
+    if method == "sqapt":
         if pt_betas is not None or pt_gammas is not None:
             if pt_betas is None or pt_gammas is None:
                 raise ValueError("pt_betas and pt_gammas must both be provided.")
@@ -384,9 +386,17 @@ def solve(problem: Any,
         else:
             ladder = auto_ladder_sqa_tuned(ising, replicas=max(2, replicas))
             lb, lg = ladder.betas, ladder.gammas
-        annealer = SQAParallelTemperingAnnealer(ising, lb, lg, trotter_slices, backend=backend)
-    else:
-        annealer = Annealer(ising, schedule, backend=backend)
+
+    def make_annealer():
+        if method == "sqa":
+            return SQAAnnealer(ising, schedule, trotter_slices, replicas, backend=backend)
+        if method == "sqa_chi":
+            return SQAChiAnnealer(ising, trotter_slices, replicas, backend=backend)
+        if method == "sqapt":
+            return SQAParallelTemperingAnnealer(ising, lb, lg, trotter_slices, backend=backend)
+        return Annealer(ising, schedule, backend=backend)
+
+    annealer = make_annealer()
 
     opt = {}
     if schedule_type == "optimal" and method in ("sqa", "sqapt"):
@@ -429,53 +439,84 @@ def solve(problem: Any,
         best = int(np.argmin(energies))
         return SolveResult(method=method, samples=samples, energies=energies, best_sample=samples[best],
                            best_energy=energies[best], trace=trace, var_order=var_order, return_bits=return_bits)
-    for r in _iter_reads(reads, progress, logger):
+
+    def one_read(ann, r):
         if seed is not None:
-            annealer.set_seed(seed + int(r))
+            ann.set_seed(seed + int(r))
         if schedule_type == "optimal" and method == "sqa":
-            res = annealer.run_optimal(beta=opt["beta"], j_perp_start=opt["jps"], j_perp_end=opt["jpe"],
-                                       eps_tilde=optimal_eps_tilde, alpha=optimal_alpha, num_steps=opt["steps"],
-                                       sweeps_per_step=sweeps_per_beta, worldline_sweeps=worldline_sweeps,
-                                       cluster_sweeps=cluster_sweeps, calib_probes=int(optimal_calib_probes),
-                                       calib_sweeps=int(optimal_calib_sweeps),
-                                       debug_csv_path=(optimal_debug_csv or ""),
-                                       beta_ramp_fraction=float(optimal_beta_ramp_fraction))
+            res = ann.run_optimal(beta=opt["beta"], j_perp_start=opt["jps"], j_perp_end=opt["jpe"],
+                                  eps_tilde=optimal_eps_tilde, alpha=optimal_alpha, num_steps=opt["steps"],
+                                  sweeps_per_step=sweeps_per_beta, worldline_sweeps=worldline_sweeps,
+                                  cluster_sweeps=cluster_sweeps, calib_probes=int(optimal_calib_probes),
+                                  calib_sweeps=int(optimal_calib_sweeps),
+                                  debug_csv_path=(optimal_debug_csv or ""),
+                                  beta_ramp_fraction=float(optimal_beta_ramp_fraction))
             local = list(res.energy_trace)
         elif method == "sqa_chi":
-            res = annealer.run_chi(beta=chi["beta"], gamma_start=chi["gs"], gamma_end=chi["ge"],
-                                   num_steps=chi["steps"], sweeps_per_step=sweeps_per_beta,
-                                   scan_points=int(chi_scan_points), scan_sweeps=int(chi_scan_sweeps),
-                                   scan_burn=int(chi_scan_burn), chi_floor_fraction=float(chi_floor_fraction),
-                                   driver_A0=float(chi_driver_A0),
-                                   beta_ramp_fraction=float(optimal_beta_ramp_fraction),
-                                   debug_csv_path=(optimal_debug_csv or ""))
+            res = ann.run_chi(beta=chi["beta"], gamma_start=chi["gs"], gamma_end=chi["ge"],
+                              num_steps=chi["steps"], sweeps_per_step=sweeps_per_beta,
+                              scan_points=int(chi_scan_points), scan_sweeps=int(chi_scan_sweeps),
+                              scan_burn=int(chi_scan_burn), chi_floor_fraction=float(chi_floor_fraction),
+                              driver_A0=float(chi_driver_A0),
+                              beta_ramp_fraction=float(optimal_beta_ramp_fraction),
+                              debug_csv_path=(optimal_debug_csv or ""))
             local = list(res.energy_trace)
         elif schedule_type == "optimal" and method == "sqapt":
-            res = annealer.run_optimal(num_steps=opt["steps"], sweeps_per_step=sweeps_per_beta,
-                                       worldline_sweeps=worldline_sweeps, eps_tilde=optimal_eps_tilde,
-                                       alpha=optimal_alpha, j_perp_end=opt["jpe"], cluster_sweeps=cluster_sweeps,
-                                       swap_interval=swap_interval, continuous_time_slices=continuous_time_slices,
-                                       calib_probes=int(optimal_calib_probes),
-                                       calib_sweeps=int(optimal_calib_sweeps),
-                                       debug_csv_path=(optimal_debug_csv or ""))
+            res = ann.run_optimal(num_steps=opt["steps"], sweeps_per_step=sweeps_per_beta,
+                                  worldline_sweeps=worldline_sweeps, eps_tilde=optimal_eps_tilde,
+                                  alpha=optimal_alpha, j_perp_end=opt["jpe"], cluster_sweeps=cluster_sweeps,
+                                  swap_interval=swap_interval, continuous_time_slices=continuous_time_slices,
+                                  calib_probes=int(optimal_calib_probes),
+                                  calib_sweeps=int(optimal_calib_sweeps),
+                                  debug_csv_path=(optimal_debug_csv or ""))
             local = list(res.average_energy_trace)
         elif method == "sqa":
-            res = annealer.run(sweeps_per_beta=sweeps_per_beta, worldline_sweeps=worldline_sweeps,
-                               cluster_sweeps=cluster_sweeps, continuous_time_slices=continuous_time_slices)
+            res = ann.run(sweeps_per_beta=sweeps_per_beta, worldline_sweeps=worldline_sweeps,
+                          cluster_sweeps=cluster_sweeps, continuous_time_slices=continuous_time_slices)
             local = list(res.energy_trace)
         elif method == "sqapt":
-            res = annealer.run(sweeps_per_step=sweeps_per_beta, worldline_sweeps=worldline_sweeps, steps=pt_steps,
-                               swap_interval=swap_interval, cluster_sweeps=cluster_sweeps,
-                               continuous_time_slices=continuous_time_slices)
+            res = ann.run(sweeps_per_step=sweeps_per_beta, worldline_sweeps=worldline_sweeps, steps=pt_steps,
+                          swap_interval=swap_interval, cluster_sweeps=cluster_sweeps,
+                          continuous_time_slices=continuous_time_slices)
             local = list(res.average_energy_trace)
         else:
-            res = annealer.run(sweeps_per_beta=sweeps_per_beta)
+            res = ann.run(sweeps_per_beta=sweeps_per_beta)
             local = list(res.energy_trace)
+        return res, local
+
+    def collect(res, local):
+        nonlocal trace
         spins = np.array(res.best_state.spins, dtype=int)
         samples.append(((spins + 1) // 2).astype(int) if (return_bits and is_qubo) else spins)
         energies.append(float(res.best_energy))
         if trace is None:
             trace = local
+
+    workers = min(reads, hip_context_count()) if (seed is not None and not optimal_debug_csv) else 1
+    if workers > 1:
+        # Seeded reads are independent: read r is set_seed(seed + r) and one run, whatever ran
+        # before on the engine. One engine per HIP context, each driven by its own thread (the
+        # bindings release the GIL), runs the reads concurrently on the GPU; results are
+        # collected in read order.
+        engines = queue.SimpleQueue()
+        engines.put(annealer)
+        for _ in range(workers - 1):
+            engines.put(make_annealer())
+
+        def task(r):
+            ann = engines.get()
+            try:
+                return one_read(ann, r)
+            finally:
+                engines.put(ann)
+
+        with ThreadPoolExecutor(workers) as ex:
+            futures = [ex.submit(task, r) for r in range(reads)]
+            for r in _iter_reads(reads, progress, logger):
+                collect(*futures[r].result())
+    else:
+        for r in _iter_reads(reads, progress, logger):
+            collect(*one_read(annealer, r))
 
     best = int(np.argmin(energies))
     return SolveResult(method=method, samples=samples, energies=energies, best_sample=samples[best],
