@@ -2,9 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <memory>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -15,6 +17,106 @@ namespace qa {
 thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+// ---- run-scoped device buffer cache ----------------------------------------------------------
+struct BufCache {
+    std::mutex mu;
+    std::map<size_t, std::vector<void *>> free;  // by size class
+    size_t held = 0;      // bytes in the free lists
+    int64_t live = 0;     // buffers handed out and not yet returned
+    bool orphan = false;  // the context is gone: returned buffers are freed
+};
+
+thread_local BufCache *t_run_cache = nullptr;
+
+namespace {
+constexpr size_t kCacheMaxBuf = size_t(1) << 30;   // larger buffers bypass the cache
+constexpr size_t kCacheMaxHeld = size_t(4) << 30;  // bytes a context keeps for reuse
+
+// 256 bytes at least, then quarter steps of each power of two: 2^k * {5, 6, 7, 8} / 4
+size_t size_class(size_t b) {
+    if (b <= 256) return 256;
+    const int k = 63 - __builtin_clzll(static_cast<unsigned long long>(b - 1));  // 2^k < b <= 2^(k+1)
+    const size_t base = size_t(1) << k;
+    for (int q = 5; q <= 8; ++q)
+        if (base / 4 * q >= b) return base / 4 * q;
+    return base * 2;
+}
+
+void free_held(BufCache *c) {  // caller holds c->mu
+    for (auto &kv : c->free)
+        for (void *p : kv.second) (void)hipFree(p);
+    c->free.clear();
+    c->held = 0;
+}
+}  // namespace
+
+BufCache *cache_create() { return new BufCache(); }
+
+void cache_destroy(BufCache *c) {
+    if (!c) return;
+    bool del;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        free_held(c);
+        c->orphan = true;
+        del = c->live == 0;
+    }
+    if (del) delete c;
+}
+
+void *cache_alloc(BufCache *c, size_t bytes, size_t *cls) {
+    void *p = nullptr;
+    if (bytes > kCacheMaxBuf) {
+        QA_HIP(hipMalloc(&p, bytes));
+        *cls = 0;
+        return p;
+    }
+    const size_t k = size_class(bytes);
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        auto it = c->free.find(k);
+        if (it != c->free.end() && !it->second.empty()) {
+            p = it->second.back();
+            it->second.pop_back();
+            c->held -= k;
+            ++c->live;
+            *cls = k;
+            return p;
+        }
+    }
+    hipError_t e = hipMalloc(&p, k);
+    if (e == hipErrorOutOfMemory) {  // give the held buffers back (hipFree waits for the device) and retry
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> g(c->mu);
+            free_held(c);
+        }
+        e = hipMalloc(&p, k);
+    }
+    QA_HIP(e);
+    std::lock_guard<std::mutex> g(c->mu);
+    ++c->live;
+    *cls = k;
+    return p;
+}
+
+void cache_release(BufCache *c, void *p, size_t k) {
+    bool drop = false, del = false;
+    {
+        std::lock_guard<std::mutex> g(c->mu);
+        --c->live;
+        if (c->orphan || c->held + k > kCacheMaxHeld) {
+            drop = true;
+        } else {
+            c->free[k].push_back(p);
+            c->held += k;
+        }
+        del = c->orphan && c->live == 0;
+    }
+    if (drop) (void)hipFree(p);
+    if (del) delete c;
+}
 
 double trotter_coupling(double beta, double gamma, int64_t slices) {
     const double x = std::max(beta * gamma / static_cast<double>(slices), 1e-12);
@@ -178,6 +280,7 @@ qa_status qa_ctx_create(int device, qa_ctx **out) {
         QA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         QA_HIP(hipEventCreate(&c->ev0));
         QA_HIP(hipEventCreate(&c->ev1));
+        c->cache = cache_create();
         *out = c;
     });
 }
@@ -187,6 +290,7 @@ qa_status qa_ctx_destroy(qa_ctx *ctx) {
         if (!ctx) return;
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        cache_destroy(ctx->cache);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);
@@ -405,6 +509,7 @@ qa_status qa_problem_destroy(qa_problem *p) {
 
 qa_status qa_energy(const qa_problem *p, int64_t count, const int8_t *spins, double *out) {
     return guarded([&] {
+        RunScope scope(p ? p->ctx : nullptr);
         QA_REQUIRE(p && spins && out && count >= 0, "qa_energy: bad argument");
         check_spins(spins, count * p->n);
         QA_HIP(hipSetDevice(p->ctx->device));
@@ -420,6 +525,7 @@ qa_status qa_energy(const qa_problem *p, int64_t count, const int8_t *spins, dou
 
 qa_status qa_local_fields(const qa_problem *p, int64_t count, const int8_t *spins, float *fields) {
     return guarded([&] {
+        RunScope scope(p ? p->ctx : nullptr);
         QA_REQUIRE(p && spins && fields && count >= 0, "qa_local_fields: bad argument");
         check_spins(spins, count * p->n);
         QA_HIP(hipSetDevice(p->ctx->device));
@@ -475,6 +581,7 @@ qa_status qa_chi_features(const qa_problem *p, double beta, int64_t trotter_slic
 qa_status qa_chi_pilot(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, double *scan_s,
                        double *scan_gamma, double *scan_chi_B, double *kernel_ms) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && scan_s && scan_gamma && scan_chi_B, "qa_chi_pilot: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_chi_pilot: problem belongs to another context");
         QA_REQUIRE(prm->trotter_slices > 0 && prm->replicas > 0, "trotter_slices and replicas must be > 0.");
@@ -566,6 +673,7 @@ qa_status qa_mlp_set_state(qa_mlp *m, const float *params, const double *norm) {
 
 qa_status qa_mlp_forward(qa_mlp *m, int64_t batch, const double *x, float *y) {
     return guarded([&] {
+        RunScope scope(m ? m->ctx : nullptr);
         QA_REQUIRE(m && x && y && batch >= 0, "qa_mlp_forward: bad argument");
         check_finite(x, batch * m->n_in, "features");
         QA_HIP(hipSetDevice(m->ctx->device));
@@ -575,6 +683,7 @@ qa_status qa_mlp_forward(qa_mlp *m, int64_t batch, const double *x, float *y) {
 
 qa_status qa_mlp_loss_grad(qa_mlp *m, int64_t batch, const double *x, const float *t, double *loss, float *grad) {
     return guarded([&] {
+        RunScope scope(m ? m->ctx : nullptr);
         QA_REQUIRE(m && x && t && batch > 0, "qa_mlp_loss_grad: bad argument");
         check_finite(x, batch * m->n_in, "features");
         QA_HIP(hipSetDevice(m->ctx->device));
@@ -585,6 +694,7 @@ qa_status qa_mlp_loss_grad(qa_mlp *m, int64_t batch, const double *x, const floa
 qa_status qa_mlp_train(qa_mlp *m, int64_t samples, const double *x, const float *t, const qa_mlp_train_params *tp,
                        double *loss_history) {
     return guarded([&] {
+        RunScope scope(m ? m->ctx : nullptr);
         QA_REQUIRE(m && x && t && tp, "qa_mlp_train: NULL argument");
         QA_REQUIRE(tp->lr > 0.0 && tp->weight_decay >= 0.0, "qa_mlp_train: lr must be > 0, weight_decay >= 0");
         check_finite(x, samples * m->n_in, "features");
@@ -597,6 +707,7 @@ qa_status qa_mlp_train(qa_mlp *m, int64_t samples, const double *x, const float 
 
 qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_result *res) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && res, "qa_sa_run: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_sa_run: problem belongs to another context");
         QA_REQUIRE(prm->steps > 0 && prm->betas, "Schedule must contain at least one beta.");
@@ -616,6 +727,7 @@ qa_status qa_sa_run(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, q
 
 qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && res, "qa_sqa_run: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_sqa_run: problem belongs to another context");
         QA_REQUIRE(prm->steps > 0 && prm->betas && prm->gammas, "SQA schedule must contain betas.");
@@ -638,6 +750,7 @@ qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
 
 qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && res, "qa_sqapt_run: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_sqapt_run: problem belongs to another context");
         QA_REQUIRE(prm->rungs >= 2 && prm->betas && prm->gammas, "SQA parallel tempering requires >=2 beta/gamma pairs.");
@@ -661,6 +774,7 @@ qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *
 qa_status qa_sqa_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_opt_params *prm,
                              qa_sqa_opt_result *res) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && res, "qa_sqa_optimal_run: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_sqa_optimal_run: problem belongs to another context");
         QA_REQUIRE(res->best_state && res->best_energy && res->energy_trace && res->sched.j_perp_trace &&
@@ -685,6 +799,7 @@ qa_status qa_sqa_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_opt_
 qa_status qa_sqapt_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_opt_params *prm,
                                qa_sqapt_opt_result *res) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && res, "qa_sqapt_optimal_run: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_sqapt_optimal_run: problem belongs to another context");
         QA_REQUIRE(res->final_states && res->final_energies && res->best_state && res->best_energy &&
@@ -710,6 +825,7 @@ qa_status qa_sqapt_optimal_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_
 
 qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
     return guarded([&] {
+        RunScope scope(ctx);
         QA_REQUIRE(ctx && p && prm && res, "qa_sqa_chi_run: NULL argument");
         QA_REQUIRE(p->ctx == ctx, "qa_sqa_chi_run: problem belongs to another context");
         QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");

@@ -37,29 +37,70 @@ struct Error {
     } while (0)
 
 // ---- device buffers ------------------------------------------------------------------------
+// hipFree synchronizes the whole device, so a run that freed its scratch with it would wait for
+// the kernels of every other stream. While a run is active on a thread (RunScope), device buffers
+// come from and go back to the cache of the run's context instead. A context's buffers are only
+// handed out again to work of that context, i.e. on its one stream, so stream order makes a
+// reused buffer safe without any host synchronization. Outside a run (problem uploads and other
+// long-lived buffers) allocation is plain hipMalloc / hipFree.
+struct BufCache;
+extern thread_local BufCache *t_run_cache;
+void *cache_alloc(BufCache *c, size_t bytes, size_t *cls);  // cls: 0 = not cached (plain hipMalloc)
+void cache_release(BufCache *c, void *ptr, size_t cls);
+BufCache *cache_create();
+void cache_destroy(BufCache *c);  // frees the cached buffers; live ones are freed on release
+
 template <class T>
 struct DevBuf {
     T *ptr = nullptr;
     size_t count = 0;
+    BufCache *cache = nullptr;
+    size_t cls = 0;
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DevBuf(DevBuf &&o) noexcept : ptr(o.ptr), count(o.count), cache(o.cache), cls(o.cls) {
+        o.ptr = nullptr;
+        o.count = 0;
+        o.cache = nullptr;
+        o.cls = 0;
+    }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); ptr = o.ptr; count = o.count; o.ptr = nullptr; o.count = 0; }
+        if (this != &o) {
+            release();
+            ptr = o.ptr;
+            count = o.count;
+            cache = o.cache;
+            cls = o.cls;
+            o.ptr = nullptr;
+            o.count = 0;
+            o.cache = nullptr;
+            o.cls = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t n) {
         release();
         count = n;
-        if (n) QA_HIP(hipMalloc(&ptr, n * sizeof(T)));
+        if (!n) return;
+        if (t_run_cache) {
+            ptr = static_cast<T *>(cache_alloc(t_run_cache, n * sizeof(T), &cls));
+            cache = cls ? t_run_cache : nullptr;
+        } else {
+            QA_HIP(hipMalloc(&ptr, n * sizeof(T)));
+        }
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            if (cache) cache_release(cache, ptr, cls);
+            else (void)hipFree(ptr);
+        }
         ptr = nullptr;
         count = 0;
+        cache = nullptr;
+        cls = 0;
     }
     void upload(const T *src, size_t n, hipStream_t s) {
         if (n) QA_HIP(hipMemcpyAsync(ptr, src, n * sizeof(T), hipMemcpyHostToDevice, s));
@@ -74,12 +115,24 @@ struct DevBuf {
 struct qa_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    qa::BufCache *cache = nullptr;  // run-scoped device buffers of this context's stream
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int num_cus = 256;
     size_t lds_per_block = 65536;
     // batch autotuner: replicas per group chosen for (model kind, n rounded up to 64, P)
     std::map<std::tuple<int, int64_t, int64_t>, int64_t> tuned_rg;
 };
+
+namespace qa {
+// Marks the calling thread as running work of `ctx` (see DevBuf); scopes nest.
+struct RunScope {
+    BufCache *prev;
+    explicit RunScope(const qa_ctx *ctx) : prev(t_run_cache) { t_run_cache = ctx ? ctx->cache : nullptr; }
+    ~RunScope() { t_run_cache = prev; }
+    RunScope(const RunScope &) = delete;
+    RunScope &operator=(const RunScope &) = delete;
+};
+}  // namespace qa
 
 enum { QA_KIND_DENSE = 0, QA_KIND_SPARSE = 1, QA_KIND_HUBO = 2 };
 

@@ -385,8 +385,9 @@ void sqapt_optimal(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_opt_params *
     };
     auto slice_state = [&](int64_t r, int64_t t, int8_t *out) {
         if (main.f) {
-            QA_HIP(hipMemcpy(out, main.f->device_spins() + (static_cast<int64_t>(main.slot[r]) * P + t) * n, n,
-                             hipMemcpyDeviceToHost));
+            QA_HIP(hipMemcpyAsync(out, main.f->device_spins() + (static_cast<int64_t>(main.slot[r]) * P + t) * n, n,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+            QA_HIP(hipStreamSynchronize(ctx->stream));
         } else {
             main.v->slice_state(r, t, out);
         }

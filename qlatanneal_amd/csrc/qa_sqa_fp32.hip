@@ -1312,7 +1312,8 @@ void sqa_fp32_observed(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *pr
         const double jp = trotter_coupling(prm->betas[step], prm->gammas[step], P);
         f.run(1, &bs, &jp, 0, nullptr, prm->sweeps_per_beta, prm->cluster_sweeps, prm->worldline_sweeps, 0, nullptr,
               res->energy_trace + step);
-        QA_HIP(hipMemcpy(h.data(), f.device_spins(), R * P * n, hipMemcpyDeviceToHost));
+        QA_HIP(hipMemcpyAsync(h.data(), f.device_spins(), R * P * n, hipMemcpyDeviceToHost, ctx->stream));
+        QA_HIP(hipStreamSynchronize(ctx->stream));
         prm->on_step(prm->callback_user, step, res->energy_trace[step], h.data());
     }
     f.best(res->best_state, res->best_energy);
@@ -1348,11 +1349,19 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     const int64_t chunk = ri > 0 ? std::max<int64_t>(1, ri / per_step) : prm->steps;
     const int64_t nchunks = (prm->steps + chunk - 1) / chunk;
     const int64_t nchain = R * P, nsample = std::min<int64_t>(nchain, 64);
+    // Steps that mix slice sweeps with cluster or worldline sweeps run as two launches when the
+    // slice sweeps qualify for the pipelined kernel: the slice sweeps there (no step tail), then the
+    // cluster and worldline sweeps and the step tail in the two-barrier kernel, the LDS state
+    // passed through gS/gF. Sweep ids, Philox counters and arithmetic are those of one launch.
+    const char *split_env = std::getenv("QANNEAL_SQA_SPLIT");
+    const bool split = pl.pipe && prm->sweeps_per_beta > 0 &&
+                       (prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0) && !(split_env && split_env[0] == '0');
+    const int64_t nacc_slots = split ? 2 * prm->steps : nchunks;
     DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
-    DevBuf<float> gF((use_lds && nchunks == 1) ? 1 : R * P * n);
+    DevBuf<float> gF((use_lds && nchunks == 1 && !split) ? 1 : R * P * n);
     DevBuf<double> tpart(prm->steps * ngroups), be(ngroups);
     DevBuf<int> bstep(ngroups), bchain(ngroups);
-    DevBuf<unsigned long long> accd(ngroups * nchunks), drift(2);
+    DevBuf<unsigned long long> accd(ngroups * nacc_slots), drift(2);
     const bool mfma_refresh = p->kind == QA_KIND_DENSE && p->symmetric;
     DevBuf<double> scratch(nchunks > 1 ? (mfma_refresh ? nsample : nchain) * n : 0);
     QA_HIP(hipMemsetAsync(drift.ptr, 0, 2 * sizeof(unsigned long long), s));
@@ -1396,7 +1405,37 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
         a.save_fields = ci + 1 < nchunks ? 1 : 0;
         a.sweep_base = static_cast<uint32_t>(start * per_step);
         a.step_base = static_cast<int>(start);
-        launch_sqa(ctx, pl, a);
+        if (!split) {
+            launch_sqa(ctx, pl, a);
+        } else {
+            for (int64_t k = start; k < start + len; ++k) {
+                SqaArgs sl = a;  // slice sweeps of step k
+                sl.steps = 1;
+                sl.bs = dbs.ptr + k;
+                sl.jp = djp.ptr + k;
+                sl.join = djoin.ptr + k;
+                sl.trace_part = tpart.ptr + k * ngroups;
+                sl.wl = 0;
+                sl.cl = 0;
+                sl.tail_mode = 2;
+                sl.resume = k > 0 ? 1 : 0;
+                sl.save_fields = 1;
+                sl.sweep_base = static_cast<uint32_t>(k * per_step);
+                sl.step_base = static_cast<int>(k);
+                sl.accepted = accd.ptr + 2 * k * ngroups;
+                launch_sqa(ctx, pl, sl);
+                SqaArgs rest = sl;  // cluster, then worldline sweeps, then the step tail
+                rest.spb = 0;
+                rest.wl = a.wl;
+                rest.cl = a.cl;
+                rest.tail_mode = 0;
+                rest.resume = 1;
+                rest.save_fields = k + 1 < prm->steps ? 1 : 0;
+                rest.sweep_base = static_cast<uint32_t>(k * per_step + prm->sweeps_per_beta);
+                rest.accepted = accd.ptr + (2 * k + 1) * ngroups;
+                launch_sqa(ctx, pl, rest);
+            }
+        }
         if (ci + 1 < nchunks) {  // drift of the running fields, matrix-core refresh, drift after it
             field_drift_launch(p, gS.ptr, gF.ptr, n, nsample, scratch.ptr, drift.ptr, s);
             fields_fp32_launch(p, gS.ptr, nchain, gF.ptr, n, scratch.ptr, s);
@@ -1407,12 +1446,12 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
 
     std::vector<double> hpart(prm->steps * ngroups), hbe(ngroups);
     std::vector<int> hstep(ngroups), hchain(ngroups);
-    std::vector<unsigned long long> hacc(ngroups * nchunks), hdrift(2);
+    std::vector<unsigned long long> hacc(ngroups * nacc_slots), hdrift(2);
     tpart.download(hpart.data(), hpart.size(), s);
     be.download(hbe.data(), ngroups, s);
     bstep.download(hstep.data(), ngroups, s);
     bchain.download(hchain.data(), ngroups, s);
-    accd.download(hacc.data(), ngroups * nchunks, s);
+    accd.download(hacc.data(), ngroups * nacc_slots, s);
     drift.download(hdrift.data(), 2, s);
     QA_HIP(hipStreamSynchronize(s));
     if (res->field_drift) {

@@ -173,6 +173,30 @@ def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
             assert bits_equal(res["lockstep"][key].ravel(), res[name][key].ravel()), (name, key)
 
 
+def test_sqa_fp32_split_step_launches_bit_identical(ctx, monkeypatch):
+    """Steps with cluster or worldline sweeps run their slice sweeps in the pipelined kernel and the
+    rest in the two-barrier kernel (two launches per step, state passed through global memory); the
+    result must equal the single two-barrier launch bit for bit, with and without field refreshes."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(47)
+    cases = [(200, 16, 64, 2, 1, 0, 0), (97, 5, 7, 3, 1, 1, 0), (550, 16, 64, 4, 2, 0, 0), (64, 64, 2, 1, 0, 1, 0),
+             (301, 8, 9, 2, 1, 1, 6), (1465, 16, 8, 1, 1, 0, 0)]
+    for n, P, R, spb, wl, cl, ri in cases:
+        print("split-step case", n, P, R, spb, wl, cl, ri, flush=True)
+        J = _sk(rng, n)
+        dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), J, 0.5)
+        betas, gammas = np.geomspace(0.2, 3.0, 5), np.geomspace(3.0, 0.02, 5)
+        res = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_SQA_SPLIT", v)
+            res.append(abi.sqa_run(dp, betas, gammas, P, R, spb, wl, cl, abi.MODE_FP32, 23, refresh_interval=ri))
+        a, b = res
+        assert a["accept_rate"] == b["accept_rate"]
+        assert np.array_equal(a["best_state"], b["best_state"]) and bits_equal([a["best_energy"]], [b["best_energy"]])
+        assert bits_equal(a["energy_trace"], b["energy_trace"])
+    monkeypatch.delenv("QANNEAL_SQA_SPLIT", raising=False)
+
+
 def test_sqa_fp32_distribution_vs_oracle(ctx):
     from scipy.stats import ks_2samp
     from qlatanneal_amd import abi
