@@ -226,8 +226,9 @@ def bench_c4(args):
     switched off): SQA P=16, 64 replicas per instance (the 64 reads of an instance run together),
     auto_schedule_sqa (50 steps x 20 slice sweeps + 5 worldline sweeps). One timed step = a
     seeded sample of `--c4-instances` instances of the C4 mix, driven through the SQAAnnealer
-    class by `--c4-streams` host threads (1: one after another); algorithmic bytes ~10 B per
-    attempt (fields and spins read and written) plus J once per instance sweep."""
+    class by `--c4-streams` host threads, largest instance first (1: one after another);
+    algorithmic bytes ~10 B per attempt (fields and spins read and written) plus J once per
+    instance sweep."""
     import qlatanneal_amd as qa
     P, R, spb, wl = 16, 64, 20, 5
     sched = qa.auto_schedule_sqa()
@@ -236,29 +237,27 @@ def bench_c4(args):
     attempts = sum(steps * R * (spb * P * n + wl * n) for _, n, _ in inst)
     nbytes = sum(steps * (spb + wl) * (R * P * n * 10 + n * n * 4) for _, n, _ in inst)
     kms, best = [], []
-    # Each annealer's backend is bound round-robin to one of the library's HIP contexts (one stream
-    # each); K host threads each drive every K-th instance, so with K > 1 the runs of different
-    # streams execute concurrently on the GPU's hardware queues (the bindings release the GIL).
+    # K host threads take the instances largest first; each run leases a free one of the library's
+    # HIP contexts (one stream each), so with K > 1 runs execute concurrently on the GPU's hardware
+    # queues (the bindings release the GIL during a run).
     anns = [qa.SQAAnnealer(h.ham if kind == "hubo" else h, sched, P, R, backend="hip") for kind, _, h in inst]
     K = max(1, args.c4_streams)
+    order = sorted(range(len(inst)), key=lambda i: -inst[i][1])
 
-    def lane(j, seed0, limit=None):
-        out = []
-        for i in list(range(j, len(anns), K))[:limit]:
-            anns[i].set_seed(seed0 + i)
-            out.append((anns[i].run(spb, wl).best_energy, anns[i].last_kernel_ms()))
-        return out
+    def one(i, seed0):
+        anns[i].set_seed(seed0 + i)
+        return anns[i].run(spb, wl).best_energy, anns[i].last_kernel_ms()
 
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(K) as pool:
         for w in range(args.warmup):
-            list(pool.map(lambda j: lane(j, 10 + w, 1), range(K)))
+            list(pool.map(lambda i: one(i, 10 + w), order[:K]))
         t0 = time.perf_counter()
         for k in range(args.steps):
-            res = [x for part in pool.map(lambda j: lane(j, 1000 + 97 * k), range(K)) for x in part]
+            res = list(pool.map(lambda i: one(i, 1000 + 97 * k), order))
             best.extend(e for e, _ in res)
             kms.append(sum(m for _, m in res))
-            print(f"step {k}: {K} stream(s), sweep kernels {kms[-1]:.1f} ms summed over {len(anns)} instances",
+            print(f"step {k}: {K} thread(s), sweep kernels {kms[-1]:.1f} ms summed over {len(anns)} instances",
                   file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
     kern_s = float(np.mean(kms)) / 1e3 if K == 1 else wall / args.steps

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <memory>
 #include <mutex>
+#include <vector>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -81,9 +82,9 @@ protected:
 };
 
 // HIP backend: the per-spin host methods of CPUBackend plus a device copy of the problem,
-// uploaded on first use and reused by every run. Each backend is bound at construction to one of
-// a small pool of contexts (one HIP stream each) on the selected GPU, assigned round-robin, so
-// runs of different backends from different host threads execute concurrently on the GPU.
+// uploaded on first use and reused by every run. Runs take one of a small pool of contexts (one
+// HIP stream each) on the selected GPU: each backend prefers its own, assigned round-robin at
+// construction, so runs of different backends from different host threads execute concurrently.
 class HIPBackend final : public CPUBackend {
 public:
     HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision);
@@ -93,14 +94,24 @@ public:
     DevicePrecision precision() const { return precision_; }
     void set_precision(DevicePrecision p) { precision_ = p; }
 
-    qa_ctx *context() const;
-    qa_problem *device_problem() const;  // uploads on first call
-    std::mutex &device_mutex() const;    // serializes runs on this backend's context
+    // A context to run on and the device problem in it, held until the lease ends: the backend's
+    // own context when it is free, else another free one (the problem is uploaded there on first
+    // use, when it is small), else the backend's own context once it frees up.
+    struct Lease {
+        std::unique_lock<std::mutex> lock;
+        qa_ctx *ctx = nullptr;
+        qa_problem *problem = nullptr;
+    };
+    Lease lease() const;
 
 private:
+    qa_problem *problem_on(int slot) const;  // uploads on first use; caller holds the slot
+    bool has_problem(int slot) const;
+
     DevicePrecision precision_;
-    int slot_;                           // context pool slot
-    mutable qa_problem *problem_ = nullptr;
+    int slot_;                               // own context pool slot
+    bool movable_;                           // small enough to copy into other contexts
+    mutable std::vector<qa_problem *> problems_;  // per pool slot
     mutable std::mutex upload_mu_;
 };
 

@@ -73,7 +73,7 @@ AnnealResult sa_observed(HIPBackend &hb, const AnnealSchedule &sched, std::mt199
     double be = 0.0, tr = 0.0, fe = 0.0;
     int64_t used = 0;
     uint64_t launch = 0;
-    std::lock_guard<std::mutex> lock(hb.device_mutex());
+    const auto lease = hb.lease();
     for (std::size_t step = 0; step < steps; ++step) {
         const double beta = sched.betas[step];
         const std::size_t per = sweep_obs ? 1 : spb;
@@ -103,7 +103,7 @@ AnnealResult sa_observed(HIPBackend &hb, const AnnealSchedule &sched, std::mt199
             out.uniforms_used = &used;
             out.final_states = fs.data();
             out.final_energies = &fe;
-            throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+            throw_on_status(qa_sa_run(lease.ctx, lease.problem, &prm, &out));
             if (mode != QA_MODE_FP32) rng.discard(static_cast<unsigned long long>(used));
             if (be < res.best_energy) {
                 res.best_energy = be;
@@ -189,8 +189,8 @@ AnnealResult Annealer::run(std::size_t sweeps_per_beta, Observer *observer) {
     int64_t used = 0;
     qa_sa_result out{res.best_state.spins.data(), &res.best_energy, res.energy_trace.data(), &used, nullptr};
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sa_run(lease.ctx, lease.problem, &prm, &out));
     }
     if (prm.mode != QA_MODE_FP32) rng_.discard(static_cast<unsigned long long>(used));
     return res;
@@ -229,8 +229,8 @@ std::vector<AnnealResult> Annealer::run_reads(std::size_t sweeps_per_beta, std::
     std::vector<double> be(reads), tr(reads * steps);
     qa_sa_result out{best.data(), be.data(), tr.data(), nullptr, nullptr};
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sa_run(lease.ctx, lease.problem, &prm, &out));
     }
     std::vector<AnnealResult> all(reads);
     for (std::size_t r = 0; r < reads; ++r) {
@@ -303,8 +303,8 @@ MultiAnnealResult ReplicaAnnealer::run(std::size_t sweeps_per_beta) {
     out.magnetization_trace = mg.data();
     out.best_energy_trace = bt.data();
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sa_run(lease.ctx, lease.problem, &prm, &out));
     }
     MultiAnnealResult res;
     res.replicas.resize(R);
@@ -378,7 +378,7 @@ ParallelTemperingResult ParallelTemperingAnnealer::run(std::size_t sweeps_per_st
     res.average_energy_trace.reserve(steps);
     res.swap_acceptance_trace.reserve(steps);
     std::uniform_real_distribution<double> uniform(0.0, 1.0);
-    std::lock_guard<std::mutex> lock(hb.device_mutex());
+    const auto lease = hb.lease();
     for (std::size_t step = 0; step < steps; ++step) {
         qa_sa_params prm{};
         prm.steps = 1;
@@ -398,7 +398,7 @@ ParallelTemperingResult ParallelTemperingAnnealer::run(std::size_t sweeps_per_st
             out.energy_trace = tr.data();
             out.final_states = nxt.data();
             out.final_energies = fe.data();
-            throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+            throw_on_status(qa_sa_run(lease.ctx, lease.problem, &prm, &out));
         } else {
             for (std::size_t r = 0; r < R; ++r) {
                 const auto uni = uniform_stream(rng_, sweeps_per_step * n);
@@ -415,7 +415,7 @@ ParallelTemperingResult ParallelTemperingAnnealer::run(std::size_t sweeps_per_st
                 out.uniforms_used = &used;
                 out.final_states = &nxt[r * n];
                 out.final_energies = &fe[r];
-                throw_on_status(qa_sa_run(hb.context(), hb.device_problem(), &prm, &out));
+                throw_on_status(qa_sa_run(lease.ctx, lease.problem, &prm, &out));
                 rng_.discard(static_cast<unsigned long long>(used));
             }
         }
@@ -532,8 +532,8 @@ SQAResult SQAAnnealer::run(std::size_t sweeps_per_beta, std::size_t worldline_sw
         out.trace_engine_out_len = static_cast<int64_t>(engine_out.size());
     }
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sqa_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sqa_run(lease.ctx, lease.problem, &prm, &out));
     }
     if (sweep_obs) {
         std::istringstream in(engine_out.data());
@@ -601,8 +601,8 @@ SQAResult SQAAnnealer::run_optimal(double beta, double j_perp_start, double j_pe
                                 &res.j_perp_start, &res.resolved_j_perp_end, &res.final_j_perp};
     out.kernel_ms = &kms;
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sqa_optimal_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sqa_optimal_run(lease.ctx, lease.problem, &prm, &out));
     }
     last_kernel_ms_ = kms;
     last_accept_rate_ = 0.0;
@@ -684,8 +684,8 @@ SQAParallelTemperingResult SQAParallelTemperingAnnealer::run(std::size_t sweeps_
     int64_t used = 0;
     qa_sqapt_result out{fs.data(), fe.data(), bs.data(), &be, at.data(), st.data(), &used, nullptr};
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sqapt_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sqapt_run(lease.ctx, lease.problem, &prm, &out));
     }
     if (prm.mode != QA_MODE_FP32) rng_.discard(static_cast<unsigned long long>(used));
     SQAParallelTemperingResult res;
@@ -725,8 +725,8 @@ std::vector<SQAParallelTemperingResult> SQAParallelTemperingAnnealer::run_ladder
     std::vector<double> fe(L * R), be(L), at(L * steps), st(L * steps);
     qa_sqapt_result out{fs.data(), fe.data(), bs.data(), be.data(), at.data(), st.data(), nullptr, nullptr};
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sqapt_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sqapt_run(lease.ctx, lease.problem, &prm, &out));
     }
     std::vector<SQAParallelTemperingResult> all(L);
     for (std::size_t l = 0; l < L; ++l) {
@@ -816,8 +816,8 @@ SQAParallelTemperingResult SQAParallelTemperingAnnealer::run_optimal(
                                 &res.j_perp_start, &res.resolved_j_perp_end, &res.final_j_perp};
     out.swap_uniforms_used = &used;
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sqapt_optimal_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sqapt_optimal_run(lease.ctx, lease.problem, &prm, &out));
     }
     if (prm.mode != QA_MODE_FP32) rng_.discard(static_cast<unsigned long long>(used));
     for (std::size_t r = 0; r < R; ++r) res.final_states.push_back(state_from(&fs[r * n], n));
@@ -878,8 +878,8 @@ SQAChiResult chi_run(HIPBackend &hb, std::mt19937_64 &rng, std::size_t P, std::s
                       res.beta_schedule.data(), res.gamma_schedule.data(), res.s_schedule.data(),
                       res.scan_s.data(), res.scan_gamma.data(), res.scan_chi_B.data(), sc, nullptr};
     {
-        std::lock_guard<std::mutex> lock(hb.device_mutex());
-        throw_on_status(qa_sqa_chi_run(hb.context(), hb.device_problem(), &prm, &out));
+        const auto lease = hb.lease();
+        throw_on_status(qa_sqa_chi_run(lease.ctx, lease.problem, &prm, &out));
     }
     res.s_star = sc[0];
     res.gamma_star = sc[1];

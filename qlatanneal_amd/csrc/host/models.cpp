@@ -333,22 +333,18 @@ DeviceContext &device_context(int slot) {
 
 int hip_context_count() { return pool_size(); }
 
-HIPBackend::HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision)
-    : CPUBackend(std::move(ham)), precision_(precision), slot_(next_slot()) {}
-
-HIPBackend::~HIPBackend() {
-    if (problem_) qa_problem_destroy(problem_);
+namespace {
+// Device bytes of a problem's copy (dense: J in FP64 and FP32 plus the bf16 split), for the choice
+// of whether runs may copy it into another context.
+bool small_problem(const Hamiltonian *h) {
+    if (auto *d = dynamic_cast<const DenseIsing *>(h)) {
+        const double n = static_cast<double>(d->size());
+        return 16.0 * n * n <= 256.0 * 1024 * 1024;
+    }
+    return true;
 }
 
-qa_ctx *HIPBackend::context() const { return device_context(slot_).ctx; }
-
-std::mutex &HIPBackend::device_mutex() const { return device_context(slot_).run_mu; }
-
-qa_problem *HIPBackend::device_problem() const {
-    std::lock_guard<std::mutex> lock(upload_mu_);
-    if (problem_) return problem_;
-    qa_ctx *ctx = context();
-    const Hamiltonian *h = ham_.get();
+qa_problem *upload_problem(qa_ctx *ctx, const Hamiltonian *h) {
     qa_problem *p = nullptr;
     if (auto *d = dynamic_cast<const DenseIsing *>(h)) {
         throw_on_status(qa_problem_dense(ctx, static_cast<int64_t>(d->size()), d->h().data(), d->J().data(),
@@ -373,8 +369,46 @@ qa_problem *HIPBackend::device_problem() const {
     } else {
         throw std::invalid_argument("the HIP backend runs DenseIsing, SparseIsing and HigherOrderIsing models");
     }
-    problem_ = p;
-    return problem_;
+    return p;
+}
+}  // namespace
+
+HIPBackend::HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision)
+    : CPUBackend(std::move(ham)), precision_(precision), slot_(next_slot()), movable_(small_problem(ham_.get())) {}
+
+HIPBackend::~HIPBackend() {
+    for (qa_problem *p : problems_)
+        if (p) qa_problem_destroy(p);
+}
+
+bool HIPBackend::has_problem(int slot) const {
+    std::lock_guard<std::mutex> lock(upload_mu_);
+    return !problems_.empty() && problems_[static_cast<std::size_t>(slot)] != nullptr;
+}
+
+qa_problem *HIPBackend::problem_on(int slot) const {
+    qa_ctx *ctx = device_context(slot).ctx;
+    std::lock_guard<std::mutex> lock(upload_mu_);
+    if (problems_.empty()) problems_.assign(static_cast<std::size_t>(pool_size()), nullptr);
+    qa_problem *&p = problems_[static_cast<std::size_t>(slot)];
+    if (!p) p = upload_problem(ctx, ham_.get());
+    return p;
+}
+
+HIPBackend::Lease HIPBackend::lease() const {
+    const int k = pool_size();
+    for (int d = 0; d < k; ++d) {
+        const int s = (slot_ + d) % k;
+        if (d > 0 && !movable_ && !has_problem(s)) continue;
+        std::unique_lock<std::mutex> lk(device_context(s).run_mu, std::try_to_lock);
+        if (lk.owns_lock()) {
+            qa_problem *p = problem_on(s);
+            return Lease{std::move(lk), device_context(s).ctx, p};
+        }
+    }
+    std::unique_lock<std::mutex> lk(device_context(slot_).run_mu);
+    qa_problem *p = problem_on(slot_);
+    return Lease{std::move(lk), device_context(slot_).ctx, p};
 }
 
 std::shared_ptr<Backend> make_backend(BackendSpec spec, std::shared_ptr<const Hamiltonian> ham) {

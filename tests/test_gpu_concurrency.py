@@ -52,9 +52,14 @@ def test_threaded_engines_equal_sequential(backend):
     print("threaded engines", backend, len(models), flush=True)
     alone = [_run_sqa(_sqa(m, backend, 200 + i)) for i, m in enumerate(models)]
     pairs = [_sqa(m, backend, 200 + i) for i, m in enumerate(models)]
-    with ThreadPoolExecutor(8) as ex:  # more threads than contexts: engines sharing one wait for it
+    with ThreadPoolExecutor(8) as ex:  # more threads than contexts: runs wait for a free one
         together = list(ex.map(_run_sqa, pairs))
     assert together == alone
+    # a second round on the same engines: runs may now land on other contexts than the first time
+    # (the problem is copied into whichever context is free), with the same results
+    with ThreadPoolExecutor(3) as ex:
+        again = list(ex.map(_run_sqa, pairs[::-1]))
+    assert again[::-1] == alone
     sched = qa.AnnealSchedule.linear(0.2, 3.0, 16)
     alone = [_run_sa((qa.Annealer(m, sched, backend=backend), 30 + i)) for i, m in enumerate(models)]
     with ThreadPoolExecutor(4) as ex:
