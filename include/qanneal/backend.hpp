@@ -107,10 +107,12 @@ public:
 private:
     qa_problem *problem_on(int slot) const;  // uploads on first use; caller holds the slot
     bool has_problem(int slot) const;
+    bool reserve_extra_copy() const;
 
     DevicePrecision precision_;
     int slot_;                               // own context pool slot
-    bool movable_;                           // small enough to copy into other contexts
+    int64_t bytes_;                          // approximate device bytes of one problem copy
+    mutable int64_t extra_bytes_ = 0;        // bytes of this backend's copies in other contexts
     mutable std::vector<qa_problem *> problems_;  // per pool slot
     mutable std::mutex upload_mu_;
 };

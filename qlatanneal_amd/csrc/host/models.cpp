@@ -334,15 +334,21 @@ DeviceContext &device_context(int slot) {
 int hip_context_count() { return pool_size(); }
 
 namespace {
-// Device bytes of a problem's copy (dense: J in FP64 and FP32 plus the bf16 split), for the choice
-// of whether runs may copy it into another context.
-bool small_problem(const Hamiltonian *h) {
-    if (auto *d = dynamic_cast<const DenseIsing *>(h)) {
-        const double n = static_cast<double>(d->size());
-        return 16.0 * n * n <= 256.0 * 1024 * 1024;
-    }
-    return true;
+// Approximate device bytes of a problem's copy (dense: J in FP64 and FP32 plus the bf16 split;
+// sparse and HUBO: their coupling lists and incidence arrays), for the choice of whether runs may
+// copy it into another context.
+int64_t problem_bytes(const Hamiltonian *h) {
+    const auto n = static_cast<int64_t>(h->size());
+    if (dynamic_cast<const DenseIsing *>(h)) return 16 * n * n + 32 * n;
+    if (auto *sp = dynamic_cast<const SparseIsing *>(h)) return 40 * static_cast<int64_t>(sp->edges().size()) + 32 * n;
+    if (auto *hb = dynamic_cast<const HigherOrderIsing *>(h))
+        return 24 * static_cast<int64_t>(hb->term_variables().size()) + 24 * static_cast<int64_t>(hb->num_terms()) + 32 * n;
+    return 32 * n;
 }
+
+constexpr int64_t kMaxMovable = int64_t(256) << 20;    // larger problems stay in their own context
+constexpr int64_t kExtraCopyBudget = int64_t(16) << 30;  // all extra copies of all backends together
+std::atomic<int64_t> g_extra_copy_bytes{0};
 
 qa_problem *upload_problem(qa_ctx *ctx, const Hamiltonian *h) {
     qa_problem *p = nullptr;
@@ -374,11 +380,12 @@ qa_problem *upload_problem(qa_ctx *ctx, const Hamiltonian *h) {
 }  // namespace
 
 HIPBackend::HIPBackend(std::shared_ptr<const Hamiltonian> ham, DevicePrecision precision)
-    : CPUBackend(std::move(ham)), precision_(precision), slot_(next_slot()), movable_(small_problem(ham_.get())) {}
+    : CPUBackend(std::move(ham)), precision_(precision), slot_(next_slot()), bytes_(problem_bytes(ham_.get())) {}
 
 HIPBackend::~HIPBackend() {
     for (qa_problem *p : problems_)
         if (p) qa_problem_destroy(p);
+    g_extra_copy_bytes.fetch_sub(extra_bytes_);
 }
 
 bool HIPBackend::has_problem(int slot) const {
@@ -395,16 +402,29 @@ qa_problem *HIPBackend::problem_on(int slot) const {
     return p;
 }
 
+// Reserves room for one more copy of this problem in another context; false when the problem is
+// too large to copy or the copies of all backends together would exceed their budget.
+bool HIPBackend::reserve_extra_copy() const {
+    if (bytes_ > kMaxMovable) return false;
+    if (g_extra_copy_bytes.fetch_add(bytes_) + bytes_ > kExtraCopyBudget) {
+        g_extra_copy_bytes.fetch_sub(bytes_);
+        return false;
+    }
+    std::lock_guard<std::mutex> lock(upload_mu_);
+    extra_bytes_ += bytes_;
+    return true;
+}
+
 HIPBackend::Lease HIPBackend::lease() const {
     const int k = pool_size();
     for (int d = 0; d < k; ++d) {
         const int s = (slot_ + d) % k;
-        if (d > 0 && !movable_ && !has_problem(s)) continue;
+        if (d > 0 && bytes_ > kMaxMovable && !has_problem(s)) continue;
         std::unique_lock<std::mutex> lk(device_context(s).run_mu, std::try_to_lock);
-        if (lk.owns_lock()) {
-            qa_problem *p = problem_on(s);
-            return Lease{std::move(lk), device_context(s).ctx, p};
-        }
+        if (!lk.owns_lock()) continue;
+        if (d > 0 && !has_problem(s) && !reserve_extra_copy()) continue;
+        qa_problem *p = problem_on(s);
+        return Lease{std::move(lk), device_context(s).ctx, p};
     }
     std::unique_lock<std::mutex> lk(device_context(slot_).run_mu);
     qa_problem *p = problem_on(slot_);
