@@ -1872,6 +1872,105 @@ void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int6
 }
 
 // sqa_chi production anneal: R replicas, checkerboard sweeps along (beta_k, Gamma_k).
+namespace {
+// ---- sqa_chi production sweeps: one wavefront per chain -----------------------------------------
+// In a checkerboard phase (sqa_chi_annealer.cpp:81-140) every chain of the phase's colour sweeps
+// sites 0..n-1 in order while the chains of the other colours are held fixed, so the chains of a
+// phase are independent and each one's sites depend only on its own flips. One launch per phase,
+// one wavefront per chain: the chain's fields and spins, the fixed neighbour sums and the phase's
+// Philox words sit in the wave's LDS, the J rows stream in site order with the loads issued two
+// sites ahead (register sets A and B, the site loop unrolled by two), and an accepted flip updates
+// the fields with the same fmaf per element, in the same site order, as the two-barrier kernel's
+// checkerboard sweep: the trajectory is the same bit for bit.
+struct ChkArgs {
+    const float *J4rows;  // J32p: n rows of rn floats (rn = n rounded up to 4)
+    int8_t *gS;           // [R*P][n]
+    float *gF;            // [R*P][n]
+    int P, n, rn, ph;
+    uint32_t sweep_id, k0, k1;
+    float bs, jp;
+};
+
+template <int NQ>  // float4 groups of a row per lane: rn <= 256 * NQ
+__global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int P = a.P, n = a.n, rn = a.rn, ng = a.rn >> 2;
+    const int r = c / P, t = c - r * P;
+    const int colour = P == 1 ? 0 : ((P & 1) ? (t == P - 1 ? 1 : ((t & 1) ? 2 : 0)) : (t & 1));
+    if (colour != a.ph) return;  // the whole wave: this kernel has no workgroup barrier after here
+    const int cprev = r * P + (t == 0 ? P - 1 : t - 1), cnext = r * P + (t + 1 == P ? 0 : t + 1);
+    float *F = reinterpret_cast<float *>(smem);           // [rn]
+    f4 *F4 = reinterpret_cast<f4 *>(smem);
+    uint32_t *W = reinterpret_cast<uint32_t *>(F + rn);   // [n] uniform words of this phase's sites
+    int8_t *S = reinterpret_cast<int8_t *>(W + n);        // [n]
+    int8_t *NB = S + n;                                   // [n] s_{t-1,i} + s_{t+1,i} (fixed in the phase)
+    const int64_t off = static_cast<int64_t>(c) * n;
+    for (int i = lane; i < n; i += 64) {
+        S[i] = a.gS[off + i];
+        NB[i] = static_cast<int8_t>(a.gS[static_cast<int64_t>(cprev) * n + i] + a.gS[static_cast<int64_t>(cnext) * n + i]);
+        F[i] = a.gF[off + i];
+    }
+    for (int i = n + lane; i < rn; i += 64) F[i] = 0.0f;
+    // site k of phase ph is step tau = ph * n + k of the sweep: word tau & 3 of block tau >> 2
+    const uint32_t tau0 = static_cast<uint32_t>(a.ph) * static_cast<uint32_t>(n);
+    for (uint32_t b = (tau0 >> 2) + lane; b <= ((tau0 + n - 1) >> 2); b += 64) {
+        const Philox4 w = philox4x32_10(static_cast<uint32_t>(c), a.sweep_id, b, TAG_CHECKER, a.k0, a.k1);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t tau = 4 * b + q;
+            if (tau >= tau0 && tau < tau0 + static_cast<uint32_t>(n)) W[tau - tau0] = ws[q];
+        }
+    }
+    __syncthreads();
+    const f4 *J4 = reinterpret_cast<const f4 *>(a.J4rows);
+    auto load_row = [&](f4 (&h)[NQ], int k) {
+        const f4 *row = J4 + static_cast<int64_t>(min(k, n - 1)) * ng;
+        static_for<NQ>([&](auto q) { h[q] = row[min(lane + 64 * q, ng - 1)]; });
+    };
+    auto site = [&](const f4 (&h)[NQ], int k) {
+        const float f = F[k];
+        const int8_t s = S[k];
+        const int nb = NB[k];
+        const uint32_t word = W[k];
+        const float fs = static_cast<float>(s);
+        const float de = fmaf(a.bs, -2.0f * fs * f, 2.0f * a.jp * fs * static_cast<float>(nb));
+        const bool acc = de <= 0.0f || u01(word) < __expf(-de);
+        if (acc) {
+            if (lane == 0) S[k] = static_cast<int8_t>(-s);
+            const float ds = -2.0f * fs;
+            static_for<NQ>([&](auto q) {
+                const int gi = lane + 64 * q;
+                if (gi < ng) {
+                    const f4 x = h[q], y = F4[gi];
+                    F4[gi] = f4{fmaf(ds, x.x, y.x), fmaf(ds, x.y, y.y), fmaf(ds, x.z, y.z), fmaf(ds, x.w, y.w)};
+                }
+            });
+        }
+        __syncthreads();  // this site's field and spin writes before the next site's reads
+    };
+    f4 A[NQ], B[NQ];
+    load_row(A, 0);
+    load_row(B, 1);
+    for (int k = 0; k < n; k += 2) {
+        site(A, k);
+        load_row(A, k + 2);
+        if (k + 1 < n) {
+            site(B, k + 1);
+            load_row(B, k + 3);
+        }
+    }
+    for (int i = lane; i < n; i += 64) {
+        a.gS[off + i] = S[i];
+        a.gF[off + i] = F[i];
+    }
+}
+
+// Dense problems up to this size run sqa_chi production sweeps with chk_wave_kernel.
+constexpr int kChkWaveMaxN = 2048;
+}  // namespace
+
 void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t steps,
                              const double *betas, const double *gammas, int64_t sweeps, uint64_t seed,
                              int8_t *best_state, double *best_energy, double *trace, float *ms) {
@@ -1886,7 +1985,9 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
         hjp[k] = static_cast<float>(jp);
         hjoin[k] = static_cast<float>(1.0 - std::exp(-2.0 * jp));
     }
-    DevBuf<float> dbs(steps), djp(steps), djoin(steps), gF(pl.use_lds ? 1 : R * P * n);
+    const char *wave_env = std::getenv("QANNEAL_CHI_WAVE");
+    const bool wave = p->kind == QA_KIND_DENSE && n <= kChkWaveMaxN && !(wave_env && wave_env[0] == '0');
+    DevBuf<float> dbs(steps), djp(steps), djoin(steps), gF(pl.use_lds && !wave ? 1 : R * P * n);
     DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
     DevBuf<double> tpart(steps * ngroups), be(ngroups), ex(1);
     DevBuf<int> bstep(ngroups), bchain(ngroups);
@@ -1919,7 +2020,61 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
     a.accepted = accd.ptr;
     a.sweep_kind = 1;
     QA_HIP(hipEventRecord(ctx->ev0, s));
-    launch_sqa(ctx, pl, a);
+    if (!wave) {
+        launch_sqa(ctx, pl, a);
+    } else {
+        // random start state and its fields (no sweeps, no tail), then per schedule step the
+        // phases of every sweep in chk_wave_kernel and the step tail in the two-barrier kernel
+        SqaArgs init = a;
+        init.steps = 1;
+        init.spb = 0;
+        init.tail_mode = 2;
+        init.save_fields = 1;
+        launch_sqa(ctx, pl, init);
+        const int nph = P == 1 ? 1 : ((P & 1) ? 3 : 2);
+        const int rn = static_cast<int>((n + 3) / 4 * 4), ngr = rn / 4;
+        const void *kern = ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
+                         : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2>)
+                         : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4>)
+                                      : reinterpret_cast<const void *>(chk_wave_kernel<8>);
+        QA_REQUIRE(ngr <= 512, "chk_wave_kernel: row too long");
+        const size_t shmem = lds_align16(static_cast<size_t>(rn) * 4 + static_cast<size_t>(n) * 6);
+        QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+        ChkArgs ca{};
+        ca.J4rows = a.pv.J32p;
+        ca.gS = gS.ptr;
+        ca.gF = gF.ptr;
+        ca.P = static_cast<int>(P);
+        ca.n = static_cast<int>(n);
+        ca.rn = rn;
+        ca.k0 = a.k0;
+        ca.k1 = a.k1;
+        for (int64_t k = 0; k < steps; ++k) {
+            ca.bs = hbs[k];
+            ca.jp = hjp[k];
+            for (int64_t sw = 0; sw < sweeps; ++sw) {
+                ca.sweep_id = static_cast<uint32_t>(k * sweeps + sw);
+                for (int ph = 0; ph < nph; ++ph) {
+                    ca.ph = ph;
+                    void *params[] = {&ca};
+                    QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(R * P)), dim3(64), params, shmem, s));
+                }
+            }
+            SqaArgs tl = a;
+            tl.steps = 1;
+            tl.spb = 0;
+            tl.resume = 1;
+            tl.save_fields = 0;
+            tl.tail_mode = 0;
+            tl.bs = dbs.ptr + k;
+            tl.jp = djp.ptr + k;
+            tl.join = djoin.ptr + k;
+            tl.trace_part = tpart.ptr + k * ngroups;
+            tl.step_base = static_cast<int>(k);
+            launch_sqa(ctx, pl, tl);
+        }
+        QA_HIP(hipGetLastError());
+    }
     QA_HIP(hipEventRecord(ctx->ev1, s));
     std::vector<double> hpart(steps * ngroups), hbe(ngroups);
     std::vector<int> hstep(ngroups), hchain(ngroups);

@@ -197,6 +197,33 @@ def test_sqa_fp32_split_step_launches_bit_identical(ctx, monkeypatch):
     monkeypatch.delenv("QANNEAL_SQA_SPLIT", raising=False)
 
 
+def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
+    """sqa_chi production sweeps of dense problems run one wavefront per chain and one launch per
+    checkerboard phase; the result must equal the two-barrier kernel's single launch bit for bit:
+    odd and even P (three and two colours), P = 1 and 2, n not a multiple of 4, LDS-resident and
+    global-memory state in the two-barrier kernel, and the largest dense size of the wave path."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(53)
+    cases = [(12, 8, 4), (33, 5, 3), (100, 1, 2), (130, 2, 3), (257, 7, 2), (512, 32, 4), (1100, 16, 2),
+             (2048, 32, 4)]
+    for n, P, R in cases:
+        print("chi production case", n, P, R, flush=True)
+        J = _sk(rng, n)
+        dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), J, 0.25)
+        kw = dict(beta=2.5, gamma_start=3.0, gamma_end=0.05, num_steps=6, sweeps_per_step=2, scan_points=4,
+                  scan_sweeps=2, scan_burn=1, seed=29)
+        res = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_CHI_WAVE", v)
+            res.append(abi.sqa_chi_run(dp, P, R, **kw))
+        a, b = res
+        assert np.array_equal(a["best_state"], b["best_state"]), (n, P, R)
+        assert bits_equal([a["best_energy"]], [b["best_energy"]]), (n, P, R)
+        assert bits_equal(a["energy_trace"], b["energy_trace"]), (n, P, R)
+        assert bits_equal(a["gamma_schedule"], b["gamma_schedule"]), (n, P, R)
+    monkeypatch.delenv("QANNEAL_CHI_WAVE", raising=False)
+
+
 def test_sqa_fp32_distribution_vs_oracle(ctx):
     from scipy.stats import ks_2samp
     from qlatanneal_amd import abi
