@@ -418,6 +418,17 @@ qa_status qa_problem_sparse(qa_ctx *ctx, int64_t n, const double *h, int64_t ned
         p->ei.alloc(std::max<int64_t>(nedges, 1)); p->ei.upload(e_i.data(), nedges, s);
         p->ej.alloc(std::max<int64_t>(nedges, 1)); p->ej.upload(e_j.data(), nedges, s);
         p->ev64.alloc(std::max<int64_t>(nedges, 1)); p->ev64.upload(ev, nedges, s);
+        {  // no spin listed twice in one adjacency row (duplicate edges make such rows)
+            std::vector<int64_t> seen(n, -1);
+            for (int64_t i = 0; i < n && p->adj_unique; ++i)
+                for (int32_t q = ptr[i]; q < ptr[i + 1]; ++q) {
+                    if (seen[idx[q]] == i) {
+                        p->adj_unique = false;
+                        break;
+                    }
+                    seen[idx[q]] = i;
+                }
+        }
         p->adj_ptr.alloc(n + 1); p->adj_ptr.upload(ptr.data(), n + 1, s);
         p->adj_idx.alloc(std::max<int64_t>(2 * nedges, 1)); p->adj_idx.upload(idx.data(), 2 * nedges, s);
         p->adj_val64.alloc(std::max<int64_t>(2 * nedges, 1)); p->adj_val64.upload(val.data(), 2 * nedges, s);

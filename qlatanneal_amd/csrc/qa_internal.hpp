@@ -157,6 +157,7 @@ struct qa_problem {
     qa::DevBuf<int32_t> ei, ej;
     qa::DevBuf<double> ev64;
     qa::DevBuf<int32_t> adj_ptr, adj_idx;  // [n+1], [2*nedges]
+    bool adj_unique = true;                // no spin appears twice in one adjacency row
     qa::DevBuf<double> adj_val64;
     qa::DevBuf<float> adj_val32;
     // hubo: consolidated terms + incidence CSR (term ids per variable, ascending)

@@ -1881,17 +1881,27 @@ namespace {
 // Philox words sit in the wave's LDS, the J rows stream in site order with the loads issued two
 // sites ahead (register sets A and B, the site loop unrolled by two), and an accepted flip updates
 // the fields with the same fmaf per element, in the same site order, as the two-barrier kernel's
-// checkerboard sweep: the trajectory is the same bit for bit.
+// checkerboard sweep: the trajectory is the same bit for bit. Sparse problems stage their
+// adjacency (CSR) in the wave's LDS instead; a flip's neighbour updates run one per lane when no
+// adjacency row lists a spin twice, else one after another on lane 0, in row order.
 struct ChkArgs {
-    const float *J4rows;  // J32p: n rows of rn floats (rn = n rounded up to 4)
+    const float *J4rows;  // dense: J32p, n rows of rn floats (rn = n rounded up to 4)
+    const int32_t *adj_ptr, *adj_idx;  // sparse: CSR [n + 1], [nnz]
+    const float *adj_val;              // sparse: [nnz]
     int8_t *gS;           // [R*P][n]
     float *gF;            // [R*P][n]
-    int P, n, rn, ph;
+    int P, n, rn, ph, nnz, unique;
     uint32_t sweep_id, k0, k1;
     float bs, jp;
 };
 
-template <int NQ>  // float4 groups of a row per lane: rn <= 256 * NQ
+// LDS bytes of chk_wave_kernel: fields, words, spins, neighbour sums (+ the CSR when sparse)
+__host__ __device__ constexpr size_t chk_lds_bytes(int n, int rn, int nnz, bool sparse) {
+    return lds_align16(static_cast<size_t>(rn) * 4 + static_cast<size_t>(n) * 6) +
+           (sparse ? lds_align16(static_cast<size_t>(n + 1) * 4) + static_cast<size_t>(nnz) * 8 : 0);
+}
+
+template <int NQ>  // dense: float4 groups of a row per lane (rn <= 256 * NQ); 0: sparse
 __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blockIdx.x, lane = threadIdx.x;
@@ -1905,6 +1915,16 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     uint32_t *W = reinterpret_cast<uint32_t *>(F + rn);   // [n] uniform words of this phase's sites
     int8_t *S = reinterpret_cast<int8_t *>(W + n);        // [n]
     int8_t *NB = S + n;                                   // [n] s_{t-1,i} + s_{t+1,i} (fixed in the phase)
+    int32_t *AP = reinterpret_cast<int32_t *>(smem + chk_lds_bytes(n, rn, 0, false));  // sparse: [n + 1]
+    int32_t *AI = AP + lds_align16(static_cast<size_t>(n + 1) * 4) / 4;                  // sparse: [nnz]
+    float *AV = reinterpret_cast<float *>(AI + a.nnz);                                   // sparse: [nnz]
+    if constexpr (NQ == 0) {
+        for (int i = lane; i <= n; i += 64) AP[i] = a.adj_ptr[i];
+        for (int q = lane; q < a.nnz; q += 64) {
+            AI[q] = a.adj_idx[q];
+            AV[q] = a.adj_val[q];
+        }
+    }
     const int64_t off = static_cast<int64_t>(c) * n;
     for (int i = lane; i < n; i += 64) {
         S[i] = a.gS[off + i];
@@ -1924,12 +1944,13 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
         }
     }
     __syncthreads();
+    constexpr int NR = NQ > 0 ? NQ : 1;
     const f4 *J4 = reinterpret_cast<const f4 *>(a.J4rows);
-    auto load_row = [&](f4 (&h)[NQ], int k) {
+    auto load_row = [&](f4 (&h)[NR], int k) {
         const f4 *row = J4 + static_cast<int64_t>(min(k, n - 1)) * ng;
         static_for<NQ>([&](auto q) { h[q] = row[min(lane + 64 * q, ng - 1)]; });
     };
-    auto site = [&](const f4 (&h)[NQ], int k) {
+    auto site = [&](const f4 (&h)[NR], int k) {
         const float f = F[k];
         const int8_t s = S[k];
         const int nb = NB[k];
@@ -1940,25 +1961,38 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
         if (acc) {
             if (lane == 0) S[k] = static_cast<int8_t>(-s);
             const float ds = -2.0f * fs;
-            static_for<NQ>([&](auto q) {
-                const int gi = lane + 64 * q;
-                if (gi < ng) {
-                    const f4 x = h[q], y = F4[gi];
-                    F4[gi] = f4{fmaf(ds, x.x, y.x), fmaf(ds, x.y, y.y), fmaf(ds, x.z, y.z), fmaf(ds, x.w, y.w)};
+            if constexpr (NQ == 0) {
+                const int q0 = AP[k], q1 = AP[k + 1];
+                if (a.unique) {
+                    for (int q = q0 + lane; q < q1; q += 64) F[AI[q]] += AV[q] * ds;
+                } else if (lane == 0) {
+                    for (int q = q0; q < q1; ++q) F[AI[q]] += AV[q] * ds;
                 }
-            });
+            } else {
+                static_for<NQ>([&](auto q) {
+                    const int gi = lane + 64 * q;
+                    if (gi < ng) {
+                        const f4 x = h[q], y = F4[gi];
+                        F4[gi] = f4{fmaf(ds, x.x, y.x), fmaf(ds, x.y, y.y), fmaf(ds, x.z, y.z), fmaf(ds, x.w, y.w)};
+                    }
+                });
+            }
         }
         __syncthreads();  // this site's field and spin writes before the next site's reads
     };
-    f4 A[NQ], B[NQ];
-    load_row(A, 0);
-    load_row(B, 1);
-    for (int k = 0; k < n; k += 2) {
-        site(A, k);
-        load_row(A, k + 2);
-        if (k + 1 < n) {
-            site(B, k + 1);
-            load_row(B, k + 3);
+    f4 A[NR], B[NR];
+    if constexpr (NQ == 0) {
+        for (int k = 0; k < n; ++k) site(A, k);
+    } else {
+        load_row(A, 0);
+        load_row(B, 1);
+        for (int k = 0; k < n; k += 2) {
+            site(A, k);
+            load_row(A, k + 2);
+            if (k + 1 < n) {
+                site(B, k + 1);
+                load_row(B, k + 3);
+            }
         }
     }
     for (int i = lane; i < n; i += 64) {
@@ -1986,7 +2020,9 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
         hjoin[k] = static_cast<float>(1.0 - std::exp(-2.0 * jp));
     }
     const char *wave_env = std::getenv("QANNEAL_CHI_WAVE");
-    const bool wave = p->kind == QA_KIND_DENSE && n <= kChkWaveMaxN && !(wave_env && wave_env[0] == '0');
+    const int rn = static_cast<int>((n + 3) / 4 * 4), nnz = static_cast<int>(2 * p->nedges);
+    const bool sparse_fits = p->kind == QA_KIND_SPARSE && chk_lds_bytes(static_cast<int>(n), rn, nnz, true) <= 160 * 1024;
+    const bool wave = ((p->kind == QA_KIND_DENSE && n <= kChkWaveMaxN) || sparse_fits) && !(wave_env && wave_env[0] == '0');
     DevBuf<float> dbs(steps), djp(steps), djoin(steps), gF(pl.use_lds && !wave ? 1 : R * P * n);
     DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
     DevBuf<double> tpart(steps * ngroups), be(ngroups), ex(1);
@@ -2032,16 +2068,23 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
         init.save_fields = 1;
         launch_sqa(ctx, pl, init);
         const int nph = P == 1 ? 1 : ((P & 1) ? 3 : 2);
-        const int rn = static_cast<int>((n + 3) / 4 * 4), ngr = rn / 4;
-        const void *kern = ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
+        const int ngr = rn / 4;
+        const bool sparse = p->kind == QA_KIND_SPARSE;
+        const void *kern = sparse ? reinterpret_cast<const void *>(chk_wave_kernel<0>)
+                         : ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
                          : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2>)
                          : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4>)
                                       : reinterpret_cast<const void *>(chk_wave_kernel<8>);
-        QA_REQUIRE(ngr <= 512, "chk_wave_kernel: row too long");
-        const size_t shmem = lds_align16(static_cast<size_t>(rn) * 4 + static_cast<size_t>(n) * 6);
+        QA_REQUIRE(sparse || ngr <= 512, "chk_wave_kernel: row too long");
+        const size_t shmem = chk_lds_bytes(static_cast<int>(n), rn, nnz, sparse);
         QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
         ChkArgs ca{};
         ca.J4rows = a.pv.J32p;
+        ca.adj_ptr = a.pv.adj_ptr;
+        ca.adj_idx = a.pv.adj_idx;
+        ca.adj_val = a.pv.adj_val32;
+        ca.nnz = nnz;
+        ca.unique = p->adj_unique ? 1 : 0;
         ca.gS = gS.ptr;
         ca.gF = gF.ptr;
         ca.P = static_cast<int>(P);

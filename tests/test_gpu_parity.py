@@ -198,10 +198,11 @@ def test_sqa_fp32_split_step_launches_bit_identical(ctx, monkeypatch):
 
 
 def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
-    """sqa_chi production sweeps of dense problems run one wavefront per chain and one launch per
-    checkerboard phase; the result must equal the two-barrier kernel's single launch bit for bit:
-    odd and even P (three and two colours), P = 1 and 2, n not a multiple of 4, LDS-resident and
-    global-memory state in the two-barrier kernel, and the largest dense size of the wave path."""
+    """sqa_chi production sweeps of dense and sparse problems run one wavefront per chain and one
+    launch per checkerboard phase; the result must equal the two-barrier kernel's single launch bit
+    for bit: odd and even P (three and two colours), P = 1 and 2, n not a multiple of 4, LDS-resident
+    and global-memory state in the two-barrier kernel, the largest dense size of the wave path, and
+    sparse graphs with and without repeated edges."""
     from qlatanneal_amd import abi
     rng = np.random.default_rng(53)
     cases = [(12, 8, 4), (33, 5, 3), (100, 1, 2), (130, 2, 3), (257, 7, 2), (512, 32, 4), (1100, 16, 2),
@@ -221,6 +222,31 @@ def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
         assert bits_equal([a["best_energy"]], [b["best_energy"]]), (n, P, R)
         assert bits_equal(a["energy_trace"], b["energy_trace"]), (n, P, R)
         assert bits_equal(a["gamma_schedule"], b["gamma_schedule"]), (n, P, R)
+    # sparse: a 3-regular graph with unit weights (MAX-CUT, rows without repeated spins: parallel
+    # neighbour updates) and a graph with repeated edges (sequential updates in row order)
+    for n, P, R, dup in [(400, 32, 4, False), (257, 5, 3, False), (150, 8, 2, True), (4096, 32, 4, False)]:
+        print("chi production sparse case", n, P, R, dup, flush=True)
+        edges = set()
+        while len(edges) < 3 * n // 2:
+            perm = rng.permutation(n)
+            for i in range(0, n - 1, 2):
+                u, v = sorted((int(perm[i]), int(perm[i + 1])))
+                if len(edges) < 3 * n // 2:
+                    edges.add((u, v))
+        el = [(u, v, 1.0) for u, v in sorted(edges)]
+        if dup:
+            el += [(u, v, float(rng.normal())) for u, v, _ in el[::5]]
+        dp = abi.DeviceProblem.sparse(ctx, rng.normal(0, 0.1, n), el, -0.5)
+        kw = dict(beta=3.0, gamma_start=3.0, gamma_end=0.05, num_steps=5, sweeps_per_step=2, scan_points=4,
+                  scan_sweeps=2, scan_burn=1, seed=31)
+        res = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_CHI_WAVE", v)
+            res.append(abi.sqa_chi_run(dp, P, R, **kw))
+        a, b = res
+        assert np.array_equal(a["best_state"], b["best_state"]), (n, P, R, dup)
+        assert bits_equal([a["best_energy"]], [b["best_energy"]]), (n, P, R, dup)
+        assert bits_equal(a["energy_trace"], b["energy_trace"]), (n, P, R, dup)
     monkeypatch.delenv("QANNEAL_CHI_WAVE", raising=False)
 
 
