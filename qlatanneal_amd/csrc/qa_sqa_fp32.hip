@@ -1811,67 +1811,6 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
 }
 
 
-// sqa_chi pilot scan: `points` x R replicas in one launch; replica rep belongs to point rep / R
-// and sweeps (burn + sweeps) checkerboard sweeps on a fresh worldline at that point's J_perp,
-// accumulating sum m^2 and sum |m| over the measured sweeps.
-void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t points,
-                    const std::vector<double> &bs, const std::vector<double> &jp, int64_t burn, int64_t sweeps,
-                    uint64_t seed, std::vector<double> &m2, std::vector<double> &mabs, float *ms) {
-    const int64_t n = p->n, NS = points * R;
-    const SqaPlan pl = plan_sqa(ctx, p, NS, P);
-    hipStream_t s = ctx->stream;
-    std::vector<float> hbs(points), hjp(points), hjoin(points);
-    std::vector<int> pof(NS);
-    for (int64_t q = 0; q < points; ++q) {
-        hbs[q] = static_cast<float>(bs[q]);
-        hjp[q] = static_cast<float>(jp[q]);
-        hjoin[q] = static_cast<float>(1.0 - std::exp(-2.0 * jp[q]));
-    }
-    for (int64_t k = 0; k < NS; ++k) pof[k] = static_cast<int>(k / R);
-    DevBuf<float> dbs(points), djp(points), djoin(points), gF(pl.use_lds ? 1 : NS * P * n);
-    DevBuf<int> dpof(NS);
-    DevBuf<int8_t> gS(NS * P * n);
-    DevBuf<double> dm2(NS), dma(NS);
-    DevBuf<unsigned long long> accd(pl.ngroups);
-    dbs.upload(hbs.data(), points, s);
-    djp.upload(hjp.data(), points, s);
-    djoin.upload(hjoin.data(), points, s);
-    dpof.upload(pof.data(), NS, s);
-    QA_HIP(hipMemsetAsync(dm2.ptr, 0, NS * sizeof(double), s));
-    QA_HIP(hipMemsetAsync(dma.ptr, 0, NS * sizeof(double), s));
-    SqaArgs a{};
-    a.pv = view_of(p);
-    a.P = static_cast<int>(P);
-    a.R = static_cast<int>(NS);
-    a.rg = static_cast<int>(pl.rg);
-    a.steps = 1;
-    a.spb = static_cast<int>(burn + sweeps);
-    a.bs = dbs.ptr;
-    a.jp = djp.ptr;
-    a.join = djoin.ptr;
-    a.k0 = static_cast<uint32_t>(seed);
-    a.k1 = static_cast<uint32_t>(seed >> 32);
-    a.gS = gS.ptr;
-    a.gF = gF.ptr;
-    a.accepted = accd.ptr;
-    a.param_of_rep = dpof.ptr;
-    a.tail_mode = 2;
-    a.sweep_kind = 1;
-    a.mag_burn = static_cast<int>(burn);
-    a.mag_m2 = dm2.ptr;
-    a.mag_abs = dma.ptr;
-    QA_HIP(hipEventRecord(ctx->ev0, s));
-    launch_sqa(ctx, pl, a);
-    QA_HIP(hipEventRecord(ctx->ev1, s));
-    m2.resize(NS);
-    mabs.resize(NS);
-    dm2.download(m2.data(), NS, s);
-    dma.download(mabs.data(), NS, s);
-    QA_HIP(hipStreamSynchronize(s));
-    QA_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
-}
-
-// sqa_chi production anneal: R replicas, checkerboard sweeps along (beta_k, Gamma_k).
 namespace {
 // ---- sqa_chi production sweeps: one wavefront per chain -----------------------------------------
 // In a checkerboard phase (sqa_chi_annealer.cpp:81-140) every chain of the phase's colour sweeps
@@ -1890,9 +1829,11 @@ struct ChkArgs {
     const float *adj_val;              // sparse: [nnz]
     int8_t *gS;           // [R*P][n]
     float *gF;            // [R*P][n]
+    const float *bsv, *jpv;  // beta / P and J_perp by parameter index
+    const int *pof;          // non-null: replica r uses index pof[r] (pilot scan points), else pidx
+    int pidx;
     int P, n, rn, ph, nnz, unique;
     uint32_t sweep_id, k0, k1;
-    float bs, jp;
 };
 
 // LDS bytes of chk_wave_kernel: fields, words, spins, neighbour sums (+ the CSR when sparse)
@@ -1910,6 +1851,8 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     const int colour = P == 1 ? 0 : ((P & 1) ? (t == P - 1 ? 1 : ((t & 1) ? 2 : 0)) : (t & 1));
     if (colour != a.ph) return;  // the whole wave: this kernel has no workgroup barrier after here
     const int cprev = r * P + (t == 0 ? P - 1 : t - 1), cnext = r * P + (t + 1 == P ? 0 : t + 1);
+    const int pq = a.pof ? a.pof[r] : a.pidx;
+    const float bs = a.bsv[pq], jp = a.jpv[pq];
     float *F = reinterpret_cast<float *>(smem);           // [rn]
     f4 *F4 = reinterpret_cast<f4 *>(smem);
     uint32_t *W = reinterpret_cast<uint32_t *>(F + rn);   // [n] uniform words of this phase's sites
@@ -1956,7 +1899,7 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
         const int nb = NB[k];
         const uint32_t word = W[k];
         const float fs = static_cast<float>(s);
-        const float de = fmaf(a.bs, -2.0f * fs * f, 2.0f * a.jp * fs * static_cast<float>(nb));
+        const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(nb));
         const bool acc = de <= 0.0f || u01(word) < __expf(-de);
         if (acc) {
             if (lane == 0) S[k] = static_cast<int8_t>(-s);
@@ -2001,9 +1944,166 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     }
 }
 
-// Dense problems up to this size run sqa_chi production sweeps with chk_wave_kernel.
+// Pilot magnetisation after a measured sweep (the checkerboard kernel's measurement, same
+// arithmetic): one workgroup per replica, m_r = sum of the replica's spins / (n P), accumulated
+// as m^2 and |m|.
+__global__ void __launch_bounds__(256) chk_mag_kernel(const int8_t *gS, int P, int n, double *m2, double *mabs) {
+    __shared__ double ered[kMaxChains];
+    const int r = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int t = wave; t < P; t += 4) {
+        const int8_t *sc = gS + (static_cast<int64_t>(r) * P + t) * n;
+        int sum = 0;
+        for (int i = lane; i < n; i += 64) sum += sc[i];
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (lane == 0) ered[t] = static_cast<double>(sum);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double tot = 0.0;
+        for (int t = 0; t < P; ++t) tot += ered[t];
+        const double m = tot / (static_cast<double>(n) * P);
+        m2[r] += m * m;
+        mabs[r] += fabs(m);
+    }
+}
+
+// Problems whose checkerboard sweeps run in chk_wave_kernel: dense up to this size, and, for the
+// production anneal only, sparse ones whose adjacency fits the wave's LDS. (The pilot scan runs
+// 64x more chains; a sparse wave's LDS footprint then leaves too few waves per CU, and the
+// two-barrier kernel's 64 chains per workgroup measured faster: MAX-CUT N=4096 529 ms vs 1718 ms.)
 constexpr int kChkWaveMaxN = 2048;
+
+bool chk_wave_applies(const qa_problem *p, bool pilot) {
+    const char *env = std::getenv("QANNEAL_CHI_WAVE");
+    if (env && env[0] == '0') return false;
+    const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4, nnz = static_cast<int>(2 * p->nedges);
+    if (p->kind == QA_KIND_DENSE) return n <= kChkWaveMaxN;
+    if (p->kind == QA_KIND_SPARSE) return !pilot && chk_lds_bytes(n, rn, nnz, true) <= 160 * 1024;
+    return false;
+}
+
+// Launches the phases of one checkerboard sweep (sweep_id) of every chain.
+void chk_sweep(const qa_problem *p, ChkArgs &ca, int64_t chains, int P, hipStream_t s) {
+    const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4, ngr = rn / 4;
+    const int nnz = static_cast<int>(2 * p->nedges);
+    const bool sparse = p->kind == QA_KIND_SPARSE;
+    const void *kern = sparse ? reinterpret_cast<const void *>(chk_wave_kernel<0>)
+                     : ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
+                     : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2>)
+                     : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4>)
+                                  : reinterpret_cast<const void *>(chk_wave_kernel<8>);
+    QA_REQUIRE(sparse || ngr <= 512, "chk_wave_kernel: row too long");
+    const size_t shmem = chk_lds_bytes(n, rn, nnz, sparse);
+    QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    const ProblemView pv = view_of(p);
+    ca.J4rows = pv.J32p;
+    ca.adj_ptr = pv.adj_ptr;
+    ca.adj_idx = pv.adj_idx;
+    ca.adj_val = pv.adj_val32;
+    ca.nnz = nnz;
+    ca.unique = p->adj_unique ? 1 : 0;
+    ca.P = P;
+    ca.n = n;
+    ca.rn = rn;
+    const int nph = P == 1 ? 1 : ((P & 1) ? 3 : 2);
+    for (int ph = 0; ph < nph; ++ph) {
+        ca.ph = ph;
+        void *params[] = {&ca};
+        QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(chains)), dim3(64), params, shmem, s));
+    }
+}
 }  // namespace
+
+// sqa_chi pilot scan: `points` x R replicas in one launch; replica rep belongs to point rep / R
+// and sweeps (burn + sweeps) checkerboard sweeps on a fresh worldline at that point's J_perp,
+// accumulating sum m^2 and sum |m| over the measured sweeps.
+void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t points,
+                    const std::vector<double> &bs, const std::vector<double> &jp, int64_t burn, int64_t sweeps,
+                    uint64_t seed, std::vector<double> &m2, std::vector<double> &mabs, float *ms) {
+    const int64_t n = p->n, NS = points * R;
+    const SqaPlan pl = plan_sqa(ctx, p, NS, P);
+    hipStream_t s = ctx->stream;
+    std::vector<float> hbs(points), hjp(points), hjoin(points);
+    std::vector<int> pof(NS);
+    for (int64_t q = 0; q < points; ++q) {
+        hbs[q] = static_cast<float>(bs[q]);
+        hjp[q] = static_cast<float>(jp[q]);
+        hjoin[q] = static_cast<float>(1.0 - std::exp(-2.0 * jp[q]));
+    }
+    for (int64_t k = 0; k < NS; ++k) pof[k] = static_cast<int>(k / R);
+    const bool wave = chk_wave_applies(p, true);
+    DevBuf<float> dbs(points), djp(points), djoin(points), gF(pl.use_lds && !wave ? 1 : NS * P * n);
+    DevBuf<int> dpof(NS);
+    DevBuf<int8_t> gS(NS * P * n);
+    DevBuf<double> dm2(NS), dma(NS);
+    DevBuf<unsigned long long> accd(pl.ngroups);
+    dbs.upload(hbs.data(), points, s);
+    djp.upload(hjp.data(), points, s);
+    djoin.upload(hjoin.data(), points, s);
+    dpof.upload(pof.data(), NS, s);
+    QA_HIP(hipMemsetAsync(dm2.ptr, 0, NS * sizeof(double), s));
+    QA_HIP(hipMemsetAsync(dma.ptr, 0, NS * sizeof(double), s));
+    SqaArgs a{};
+    a.pv = view_of(p);
+    a.P = static_cast<int>(P);
+    a.R = static_cast<int>(NS);
+    a.rg = static_cast<int>(pl.rg);
+    a.steps = 1;
+    a.spb = static_cast<int>(burn + sweeps);
+    a.bs = dbs.ptr;
+    a.jp = djp.ptr;
+    a.join = djoin.ptr;
+    a.k0 = static_cast<uint32_t>(seed);
+    a.k1 = static_cast<uint32_t>(seed >> 32);
+    a.gS = gS.ptr;
+    a.gF = gF.ptr;
+    a.accepted = accd.ptr;
+    a.param_of_rep = dpof.ptr;
+    a.tail_mode = 2;
+    a.sweep_kind = 1;
+    a.mag_burn = static_cast<int>(burn);
+    a.mag_m2 = dm2.ptr;
+    a.mag_abs = dma.ptr;
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    if (!wave) {
+        launch_sqa(ctx, pl, a);
+    } else {
+        // random start state and its fields, then each sweep's phases in chk_wave_kernel and,
+        // from sweep `burn` on, the magnetisation sample
+        SqaArgs init = a;
+        init.spb = 0;
+        init.save_fields = 1;
+        init.mag_m2 = nullptr;
+        init.mag_abs = nullptr;
+        launch_sqa(ctx, pl, init);
+        ChkArgs ca{};
+        ca.gS = gS.ptr;
+        ca.gF = gF.ptr;
+        ca.bsv = dbs.ptr;
+        ca.jpv = djp.ptr;
+        ca.pof = dpof.ptr;
+        ca.k0 = a.k0;
+        ca.k1 = a.k1;
+        for (int64_t sw = 0; sw < burn + sweeps; ++sw) {
+            ca.sweep_id = static_cast<uint32_t>(sw);
+            chk_sweep(p, ca, NS * P, static_cast<int>(P), s);
+            if (sw >= burn) {
+                hipLaunchKernelGGL(chk_mag_kernel, dim3(static_cast<unsigned>(NS)), dim3(256), 0, s, gS.ptr,
+                                   static_cast<int>(P), static_cast<int>(n), dm2.ptr, dma.ptr);
+                QA_HIP(hipGetLastError());
+            }
+        }
+    }
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    m2.resize(NS);
+    mabs.resize(NS);
+    dm2.download(m2.data(), NS, s);
+    dma.download(mabs.data(), NS, s);
+    QA_HIP(hipStreamSynchronize(s));
+    QA_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+}
+
+// sqa_chi production anneal: R replicas, checkerboard sweeps along (beta_k, Gamma_k).
 
 void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int64_t steps,
                              const double *betas, const double *gammas, int64_t sweeps, uint64_t seed,
@@ -2019,10 +2119,7 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
         hjp[k] = static_cast<float>(jp);
         hjoin[k] = static_cast<float>(1.0 - std::exp(-2.0 * jp));
     }
-    const char *wave_env = std::getenv("QANNEAL_CHI_WAVE");
-    const int rn = static_cast<int>((n + 3) / 4 * 4), nnz = static_cast<int>(2 * p->nedges);
-    const bool sparse_fits = p->kind == QA_KIND_SPARSE && chk_lds_bytes(static_cast<int>(n), rn, nnz, true) <= 160 * 1024;
-    const bool wave = ((p->kind == QA_KIND_DENSE && n <= kChkWaveMaxN) || sparse_fits) && !(wave_env && wave_env[0] == '0');
+    const bool wave = chk_wave_applies(p, false);
     DevBuf<float> dbs(steps), djp(steps), djoin(steps), gF(pl.use_lds && !wave ? 1 : R * P * n);
     DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
     DevBuf<double> tpart(steps * ngroups), be(ngroups), ex(1);
@@ -2067,41 +2164,18 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
         init.tail_mode = 2;
         init.save_fields = 1;
         launch_sqa(ctx, pl, init);
-        const int nph = P == 1 ? 1 : ((P & 1) ? 3 : 2);
-        const int ngr = rn / 4;
-        const bool sparse = p->kind == QA_KIND_SPARSE;
-        const void *kern = sparse ? reinterpret_cast<const void *>(chk_wave_kernel<0>)
-                         : ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
-                         : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2>)
-                         : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4>)
-                                      : reinterpret_cast<const void *>(chk_wave_kernel<8>);
-        QA_REQUIRE(sparse || ngr <= 512, "chk_wave_kernel: row too long");
-        const size_t shmem = chk_lds_bytes(static_cast<int>(n), rn, nnz, sparse);
-        QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
         ChkArgs ca{};
-        ca.J4rows = a.pv.J32p;
-        ca.adj_ptr = a.pv.adj_ptr;
-        ca.adj_idx = a.pv.adj_idx;
-        ca.adj_val = a.pv.adj_val32;
-        ca.nnz = nnz;
-        ca.unique = p->adj_unique ? 1 : 0;
         ca.gS = gS.ptr;
         ca.gF = gF.ptr;
-        ca.P = static_cast<int>(P);
-        ca.n = static_cast<int>(n);
-        ca.rn = rn;
+        ca.bsv = dbs.ptr;
+        ca.jpv = djp.ptr;
         ca.k0 = a.k0;
         ca.k1 = a.k1;
         for (int64_t k = 0; k < steps; ++k) {
-            ca.bs = hbs[k];
-            ca.jp = hjp[k];
+            ca.pidx = static_cast<int>(k);
             for (int64_t sw = 0; sw < sweeps; ++sw) {
                 ca.sweep_id = static_cast<uint32_t>(k * sweeps + sw);
-                for (int ph = 0; ph < nph; ++ph) {
-                    ca.ph = ph;
-                    void *params[] = {&ca};
-                    QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(R * P)), dim3(64), params, shmem, s));
-                }
+                chk_sweep(p, ca, R * P, static_cast<int>(P), s);
             }
             SqaArgs tl = a;
             tl.steps = 1;

@@ -198,9 +198,9 @@ def test_sqa_fp32_split_step_launches_bit_identical(ctx, monkeypatch):
 
 
 def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
-    """sqa_chi production sweeps of dense and sparse problems run one wavefront per chain and one
-    launch per checkerboard phase; the result must equal the two-barrier kernel's single launch bit
-    for bit: odd and even P (three and two colours), P = 1 and 2, n not a multiple of 4, LDS-resident
+    """sqa_chi pilot and production sweeps of dense and sparse problems run one wavefront per chain
+    and one launch per checkerboard phase; the result must equal the two-barrier kernel's single
+    launch bit for bit (pilot chi_B scan included): odd and even P (three and two colours), P = 1 and 2, n not a multiple of 4, LDS-resident
     and global-memory state in the two-barrier kernel, the largest dense size of the wave path, and
     sparse graphs with and without repeated edges."""
     from qlatanneal_amd import abi
@@ -222,6 +222,13 @@ def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
         assert bits_equal([a["best_energy"]], [b["best_energy"]]), (n, P, R)
         assert bits_equal(a["energy_trace"], b["energy_trace"]), (n, P, R)
         assert bits_equal(a["gamma_schedule"], b["gamma_schedule"]), (n, P, R)
+        assert bits_equal(a["scan_chi_B"], b["scan_chi_B"]), (n, P, R)
+        if n in (257, 1100):  # the pilot scan alone, more points and measured sweeps
+            pil = []
+            for v in ("1", "0"):
+                monkeypatch.setenv("QANNEAL_CHI_WAVE", v)
+                pil.append(abi.chi_pilot(dp, P, R, 2.0, 3.0, 1e-3, 16, 5, 3, seed=41))
+            assert bits_equal(pil[0]["scan_chi_B"], pil[1]["scan_chi_B"]), (n, P, R)
     # sparse: a 3-regular graph with unit weights (MAX-CUT, rows without repeated spins: parallel
     # neighbour updates) and a graph with repeated edges (sequential updates in row order)
     for n, P, R, dup in [(400, 32, 4, False), (257, 5, 3, False), (150, 8, 2, True), (4096, 32, 4, False)]:
@@ -247,6 +254,12 @@ def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
         assert np.array_equal(a["best_state"], b["best_state"]), (n, P, R, dup)
         assert bits_equal([a["best_energy"]], [b["best_energy"]]), (n, P, R, dup)
         assert bits_equal(a["energy_trace"], b["energy_trace"]), (n, P, R, dup)
+        assert bits_equal(a["scan_chi_B"], b["scan_chi_B"]), (n, P, R, dup)
+        pil = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_CHI_WAVE", v)
+            pil.append(abi.chi_pilot(dp, P, R, 2.0, 3.0, 1e-3, 16, 5, 3, seed=43))
+        assert bits_equal(pil[0]["scan_chi_B"], pil[1]["scan_chi_B"]), (n, P, R, dup)
     monkeypatch.delenv("QANNEAL_CHI_WAVE", raising=False)
 
 
