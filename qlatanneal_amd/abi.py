@@ -148,6 +148,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
+        self.device = device
         check(lib().qa_ctx_create(C.c_int(device), C.byref(self.h)))
 
     def close(self):

@@ -113,20 +113,44 @@ def pilot_profile(prob: abi.DeviceProblem, beta: float, cfg: PilotConfig, seed: 
     return prof, r
 
 
-def make_dataset(ctx: abi.Context, specs: Sequence[InstanceSpec], cfg: PilotConfig, log=None):
+def make_dataset(ctx: abi.Context, specs: Sequence[InstanceSpec], cfg: PilotConfig, log=None, threads: int = 1):
     """Features X [S][NFEAT] and labels Y [S][scan_points] from GPU pilot scans (flat scans dropped),
-    plus the pilot device time in ms."""
-    X, Y, ms = [], [], 0.0
-    for k, sp in enumerate(specs):
-        prob = build(ctx, sp)
+    plus the pilot device time in ms. With threads > 1 the scans run on that many contexts (one HIP
+    stream each) at once; the dataset is the same, in spec order."""
+    def one(c: abi.Context, sp: InstanceSpec):
+        prob = build(c, sp)
         prof, r = pilot_profile(prob, sp.beta, cfg, sp.seed ^ 0x5DEECE66D)
-        ms += r["kernel_ms"]
-        if prof is not None:
-            X.append(features(prob, sp.beta, cfg))
-            Y.append(prof)
-        if log and (k + 1) % max(1, len(specs) // 10) == 0:
-            log(f"pilot scans {k + 1}/{len(specs)}")
+        x = features(prob, sp.beta, cfg) if prof is not None else None
         del prob
+        return x, prof, r["kernel_ms"]
+
+    X, Y, ms = [], [], 0.0
+    step = max(1, len(specs) // 10)
+    if threads <= 1:
+        results = (one(ctx, sp) for sp in specs)
+    else:
+        import threading
+        from concurrent.futures import ThreadPoolExecutor
+        local = threading.local()
+
+        def task(sp):
+            if not hasattr(local, "ctx"):
+                local.ctx = abi.Context(ctx.device)
+            return one(local.ctx, sp)
+
+        pool = ThreadPoolExecutor(threads)
+        results = pool.map(task, specs)
+    try:
+        for k, (x, prof, t) in enumerate(results):
+            ms += t
+            if prof is not None:
+                X.append(x)
+                Y.append(prof)
+            if log and (k + 1) % step == 0:
+                log(f"pilot scans {k + 1}/{len(specs)}")
+    finally:
+        if threads > 1:
+            pool.shutdown()
     return np.array(X).reshape(-1, NFEAT), np.array(Y).reshape(-1, cfg.scan_points), ms
 
 
@@ -240,6 +264,15 @@ def _main(argv=None) -> int:
     t.add_argument("--members", type=int, default=4)
     t.add_argument("--seed", type=int, default=2024)
     t.add_argument("--data", default=None, help="also save the (X, Y) dataset here (.npz)")
+    t.add_argument("--from-data", nargs="+", default=None,
+                   help="train on these saved datasets (from `generate` or --data) instead of new scans")
+    t.add_argument("--threads", type=int, default=1, help="concurrent pilot scans (one context each)")
+    g = sub.add_parser("generate", help="pilot scans only, saved as a dataset chunk")
+    g.add_argument("--count", type=int, required=True)
+    g.add_argument("--seed", type=int, default=2024)
+    g.add_argument("--chunk", type=int, default=0, help="chunk index: chunks of one seed draw disjoint instances")
+    g.add_argument("--threads", type=int, default=4)
+    g.add_argument("--data", required=True, help="output .npz (X, Y)")
     e = sub.add_parser("evaluate")
     e.add_argument("--model", required=True)
     e.add_argument("--count", type=int, default=16)
@@ -256,10 +289,26 @@ def _main(argv=None) -> int:
     def log(msg):
         print(f"[{time.time() - t0:7.1f}s] {msg}", flush=True)
 
-    if a.cmd == "train":
+    if a.cmd == "generate":
         cfg = PilotConfig()
-        specs = random_specs(a.count, np.random.default_rng(a.seed))
-        X, Y, ms = make_dataset(ctx, specs, cfg, log=log)
+        specs = random_specs(a.count, np.random.default_rng([a.seed, a.chunk]))
+        X, Y, ms = make_dataset(ctx, specs, cfg, log=log, threads=a.threads)
+        np.savez(a.data, X=X, Y=Y)
+        log(f"chunk {a.chunk}: {X.shape[0]} scans, pilot device time {ms / 1e3:.1f} s")
+        print(json.dumps(dict(scans=int(X.shape[0]), pilot_ms=ms, wall_s=time.time() - t0)))
+    elif a.cmd == "train":
+        cfg = PilotConfig()
+        if a.from_data:
+            parts = []
+            for path in a.from_data:
+                with np.load(path, allow_pickle=False) as z:
+                    parts.append((z["X"], z["Y"]))
+            X = np.concatenate([x for x, _ in parts])
+            Y = np.concatenate([y for _, y in parts])
+            ms = float("nan")
+        else:
+            specs = random_specs(a.count, np.random.default_rng(a.seed))
+            X, Y, ms = make_dataset(ctx, specs, cfg, log=log, threads=a.threads)
         log(f"dataset {X.shape[0]} scans, pilot device time {ms / 1e3:.1f} s")
         if a.data:
             np.savez(a.data, X=X, Y=Y)
