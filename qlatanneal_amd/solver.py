@@ -512,8 +512,13 @@ def solve(problem: Any,
 
         with ThreadPoolExecutor(workers) as ex:
             futures = [ex.submit(task, r) for r in range(reads)]
-            for r in _iter_reads(reads, progress, logger):
-                collect(*futures[r].result())
+            try:
+                for r in _iter_reads(reads, progress, logger):
+                    collect(*futures[r].result())
+            except BaseException:
+                for f in futures:  # a failed read ends the call without running the queued ones
+                    f.cancel()
+                raise
     else:
         for r in _iter_reads(reads, progress, logger):
             collect(*one_read(annealer, r))
