@@ -1842,7 +1842,10 @@ __host__ __device__ constexpr size_t chk_lds_bytes(int n, int rn, int nnz, bool 
            (sparse ? lds_align16(static_cast<size_t>(n + 1) * 4) + static_cast<size_t>(nnz) * 8 : 0);
 }
 
-template <int NQ>  // dense: float4 groups of a row per lane (rn <= 256 * NQ); 0: sparse
+// kDemand (dense): load a J row only for an accepted flip instead of streaming every row ahead of
+// its site; with thousands of chains in flight (the pilot scan) this cuts the row traffic to the
+// accepted fraction while other waves hide each load's latency.
+template <int NQ, bool kDemand = false>  // dense: float4 groups of a row per lane (rn <= 256 * NQ); 0: sparse
 __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blockIdx.x, lane = threadIdx.x;
@@ -1912,10 +1915,13 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
                     for (int q = q0; q < q1; ++q) F[AI[q]] += AV[q] * ds;
                 }
             } else {
+                f4 hd[NR];
+                if constexpr (kDemand) load_row(hd, k);
+                const f4 (&hr)[NR] = kDemand ? hd : h;
                 static_for<NQ>([&](auto q) {
                     const int gi = lane + 64 * q;
                     if (gi < ng) {
-                        const f4 x = h[q], y = F4[gi];
+                        const f4 x = hr[q], y = F4[gi];
                         F4[gi] = f4{fmaf(ds, x.x, y.x), fmaf(ds, x.y, y.y), fmaf(ds, x.z, y.z), fmaf(ds, x.w, y.w)};
                     }
                 });
@@ -1924,7 +1930,7 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
         __syncthreads();  // this site's field and spin writes before the next site's reads
     };
     f4 A[NR], B[NR];
-    if constexpr (NQ == 0) {
+    if constexpr (NQ == 0 || kDemand) {
         for (int k = 0; k < n; ++k) site(A, k);
     } else {
         load_row(A, 0);
@@ -1972,6 +1978,15 @@ __global__ void __launch_bounds__(256) chk_mag_kernel(const int8_t *gS, int P, i
 // 64x more chains; a sparse wave's LDS footprint then leaves too few waves per CU, and the
 // two-barrier kernel's 64 chains per workgroup measured faster: MAX-CUT N=4096 529 ms vs 1718 ms.)
 constexpr int kChkWaveMaxN = 2048;
+// Dense J rows are loaded on demand (chk_wave_kernel<NQ, true>: only for accepted flips) rather
+// than streamed ahead of every site: the other waves in flight hide each load's latency and only
+// the accepted fraction of rows is read. Measured on SK at the C5 shape: pilot N=2048 243-354 ms
+// vs 503-597 ms streamed, N=512 13-17 vs 22-24 ms; a whole run_chi at N=2048 1.385 s vs 1.784 s.
+// QANNEAL_CHI_ROWS=stream selects the streamed rows.
+bool chk_rows_on_demand() {
+    const char *env = std::getenv("QANNEAL_CHI_ROWS");
+    return !(env && env[0] == 's');
+}
 
 bool chk_wave_applies(const qa_problem *p, bool pilot) {
     const char *env = std::getenv("QANNEAL_CHI_WAVE");
@@ -1982,12 +1997,17 @@ bool chk_wave_applies(const qa_problem *p, bool pilot) {
     return false;
 }
 
-// Launches the phases of one checkerboard sweep (sweep_id) of every chain.
-void chk_sweep(const qa_problem *p, ChkArgs &ca, int64_t chains, int P, hipStream_t s) {
+// Launches the phases of one checkerboard sweep (sweep_id) of every chain; `demand`: dense rows
+// loaded only for accepted flips (chk_wave_kernel<NQ, true>).
+void chk_sweep(const qa_problem *p, ChkArgs &ca, int64_t chains, int P, hipStream_t s, bool demand = false) {
     const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4, ngr = rn / 4;
     const int nnz = static_cast<int>(2 * p->nedges);
     const bool sparse = p->kind == QA_KIND_SPARSE;
     const void *kern = sparse ? reinterpret_cast<const void *>(chk_wave_kernel<0>)
+                     : demand ? (ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1, true>)
+                                 : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2, true>)
+                                 : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4, true>)
+                                              : reinterpret_cast<const void *>(chk_wave_kernel<8, true>))
                      : ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
                      : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2>)
                      : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4>)
@@ -2084,9 +2104,10 @@ void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int6
         ca.pof = dpof.ptr;
         ca.k0 = a.k0;
         ca.k1 = a.k1;
+        const bool demand = chk_rows_on_demand();
         for (int64_t sw = 0; sw < burn + sweeps; ++sw) {
             ca.sweep_id = static_cast<uint32_t>(sw);
-            chk_sweep(p, ca, NS * P, static_cast<int>(P), s);
+            chk_sweep(p, ca, NS * P, static_cast<int>(P), s, demand);
             if (sw >= burn) {
                 hipLaunchKernelGGL(chk_mag_kernel, dim3(static_cast<unsigned>(NS)), dim3(256), 0, s, gS.ptr,
                                    static_cast<int>(P), static_cast<int>(n), dm2.ptr, dma.ptr);
@@ -2175,7 +2196,7 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
             ca.pidx = static_cast<int>(k);
             for (int64_t sw = 0; sw < sweeps; ++sw) {
                 ca.sweep_id = static_cast<uint32_t>(k * sweeps + sw);
-                chk_sweep(p, ca, R * P, static_cast<int>(P), s);
+                chk_sweep(p, ca, R * P, static_cast<int>(P), s, chk_rows_on_demand());
             }
             SqaArgs tl = a;
             tl.steps = 1;
