@@ -186,6 +186,79 @@ def bench_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def maxcut3_edges(n: int, seed: int):
+    """3-regular-style MAX-CUT graph with unit weights: edges from random perfect matchings."""
+    rng = np.random.default_rng(seed)
+    edges = set()
+    while len(edges) < 3 * n // 2:
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            u, v = sorted((int(perm[i]), int(perm[i + 1])))
+            if len(edges) < 3 * n // 2:
+                edges.add((u, v))
+    return [(u, v, 1.0) for u, v in sorted(edges)]
+
+
+def bench_c5(args):
+    """sqa_chi at the C5 shape without a surrogate (the pilot-scheduled method the surrogate
+    replaces): run_chi on a dense SK N=2048 and a 3-regular MAX-CUT N=4096, P=32, R=4, beta=4,
+    Gamma 3.0 -> 1e-3, pilot of 64 points x (10 burn + 30 measured) sweeps, production 100 steps x
+    10 sweeps. One timed step = both runs. Algorithmic bytes: ~10 B per attempt (a chain's field and
+    spin read and written), as for the other SQA lines; the sweeps are latency-bound."""
+    from qlatanneal_amd import abi
+    P, R, beta, pts, burn, meas, steps, spb = 32, 4, 4.0, 64, 10, 30, 100, 10
+    ctx = abi.Context(0)
+    n_sk, n_mc = 2048, 4096
+    J = sk_instance(n_sk, 20261208)
+    probs = [("sk", n_sk, abi.DeviceProblem.dense(ctx, np.zeros(n_sk), J)),
+             ("maxcut3", n_mc, abi.DeviceProblem.sparse(ctx, np.zeros(n_mc), maxcut3_edges(n_mc, 20261209), 0.0))]
+    kw = dict(num_steps=steps, sweeps_per_step=spb, scan_points=pts, scan_sweeps=meas, scan_burn=burn)
+    attempts = sum(R * P * n * (pts * (burn + meas) + steps * spb) for _, n, _ in probs)
+    for w in range(args.warmup):
+        for _, _, dp in probs:
+            abi.sqa_chi_run(dp, P, R, beta, 3.0, 1e-3, num_steps=4, sweeps_per_step=2, scan_points=4, scan_sweeps=2,
+                            scan_burn=1, seed=10 + w)
+    kms, best = [], {}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        tot = 0.0
+        for name, n, dp in probs:
+            r = abi.sqa_chi_run(dp, P, R, beta, 3.0, 1e-3, seed=1000 + 97 * k, **kw)
+            tot += r["kernel_ms"]
+            best.setdefault(name, []).append(r["best_energy"] / n)
+        kms.append(tot)
+        print(f"step {k}: run_chi device {tot:.1f} ms", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    kern_s = float(np.mean(kms)) / 1e3
+    achieved = attempts * 10 / kern_s / 1e9
+    line = _line(args.steps * attempts / wall, args.steps, args.warmup, wall,
+                 {"workload": "sqa_chi_run_sk_n2048_and_maxcut3_n4096_p32_r4_pilot64x40_prod100x10_fp32",
+                  "trotter_slices": P, "replicas": R, "beta": beta, "scan_points": pts,
+                  "pilot_sweeps": burn + meas, "production": [steps, spb], "attempts_per_step": attempts,
+                  "device_ms_mean": float(np.mean(kms)),
+                  "best_energy_per_spin": {k: float(np.min(v)) for k, v in best.items()}},
+                 {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                 data="synthetic (seeded SK J_ij ~ N(0, 1/N); 3-regular unit-weight MAX-CUT)")
+    if not args.no_cpu:
+        from oracle.orc import Oracle, Problem, REF_PORTABLE_SO, ORACLE_SO
+        path, kind = (REF_PORTABLE_SO, "reference") if os.path.exists(REF_PORTABLE_SO) else (ORACLE_SO, "port")
+        cores = min(16, os.cpu_count() or 1)
+        lib = Oracle(path)
+        lib.set_threads(cores)
+        p = Problem.dense(np.zeros(n_sk), J)
+        sp, sb, sm, st, ss = 4, 1, 2, 20, 2
+        t = time.perf_counter()
+        lib.sqa_chi_run(p, P, R, beta, 3.0, 1e-3, num_steps=st, sweeps_per_step=ss, scan_points=sp, scan_sweeps=sm,
+                        scan_burn=sb, seed=7)
+        dt = time.perf_counter() - t
+        att = R * P * n_sk * (sp * (sb + sm) + st * ss)
+        line["cpu_baseline"] = {"value": att / dt, "unit": "attempts/s", "cores": cores, "kind": kind,
+                                "sample": (f"run_chi on the same SK N=2048 instance with a {sp}-point pilot of "
+                                           f"{sb}+{sm} sweeps and {st} x {ss} production sweeps ({dt:.1f} s)")}
+    print(json.dumps(line), flush=True)
+
+
 class _Hubo:
     """A HigherOrderIsing plus its (already consolidated: distinct sorted triplets) terms."""
 
@@ -300,7 +373,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c4"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c1", "c3", "c4", "c5"), default="c2",
                     help="c2 (default): the headline SQA sweep; c1: SA reads; c3: SQA-PT with field refresh; "
                          "c4: a sample of the batched-QUBO mix through the per-instance path")
     ap.add_argument("--c4-instances", type=int, default=20)
@@ -317,6 +390,9 @@ def main():
         return
     if args.workload == "c3":
         bench_c3(args)
+        return
+    if args.workload == "c5":
+        bench_c5(args)
         return
     if args.workload == "c4":
         bench_c4(args)

@@ -223,6 +223,13 @@ def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
         assert bits_equal(a["energy_trace"], b["energy_trace"]), (n, P, R)
         assert bits_equal(a["gamma_schedule"], b["gamma_schedule"]), (n, P, R)
         assert bits_equal(a["scan_chi_B"], b["scan_chi_B"]), (n, P, R)
+        if n in (130, 1100):  # dense J rows streamed ahead of every site instead of loaded on demand
+            monkeypatch.setenv("QANNEAL_CHI_WAVE", "1")
+            monkeypatch.setenv("QANNEAL_CHI_ROWS", "stream")
+            c = abi.sqa_chi_run(dp, P, R, **kw)
+            monkeypatch.delenv("QANNEAL_CHI_ROWS")
+            assert np.array_equal(a["best_state"], c["best_state"]), (n, P, R)
+            assert bits_equal(a["energy_trace"], c["energy_trace"]) and bits_equal(a["scan_chi_B"], c["scan_chi_B"])
         if n in (257, 1100):  # the pilot scan alone, more points and measured sweeps
             pil = []
             for v in ("1", "0"):
