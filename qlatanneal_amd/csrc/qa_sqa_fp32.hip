@@ -1973,6 +1973,104 @@ __global__ void __launch_bounds__(256) chk_mag_kernel(const int8_t *gS, int P, i
     }
 }
 
+// Random start state of the wavefront path for dense problems (group_init's fresh branch, same
+// arithmetic): spins from Philox (chain, i, TAG_INIT), fields f_ci = h_i + sum_j J_ji s_cj summed in
+// FP64 in j order and rounded to FP32. A workgroup takes 256 sites of kChkInitChains chains, so
+// each J column it reads serves all of them.
+constexpr int kChkInitChains = 16;
+
+__global__ void __launch_bounds__(256) chk_init_kernel(ProblemView pv, int chains, uint32_t k0, uint32_t k1,
+                                                       int8_t *gS, float *gF) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = pv.n, tid = threadIdx.x;
+    const int c0 = blockIdx.y * kChkInitChains, nc = min(kChkInitChains, chains - c0);
+    int8_t *S = reinterpret_cast<int8_t *>(smem);  // [kChkInitChains][n]
+    for (int e = tid; e < nc * n; e += 256) {
+        const int q = e / n, i = e - q * n;
+        const Philox4 r = philox4x32_10(static_cast<uint32_t>(c0 + q), static_cast<uint32_t>(i), 0u, TAG_INIT, k0, k1);
+        const int8_t v = (r.x & 1u) ? 1 : -1;
+        S[e] = v;
+        if (blockIdx.x == 0) gS[static_cast<int64_t>(c0) * n + e] = v;
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 256 + tid;
+    if (i >= n) return;
+    double acc[kChkInitChains];
+#pragma unroll
+    for (int q = 0; q < kChkInitChains; ++q) acc[q] = pv.h32[i];
+    for (int j = 0; j < n; ++j) {
+        const double jv = static_cast<double>(pv.J32[static_cast<int64_t>(j) * n + i]);
+#pragma unroll
+        for (int q = 0; q < kChkInitChains; ++q)
+            if (q < nc) acc[q] += jv * S[q * n + j];
+    }
+#pragma unroll
+    for (int q = 0; q < kChkInitChains; ++q)
+        if (q < nc) gF[static_cast<int64_t>(c0 + q) * n + i] = static_cast<float>(acc[q]);
+}
+
+void chk_init(const qa_problem *p, int64_t chains, uint32_t k0, uint32_t k1, int8_t *gS, float *gF, hipStream_t s) {
+    const int n = static_cast<int>(p->n);
+    const size_t shmem = static_cast<size_t>(kChkInitChains) * n;
+    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(chk_init_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    hipLaunchKernelGGL(chk_init_kernel, dim3(static_cast<unsigned>((n + 255) / 256),
+                                             static_cast<unsigned>((chains + kChkInitChains - 1) / kChkInitChains)),
+                       dim3(256), shmem, s, view_of(p), static_cast<int>(chains), k0, k1, gS, gF);
+    QA_HIP(hipGetLastError());
+}
+
+// Step tail of the wavefront path (group_tail's tail_mode 0 with the same arithmetic): slice
+// energies E_c = c + 1/2 sum_i s_i (h_i + f_i), one wavefront per chain (lane-strided sums, the
+// half taken per lane, then the xor-shuffle reduction), ...
+__global__ void __launch_bounds__(64) chk_energy_kernel(ProblemView pv, const int8_t *gS, const float *gF,
+                                                        double *chain_e) {
+    const int c = blockIdx.x, lane = threadIdx.x, n = pv.n;
+    const int8_t *sc = gS + static_cast<int64_t>(c) * n;
+    const float *fc = gF + static_cast<int64_t>(c) * n;
+    double acc = 0.0;
+    for (int i = lane; i < n; i += 64)
+        acc += static_cast<double>(sc[i]) * (static_cast<double>(pv.h32[i]) + static_cast<double>(fc[i]));
+    acc *= 0.5;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) chain_e[c] = pv.c + acc;
+}
+
+// ... then per replica group (rg replicas) the chain energies summed in chain order into the trace
+// partial and the group's first strict minimum kept with its state.
+__global__ void __launch_bounds__(256) chk_tail_kernel(const double *chain_e, const int8_t *gS, int R, int P, int n,
+                                                       int rg, int ngroups, int step, double *trace_part,
+                                                       double *best_e, int *best_step, int *best_chain,
+                                                       int8_t *best_state) {
+    __shared__ int s_arg;
+    const int g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = g * rg, G = min(rg, R - r0) * P;
+    if (tid == 0) {
+        double sum = 0.0, mn = best_e[g];
+        int arg = -1;
+        for (int c = 0; c < G; ++c) {
+            const double e = chain_e[static_cast<int64_t>(r0) * P + c];
+            sum += e;
+            if (e < mn) {
+                mn = e;
+                arg = c;
+            }
+        }
+        trace_part[static_cast<int64_t>(step) * ngroups + g] = sum;
+        if (arg >= 0) {
+            best_e[g] = mn;
+            best_step[g] = step;
+            best_chain[g] = r0 * P + arg;
+        }
+        s_arg = arg;
+    }
+    __syncthreads();
+    const int arg = s_arg;
+    if (arg >= 0)
+        for (int i = tid; i < n; i += 256)
+            best_state[static_cast<int64_t>(g) * n + i] = gS[(static_cast<int64_t>(r0) * P + arg) * n + i];
+}
+
 // Problems whose checkerboard sweeps run in chk_wave_kernel: dense up to this size, and, for the
 // production anneal only, sparse ones whose adjacency fits the wave's LDS. (The pilot scan runs
 // 64x more chains; a sparse wave's LDS footprint then leaves too few waves per CU, and the
@@ -2090,12 +2188,16 @@ void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int6
     } else {
         // random start state and its fields, then each sweep's phases in chk_wave_kernel and,
         // from sweep `burn` on, the magnetisation sample
-        SqaArgs init = a;
-        init.spb = 0;
-        init.save_fields = 1;
-        init.mag_m2 = nullptr;
-        init.mag_abs = nullptr;
-        launch_sqa(ctx, pl, init);
+        if (p->kind == QA_KIND_DENSE) {
+            chk_init(p, NS * P, a.k0, a.k1, gS.ptr, gF.ptr, s);
+        } else {
+            SqaArgs init = a;
+            init.spb = 0;
+            init.save_fields = 1;
+            init.mag_m2 = nullptr;
+            init.mag_abs = nullptr;
+            launch_sqa(ctx, pl, init);
+        }
         ChkArgs ca{};
         ca.gS = gS.ptr;
         ca.gF = gF.ptr;
@@ -2179,12 +2281,17 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
     } else {
         // random start state and its fields (no sweeps, no tail), then per schedule step the
         // phases of every sweep in chk_wave_kernel and the step tail in the two-barrier kernel
-        SqaArgs init = a;
-        init.steps = 1;
-        init.spb = 0;
-        init.tail_mode = 2;
-        init.save_fields = 1;
-        launch_sqa(ctx, pl, init);
+        if (p->kind == QA_KIND_DENSE) {
+            chk_init(p, R * P, a.k0, a.k1, gS.ptr, gF.ptr, s);
+        } else {
+            SqaArgs init = a;
+            init.steps = 1;
+            init.spb = 0;
+            init.tail_mode = 2;
+            init.save_fields = 1;
+            launch_sqa(ctx, pl, init);
+        }
+        DevBuf<double> ce(R * P);  // chain energies of a step tail
         ChkArgs ca{};
         ca.gS = gS.ptr;
         ca.gF = gF.ptr;
@@ -2198,18 +2305,12 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
                 ca.sweep_id = static_cast<uint32_t>(k * sweeps + sw);
                 chk_sweep(p, ca, R * P, static_cast<int>(P), s, chk_rows_on_demand());
             }
-            SqaArgs tl = a;
-            tl.steps = 1;
-            tl.spb = 0;
-            tl.resume = 1;
-            tl.save_fields = 0;
-            tl.tail_mode = 0;
-            tl.bs = dbs.ptr + k;
-            tl.jp = djp.ptr + k;
-            tl.join = djoin.ptr + k;
-            tl.trace_part = tpart.ptr + k * ngroups;
-            tl.step_base = static_cast<int>(k);
-            launch_sqa(ctx, pl, tl);
+            hipLaunchKernelGGL(chk_energy_kernel, dim3(static_cast<unsigned>(R * P)), dim3(64), 0, s, a.pv, gS.ptr,
+                               gF.ptr, ce.ptr);
+            hipLaunchKernelGGL(chk_tail_kernel, dim3(static_cast<unsigned>(ngroups)), dim3(256), 0, s, ce.ptr, gS.ptr,
+                               static_cast<int>(R), static_cast<int>(P), static_cast<int>(n),
+                               static_cast<int>(pl.rg), static_cast<int>(ngroups), static_cast<int>(k), tpart.ptr,
+                               be.ptr, bstep.ptr, bchain.ptr, bstate.ptr);
         }
         QA_HIP(hipGetLastError());
     }
