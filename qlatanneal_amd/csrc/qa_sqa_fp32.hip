@@ -1812,17 +1812,18 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
 
 
 namespace {
-// ---- sqa_chi production sweeps: one wavefront per chain -----------------------------------------
+// ---- sqa_chi checkerboard sweeps (pilot and production): one wavefront per chain --------------
 // In a checkerboard phase (sqa_chi_annealer.cpp:81-140) every chain of the phase's colour sweeps
 // sites 0..n-1 in order while the chains of the other colours are held fixed, so the chains of a
 // phase are independent and each one's sites depend only on its own flips. One launch per phase,
 // one wavefront per chain: the chain's fields and spins, the fixed neighbour sums and the phase's
-// Philox words sit in the wave's LDS, the J rows stream in site order with the loads issued two
-// sites ahead (register sets A and B, the site loop unrolled by two), and an accepted flip updates
-// the fields with the same fmaf per element, in the same site order, as the two-barrier kernel's
-// checkerboard sweep: the trajectory is the same bit for bit. Sparse problems stage their
-// adjacency (CSR) in the wave's LDS instead; a flip's neighbour updates run one per lane when no
-// adjacency row lists a spin twice, else one after another on lane 0, in row order.
+// Philox words sit in the wave's LDS; a dense J row is loaded for an accepted flip (kDemand, the
+// default) or streamed in site order with the loads issued two sites ahead (register sets A and
+// B, the site loop unrolled by two); the flip updates the fields with the same fmaf per element,
+// in the same site order, as the two-barrier kernel's checkerboard sweep: the trajectory is the
+// same bit for bit. Sparse problems stage their adjacency (CSR) in the wave's LDS instead; a
+// flip's neighbour updates run one per lane when no adjacency row lists a spin twice, else one
+// after another on lane 0, in row order.
 struct ChkArgs {
     const float *J4rows;  // dense: J32p, n rows of rn floats (rn = n rounded up to 4)
     const int32_t *adj_ptr, *adj_idx;  // sparse: CSR [n + 1], [nnz]
