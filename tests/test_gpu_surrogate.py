@@ -139,3 +139,22 @@ def test_surrogate_use_and_fallback(ctx):
         sur.save(path)
         back = surrogate.ChiSurrogate.load(ctx, path)
         assert np.array_equal(back.mlp.forward(X[:5]), sur.mlp.forward(X[:5]))
+
+
+def test_dataset_threads_equal_sequential_and_chunked_training(ctx, tmp_path):
+    """make_dataset on several contexts at once yields the sequential dataset exactly (same scans,
+    same order); `generate` chunks feed `train --from-data`."""
+    from qlatanneal_amd import surrogate as sg
+    specs = sg.random_specs(6, np.random.default_rng(77), sk_n=(64, 160), maxcut_n=(64, 160))
+    cfg = sg.PilotConfig()
+    X1, Y1, _ = sg.make_dataset(ctx, specs, cfg, threads=1)
+    X3, Y3, _ = sg.make_dataset(ctx, specs, cfg, threads=3)
+    assert X1.shape == X3.shape and np.array_equal(X1, X3) and np.array_equal(Y1, Y3)
+    a, b, out = tmp_path / "a.npz", tmp_path / "b.npz", tmp_path / "m.npz"
+    np.savez(a, X=X1[:3], Y=Y1[:3])
+    np.savez(b, X=X1[3:], Y=Y1[3:])
+    assert sg._main(["train", "--from-data", str(a), str(b), "--out", str(out), "--epochs", "2",
+                     "--batch", "4"]) == 0
+    model = sg.ChiSurrogate.load(ctx, str(out))
+    pred, spread = model.predict(X1)
+    assert pred.shape == Y1.shape and np.all(np.isfinite(pred)) and np.all(spread >= 0)
