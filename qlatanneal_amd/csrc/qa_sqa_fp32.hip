@@ -99,6 +99,8 @@ struct SqaArgs {
     // keep B). Integer-valued, so the double sums below are exact while they stay under 2^53.
     double *bstat;         // non-null: [steps][ngroups][2] sum of B and of B^2 over the group's samples
     double *bend;          // [steps]: B of replica 0 at the end of each step (group 0 writes)
+    int hubo_lds;          // HUBO: incidence and term arrays staged in LDS after the scratch
+    int hubo_nv;           // HUBO: entries of t_vars (= of inc_idx)
 };
 
 
@@ -122,7 +124,15 @@ struct SqaLds {
     unsigned long long *flipmask;  // two-barrier: [0]; pipelined: [b] flips, [2 + b] flips from s = +1
     int *scal;         // [0]: best chain of the step
     float *junk;       // [64][4]: target of stores a lane must not make (never read)
+    const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO arrays (LDS copies or global)
+    const float *t_coef;
 };
+
+// LDS bytes of the staged HUBO arrays (inc_ptr, inc_idx, t_off, t_vars, t_coef32).
+__host__ __device__ constexpr size_t hubo_lds_bytes(int n, int nterms, int nv) {
+    return lds_align16(static_cast<size_t>(n + 1) * 4) + 2 * lds_align16(static_cast<size_t>(nv) * 4) +
+           lds_align16(static_cast<size_t>(nterms + 1) * 4) + lds_align16(static_cast<size_t>(nterms) * 4);
+}
 
 __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) {
     const int n = a.pv.n;
@@ -157,7 +167,36 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
     L.scal = reinterpret_cast<int *>(p);
     p += 16;
     L.junk = reinterpret_cast<float *>(p);
+    p += 64 * 16;
+    L.inc_ptr = a.pv.inc_ptr;
+    L.inc_idx = a.pv.inc_idx;
+    L.t_off = a.pv.t_off;
+    L.t_vars = a.pv.t_vars;
+    L.t_coef = a.pv.t_coef32;
+    if (a.hubo_lds) {  // the plan placed them here, after everything above
+        L.inc_ptr = reinterpret_cast<const int32_t *>(p);
+        p += lds_align16(static_cast<size_t>(n + 1) * 4);
+        L.inc_idx = reinterpret_cast<const int32_t *>(p);
+        p += lds_align16(static_cast<size_t>(a.hubo_nv) * 4);
+        L.t_off = reinterpret_cast<const int32_t *>(p);
+        p += lds_align16(static_cast<size_t>(a.pv.nterms + 1) * 4);
+        L.t_vars = reinterpret_cast<const int32_t *>(p);
+        p += lds_align16(static_cast<size_t>(a.hubo_nv) * 4);
+        L.t_coef = reinterpret_cast<const float *>(p);
+    }
     return L;
+}
+
+// Copies the HUBO arrays into their LDS places (called before group_init's closing barrier).
+__device__ void hubo_stage(const SqaArgs &a, const SqaLds &L) {
+    const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, nterms = a.pv.nterms;
+    for (int e = tid; e <= n; e += nt) const_cast<int32_t *>(L.inc_ptr)[e] = a.pv.inc_ptr[e];
+    for (int e = tid; e < a.hubo_nv; e += nt) {
+        const_cast<int32_t *>(L.inc_idx)[e] = a.pv.inc_idx[e];
+        const_cast<int32_t *>(L.t_vars)[e] = a.pv.t_vars[e];
+    }
+    for (int e = tid; e <= nterms; e += nt) const_cast<int32_t *>(L.t_off)[e] = a.pv.t_off[e];
+    for (int e = tid; e < nterms; e += nt) const_cast<float *>(L.t_coef)[e] = a.pv.t_coef32[e];
 }
 
 // Random permutation of 0..n-1 for sweep sweep_id of the group whose first chain is cbase
@@ -331,6 +370,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 
     const uint32_t cbase = static_cast<uint32_t>(r0 * P);  // global chain id of chain 0
 
+    if (a.hubo_lds) hubo_stage(a, L);  // visible after group_init's closing barrier
     group_init(a, F, S, ldf, G, r0, kLds);
     double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
     unsigned long long nacc = 0;
@@ -445,12 +485,12 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 for (int q = a.pv.adj_ptr[i]; q < a.pv.adj_ptr[i + 1]; ++q)
                     fc[a.pv.adj_idx[q]] += a.pv.adj_val32[q] * ds;
             } else {
-                for (int q = a.pv.inc_ptr[i]; q < a.pv.inc_ptr[i + 1]; ++q) {
-                    const int t = a.pv.inc_idx[q];
-                    float prod = a.pv.t_coef32[t];
-                    for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k) prod *= sc[a.pv.t_vars[k]];
-                    for (int k = a.pv.t_off[t]; k < a.pv.t_off[t + 1]; ++k) {
-                        const int v = a.pv.t_vars[k];
+                for (int q = L.inc_ptr[i]; q < L.inc_ptr[i + 1]; ++q) {
+                    const int t = L.inc_idx[q];
+                    float prod = L.t_coef[t];
+                    for (int k = L.t_off[t]; k < L.t_off[t + 1]; ++k) prod *= sc[L.t_vars[k]];
+                    for (int k = L.t_off[t]; k < L.t_off[t + 1]; ++k) {
+                        const int v = L.t_vars[k];
                         if (v != i) fc[v] += 2.0f * prod * sc[v];
                     }
                 }
@@ -1204,6 +1244,7 @@ struct SqaPlan {
     int64_t rg = 1, ngroups = 0;
     bool use_lds = false;
     int ring = 0, pfd = 1, ldf = 0, ldr = 0, pipe = 0;
+    int hubo_lds = 0, hubo_nv = 0;
     size_t shmem = 0;
 };
 
@@ -1249,6 +1290,16 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     pl.ngroups = (R + pl.rg - 1) / pl.rg;
     pl.shmem = fixed(pl.rg, pl.use_lds) + lds_align16(static_cast<size_t>(pl.ring) * rn * 4);
     QA_REQUIRE(pl.shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
+    if (p->kind == QA_KIND_HUBO) {  // stage the incidence and term arrays in LDS when they fit
+        const int nv = static_cast<int>(p->t_vars.count);
+        const size_t hb = hubo_lds_bytes(static_cast<int>(n), static_cast<int>(p->nterms), nv);
+        const char *env = std::getenv("QANNEAL_SQA_HUBO_LDS");
+        if (pl.shmem + hb <= lds_cap && !(env && env[0] == '0')) {
+            pl.hubo_lds = 1;
+            pl.hubo_nv = nv;
+            pl.shmem += hb;
+        }
+    }
     pl.ldf = static_cast<int>(pl.use_lds ? rn : n);
     pl.ldr = static_cast<int>(rn);
     return pl;
@@ -1256,6 +1307,8 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
 
 void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     SqaArgs a = args;
+    a.hubo_lds = pl.hubo_lds;
+    a.hubo_nv = pl.hubo_nv;
     a.ngroups = static_cast<int>(pl.ngroups);
     a.ldf = pl.ldf;
     a.ldr = pl.ldr;
