@@ -101,6 +101,7 @@ struct SqaArgs {
     double *bend;          // [steps]: B of replica 0 at the end of each step (group 0 writes)
     int hubo_lds;          // HUBO: incidence and term arrays staged in LDS after the scratch
     int hubo_nv;           // HUBO: entries of t_vars (= of inc_idx)
+    int adj_lds;           // sparse: adjacency (CSR) staged in LDS after the scratch
 };
 
 
@@ -126,7 +127,14 @@ struct SqaLds {
     float *junk;       // [64][4]: target of stores a lane must not make (never read)
     const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO arrays (LDS copies or global)
     const float *t_coef;
+    const int32_t *adj_ptr, *adj_idx;                     // sparse CSR (LDS copy or global)
+    const float *adj_val;
 };
+
+// LDS bytes of a staged sparse adjacency (adj_ptr, adj_idx, adj_val32).
+__host__ __device__ constexpr size_t adj_lds_bytes(int n, int nnz) {
+    return lds_align16(static_cast<size_t>(n + 1) * 4) + 2 * lds_align16(static_cast<size_t>(nnz) * 4);
+}
 
 // LDS bytes of the staged HUBO arrays (inc_ptr, inc_idx, t_off, t_vars, t_coef32).
 __host__ __device__ constexpr size_t hubo_lds_bytes(int n, int nterms, int nv) {
@@ -173,6 +181,17 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
     L.t_off = a.pv.t_off;
     L.t_vars = a.pv.t_vars;
     L.t_coef = a.pv.t_coef32;
+    L.adj_ptr = a.pv.adj_ptr;
+    L.adj_idx = a.pv.adj_idx;
+    L.adj_val = a.pv.adj_val32;
+    if (a.adj_lds) {  // the plan placed them here, after everything above
+        const int nnz = 2 * a.pv.nedges;
+        L.adj_ptr = reinterpret_cast<const int32_t *>(p);
+        p += lds_align16(static_cast<size_t>(n + 1) * 4);
+        L.adj_idx = reinterpret_cast<const int32_t *>(p);
+        p += lds_align16(static_cast<size_t>(nnz) * 4);
+        L.adj_val = reinterpret_cast<const float *>(p);
+    }
     if (a.hubo_lds) {  // the plan placed them here, after everything above
         L.inc_ptr = reinterpret_cast<const int32_t *>(p);
         p += lds_align16(static_cast<size_t>(n + 1) * 4);
@@ -187,9 +206,18 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
     return L;
 }
 
-// Copies the HUBO arrays into their LDS places (called before group_init's closing barrier).
-__device__ void hubo_stage(const SqaArgs &a, const SqaLds &L) {
+// Copies the sparse adjacency or the HUBO arrays into their LDS places (called before
+// group_init's closing barrier).
+__device__ void lists_stage(const SqaArgs &a, const SqaLds &L) {
     const int tid = threadIdx.x, nt = blockDim.x, n = a.pv.n, nterms = a.pv.nterms;
+    if (a.adj_lds) {
+        for (int e = tid; e <= n; e += nt) const_cast<int32_t *>(L.adj_ptr)[e] = a.pv.adj_ptr[e];
+        for (int e = tid; e < 2 * a.pv.nedges; e += nt) {
+            const_cast<int32_t *>(L.adj_idx)[e] = a.pv.adj_idx[e];
+            const_cast<float *>(L.adj_val)[e] = a.pv.adj_val32[e];
+        }
+        return;
+    }
     for (int e = tid; e <= n; e += nt) const_cast<int32_t *>(L.inc_ptr)[e] = a.pv.inc_ptr[e];
     for (int e = tid; e < a.hubo_nv; e += nt) {
         const_cast<int32_t *>(L.inc_idx)[e] = a.pv.inc_idx[e];
@@ -370,7 +398,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 
     const uint32_t cbase = static_cast<uint32_t>(r0 * P);  // global chain id of chain 0
 
-    if (a.hubo_lds) hubo_stage(a, L);  // visible after group_init's closing barrier
+    if (a.hubo_lds || a.adj_lds) lists_stage(a, L);  // visible after group_init's closing barrier
     group_init(a, F, S, ldf, G, r0, kLds);
     double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
     unsigned long long nacc = 0;
@@ -482,8 +510,8 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             const int8_t *sc = S + static_cast<int64_t>(c) * n;
             if (a.pv.kind == QA_KIND_SPARSE) {
                 const float ds = DS[c];
-                for (int q = a.pv.adj_ptr[i]; q < a.pv.adj_ptr[i + 1]; ++q)
-                    fc[a.pv.adj_idx[q]] += a.pv.adj_val32[q] * ds;
+                for (int q = L.adj_ptr[i]; q < L.adj_ptr[i + 1]; ++q)
+                    fc[L.adj_idx[q]] += L.adj_val[q] * ds;
             } else {
                 for (int q = L.inc_ptr[i]; q < L.inc_ptr[i + 1]; ++q) {
                     const int t = L.inc_idx[q];
@@ -1244,7 +1272,7 @@ struct SqaPlan {
     int64_t rg = 1, ngroups = 0;
     bool use_lds = false;
     int ring = 0, pfd = 1, ldf = 0, ldr = 0, pipe = 0;
-    int hubo_lds = 0, hubo_nv = 0;
+    int hubo_lds = 0, hubo_nv = 0, adj_lds = 0;
     size_t shmem = 0;
 };
 
@@ -1290,14 +1318,24 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     pl.ngroups = (R + pl.rg - 1) / pl.rg;
     pl.shmem = fixed(pl.rg, pl.use_lds) + lds_align16(static_cast<size_t>(pl.ring) * rn * 4);
     QA_REQUIRE(pl.shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
-    if (p->kind == QA_KIND_HUBO) {  // stage the incidence and term arrays in LDS when they fit
+    // the coupling lists the field updates walk (sparse adjacency, HUBO incidence and terms) are
+    // staged in LDS after everything above when they fit
+    const char *lists_env = std::getenv("QANNEAL_SQA_LISTS_LDS");
+    const bool lists = !(lists_env && lists_env[0] == '0');
+    if (p->kind == QA_KIND_HUBO && lists) {
         const int nv = static_cast<int>(p->t_vars.count);
         const size_t hb = hubo_lds_bytes(static_cast<int>(n), static_cast<int>(p->nterms), nv);
-        const char *env = std::getenv("QANNEAL_SQA_HUBO_LDS");
-        if (pl.shmem + hb <= lds_cap && !(env && env[0] == '0')) {
+        if (pl.shmem + hb <= lds_cap) {
             pl.hubo_lds = 1;
             pl.hubo_nv = nv;
             pl.shmem += hb;
+        }
+    }
+    if (p->kind == QA_KIND_SPARSE && lists) {
+        const size_t sb = adj_lds_bytes(static_cast<int>(n), static_cast<int>(2 * p->nedges));
+        if (pl.shmem + sb <= lds_cap) {
+            pl.adj_lds = 1;
+            pl.shmem += sb;
         }
     }
     pl.ldf = static_cast<int>(pl.use_lds ? rn : n);
@@ -1309,6 +1347,7 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     SqaArgs a = args;
     a.hubo_lds = pl.hubo_lds;
     a.hubo_nv = pl.hubo_nv;
+    a.adj_lds = pl.adj_lds;
     a.ngroups = static_cast<int>(pl.ngroups);
     a.ldf = pl.ldf;
     a.ldr = pl.ldr;

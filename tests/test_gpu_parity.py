@@ -197,10 +197,10 @@ def test_sqa_fp32_split_step_launches_bit_identical(ctx, monkeypatch):
     monkeypatch.delenv("QANNEAL_SQA_SPLIT", raising=False)
 
 
-def test_sqa_fp32_hubo_lds_staging_bit_identical(ctx, monkeypatch):
-    """HUBO incidence and term arrays staged in LDS (the two-barrier kernel's field updates read
-    them there) must give the same runs, bit for bit, as reading them from global memory: slice,
-    worldline and cluster sweeps, and SQA-PT."""
+def test_sqa_fp32_lists_lds_staging_bit_identical(ctx, monkeypatch):
+    """HUBO incidence and term arrays and sparse adjacency staged in LDS (the two-barrier kernel's
+    field updates read them there) must give the same runs, bit for bit, as reading them from
+    global memory: slice, worldline and cluster sweeps, and SQA-PT."""
     from qlatanneal_amd import abi
     rng = np.random.default_rng(59)
     for n, P, R, spb, wl, cl in [(256, 16, 8, 2, 1, 0), (96, 7, 3, 1, 1, 1), (512, 32, 2, 1, 0, 0)]:
@@ -217,7 +217,7 @@ def test_sqa_fp32_hubo_lds_staging_bit_identical(ctx, monkeypatch):
         betas, gammas = np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4)
         res = []
         for v in ("1", "0"):
-            monkeypatch.setenv("QANNEAL_SQA_HUBO_LDS", v)
+            monkeypatch.setenv("QANNEAL_SQA_LISTS_LDS", v)
             res.append(abi.sqa_run(dp, betas, gammas, P, R, spb, wl, cl, abi.MODE_FP32, 61))
         a, b = res
         assert a["accept_rate"] == b["accept_rate"]
@@ -225,13 +225,28 @@ def test_sqa_fp32_hubo_lds_staging_bit_identical(ctx, monkeypatch):
         assert bits_equal(a["energy_trace"], b["energy_trace"])
         pt = []
         for v in ("1", "0"):
-            monkeypatch.setenv("QANNEAL_SQA_HUBO_LDS", v)
+            monkeypatch.setenv("QANNEAL_SQA_LISTS_LDS", v)
             pt.append(abi.sqapt_run(dp, betas, gammas, P, 1, 0, 3, 1, 0, ladders=2, mode=abi.MODE_FP32, seed=7))
         for key in ("final_states", "best_state"):
             assert np.array_equal(pt[0][key], pt[1][key]), key
         for key in ("final_energies", "average_energy_trace", "swap_acceptance_trace"):
             assert bits_equal(pt[0][key].ravel(), pt[1][key].ravel()), key
-    monkeypatch.delenv("QANNEAL_SQA_HUBO_LDS", raising=False)
+    # sparse adjacency staged in LDS (a graph with repeated edges: updates stay in row order)
+    for n, P, R, spb, wl, cl in [(300, 16, 4, 2, 1, 1), (2048, 32, 2, 1, 0, 0)]:
+        print("sparse staging case", n, P, R, spb, wl, cl, flush=True)
+        el = [(i, (i + 1) % n, float(rng.normal())) for i in range(n)]
+        el += [(i, (i + 37) % n, float(rng.normal())) for i in range(0, n, 3)]
+        el += el[::7]
+        dp = abi.DeviceProblem.sparse(ctx, rng.normal(0, 0.3, n), el, 0.0)
+        res = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_SQA_LISTS_LDS", v)
+            res.append(abi.sqa_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, R, spb, wl, cl,
+                                   abi.MODE_FP32, 67))
+        a, b = res
+        assert a["accept_rate"] == b["accept_rate"]
+        assert np.array_equal(a["best_state"], b["best_state"]) and bits_equal(a["energy_trace"], b["energy_trace"])
+    monkeypatch.delenv("QANNEAL_SQA_LISTS_LDS", raising=False)
 
 
 def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
