@@ -383,7 +383,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every run here has a single MI355X, and the sweep shards by "
-              "independent replicas/instances (see DESIGN.md); use --gpus 1", file=sys.stderr)
+              "independent replicas and instances, one GPU each; use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "c1":
         bench_c1(args)
