@@ -283,6 +283,21 @@ class Oracle:
             _i64(continuous_time_slices), C.c_uint64(seed), _b(best), C.byref(be), _d(tr)))
         return dict(best_state=best, best_energy=be.value, energy_trace=tr)
 
+    def sqa_run_repeat(self, p: Problem, runs: int, betas, gammas, trotter_slices: int, replicas: int,
+                       sweeps_per_beta: int, worldline_sweeps: int, cluster_sweeps: int = 0,
+                       continuous_time_slices: int = 0, seed: int = 0):
+        """set_seed(seed), then `runs` consecutive run() calls on one annealer; the last run's
+        outputs."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        g = np.ascontiguousarray(gammas, dtype=np.float64)
+        best, be, tr = np.zeros(p.n, np.int8), C.c_double(), np.zeros(b.shape[0])
+        st, keep = p._struct()
+        self._chk(self.lib.orc_sqa_run_repeat(
+            C.byref(st), _i64(runs), _i64(b.shape[0]), _d(b), _d(g), _i64(trotter_slices), _i64(replicas),
+            _i64(sweeps_per_beta), _i64(worldline_sweeps), _i64(cluster_sweeps),
+            _i64(continuous_time_slices), C.c_uint64(seed), _b(best), C.byref(be), _d(tr)))
+        return dict(best_state=best, best_energy=be.value, energy_trace=tr)
+
     def sqapt_run(self, p: Problem, betas, gammas, trotter_slices: int, sweeps_per_step: int,
                   worldline_sweeps: int, steps: int, swap_interval: int = 1, cluster_sweeps: int = 0,
                   continuous_time_slices: int = 0, seed: int = 0):

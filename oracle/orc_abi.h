@@ -74,6 +74,13 @@ int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const 
                 int64_t worldline_sweeps, int64_t cluster_sweeps, int64_t continuous_time_slices,
                 uint64_t seed, int8_t *best_state, double *best_energy, double *trace);
 
+/* set_seed(seed), then `runs` consecutive SQAAnnealer::run calls on one annealer (the engine
+ * continues from run to run); best_state/best_energy/trace of the last run. */
+int orc_sqa_run_repeat(const orc_problem *p, int64_t runs, int64_t steps, const double *betas,
+                       const double *gammas, int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                       int64_t worldline_sweeps, int64_t cluster_sweeps, int64_t continuous_time_slices,
+                       uint64_t seed, int8_t *best_state, double *best_energy, double *trace);
+
 /* SQAParallelTemperingAnnealer::run (sqa_parallel_tempering.cpp:81-346) after set_seed(seed).
  * final_states [rungs*n], final_energies [rungs], avg_trace/swap_trace [steps]. */
 int orc_sqapt_run(const orc_problem *p, int64_t rungs, const double *betas, const double *gammas,

@@ -5,6 +5,7 @@
 #include "orc_abi.h"
 
 #include <cstring>
+#include <stdexcept>
 #include <exception>
 #include <memory>
 #include <string>
@@ -304,6 +305,29 @@ int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const 
         put_state(res.best_state, best_state);
         *best_energy = res.best_energy;
         std::memcpy(trace, res.energy_trace.data(), res.energy_trace.size() * sizeof(double));
+    });
+}
+
+int orc_sqa_run_repeat(const orc_problem *p, int64_t runs, int64_t steps, const double *betas,
+                       const double *gammas, int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                       int64_t worldline_sweeps, int64_t cluster_sweeps, int64_t continuous_time_slices,
+                       uint64_t seed, int8_t *best_state, double *best_energy, double *trace) {
+    return guard([&] {
+        if (runs <= 0) throw std::invalid_argument("runs must be > 0.");
+        auto ham = build(p);
+        auto sched = qanneal::SQASchedule::from_vectors(std::vector<double>(betas, betas + steps),
+                                                        std::vector<double>(gammas, gammas + steps));
+        qanneal::SQAAnnealer ann(*ham, sched, static_cast<std::size_t>(trotter_slices),
+                                 static_cast<std::size_t>(replicas));
+        ann.set_seed(seed);
+        for (int64_t k = 0; k < runs; ++k) {
+            auto res = ann.run(static_cast<std::size_t>(sweeps_per_beta), static_cast<std::size_t>(worldline_sweeps),
+                               static_cast<std::size_t>(cluster_sweeps),
+                               static_cast<std::size_t>(continuous_time_slices));
+            put_state(res.best_state, best_state);
+            *best_energy = res.best_energy;
+            std::memcpy(trace, res.energy_trace.data(), res.energy_trace.size() * sizeof(double));
+        }
     });
 }
 

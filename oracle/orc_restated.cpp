@@ -721,11 +721,14 @@ int orc_pt_run(const orc_problem *p, int64_t replicas, const double *betas, int6
 }
 
 // sqa_annealer.cpp:87-477 (the cached-field, observer-free path)
-int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
-                int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
-                int64_t worldline_sweeps, int64_t cluster_sweeps, int64_t continuous_time_slices,
-                uint64_t seed, int8_t *best_state, double *best_energy, double *trace) {
-    return guard([&] {
+namespace {
+// One SQAAnnealer::run on the annealer's engine `rng` (sqa_annealer.cpp:87-477): the engine is
+// left where the run leaves it, so a second call continues the stream as a second run() does.
+void sqa_run_on(const orc_problem *p, Rng &rng, int64_t steps, const double *betas, const double *gammas,
+                int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta, int64_t worldline_sweeps,
+                int64_t cluster_sweeps, int64_t continuous_time_slices, int8_t *best_state, double *best_energy,
+                double *trace) {
+    {
         if (steps <= 0) throw std::invalid_argument("SQA schedule must contain betas.");
         if (trotter_slices <= 0) throw std::invalid_argument("trotter_slices must be > 0.");
         if (replicas <= 0) throw std::invalid_argument("replicas must be > 0.");
@@ -737,7 +740,6 @@ int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const 
         const std::size_t P = (continuous_time_slices > trotter_slices)
                                   ? static_cast<std::size_t>(continuous_time_slices)
                                   : static_cast<std::size_t>(trotter_slices);
-        Rng rng(seed);
         std::vector<int8_t> spins(R * P * n);
         random_spins(spins, rng);  // :100, before the replica seeds
         std::vector<Rng> rrng(R);
@@ -778,6 +780,32 @@ int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const 
         }
         std::memcpy(best_state, best.data(), n);
         *best_energy = best_e;
+    }
+}
+}  // namespace
+
+int orc_sqa_run(const orc_problem *p, int64_t steps, const double *betas, const double *gammas,
+                int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                int64_t worldline_sweeps, int64_t cluster_sweeps, int64_t continuous_time_slices,
+                uint64_t seed, int8_t *best_state, double *best_energy, double *trace) {
+    return guard([&] {
+        Rng rng(seed);
+        sqa_run_on(p, rng, steps, betas, gammas, trotter_slices, replicas, sweeps_per_beta, worldline_sweeps,
+                   cluster_sweeps, continuous_time_slices, best_state, best_energy, trace);
+    });
+}
+
+// set_seed(seed), then `runs` consecutive run() calls on the same annealer; outputs of the last.
+int orc_sqa_run_repeat(const orc_problem *p, int64_t runs, int64_t steps, const double *betas,
+                       const double *gammas, int64_t trotter_slices, int64_t replicas, int64_t sweeps_per_beta,
+                       int64_t worldline_sweeps, int64_t cluster_sweeps, int64_t continuous_time_slices,
+                       uint64_t seed, int8_t *best_state, double *best_energy, double *trace) {
+    return guard([&] {
+        if (runs <= 0) throw std::invalid_argument("runs must be > 0.");
+        Rng rng(seed);
+        for (int64_t k = 0; k < runs; ++k)
+            sqa_run_on(p, rng, steps, betas, gammas, trotter_slices, replicas, sweeps_per_beta, worldline_sweeps,
+                       cluster_sweeps, continuous_time_slices, best_state, best_energy, trace);
     });
 }
 

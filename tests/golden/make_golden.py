@@ -2,7 +2,8 @@
 
 Run in the container that holds the reference tree, after `make -C oracle`:
 
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py              (all fixtures but one)
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py sqa_repeat   (sqa_repeat.json)
 
 Inputs are generated here from fixed numpy seeds and stored in the fixtures themselves, so the
 tests never depend on a generator. Outputs come from oracle/_ref/libqanneal_ref.so, the
@@ -281,6 +282,23 @@ def main():
     dump("observers.json", {"sa": sa_cases, "sqa": sqa_cases})
 
 
+def main_repeat():
+    """sqa_repeat.json: consecutive SQAAnnealer::run calls without reseeding (the engine
+    continues from run to run); outputs of the last run."""
+    ref = Oracle(REF_SO)
+    ref.set_threads(1)
+    rng = np.random.default_rng(20261204)
+    out = []
+    for (n, P, R, spb, wl, cl, runs, seed) in [(16, 4, 3, 2, 1, 1, 2, 8), (20, 8, 2, 1, 1, 0, 3, 21)]:
+        p = sk(rng, n, 0.2)
+        betas, gammas = np.linspace(0.3, 3, 12), np.geomspace(3, 0.02, 12)
+        res = ref.sqa_run_repeat(p, runs, betas, gammas, P, R, spb, wl, cl, 0, seed)
+        out.append({"problem": problem_json(p), "runs": runs, "betas": enc(betas), "gammas": enc(gammas),
+                    "trotter_slices": P, "replicas": R, "sweeps_per_beta": spb, "worldline_sweeps": wl,
+                    "cluster_sweeps": cl, "seed": seed, "out": out_json(res)})
+    dump("sqa_repeat.json", {"cases": out})
+
+
 def dump(name, obj):
     obj["source"] = ("reference sources compiled by oracle/Makefile (Release flags), "
                      "OMP_NUM_THREADS=1; written by tests/golden/make_golden.py")
@@ -290,4 +308,7 @@ def dump(name, obj):
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["sqa_repeat"]:
+        main_repeat()
+    else:
+        main()

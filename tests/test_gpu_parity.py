@@ -527,10 +527,13 @@ def test_sqa_chi_fp32(ctx):
     orc.set_threads(8)
     kw = dict(beta=2.0, gamma_start=3.0, gamma_end=0.05, num_steps=20, sweeps_per_step=2, scan_points=8,
               scan_sweeps=6, scan_burn=2)
-    gpu = np.array([abi.sqa_chi_run(dp, 4, 2, seed=100 + k, **kw)["best_energy"] for k in range(256)])
+    runs = [abi.sqa_chi_run(dp, 4, 2, seed=100 + k, **kw) for k in range(256)]
+    gpu = np.array([r["best_energy"] for r in runs])
     cpu = np.array([orc.sqa_chi_run(prob, 4, 2, seed=700 + k, **kw)["best_energy"] for k in range(256)])
-    for e, s in [(gpu[0], None)]:
-        assert np.isfinite(e)
+    # every returned sample's energy is the FP64 energy of its state
+    states = np.stack([r["best_state"] for r in runs])
+    assert np.all(np.isfinite(gpu))
+    assert np.array_equal(gpu, dp.energy(states))
     p = ks_2samp(gpu, cpu).pvalue
     print("chi KS p", p, gpu.mean(), cpu.mean(), flush=True)
     assert p > 0.01
@@ -562,3 +565,87 @@ def test_sqapt_fp32_c3_shape_with_refresh(ctx):
     assert np.all((r["swap_acceptance_trace"] >= 0) & (r["swap_acceptance_trace"] <= 1))
     before, after = r["field_drift"]
     assert np.isfinite(before) and after <= 5e-4
+
+
+def _maxcut3(rng, n):
+    """Unit-weight 3-regular-style graph: edges from random perfect matchings, 3n/2 edges."""
+    edges = set()
+    while len(edges) < 3 * n // 2:
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            u, v = sorted((int(perm[i]), int(perm[i + 1])))
+            if len(edges) < 3 * n // 2:
+                edges.add((u, v))
+    return [(u, v, 1.0) for u, v in sorted(edges)]
+
+
+def test_sqa_fp32_full_c2_shape_exact_and_distribution(ctx):
+    """The headline SQA shape — dense SK N=512, P=32, 256 replicas, beta=32, linear Gamma — for a
+    20-sweep schedule: the returned best energy is the FP64 energy of the returned state, the
+    trace is finite and the acceptance is a rate. Same P on a smaller N (64): best energies over
+    256 independent reads agree with the oracle's (two-sample KS)."""
+    from scipy.stats import ks_2samp
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    rng = np.random.default_rng(512)
+    n, P, R, steps = 512, 32, 256, 20
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), _sk(rng, n))
+    betas, gammas = np.full(steps, 32.0), np.linspace(3.0, 1e-3, steps)
+    for seed in (1, 2):
+        r = abi.sqa_run(dp, betas, gammas, P, R, 1, 0, 0, abi.MODE_FP32, seed)
+        print("C2 shape seed", seed, "best", r["best_energy"], "kernel ms", r["kernel_ms"], flush=True)
+        assert r["best_energy"] == dp.energy(r["best_state"])[0]
+        assert np.all(np.isfinite(r["energy_trace"])) and r["energy_trace"][-1] < r["energy_trace"][0]
+        assert 0.0 < r["accept_rate"] < 1.0
+    n = 64
+    J = _sk(rng, n)
+    dps = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
+    orc = Oracle(ORACLE_SO)
+    orc.set_threads(8)
+    gpu = np.array([abi.sqa_run(dps, betas, gammas, P, 1, 1, 0, 0, abi.MODE_FP32, 100 + k)["best_energy"]
+                    for k in range(256)])
+    cpu = np.array([orc.sqa_run(Problem.dense(np.zeros(n), J), betas, gammas, P, 1, 1, 0, 0, 0, 900 + k)["best_energy"]
+                    for k in range(256)])
+    p = ks_2samp(gpu, cpu).pvalue
+    print("C2 P=32 N=64 KS p", p, gpu.mean(), cpu.mean(), flush=True)
+    assert p > 0.01
+
+
+def test_sqa_chi_fp32_full_c5_shapes_exact_and_distribution(ctx):
+    """sqa_chi at the C5 shapes — dense SK N=2048 and 3-regular unit MAX-CUT N=4096 (sparse, the
+    adjacency staged in LDS), P=32, R=4, 64 scan points over Gamma 3.0 -> 1e-3 — with a short pilot
+    and production: the best energy is the FP64 energy of the best state, the Gamma schedule runs
+    monotonically from gamma_start to gamma_end, the scan is nonnegative. Same P, R and scan size
+    on small instances of both families: best energies over 128 runs agree with the oracle (KS)."""
+    from scipy.stats import ks_2samp
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    rng = np.random.default_rng(2048)
+    P, R = 32, 4
+    kw = dict(num_steps=10, sweeps_per_step=2, scan_points=64, scan_sweeps=2, scan_burn=1)
+    sk_J = _sk(rng, 2048)
+    cases = [("sk2048", abi.DeviceProblem.dense(ctx, np.zeros(2048), sk_J)),
+             ("maxcut4096", abi.DeviceProblem.sparse(ctx, np.zeros(4096), _maxcut3(rng, 4096)))]
+    for name, dp in cases:
+        r = abi.sqa_chi_run(dp, P, R, 4.0, 3.0, 1e-3, seed=5, **kw)
+        print("C5 shape", name, "best", r["best_energy"], "kernel ms", r["kernel_ms"], flush=True)
+        assert r["best_energy"] == dp.energy(r["best_state"])[0], name
+        g = r["gamma_schedule"]
+        assert np.all(np.diff(g) <= 1e-12) and abs(g[0] - 3.0) < 1e-9 and abs(g[-1] - 1e-3) < 1e-9, name
+        assert np.all(r["scan_chi_B"] >= 0) and len(r["scan_chi_B"]) == 64, name
+    orc = Oracle(ORACLE_SO)
+    orc.set_threads(8)
+    for name, n in (("sk", 32), ("maxcut", 64)):
+        if name == "sk":
+            J = _sk(rng, n)
+            dp, prob = abi.DeviceProblem.dense(ctx, np.zeros(n), J), Problem.dense(np.zeros(n), J)
+        else:
+            e = _maxcut3(rng, n)
+            dp, prob = abi.DeviceProblem.sparse(ctx, np.zeros(n), e), Problem.sparse(np.zeros(n), e)
+        gpu = np.array([abi.sqa_chi_run(dp, P, R, 4.0, 3.0, 1e-3, seed=300 + k, **kw)["best_energy"]
+                        for k in range(128)])
+        cpu = np.array([orc.sqa_chi_run(prob, P, R, 4.0, 3.0, 1e-3, seed=800 + k, **kw)["best_energy"]
+                        for k in range(128)])
+        p = ks_2samp(gpu, cpu).pvalue
+        print("C5", name, n, "KS p", p, gpu.mean(), cpu.mean(), flush=True)
+        assert p > 0.01, name

@@ -209,6 +209,24 @@ def test_golden_sqa(orc):
             assert bits_equal(res[k], exp[k]), k
 
 
+def test_golden_sqa_repeat(orc):
+    """Consecutive runs without reseeding continue the annealer's engine (sqa_annealer.cpp:100,
+    108-111 draw from the member engine on every run): the oracle's last run equals the
+    reference build's, a single repeat equals sqa_run, and the second run is not the first."""
+    for c in load("sqa_repeat.json")["cases"]:
+        p = problem(c["problem"])
+        args = (dec(c["betas"]), dec(c["gammas"]), c["trotter_slices"], c["replicas"], c["sweeps_per_beta"],
+                c["worldline_sweeps"], c["cluster_sweeps"], 0, c["seed"])
+        res = orc.sqa_run_repeat(p, c["runs"], *args)
+        exp = out(c["out"])
+        for k in exp:
+            assert bits_equal(res[k], exp[k]), k
+        one = orc.sqa_run(p, *args)
+        first = orc.sqa_run_repeat(p, 1, *args)
+        assert bits_equal(one["energy_trace"], first["energy_trace"])
+        assert not bits_equal(one["energy_trace"], res["energy_trace"])
+
+
 def test_golden_sqapt(orc):
     for c in load("sqapt.json")["cases"]:
         res = orc.sqapt_run(problem(c["problem"]), dec(c["betas"]), dec(c["gammas"]), c["trotter_slices"],
