@@ -164,6 +164,17 @@ typedef struct qa_sqa_result {
 qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
                      qa_sqa_result *res);
 
+/* FP32 SQA run that allocates nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid hipMalloc). Same run and
+ * same results as qa_sqa_run in QA_MODE_FP32 (bit for bit) for refresh_interval = 0 and no
+ * callbacks; every buffer beyond the problem and the caller's result arrays lives in a static
+ * per-device scratch array of 64 MiB (spins R*P*n bytes, fields R*P*n*4 bytes when they do not
+ * fit LDS, per-group best states and trace partials, the schedule). A run that needs more is
+ * refused with QA_EINVAL before anything is launched. Calls on one device share that array and
+ * are serialized by the library; each call overwrites the previous call's contents. */
+qa_status qa_sqa_run_noalloc(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
+                             qa_sqa_result *res);
+
 /* ---- Batched SQA: many problems of uneven size in one persistent launch ---------------------
  * No reference counterpart (the reference runs instances one by one, solve() in solver.py:455-789).
  * Instance k runs what qa_sqa_run runs in FP32 mode with replicas = reads[k] and seed = seeds[k],

@@ -119,6 +119,7 @@ def lib() -> C.CDLL:
         L.qa_last_error.restype = C.c_char_p
         for name in ("qa_ctx_create", "qa_ctx_destroy", "qa_problem_dense", "qa_problem_sparse",
                      "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_local_fields", "qa_sa_run", "qa_sqa_run",
+                     "qa_sqa_run_noalloc",
                      "qa_sqapt_run", "qa_sqa_chi_run", "qa_sqa_batch_run", "qa_chi_pilot", "qa_chi_features",
                      "qa_mlp_create", "qa_mlp_destroy", "qa_mlp_info", "qa_mlp_get_state", "qa_mlp_set_state",
                      "qa_mlp_forward", "qa_mlp_loss_grad", "qa_mlp_train"):
@@ -286,6 +287,23 @@ def sqa_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: i
     check(lib().qa_sqa_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, kernel_ms=float(ms[0]),
                 accept_rate=float(ar[0]), field_drift=dr)
+
+
+def sqa_run_noalloc(prob: DeviceProblem, betas, gammas, trotter_slices: int, replicas: int, sweeps_per_beta: int,
+                    worldline_sweeps: int = 0, cluster_sweeps: int = 0, seed: int = 0):
+    """FP32 SQA run through qa_sqa_run_noalloc: no device allocation (a static per-device scratch
+    array of 64 MiB holds the run's buffers); results equal sqa_run(..., mode=MODE_FP32)."""
+    betas = np.ascontiguousarray(betas, np.float64)
+    gammas = np.ascontiguousarray(gammas, np.float64)
+    steps, n = betas.shape[0], prob.n
+    p = SqaParams(steps, _d(betas), _d(gammas), trotter_slices, replicas, sweeps_per_beta,
+                  worldline_sweeps, cluster_sweeps, MODE_FP32, seed & (2**64 - 1))
+    bs, be, tr, ms, ar = np.zeros(n, np.int8), np.zeros(1), np.zeros(steps), np.zeros(1), np.zeros(1)
+    dr = np.zeros(2)
+    r = SqaResult(_b(bs), _d(be), _d(tr), _d(ms), _d(ar), _d(dr))
+    check(lib().qa_sqa_run_noalloc(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
+    return dict(best_state=bs, best_energy=float(be[0]), energy_trace=tr, kernel_ms=float(ms[0]),
+                accept_rate=float(ar[0]))
 
 
 def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_per_step: int,

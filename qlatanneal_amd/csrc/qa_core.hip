@@ -759,6 +759,26 @@ qa_status qa_sqa_run(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm,
     });
 }
 
+qa_status qa_sqa_run_noalloc(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    return guarded([&] {
+        QA_REQUIRE(ctx && p && prm && res, "qa_sqa_run_noalloc: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_sqa_run_noalloc: problem belongs to another context");
+        QA_REQUIRE(prm->steps > 0 && prm->betas && prm->gammas, "SQA schedule must contain betas.");
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->replicas > 0, "replicas must be > 0.");
+        QA_REQUIRE(prm->sweeps_per_beta >= 0 && prm->worldline_sweeps >= 0 && prm->cluster_sweeps >= 0,
+                   "sweep counts must be >= 0.");
+        QA_REQUIRE(prm->sweeps_per_beta > 0 || prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0,
+                   "At least one of sweeps_per_beta, worldline_sweeps, or cluster_sweeps must be > 0.");
+        QA_REQUIRE(res->best_state && res->best_energy && res->energy_trace,
+                   "qa_sqa_run_noalloc: NULL result array");
+        check_finite(prm->betas, prm->steps, "betas");
+        check_finite(prm->gammas, prm->steps, "gammas");
+        QA_HIP(hipSetDevice(ctx->device));
+        sqa_fp32_noalloc(ctx, p, prm, res);
+    });
+}
+
 qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res) {
     return guarded([&] {
         RunScope scope(ctx);

@@ -279,6 +279,7 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
 void sqa_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
 void sqa_trace(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
 void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
+void sqa_fp32_noalloc(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res);
 void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
 void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
 void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);

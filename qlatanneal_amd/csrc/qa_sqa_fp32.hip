@@ -24,6 +24,8 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -1577,6 +1579,213 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
         const double att = static_cast<double>(prm->steps) * prm->sweeps_per_beta * R * P * n;
         *res->accept_rate = att > 0 ? static_cast<double>(tot) / att : 0.0;
     }
+}
+
+
+// ---- FP32 SQA run without any allocation (scratch in a static device array) ------------------
+// The device memory the run needs beyond its problem and results -- schedule scalars, spins
+// (and fields when they do not fit LDS), per-group best states, trace partials, counters, the
+// reduced trace and the best sample -- is carved from one file-scope array. Each device holds its
+// own copy; its address comes from hipGetSymbolAddress once per device and is used from then on
+// (the array's name in host code is not a device address). Calls on one device are serialized by
+// a host mutex; a call's contents are overwritten by the next call.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t{64} << 20;  // 64 MiB per device
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mutex[kMaxNoAllocDevices];
+unsigned char *g_noalloc_ptr[kMaxNoAllocDevices] = {};
+
+// Set-up: the device address of this device's scratch (the device is current).
+unsigned char *noalloc_scratch(int device) {
+    QA_REQUIRE(device >= 0 && device < kMaxNoAllocDevices, "qa_sqa_run_noalloc: device index out of range");
+    if (!g_noalloc_ptr[device]) {
+        void *ptr = nullptr;
+        QA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        QA_REQUIRE(ptr != nullptr, "qa_sqa_run_noalloc: no device address for the scratch array");
+        g_noalloc_ptr[device] = static_cast<unsigned char *>(ptr);
+    }
+    return g_noalloc_ptr[device];
+}
+
+// Per-group best energies start at +inf and best chains at -1 (no hipMemsetAsync).
+__global__ void noalloc_init_kernel(double *best_e, int *best_chain, int ngroups) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups) {
+        best_e[g] = __longlong_as_double(0x7FF0000000000000ll);
+        best_chain[g] = -1;
+    }
+}
+
+// Host-side reductions of sqa_fp32, on the device in the same order: the trace as the sequential
+// sum over groups divided by R*P (one thread per step); the best group as the first strict
+// minimum in (energy, step) over groups in order, its state copied to best_out; accepted moves
+// summed. status[0] = best group or -1.
+__global__ void noalloc_finish_kernel(const double *tpart, int steps, int ngroups, double rp, double *trace,
+                                      const double *best_e, const int *best_step, const int *best_chain,
+                                      const int8_t *bstate, int n, int8_t *best_out,
+                                      const unsigned long long *acc, unsigned long long *acc_sum, int *status) {
+    const int tid = threadIdx.x;
+    for (int k = tid; k < steps; k += blockDim.x) {
+        double sum = 0.0;
+        for (int gi = 0; gi < ngroups; ++gi) sum += tpart[static_cast<int64_t>(k) * ngroups + gi];
+        trace[k] = sum / rp;
+    }
+    __shared__ int s_bg;
+    if (tid == 0) {
+        int bg = -1;
+        unsigned long long tot = 0;
+        for (int gi = 0; gi < ngroups; ++gi) {
+            tot += acc[gi];
+            if (best_chain[gi] < 0) continue;
+            if (bg < 0 || best_e[gi] < best_e[bg] || (best_e[gi] == best_e[bg] && best_step[gi] < best_step[bg]))
+                bg = gi;
+        }
+        *acc_sum = tot;
+        status[0] = bg;
+        s_bg = bg;
+    }
+    __syncthreads();
+    const int bg = s_bg;
+    if (bg >= 0)
+        for (int i = tid; i < n; i += blockDim.x) best_out[i] = bstate[static_cast<int64_t>(bg) * n + i];
+}
+
+// Bump allocator over the scratch array (256-byte aligned pieces).
+struct Carve {
+    unsigned char *base;
+    size_t used = 0;
+    template <class T>
+    T *take(size_t count) {
+        T *p = base ? reinterpret_cast<T *>(base + used) : nullptr;
+        used += (count * sizeof(T) + 255) / 256 * 256;
+        return p;
+    }
+};
+
+}  // namespace
+
+void sqa_fp32_noalloc(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa_result *res) {
+    QA_REQUIRE(prm->mode == QA_MODE_FP32, "qa_sqa_run_noalloc: FP32 mode only");
+    QA_REQUIRE(prm->refresh_interval == 0, "qa_sqa_run_noalloc: refresh_interval is not supported");
+    QA_REQUIRE(!prm->on_step && !prm->on_sweep, "qa_sqa_run_noalloc: observer callbacks are not supported");
+    const int64_t n = p->n, R = prm->replicas, P = prm->trotter_slices, steps = prm->steps;
+    const SqaPlan pl = plan_sqa(ctx, p, R, P);
+    const int64_t ngroups = pl.ngroups;
+    // the same pieces sqa_fp32 allocates, sized first so that an oversized run is refused before
+    // anything is launched
+    auto need = [&](unsigned char *base) {
+        Carve c{base};
+        c.take<float>(steps);
+        c.take<float>(steps);
+        c.take<float>(steps);
+        c.take<int8_t>(static_cast<size_t>(R * P * n));
+        c.take<float>(pl.use_lds ? 1 : static_cast<size_t>(R * P * n));
+        c.take<int8_t>(static_cast<size_t>(ngroups * n));
+        c.take<double>(static_cast<size_t>(steps * ngroups));
+        c.take<double>(ngroups);
+        c.take<int>(ngroups);
+        c.take<int>(ngroups);
+        c.take<unsigned long long>(ngroups);
+        c.take<double>(steps);
+        c.take<int8_t>(n);
+        c.take<double>(1);
+        c.take<unsigned long long>(1);
+        c.take<int>(1);
+        return c.used;
+    };
+    const size_t bytes = need(nullptr);
+    if (bytes > kNoAllocScratchBytes)
+        throw Error{QA_EINVAL, "qa_sqa_run_noalloc: the run needs " + std::to_string(bytes) +
+                                   " bytes of scratch, more than the " + std::to_string(kNoAllocScratchBytes) +
+                                   " bytes this entry point holds per device (use qa_sqa_run)"};
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex[ctx->device]);
+    Carve c{noalloc_scratch(ctx->device)};
+    float *dbs = c.take<float>(steps), *djp = c.take<float>(steps), *djoin = c.take<float>(steps);
+    int8_t *gS = c.take<int8_t>(static_cast<size_t>(R * P * n));
+    float *gF = c.take<float>(pl.use_lds ? 1 : static_cast<size_t>(R * P * n));
+    int8_t *bstate = c.take<int8_t>(static_cast<size_t>(ngroups * n));
+    double *tpart = c.take<double>(static_cast<size_t>(steps * ngroups));
+    double *be = c.take<double>(ngroups);
+    int *bstep = c.take<int>(ngroups), *bchain = c.take<int>(ngroups);
+    unsigned long long *accd = c.take<unsigned long long>(ngroups);
+    double *dtrace = c.take<double>(steps);
+    int8_t *best_out = c.take<int8_t>(n);
+    double *dex = c.take<double>(1);
+    unsigned long long *dacc = c.take<unsigned long long>(1);
+    int *dstatus = c.take<int>(1);
+    hipStream_t s = ctx->stream;
+
+    // schedule scalars as sqa_fp32 computes them, staged through a small stack buffer
+    constexpr int64_t kStage = 256;
+    float stage[3][kStage];
+    for (int64_t k0 = 0; k0 < steps; k0 += kStage) {
+        const int64_t len = std::min(kStage, steps - k0);
+        for (int64_t q = 0; q < len; ++q) {
+            const int64_t k = k0 + q;
+            const double jp = trotter_coupling(prm->betas[k], prm->gammas[k], P);
+            stage[0][q] = static_cast<float>(prm->betas[k] / static_cast<double>(P));
+            stage[1][q] = static_cast<float>(jp);
+            stage[2][q] = static_cast<float>(1.0 - std::exp(-2.0 * jp));
+        }
+        QA_HIP(hipMemcpyAsync(dbs + k0, stage[0], len * sizeof(float), hipMemcpyHostToDevice, s));
+        QA_HIP(hipMemcpyAsync(djp + k0, stage[1], len * sizeof(float), hipMemcpyHostToDevice, s));
+        QA_HIP(hipMemcpyAsync(djoin + k0, stage[2], len * sizeof(float), hipMemcpyHostToDevice, s));
+        QA_HIP(hipStreamSynchronize(s));  // the stack buffer is reused
+    }
+    hipLaunchKernelGGL(noalloc_init_kernel, dim3(static_cast<unsigned>((ngroups + 255) / 256)), dim3(256), 0, s, be,
+                       bchain, static_cast<int>(ngroups));
+    QA_HIP(hipGetLastError());
+
+    SqaArgs a{};
+    a.pv = view_of(p);
+    a.P = static_cast<int>(P);
+    a.R = static_cast<int>(R);
+    a.rg = static_cast<int>(pl.rg);
+    a.steps = static_cast<int>(steps);
+    a.spb = static_cast<int>(prm->sweeps_per_beta);
+    a.wl = static_cast<int>(prm->worldline_sweeps);
+    a.cl = static_cast<int>(prm->cluster_sweeps);
+    a.bs = dbs;
+    a.jp = djp;
+    a.join = djoin;
+    a.k0 = static_cast<uint32_t>(prm->seed);
+    a.k1 = static_cast<uint32_t>(prm->seed >> 32);
+    a.gS = gS;
+    a.gF = gF;
+    a.trace_part = tpart;
+    a.best_e = be;
+    a.best_step = bstep;
+    a.best_chain = bchain;
+    a.best_state = bstate;
+    a.accepted = accd;
+    QA_HIP(hipEventRecord(ctx->ev0, s));
+    launch_sqa(ctx, pl, a);
+    QA_HIP(hipEventRecord(ctx->ev1, s));
+    hipLaunchKernelGGL(noalloc_finish_kernel, dim3(1), dim3(256), 0, s, tpart, static_cast<int>(steps),
+                       static_cast<int>(ngroups), static_cast<double>(R * P), dtrace, be, bstep, bchain, bstate,
+                       static_cast<int>(n), best_out, accd, dacc, dstatus);
+    QA_HIP(hipGetLastError());
+    device_energies(p, best_out, 1, dex, s);
+    int status = -1;
+    unsigned long long tot = 0;
+    QA_HIP(hipMemcpyAsync(&status, dstatus, sizeof(int), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipMemcpyAsync(&tot, dacc, sizeof(tot), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipMemcpyAsync(res->energy_trace, dtrace, steps * sizeof(double), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipMemcpyAsync(res->best_state, best_out, n, hipMemcpyDeviceToHost, s));
+    QA_HIP(hipMemcpyAsync(res->best_energy, dex, sizeof(double), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipStreamSynchronize(s));
+    QA_REQUIRE(status >= 0, "no finite slice energy was produced");
+    float ms = 0;
+    QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (res->kernel_ms) *res->kernel_ms = ms;
+    if (res->accept_rate) {
+        const double att = static_cast<double>(steps) * prm->sweeps_per_beta * R * P * n;
+        *res->accept_rate = att > 0 ? static_cast<double>(tot) / att : 0.0;
+    }
+    if (res->field_drift) res->field_drift[0] = res->field_drift[1] = std::numeric_limits<double>::quiet_NaN();
 }
 
 
