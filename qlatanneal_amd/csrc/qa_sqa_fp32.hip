@@ -681,6 +681,74 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             ring_preload();
             Philox4 rnd{0, 0, 0, 0};
             const int nsteps = n + P - 1;
+            if constexpr (!kLds && NJ > 0) {
+                // State in global memory: each chain loads the inputs of its next decision one
+                // lockstep step ahead (field, own and neighbour spins, and J[i_k][i_k+1] for its own
+                // flip), so a decision reads registers only and the loads' latency overlaps the
+                // field updates. The prefetched field lacks the chain's own flip at position k; the
+                // decision adds it with the fmaf the update applies to that element. The spin of the
+                // previous slice at the next site is decided in this very step (by lane c - 1), so it
+                // comes from that lane's decision through a shuffle instead of memory.
+                float pf_f = 0.0f, pf_j = 0.0f, ds_prev = 0.0f;
+                int pf_s = 0, pf_sp = 0, pf_sn = 0, s_left = 0;
+                bool fl_prev = false;
+                for (int tau = 0; tau < nsteps; ++tau) {
+                    if (tid < kMaxChains) {
+                        const int k = tau - t_me;
+                        const bool active = decider && k >= 0 && k < n;
+                        bool acc = false;
+                        int snew = 0;
+                        if ((tau & 3) == 0)
+                            rnd = philox4x32_10(cbase + c_me, sweep_id, static_cast<uint32_t>(tau >> 2), TAG_SLICE,
+                                                a.k0, a.k1);
+                        if (active) {
+                            const int i = order[k];
+                            int8_t *sc = S + static_cast<int64_t>(c_me) * n;
+                            int s, sp, sn;
+                            float f;
+                            if (k == 0) {
+                                s = sc[i];
+                                f = F[static_cast<int64_t>(c_me) * ldf + i];
+                                sp = S[static_cast<int64_t>(cprev) * n + i];
+                                sn = S[static_cast<int64_t>(cnext) * n + i];
+                            } else {
+                                s = pf_s;
+                                f = fl_prev ? fmaf(ds_prev, pf_j, pf_f) : pf_f;
+                                sp = t_me >= 1 ? s_left : pf_sp;
+                                sn = (P == 2 && t_me == 1) ? s_left : pf_sn;
+                            }
+                            const float fs = static_cast<float>(s);
+                            const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(sp + sn));
+                            const uint32_t q = tau & 3;
+                            const uint32_t word = q == 0 ? rnd.x : q == 1 ? rnd.y : q == 2 ? rnd.z : rnd.w;
+                            acc = de <= 0.0f || u01(word) < __expf(-de);
+                            DS[c_me] = -2.0f * fs;
+                            ROW[c_me] = i;
+                            if (acc) sc[i] = static_cast<int8_t>(-s);
+                            snew = acc ? -s : s;
+                            ds_prev = -2.0f * fs;
+                            fl_prev = acc;
+                            if (k + 1 < n) {
+                                const int i1 = order[k + 1];
+                                pf_f = F[static_cast<int64_t>(c_me) * ldf + i1];
+                                pf_j = a.pv.J32[static_cast<int64_t>(i) * n + i1];
+                                pf_s = sc[i1];
+                                pf_sp = S[static_cast<int64_t>(cprev) * n + i1];
+                                pf_sn = S[static_cast<int64_t>(cnext) * n + i1];
+                            }
+                        }
+                        s_left = __shfl_up(snew, 1);  // lane c - 1's spin at lane c's next site
+                        const unsigned long long m = __ballot(acc);
+                        if (tid == 0) *flipmask = m;
+                        nacc += acc ? 1ull : 0ull;
+                    }
+                    __syncthreads();
+                    apply_updates();
+                    __syncthreads();
+                }
+                if (a.bstat) bond_sample(true);
+                continue;
+            }
             for (int tau = 0; tau < nsteps; ++tau) {
                 ring_fetch(tau + a.pfd);
                 if (tid < kMaxChains) {
@@ -1312,7 +1380,11 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
         return w >= P + 2 ? static_cast<int>(w) : 0;
     };
     while (pl.rg > 1 && (fixed(pl.rg, true) > lds_cap || (dense_ring && ring_rows(pl.rg) == 0))) --pl.rg;
-    pl.use_lds = fixed(pl.rg, true) <= lds_cap;
+    // QANNEAL_SQA_STATE=global keeps the state in global memory even when it fits LDS (the same
+    // results bit for bit; a measurement and test switch)
+    const char *state_env = std::getenv("QANNEAL_SQA_STATE");
+    const bool force_global = state_env && std::strcmp(state_env, "global") == 0;
+    pl.use_lds = fixed(pl.rg, true) <= lds_cap && !force_global;
     pl.ring = pl.use_lds ? ring_rows(pl.rg) : 0;
     pl.pfd = pl.ring ? pl.ring - static_cast<int>(P) - 1 : 1;
     const char *env = std::getenv("QANNEAL_SQA_PIPELINE");

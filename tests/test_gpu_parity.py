@@ -129,6 +129,44 @@ def test_sqa_fp32_global_state_update_paths_bit_identical(ctx, monkeypatch):
     monkeypatch.delenv("QANNEAL_SQA_GLOBAL_F4", raising=False)
 
 
+def test_sqa_fp32_global_state_prefetch_bit_identical(ctx, monkeypatch):
+    """The global-memory slice sweep loads each chain's next decision inputs a step ahead (own flip
+    added from J[i_k][i_k+1], the previous slice's spin taken from its lane's decision). With the
+    state forced into global memory it must reproduce the LDS-state kernels and the column-loop
+    global path bit for bit, for P = 1, 2, 3 (ring closure onto the lane itself, onto the lane
+    before, and the general case), several replicas per workgroup, and SQA-PT."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(44)
+    for n, P, R in [(64, 1, 5), (96, 2, 3), (132, 3, 4), (200, 8, 9), (256, 16, 4)]:
+        print("prefetch case", n, P, R, flush=True)
+        dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), _sk(rng, n), 0.25)
+        betas, gammas = np.geomspace(0.3, 3.0, 5), np.geomspace(3.0, 0.05, 5)
+        res = []
+        for state, f4 in (("lds", "1"), ("global", "1"), ("global", "0")):
+            monkeypatch.setenv("QANNEAL_SQA_STATE", state)
+            monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", f4)
+            res.append(abi.sqa_run(dp, betas, gammas, P, R, 2, 0, 0, abi.MODE_FP32, 23))
+        for r in res[1:]:
+            assert r["accept_rate"] == res[0]["accept_rate"], (n, P, R)
+            assert np.array_equal(r["best_state"], res[0]["best_state"]), (n, P, R)
+            assert bits_equal([r["best_energy"]], [res[0]["best_energy"]]), (n, P, R)
+            assert bits_equal(r["energy_trace"], res[0]["energy_trace"]), (n, P, R)
+    n, P = 512, 32
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), _sk(rng, n))
+    res = []
+    for state in ("lds", "global"):
+        monkeypatch.setenv("QANNEAL_SQA_STATE", state)
+        monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", "1")
+        res.append(abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, 3, 0, 4, 1, 0,
+                                 ladders=3, mode=abi.MODE_FP32, seed=6))
+    for key in ("final_states", "best_state"):
+        assert np.array_equal(res[0][key], res[1][key]), key
+    for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
+        assert bits_equal(res[0][key].ravel(), res[1][key].ravel()), key
+    monkeypatch.delenv("QANNEAL_SQA_STATE", raising=False)
+    monkeypatch.delenv("QANNEAL_SQA_GLOBAL_F4", raising=False)
+
+
 def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
     """The one-barrier pipelined sweep (decisions of step t overlapped with the field updates of
     step t-1) must reproduce the two-barrier lockstep sweep bit for bit: same spins, same FP32
