@@ -30,8 +30,10 @@ public:
 
     AnnealResult run(std::size_t sweeps_per_beta, Observer *observer = nullptr);
 
-    // `reads` independent chains in one device launch; read r uses the engine state the
-    // reference would have after set_seed(seed + r) (what solve() does per read).
+    // `reads` independent chains in one device launch. Validation backend: read r replays the
+    // engine state the reference has after set_seed(seed + r) (what solve() does per read). FP32
+    // backend: the launch's Philox stream is keyed by (seed, read index) -- reproducible, but not
+    // the stream of a separate set_seed(seed + r) run.
     std::vector<AnnealResult> run_reads(std::size_t sweeps_per_beta, std::size_t reads, std::uint64_t seed);
 
 private:

@@ -344,9 +344,11 @@ def solve(problem: Any,
 
     Read r of a seeded call is set_seed(seed + r) and one run, as in the reference's read loop, so
     the call is reproducible. Seeded reads run concurrently, on one engine per HIP context
-    (hip_context_count()), unless a debug CSV is requested; seeded SA reads run in one launch.
-    Unseeded reads run one after another on one engine, continuing its random stream. Every run
-    executes on the GPU.
+    (hip_context_count()), unless a debug CSV is requested. Seeded SA reads run in one launch
+    (Annealer.run_reads): with backend "hip_fp64" read r is then exactly set_seed(seed + r) and one
+    run; with the FP32 backend the launch's random stream is keyed by (seed, read index), which is
+    reproducible but not the stream a separate set_seed(seed + r) run would draw. Unseeded reads
+    run one after another on one engine, continuing its random stream. Every run executes on the GPU.
     """
     method = method.lower().strip()
     schedule_type = schedule_type.lower().strip()
@@ -426,8 +428,9 @@ def solve(problem: Any,
     energies: List[float] = []
     trace: Optional[List[float]] = None
     if method == "sa" and seed is not None and reads > 1:
-        # Seeded SA reads are independent chains: run them in one launch. Read r starts from the
-        # engine state set_seed(seed + r) gives (Annealer.run_reads), as the loop below would.
+        # Seeded SA reads are independent chains: run them in one launch (Annealer.run_reads). In
+        # validation mode read r replays set_seed(seed + r) exactly, as the loop below would; the
+        # FP32 launch keys its Philox stream with (seed, read index) instead.
         batch = annealer.run_reads(sweeps_per_beta, reads, seed)
         for r in _iter_reads(reads, progress, logger):
             res = batch[r]
