@@ -26,9 +26,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "qa_internal.hpp"
@@ -115,6 +117,78 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs<TB> g) {
             if (n >= g.N) continue;
             const float v = acc[i][j] + (bias ? bias[n] : 0.0f);
             C[m * g.c_rs + n] = activate(v, g.act);
+        }
+    }
+}
+
+// The same products on the matrix cores: a 64 x 64 tile of C per workgroup of four waves (2 x 2
+// waves of 32 x 32), v_mfma_f32_32x32x16_bf16 with FP32 accumulation. An FP32 operand x enters as
+// two bf16 terms, hi = bf16(x) and lo = bf16(x - hi) (|x - hi - lo| <= 2^-16 |x| ulp-wise); bf16
+// weights enter as they are. With bf16 B: C = A_hi B + A_lo B; with FP32 B (the weight gradient):
+// C = A_hi B_hi + A_hi B_lo + A_lo B_hi (the lo x lo term, below 2^-16 of the product, dropped).
+// Operand lanes: A row (lane % 32) and B column (lane % 32) over k = 8 (lane / 32) .. +7; D column
+// (lane % 32), rows 8 (v / 4) + 4 (lane / 32) + v % 4 of accumulator register v.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_bf16(float x, uint32_t &hi, uint32_t &lo) {
+    hi = bf16_bits(x);
+    lo = bf16_bits(x - __uint_as_float(hi << 16));
+}
+
+template <class TB>
+__global__ void __launch_bounds__(256) gemm_mfma_kernel(GemmArgs<TB> g) {
+    __shared__ float As[kBK][kT + 4];
+    __shared__ float Bs[kBK][kT + 4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * kT, n0 = blockIdx.x * kT;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int64_t z = blockIdx.z;
+    const float *A = g.A + z * g.a_mz;
+    const TB *B = g.B + z * g.b_mz;
+    const int kk = (lane >> 5) * 8, rl = lane & 31;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+        for (int e = tid; e < kT * kBK; e += 256) {
+            const int r = e / kBK, k = e - r * kBK;  // A tile row r (m), column k
+            const int m = m0 + r, kk1 = k0 + k;
+            As[k][r] = (m < g.M && kk1 < g.K) ? A[m * g.a_rs + kk1 * g.a_cs] : 0.0f;
+            const int c = e % kT, kb = e / kT;        // B tile row kb (k), column c (n)
+            const int n = n0 + c, kk2 = k0 + kb;
+            Bs[kb][c] = (n < g.N && kk2 < g.K) ? load_b(B + kk2 * g.b_rs + n * g.b_cs) : 0.0f;
+        }
+        __syncthreads();
+        u32x4 ah, al, bh, bl;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t h0, l0, h1, l1;
+            split_bf16(As[kk + 2 * q][wm + rl], h0, l0);
+            split_bf16(As[kk + 2 * q + 1][wm + rl], h1, l1);
+            ah[q] = h0 | (h1 << 16);
+            al[q] = l0 | (l1 << 16);
+            split_bf16(Bs[kk + 2 * q][wn + rl], h0, l0);
+            split_bf16(Bs[kk + 2 * q + 1][wn + rl], h1, l1);
+            bh[q] = h0 | (h1 << 16);
+            bl[q] = l0 | (l1 << 16);
+        }
+        const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah), Al = __builtin_bit_cast(bf16x8, al);
+        const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bh, acc, 0, 0, 0);
+        if constexpr (!std::is_same<TB, uint16_t>::value)  // bf16 weights have no lo part
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bl, acc, 0, 0, 0);
+        __syncthreads();
+    }
+    float *C = g.C + z * g.c_mz;
+    const float *bias = g.bias ? g.bias + z * g.bias_mz : nullptr;
+    const int n = n0 + wn + rl;
+    if (n < g.N) {
+        const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int m = m0 + wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            if (m < g.M) C[m * g.c_rs + n] = activate(acc[v] + bn, g.act);
         }
     }
 }
@@ -206,7 +280,13 @@ template <class TB>
 void launch_gemm(const GemmArgs<TB> &g, int members, hipStream_t s) {
     const dim3 grid(static_cast<unsigned>((g.N + kT - 1) / kT), static_cast<unsigned>((g.M + kT - 1) / kT),
                     static_cast<unsigned>(members));
-    hipLaunchKernelGGL(gemm_kernel<TB>, grid, dim3(256), 0, s, g);
+    // QANNEAL_MLP_GEMM=valu: the VALU kernel (FP32 activations times bf16 weights in FMA order)
+    static const bool valu = [] {
+        const char *e = std::getenv("QANNEAL_MLP_GEMM");
+        return e && std::strcmp(e, "valu") == 0;
+    }();
+    if (valu) hipLaunchKernelGGL(gemm_kernel<TB>, grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL(gemm_mfma_kernel<TB>, grid, dim3(256), 0, s, g);
     QA_HIP(hipGetLastError());
 }
 

@@ -194,6 +194,51 @@ class ChiSurrogate:
         return s
 
 
+def scan_grid(cfg: PilotConfig) -> Tuple[np.ndarray, np.ndarray]:
+    """The pilot's s grid (uniform over [s(gamma_start), s(gamma_end)], s = A0 / (A0 + Gamma)) and its
+    Gamma values, as run_chi lays them out (sqa_chi_annealer.cpp:232-281)."""
+    A0 = cfg.driver_A0 if cfg.driver_A0 > 0 else cfg.gamma_start
+    s_lo, s_hi = A0 / (A0 + cfg.gamma_start), A0 / (A0 + cfg.gamma_end)
+    s = s_lo + (s_hi - s_lo) * np.arange(cfg.scan_points) / (cfg.scan_points - 1)
+    return s, A0 * (1.0 - s) / s
+
+
+def chi_profile(sur: Optional["ChiSurrogate"], prob: abi.DeviceProblem, beta: float, cfg: PilotConfig,
+                max_spread: float = 0.05, seed: int = 0) -> Tuple[np.ndarray, bool, float]:
+    """Normalized chi_B profile on the pilot grid: the surrogate's ensemble mean when the instance's
+    features lie within a quarter width of the training range and the largest per-point member
+    spread is at most max_spread, else the real pilot scan (qa_chi_pilot). Returns (profile,
+    surrogate used, spread)."""
+    if sur is not None:
+        f = features(prob, beta, cfg)
+        _, norm = sur.mlp.get_state()
+        mean, scale, lo, hi = norm
+        v = (f - mean) / scale
+        w = 0.25 * (hi - lo)
+        if np.all((v >= lo - w) & (v <= hi + w)):
+            pred, spread = sur.predict(f[None, :])
+            if spread[0] <= max_spread:
+                return pred[0], True, float(spread[0])
+    prof, r = pilot_profile(prob, beta, cfg, seed)
+    return (prof if prof is not None else np.ones(cfg.scan_points)), False, float("nan")
+
+
+def chi_ladder(profile: np.ndarray, cfg: PilotConfig, rungs: int, floor_fraction: float = 0.1) -> np.ndarray:
+    """Gamma values of an SQA-PT ladder (hot to cold) at equal-weight quantiles of the chi_B profile
+    over s: rungs crowd where chi_B is large (the quantum critical region, where neighbouring rungs
+    must be close for exchanges to pass) and thin out where it is small. The weight is the
+    normalized profile floored at floor_fraction, integrated by trapezoids on the pilot grid; the
+    end rungs sit at gamma_start and gamma_end."""
+    s, _ = scan_grid(cfg)
+    w = np.maximum(np.asarray(profile, np.float64) / max(float(np.max(profile)), 1e-300), floor_fraction)
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (w[1:] + w[:-1]) * np.diff(s))])
+    cdf /= cdf[-1]
+    s_r = np.interp(np.linspace(0.0, 1.0, rungs), cdf, s)
+    A0 = cfg.driver_A0 if cfg.driver_A0 > 0 else cfg.gamma_start
+    g = A0 * (1.0 - s_r) / s_r
+    return np.clip(g, cfg.gamma_end, cfg.gamma_start)
+
+
 def evaluate(ctx: abi.Context, sur: ChiSurrogate, specs: Sequence[InstanceSpec], num_steps: int = 100,
              sweeps_per_step: int = 10, max_spread: float = 0.05, seed: int = 12345, log=None) -> dict:
     """Production run_chi on each instance under four schedules, same production seed unless noted:
