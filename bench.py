@@ -195,9 +195,9 @@ def run_c3(steps, warmup, cpu=True):
     per step, 500 steps (5000 sweeps), swaps every step, FP32 fields refreshed every 100 sweeps (bf16
     hi+lo S.J, FP32 accumulate) with the FP64 drift check. Algorithmic bytes per sweep (128
     replicas): fields r+w 134.2 MB + spins r+w 33.6 MB + J once 16.8 MB = 11.0 B/attempt. One sweep
-    launch = 10 sweeps of all 128 replicas (one PT step); its average duration is taken as the run's
-    device time (hipEvents on the library's stream around all launches) over the 500 steps -- the
-    sweep kernel is 97 % of that time, the refresh, drift check and swap kernels the rest."""
+    launch = 10 sweeps of all 128 replicas (one PT step); its average duration comes from hipEvent
+    pairs the library records around each sweep launch on its stream (sweep_ms); the run's device
+    time adds the exchange, refresh and drift kernels (3 %)."""
     from qlatanneal_amd import abi
     n, P, rungs, L, spb, nsteps = 2048, 64, 32, 4, 10, 500
     J = sk_instance(n, 20261204)
@@ -208,23 +208,24 @@ def run_c3(steps, warmup, cpu=True):
     attempts = L * rungs * P * n * spb * nsteps
     for w in range(warmup):
         abi.sqapt_run(dp, betas, gammas, P, spb, 0, 5, 1, 0, L, abi.MODE_FP32, 10 + w, refresh_interval=100)
-    kms, drift, best = [], [], []
+    kms, sws, drift, best = [], [], [], []
     t0 = time.perf_counter()
     for k in range(steps):
         r = abi.sqapt_run(dp, betas, gammas, P, spb, 0, nsteps, 1, 0, L, abi.MODE_FP32, 1000 + k,
                           refresh_interval=100)
         kms.append(r["kernel_ms"])
+        sws.append(r["sweep_ms"])
         drift.append(r["field_drift"].tolist())
         best.append(float(np.min(r["best_energy"])))
         log(f"c3 step {k}: device {kms[-1]:.1f} ms best {best[-1]:.4f} drift {drift[-1]}")
     wall = time.perf_counter() - t0
     bytes_per_launch = (2 * L * rungs * P * n * 4 + 2 * L * rungs * P * n + n * n * 4) * spb
-    launch_s = float(np.mean(kms)) / 1e3 / nsteps
+    launch_s = float(np.mean(sws)) / 1e3 / nsteps  # hipEvent pairs around each sweep launch
     line = _line(steps * attempts / wall, steps, warmup, wall,
                  {"workload": "sqapt_dense_sk_n2048_p64_32rungs_x4_5000sweeps_refresh100_fp32", "N": n,
                   "trotter_slices": P, "rungs": rungs, "ladders": L, "sweeps": spb * nsteps,
                   "attempts_per_step": attempts, "device_ms_mean": float(np.mean(kms)),
-                  "sweep_launches_per_step": nsteps,
+                  "sweep_launches_per_step": nsteps, "sweep_launch_ms_mean": launch_s * 1e3,
                   "field_drift_before_after_refresh": drift[-1], "best_energy_per_spin": float(np.min(best)) / n},
                  _roof(bytes_per_launch, launch_s, traffic_from_profile("sqa_fp32_kernel_c3_traffic.md"),
                        lockstep=spb * (n + P - 1)))
@@ -304,8 +305,8 @@ def run_c4(steps, warmup, cpu=True, instances=20, streams=4):
 
     kms, best = [], []
     with ThreadPoolExecutor(K) as pool:
-        for w in range(warmup):
-            list(pool.map(lambda i: one(i, 10 + w), order[:K]))
+        for w in range(warmup):  # every instance once: its first run on a context uploads the problem there
+            list(pool.map(lambda i: one(i, 10 + w), order))
         t0 = time.perf_counter()
         for k in range(steps):
             res = list(pool.map(lambda i: one(i, 1000 + 97 * k), order))
@@ -601,6 +602,10 @@ def main():
     ap.add_argument("--c4-streams", type=int, default=4,
                     help="c4: host threads driving the instances (each annealer runs on one of the "
                          "library's HIP streams; 1 runs the instances one after another)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a device copy of known size (torch elementwise multiply by 1 of 256 MiB, "
+                         "4 B read + 4 B written per element, 3 times): the FETCH_SIZE / WRITE_SIZE "
+                         "calibration pass for the counter-measured HBM traffic")
     ap.add_argument("--tts-instances", type=int, default=2)
     ap.add_argument("--tts-reads", type=int, default=4, help="tts: plain-SQA reads per instance")
     ap.add_argument("--tts-runs", type=int, default=8, help="tts: surrogate-scheduled sqa_chi runs per instance")
@@ -609,6 +614,16 @@ def main():
         print("bench.py measures one GPU: every run here has a single MI355X, and the sweep shards by "
               "independent replicas and instances, one GPU each; use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_dispatch": a.numel() * 4,
+                          "bytes_written_per_dispatch": b.numel() * 4, "dispatches": 3}), flush=True)
+        return
     cpu = not args.no_cpu
     w = args.workload
     if w == "c1":

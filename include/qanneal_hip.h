@@ -241,6 +241,9 @@ typedef struct qa_sqapt_result {
     int64_t *swap_uniforms_used; /* validation (may be NULL)                                 */
     double *kernel_ms;
     double *field_drift;         /* optional [2]: as in qa_sqa_result                         */
+    double *sweep_ms;            /* optional (FP32): device time of the sweep launches alone,
+                                    summed over the run (kernel_ms adds the exchange, refresh
+                                    and drift kernels)                                        */
 } qa_sqapt_result;
 
 qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm,

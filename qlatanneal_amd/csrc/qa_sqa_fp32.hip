@@ -2112,12 +2112,20 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     const size_t wsh = static_cast<size_t>(R) * 24 + 16;
     const uint32_t per_step = static_cast<uint32_t>(prm->sweeps_per_step + prm->worldline_sweeps + prm->cluster_sweeps);
     int64_t refreshes = 0;
+    // optional: device time of the sweep launches alone (an event pair around each)
+    std::vector<hipEvent_t> sev;
+    if (res->sweep_ms) {
+        sev.resize(static_cast<size_t>(2 * steps));
+        for (auto &e : sev) QA_HIP(hipEventCreate(&e));
+    }
     QA_HIP(hipEventRecord(ctx->ev0, s));
     for (int64_t step = 0; step < steps; ++step) {
         a.resume = step > 0 ? 1 : 0;
         a.sweep_base = static_cast<uint32_t>(step) * per_step;
         a.step_base = static_cast<int>(step);
+        if (res->sweep_ms) QA_HIP(hipEventRecord(sev[2 * step], s));
         launch_sqa(ctx, pl, a);
+        if (res->sweep_ms) QA_HIP(hipEventRecord(sev[2 * step + 1], s));
         w.step = static_cast<int>(step);
         w.do_swap = ((step + 1) % prm->swap_interval == 0) ? 1 : 0;
         hipLaunchKernelGGL(pt_swap_kernel, dim3(static_cast<unsigned>(L)), dim3(64), wsh, s, w);
@@ -2180,6 +2188,16 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
         float ms = 0;
         QA_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         *res->kernel_ms = ms;
+    }
+    if (res->sweep_ms) {
+        double tot = 0.0;
+        for (int64_t step = 0; step < steps; ++step) {
+            float ms = 0;
+            QA_HIP(hipEventElapsedTime(&ms, sev[2 * step], sev[2 * step + 1]));
+            tot += ms;
+        }
+        for (auto &e : sev) (void)hipEventDestroy(e);
+        *res->sweep_ms = tot;
     }
 }
 
