@@ -43,7 +43,10 @@ constexpr int kThreads = 384;
 constexpr int kUpd = 256;        // update threads (tid 64..319)
 constexpr int kHelperWave = 5;
 constexpr int kPreH = 32;        // helper registers for one staged row (n <= 64 * kPreH)
-constexpr int kMaxChains = 64;
+constexpr int kMaxChains = 64;  // chains of a group (rg * P) in one wavefront: pipelined sweeps, rg > 1
+// Trotter slices per replica of the two-barrier kernel: one thread per chain, one replica per
+// group once P > 64 (the reference accepts any count, sqa_annealer.cpp:28-30, 97-99).
+constexpr int kMaxSlices = kThreads;
 
 // LDS regions start on 16-byte boundaries (the per-chain double arrays need 8).
 __host__ __device__ constexpr size_t lds_align16(size_t b) { return (b + 15) / 16 * 16; }
@@ -104,6 +107,7 @@ struct SqaArgs {
     int hubo_lds;          // HUBO: incidence and term arrays staged in LDS after the scratch
     int hubo_nv;           // HUBO: entries of t_vars (= of inc_idx)
     int adj_lds;           // sparse: adjacency (CSR) staged in LDS after the scratch
+    int cmax;              // per-chain scratch entries: 64, or G rounded up to 64 when P > 64
 };
 
 
@@ -117,14 +121,14 @@ struct SqaLds {
     int *order;        // two visit-order buffers of obstride ints
     int obstride;
     float *Jring;      // [ring][ldf]: J rows of upcoming order positions
-    int *RSLOT;        // [kMaxChains] ring slot of each chain's site (two-barrier sweeps)
-    float *DS;         // [kMaxChains] -2 s_old of each chain's attempt (two-barrier sweeps)
-    int *ROW;          // [kMaxChains] site of each chain's attempt (two-barrier sweeps)
-    double *ered;      // [kMaxChains] slice energies of the step tail
-    float *wred;       // [kMaxChains] worldline partial sums
-    unsigned long long *cmask;  // [kMaxChains] cluster memberships
-    int *rflag;        // [kMaxChains] worldline decisions
-    unsigned long long *flipmask;  // two-barrier: [0]; pipelined: [b] flips, [2 + b] flips from s = +1
+    int *RSLOT;        // [cmax] ring slot of each chain's site (two-barrier sweeps)
+    float *DS;         // [cmax] -2 s_old of each chain's attempt (two-barrier sweeps)
+    int *ROW;          // [cmax] site of each chain's attempt (two-barrier sweeps)
+    double *ered;      // [cmax] slice energies of the step tail
+    float *wred;       // [cmax] worldline partial sums
+    unsigned long long *cmask;  // [cmax] cluster memberships
+    int *rflag;        // [cmax] worldline decisions
+    unsigned long long *flipmask;  // two-barrier: [w] flips of wave w's chains; pipelined: [b] flips, [2 + b] flips from s = +1
     int *scal;         // [0]: best chain of the step
     float *junk;       // [64][4]: target of stores a lane must not make (never read)
     const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO arrays (LDS copies or global)
@@ -144,6 +148,17 @@ __host__ __device__ constexpr size_t hubo_lds_bytes(int n, int nterms, int nv) {
            lds_align16(static_cast<size_t>(nterms + 1) * 4) + lds_align16(static_cast<size_t>(nterms) * 4);
 }
 
+// Flip-mask words: one per deciding wavefront (cmax / 64), at least the pipelined kernel's four.
+__host__ __device__ constexpr size_t flipmask_bytes(int cmax) {
+    return lds_align16(static_cast<size_t>(cmax / 64 > 4 ? cmax / 64 : 4) * 8);
+}
+
+// LDS of the per-chain scratch (RSLOT, DS, ROW, ered, wred, cmask, rflag), the flip masks, scal
+// and junk, for cmax chain entries.
+__host__ __device__ constexpr size_t chain_scratch_bytes(int cmax) {
+    return static_cast<size_t>(cmax) * (4 + 4 + 4 + 8 + 4 + 8 + 4) + flipmask_bytes(cmax) + 16 + 64 * 16;
+}
+
 __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) {
     const int n = a.pv.n;
     SqaLds L{};
@@ -158,22 +173,23 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
     p += 2 * lds_align16(static_cast<size_t>(n) * sizeof(int));
     L.Jring = reinterpret_cast<float *>(p);
     p += lds_align16(static_cast<size_t>(a.ring) * a.ldr * sizeof(float));
+    const int cm = a.cmax;
     L.RSLOT = reinterpret_cast<int *>(p);
-    p += kMaxChains * sizeof(int);
+    p += cm * sizeof(int);
     L.DS = reinterpret_cast<float *>(p);
-    p += kMaxChains * sizeof(float);
+    p += cm * sizeof(float);
     L.ROW = reinterpret_cast<int *>(p);
-    p += kMaxChains * sizeof(int);
+    p += cm * sizeof(int);
     L.ered = reinterpret_cast<double *>(p);
-    p += kMaxChains * sizeof(double);
+    p += cm * sizeof(double);
     L.wred = reinterpret_cast<float *>(p);
-    p += kMaxChains * sizeof(float);
+    p += cm * sizeof(float);
     L.cmask = reinterpret_cast<unsigned long long *>(p);
-    p += kMaxChains * sizeof(unsigned long long);
+    p += cm * sizeof(unsigned long long);
     L.rflag = reinterpret_cast<int *>(p);
-    p += kMaxChains * sizeof(int);
+    p += cm * sizeof(int);
     L.flipmask = reinterpret_cast<unsigned long long *>(p);
-    p += 32;
+    p += flipmask_bytes(cm);
     L.scal = reinterpret_cast<int *>(p);
     p += 16;
     L.junk = reinterpret_cast<float *>(p);
@@ -415,8 +431,8 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     // Dense: the flipped chains are taken eight at a time with a wave-uniform mask; every LDS
     // load of a batch (ring row value, field) is issued before any of its stores, so one batch
     // costs one LDS round trip instead of one per chain and column.
-    auto apply_updates = [&]() {
-        const unsigned long long m_all = *flipmask;
+    // One flip-mask word: chains cw .. cw + 63.
+    auto apply_word = [&](const unsigned long long m_all, const int cw) {
         if (m_all == 0ull) return;
         if constexpr (!kLds && NJ > 0) {
             constexpr int B = NJ >= 8 ? 1 : 8 / NJ;
@@ -435,7 +451,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                     dsv[q] = 0.0f;
                     rows[q] = nullptr;
                     if (m) {
-                        const int c = __ffsll(static_cast<long long>(m)) - 1;
+                        const int c = cw + __ffsll(static_cast<long long>(m)) - 1;
                         m &= m - 1ull;
                         cs[q] = c;
                         dsv[q] = DS[c];
@@ -477,7 +493,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     if (m) {
-                        cs[q] = __ffsll(static_cast<long long>(m)) - 1;
+                        cs[q] = cw + __ffsll(static_cast<long long>(m)) - 1;
                         m &= m - 1ull;
                     } else {
                         cs[q] = -1;
@@ -506,7 +522,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                         if (cs[q] >= 0) F[static_cast<int64_t>(cs[q]) * ldf + j] = fmaf(dsv[q], rv[q], fv[q]);
                 }
             }
-        } else if (tid < G && ((m_all >> tid) & 1ull)) {
+        } else if (tid >= cw && tid < cw + 64 && tid < G && ((m_all >> (tid - cw)) & 1ull)) {
             const int c = tid, i = ROW[c];
             float *fc = F + static_cast<int64_t>(c) * ldf;
             const int8_t *sc = S + static_cast<int64_t>(c) * n;
@@ -526,6 +542,11 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 }
             }
         }
+    };
+    // Apply the field updates of every flagged chain: one mask word per deciding wavefront.
+    const int nwords = a.cmax >> 6;
+    auto apply_updates = [&]() {
+        for (int w = 0; w < nwords; ++w) apply_word(flipmask[w], w << 6);
     };
 
     // Random permutation of 0..n-1 for this sweep (Fisher-Yates on one lane, Philox draws).
@@ -624,7 +645,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                     for (int k = 0; k < n; ++k) {
                         ring_fetch(k + a.pfd);
                         const int tau = ph * n + k;
-                        if (tid < kMaxChains) {
+                        if (tid < a.cmax) {
                             const bool active = decider && colour == ph;
                             bool acc = false;
                             if ((tau & 3) == 0)
@@ -646,7 +667,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                                 if (acc) sc[k] = static_cast<int8_t>(-s);
                             }
                             const unsigned long long m = __ballot(acc);
-                            if (tid == 0) *flipmask = m;
+                            if (lane == 0) flipmask[wave] = m;
                             nacc += acc ? 1ull : 0ull;
                         }
                         __syncthreads();
@@ -693,7 +714,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 int pf_s = 0, pf_sp = 0, pf_sn = 0, s_left = 0;
                 bool fl_prev = false;
                 for (int tau = 0; tau < nsteps; ++tau) {
-                    if (tid < kMaxChains) {
+                    if (tid < a.cmax) {
                         const int k = tau - t_me;
                         const bool active = decider && k >= 0 && k < n;
                         bool acc = false;
@@ -714,7 +735,9 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                             } else {
                                 s = pf_s;
                                 f = fl_prev ? fmaf(ds_prev, pf_j, pf_f) : pf_f;
-                                sp = t_me >= 1 ? s_left : pf_sp;
+                                // (lane 0 of a later wavefront: lane c - 1 is in the previous
+                                // wavefront, so its spin, written last step, comes from memory)
+                                sp = t_me >= 1 ? (lane == 0 ? S[static_cast<int64_t>(cprev) * n + i] : s_left) : pf_sp;
                                 sn = (P == 2 && t_me == 1) ? s_left : pf_sn;
                             }
                             const float fs = static_cast<float>(s);
@@ -739,7 +762,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                         }
                         s_left = __shfl_up(snew, 1);  // lane c - 1's spin at lane c's next site
                         const unsigned long long m = __ballot(acc);
-                        if (tid == 0) *flipmask = m;
+                        if (lane == 0) flipmask[wave] = m;
                         nacc += acc ? 1ull : 0ull;
                     }
                     __syncthreads();
@@ -751,7 +774,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             }
             for (int tau = 0; tau < nsteps; ++tau) {
                 ring_fetch(tau + a.pfd);
-                if (tid < kMaxChains) {
+                if (tid < a.cmax) {
                     const int k = tau - t_me;
                     const bool active = decider && k >= 0 && k < n;
                     bool acc = false;
@@ -775,7 +798,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                         if (acc) sc[i] = static_cast<int8_t>(-s);
                     }
                     const unsigned long long m = __ballot(acc);
-                    if (tid == 0) *flipmask = m;
+                    if (lane == 0) flipmask[wave] = m;
                     nacc += acc ? 1ull : 0ull;
                 }
                 __syncthreads();
@@ -793,8 +816,8 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 const int i = order[k];
                 ring_fetch(k + a.pfd);
                 if (decider && t_me == 0) {
-                    // leader lane of replica rr_me grows the imaginary-time cluster (LIFO, prev
-                    // before next), membership as a bit mask over the replica's P <= 64 slices
+                    // leader lane of replica rr_me grows the imaginary-time cluster (prev before
+                    // next), membership as a bit mask over the replica's slices
                     const int base = rr_me * P;
                     uint32_t draw = 0;
                     auto next_u = [&]() {
@@ -805,32 +828,69 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                     };
                     const int seed = static_cast<int>(below(next_u(), static_cast<uint32_t>(P)));
                     const int8_t s0 = S[static_cast<int64_t>(base + seed) * n + i];
-                    unsigned long long inc = 1ull << seed, stack = 1ull << seed;
-                    while (stack) {
-                        const int t = 63 - __clzll(static_cast<long long>(stack));
-                        stack &= ~(1ull << t);
-                        const int pv = t == 0 ? P - 1 : t - 1, nx = t + 1 == P ? 0 : t + 1;
-                        if (!((inc >> pv) & 1ull) && S[static_cast<int64_t>(base + pv) * n + i] == s0 &&
-                            u01(next_u()) < join) { inc |= 1ull << pv; stack |= 1ull << pv; }
-                        if (!((inc >> nx) & 1ull) && S[static_cast<int64_t>(base + nx) * n + i] == s0 &&
-                            u01(next_u()) < join) { inc |= 1ull << nx; stack |= 1ull << nx; }
+                    if (P <= 64) {
+                        unsigned long long inc = 1ull << seed, stack = 1ull << seed;
+                        while (stack) {
+                            const int t = 63 - __clzll(static_cast<long long>(stack));
+                            stack &= ~(1ull << t);
+                            const int pv = t == 0 ? P - 1 : t - 1, nx = t + 1 == P ? 0 : t + 1;
+                            if (!((inc >> pv) & 1ull) && S[static_cast<int64_t>(base + pv) * n + i] == s0 &&
+                                u01(next_u()) < join) { inc |= 1ull << pv; stack |= 1ull << pv; }
+                            if (!((inc >> nx) & 1ull) && S[static_cast<int64_t>(base + nx) * n + i] == s0 &&
+                                u01(next_u()) < join) { inc |= 1ull << nx; stack |= 1ull << nx; }
+                        }
+                        float dcl = 0.0f, dtm = 0.0f;
+                        for (int t = 0; t < P; ++t) {
+                            if (!((inc >> t) & 1ull)) continue;
+                            dcl += -2.0f * s0 * F[static_cast<int64_t>(base + t) * ldf + i];
+                            const int pv = t == 0 ? P - 1 : t - 1, nx = t + 1 == P ? 0 : t + 1;
+                            if (!((inc >> pv) & 1ull)) dtm += 2.0f * jp * s0 * S[static_cast<int64_t>(base + pv) * n + i];
+                            if (!((inc >> nx) & 1ull)) dtm += 2.0f * jp * s0 * S[static_cast<int64_t>(base + nx) * n + i];
+                        }
+                        const float de = fmaf(bs, dcl, dtm);
+                        const bool ok = de <= 0.0f || u01(next_u()) < __expf(-de);
+                        cmask[rr_me] = ok ? inc : 0ull;
+                    } else {
+                        // P > 64 (one replica per group): the same growth with multi-word masks in
+                        // LDS, membership in cmask[0 .. nw), the stack in cmask[8 .. 8 + nw); the
+                        // highest stacked slice is taken first, as with one word
+                        unsigned long long *inc = cmask, *stk = cmask + 8;
+                        const int nw = (P + 63) >> 6;
+                        for (int w = 0; w < nw; ++w) inc[w] = stk[w] = 0ull;
+                        auto has = [&](const unsigned long long *m, int t) { return (m[t >> 6] >> (t & 63)) & 1ull; };
+                        auto put = [&](unsigned long long *m, int t) { m[t >> 6] |= 1ull << (t & 63); };
+                        put(inc, seed);
+                        put(stk, seed);
+                        int top = seed >> 6;
+                        for (;;) {
+                            while (top >= 0 && stk[top] == 0ull) --top;
+                            if (top < 0) break;
+                            const int t = (top << 6) + 63 - __clzll(static_cast<long long>(stk[top]));
+                            stk[top] &= ~(1ull << (t & 63));
+                            const int pv = t == 0 ? P - 1 : t - 1, nx = t + 1 == P ? 0 : t + 1;
+                            if (!has(inc, pv) && S[static_cast<int64_t>(base + pv) * n + i] == s0 &&
+                                u01(next_u()) < join) { put(inc, pv); put(stk, pv); top = max(top, pv >> 6); }
+                            if (!has(inc, nx) && S[static_cast<int64_t>(base + nx) * n + i] == s0 &&
+                                u01(next_u()) < join) { put(inc, nx); put(stk, nx); top = max(top, nx >> 6); }
+                        }
+                        float dcl = 0.0f, dtm = 0.0f;
+                        for (int t = 0; t < P; ++t) {
+                            if (!has(inc, t)) continue;
+                            dcl += -2.0f * s0 * F[static_cast<int64_t>(base + t) * ldf + i];
+                            const int pv = t == 0 ? P - 1 : t - 1, nx = t + 1 == P ? 0 : t + 1;
+                            if (!has(inc, pv)) dtm += 2.0f * jp * s0 * S[static_cast<int64_t>(base + pv) * n + i];
+                            if (!has(inc, nx)) dtm += 2.0f * jp * s0 * S[static_cast<int64_t>(base + nx) * n + i];
+                        }
+                        const float de = fmaf(bs, dcl, dtm);
+                        const bool ok = de <= 0.0f || u01(next_u()) < __expf(-de);
+                        if (!ok)
+                            for (int w = 0; w < nw; ++w) inc[w] = 0ull;
                     }
-                    float dcl = 0.0f, dtm = 0.0f;
-                    for (int t = 0; t < P; ++t) {
-                        if (!((inc >> t) & 1ull)) continue;
-                        dcl += -2.0f * s0 * F[static_cast<int64_t>(base + t) * ldf + i];
-                        const int pv = t == 0 ? P - 1 : t - 1, nx = t + 1 == P ? 0 : t + 1;
-                        if (!((inc >> pv) & 1ull)) dtm += 2.0f * jp * s0 * S[static_cast<int64_t>(base + pv) * n + i];
-                        if (!((inc >> nx) & 1ull)) dtm += 2.0f * jp * s0 * S[static_cast<int64_t>(base + nx) * n + i];
-                    }
-                    const float de = fmaf(bs, dcl, dtm);
-                    const bool ok = de <= 0.0f || u01(next_u()) < __expf(-de);
-                    cmask[rr_me] = ok ? inc : 0ull;
                 }
                 __syncthreads();
                 bool acc = false;
                 if (decider) {
-                    acc = (cmask[rr_me] >> t_me) & 1ull;
+                    acc = P <= 64 ? (cmask[rr_me] >> t_me) & 1ull : (cmask[t_me >> 6] >> (t_me & 63)) & 1ull;
                     int8_t *sc = S + static_cast<int64_t>(c_me) * n;
                     const int8_t s = sc[i];
                     DS[c_me] = -2.0f * s;
@@ -838,9 +898,9 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                     if (a.ring) RSLOT[c_me] = k % a.ring;
                     if (acc) sc[i] = static_cast<int8_t>(-s);
                 }
-                if (tid < kMaxChains) {
+                if (tid < a.cmax) {
                     const unsigned long long m = __ballot(acc);
-                    if (tid == 0) *flipmask = m;
+                    if (lane == 0) flipmask[wave] = m;
                 }
                 __syncthreads();
                 apply_updates();
@@ -877,9 +937,9 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                     if (a.ring) RSLOT[c_me] = k % a.ring;
                     if (acc) sc[i] = static_cast<int8_t>(-s);
                 }
-                if (tid < kMaxChains) {
+                if (tid < a.cmax) {
                     const unsigned long long m = __ballot(acc);
-                    if (tid == 0) *flipmask = m;
+                    if (lane == 0) flipmask[wave] = m;
                 }
                 __syncthreads();
                 apply_updates();
@@ -910,9 +970,23 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     }
     // final spins out (for the caller's state readback / diagnostics)
     if constexpr (kLds) group_save(a, F, S, ldf, G, r0);
-    if (tid < kMaxChains) {
-        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
-        if (tid == 0) a.accepted[g] = nacc;
+    if (a.cmax == kMaxChains) {
+        if (tid < kMaxChains) {
+            for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+            if (tid == 0) a.accepted[g] = nacc;
+        }
+    } else {
+        // one partial per deciding wavefront (cmask is free after the last step), summed by thread 0
+        if (tid < a.cmax) {
+            for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+            if (lane == 0) cmask[wave] = nacc;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long tot = 0;
+            for (int w = 0; w < nwords; ++w) tot += cmask[w];
+            a.accepted[g] = tot;
+        }
     }
 }
 
@@ -1343,6 +1417,7 @@ struct SqaPlan {
     bool use_lds = false;
     int ring = 0, pfd = 1, ldf = 0, ldr = 0, pipe = 0;
     int hubo_lds = 0, hubo_nv = 0, adj_lds = 0;
+    int cmax = kMaxChains;
     size_t shmem = 0;
 };
 
@@ -1351,8 +1426,8 @@ struct SqaPlan {
 SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, int64_t rg_force = 0,
                  size_t lds_cap = 160 * 1024) {
     const int64_t n = p->n;
-    if (P > kMaxChains)
-        throw Error{QA_EUNSUPPORTED, "FP32 mode runs at most 64 Trotter slices per replica in this build"};
+    if (P > kMaxSlices)
+        throw Error{QA_EUNSUPPORTED, "FP32 mode runs at most 384 Trotter slices per replica in this build"};
     if (p->kind == QA_KIND_DENSE && !p->symmetric)
         throw Error{QA_EUNSUPPORTED, "FP32 mode needs a symmetric J (the reference's field updates assume it); "
                                      "use the FP64 validation mode for an asymmetric J"};
@@ -1363,10 +1438,11 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     pl.rg = std::max<int64_t>(1, kMaxChains / P);
     while (pl.rg > 1 && (R + pl.rg - 1) / pl.rg < ctx->num_cus) --pl.rg;
     if (rg_force > 0) pl.rg = std::min<int64_t>(rg_force, std::max<int64_t>(1, kMaxChains / P));
+    // per-chain scratch: one wavefront of entries, or P rounded up to whole wavefronts (rg = 1)
+    pl.cmax = P <= kMaxChains ? kMaxChains : static_cast<int>((P + 63) / 64 * 64);
     // LDS without the row ring: two visit orders, per-chain scratch, and (LDS layout) the state
     auto fixed = [&](int64_t r, bool lds) {
-        size_t b = 2 * lds_align16(static_cast<size_t>(n) * 4) + kMaxChains * (4 + 4 + 4 + 8 + 4 + 8 + 4) + 32 + 16 +
-                   64 * 16;
+        size_t b = 2 * lds_align16(static_cast<size_t>(n) * 4) + chain_scratch_bytes(pl.cmax);
         if (lds) b += lds_align16(static_cast<size_t>(r * P * rn) * 4) + lds_align16(static_cast<size_t>(r * P * n));
         return b;
     };
@@ -1388,7 +1464,7 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     pl.ring = pl.use_lds ? ring_rows(pl.rg) : 0;
     pl.pfd = pl.ring ? pl.ring - static_cast<int>(P) - 1 : 1;
     const char *env = std::getenv("QANNEAL_SQA_PIPELINE");
-    pl.pipe = (pl.use_lds && pl.ring > 0 && !(env && env[0] == '0')) ? 1 : 0;
+    pl.pipe = (pl.use_lds && pl.ring > 0 && pl.cmax == kMaxChains && !(env && env[0] == '0')) ? 1 : 0;
     pl.ngroups = (R + pl.rg - 1) / pl.rg;
     pl.shmem = fixed(pl.rg, pl.use_lds) + lds_align16(static_cast<size_t>(pl.ring) * rn * 4);
     QA_REQUIRE(pl.shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
@@ -1422,14 +1498,15 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     a.hubo_lds = pl.hubo_lds;
     a.hubo_nv = pl.hubo_nv;
     a.adj_lds = pl.adj_lds;
+    a.cmax = pl.cmax;
     a.ngroups = static_cast<int>(pl.ngroups);
     a.ldf = pl.ldf;
     a.ldr = pl.ldr;
     a.ring = pl.ring;
     a.pfd = pl.pfd;
     // the pipelined kernel runs plain slice sweeps over dense couplings with an LDS state
-    const bool pipe = pl.pipe && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 && a.pv.kind == QA_KIND_DENSE &&
-                      a.bstat == nullptr;
+    const bool pipe = pl.pipe && a.cmax == kMaxChains && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 &&
+                      a.pv.kind == QA_KIND_DENSE && a.bstat == nullptr;
     const void *kern;
     unsigned threads = kThreads;
     if (pipe) {
@@ -2346,7 +2423,7 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
 // arithmetic): one workgroup per replica, m_r = sum of the replica's spins / (n P), accumulated
 // as m^2 and |m|.
 __global__ void __launch_bounds__(256) chk_mag_kernel(const int8_t *gS, int P, int n, double *m2, double *mabs) {
-    __shared__ double ered[kMaxChains];
+    __shared__ double ered[kMaxSlices];
     const int r = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (int t = wave; t < P; t += 4) {
         const int8_t *sc = gS + (static_cast<int64_t>(r) * P + t) * n;
@@ -2917,6 +2994,7 @@ void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const
         a.ldr = x.pl.ldr;
         a.ring = x.pl.ring;
         a.pfd = x.pl.pfd;
+        a.cmax = x.pl.cmax;
         const double cost = static_cast<double>(x.pl.rg * P * x.n) * (probs[k]->kind == QA_KIND_DENSE ? x.n : 16);
         for (int64_t gi = 0; gi < x.pl.ngroups; ++gi)
             work[x.variant].push_back({cost, BatchItem{static_cast<int>(k), static_cast<int>(gi)}});
