@@ -2296,25 +2296,37 @@ struct ChkArgs {
     const float *J4rows;  // dense: J32p, n rows of rn floats (rn = n rounded up to 4)
     const int32_t *adj_ptr, *adj_idx;  // sparse: CSR [n + 1], [nnz]
     const float *adj_val;              // sparse: [nnz]
+    const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO: incidence CSR and term lists
+    const float *t_coef;                                // HUBO: [nterms]
     int8_t *gS;           // [R*P][n]
     float *gF;            // [R*P][n]
     const float *bsv, *jpv;  // beta / P and J_perp by parameter index
     const int *pof;          // non-null: replica r uses index pof[r] (pilot scan points), else pidx
     int pidx;
-    int P, n, rn, ph, nnz, unique;
+    int P, n, rn, ph, nnz, unique, nterms, nv;
+    int lists_lds;           // sparse / HUBO: 1 = coupling lists staged in the wave's LDS
     uint32_t sweep_id, k0, k1;
 };
 
-// LDS bytes of chk_wave_kernel: fields, words, spins, neighbour sums (+ the CSR when sparse)
-__host__ __device__ constexpr size_t chk_lds_bytes(int n, int rn, int nnz, bool sparse) {
-    return lds_align16(static_cast<size_t>(rn) * 4 + static_cast<size_t>(n) * 6) +
-           (sparse ? lds_align16(static_cast<size_t>(n + 1) * 4) + static_cast<size_t>(nnz) * 8 : 0);
+// Random words of a chunk of sites: 64 Philox blocks, one per lane (>= 253 consecutive sites).
+constexpr int kChkWords = 256;
+
+// LDS bytes of chk_wave_kernel: fields, a chunk of random words, spins, neighbour sums
+__host__ __device__ constexpr size_t chk_lds_bytes(int n, int rn) {
+    return lds_align16(static_cast<size_t>(rn) * 4 + kChkWords * 4 + static_cast<size_t>(n) * 2);
+}
+// ... plus the coupling lists when staged: sparse CSR, or the HUBO arrays
+__host__ __device__ constexpr size_t chk_list_bytes(int kind, int n, int nnz, int nterms, int nv) {
+    return kind == QA_KIND_SPARSE ? lds_align16(static_cast<size_t>(n + 1) * 4) + static_cast<size_t>(nnz) * 8
+         : kind == QA_KIND_HUBO   ? hubo_lds_bytes(n, nterms, nv)
+                                  : 0;
 }
 
 // kDemand (dense): load a J row only for an accepted flip instead of streaming every row ahead of
 // its site; with thousands of chains in flight (the pilot scan) this cuts the row traffic to the
 // accepted fraction while other waves hide each load's latency.
-template <int NQ, bool kDemand = false>  // dense: float4 groups of a row per lane (rn <= 256 * NQ); 0: sparse
+// NQ > 0: dense, float4 groups of a row per lane (rn <= 256 * NQ); 0: sparse; -1: HUBO.
+template <int NQ, bool kDemand = false>
 __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int c = blockIdx.x, lane = threadIdx.x;
@@ -2327,17 +2339,51 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
     const float bs = a.bsv[pq], jp = a.jpv[pq];
     float *F = reinterpret_cast<float *>(smem);           // [rn]
     f4 *F4 = reinterpret_cast<f4 *>(smem);
-    uint32_t *W = reinterpret_cast<uint32_t *>(F + rn);   // [n] uniform words of this phase's sites
-    int8_t *S = reinterpret_cast<int8_t *>(W + n);        // [n]
+    uint32_t *W = reinterpret_cast<uint32_t *>(F + rn);   // [kChkWords] random words of the current chunk
+    int8_t *S = reinterpret_cast<int8_t *>(W + kChkWords);  // [n]
     int8_t *NB = S + n;                                   // [n] s_{t-1,i} + s_{t+1,i} (fixed in the phase)
-    int32_t *AP = reinterpret_cast<int32_t *>(smem + chk_lds_bytes(n, rn, 0, false));  // sparse: [n + 1]
-    int32_t *AI = AP + lds_align16(static_cast<size_t>(n + 1) * 4) / 4;                  // sparse: [nnz]
-    float *AV = reinterpret_cast<float *>(AI + a.nnz);                                   // sparse: [nnz]
+    unsigned char *lists = smem + chk_lds_bytes(n, rn);
+    // sparse CSR and HUBO arrays: LDS copies (staged below) or the global arrays
+    const int32_t *AP = a.adj_ptr, *AI = a.adj_idx, *IP = a.inc_ptr, *II = a.inc_idx, *TO = a.t_off, *TV = a.t_vars;
+    const float *AV = a.adj_val, *TC = a.t_coef;
     if constexpr (NQ == 0) {
-        for (int i = lane; i <= n; i += 64) AP[i] = a.adj_ptr[i];
-        for (int q = lane; q < a.nnz; q += 64) {
-            AI[q] = a.adj_idx[q];
-            AV[q] = a.adj_val[q];
+        if (a.lists_lds) {
+            int32_t *ap = reinterpret_cast<int32_t *>(lists);
+            int32_t *ai = ap + lds_align16(static_cast<size_t>(n + 1) * 4) / 4;
+            float *av = reinterpret_cast<float *>(ai + a.nnz);
+            for (int i = lane; i <= n; i += 64) ap[i] = a.adj_ptr[i];
+            for (int q = lane; q < a.nnz; q += 64) {
+                ai[q] = a.adj_idx[q];
+                av[q] = a.adj_val[q];
+            }
+            AP = ap;
+            AI = ai;
+            AV = av;
+        }
+    } else if constexpr (NQ < 0) {
+        if (a.lists_lds) {  // the layout of carve_lds's staged HUBO arrays
+            unsigned char *q = lists;
+            int32_t *ip = reinterpret_cast<int32_t *>(q);
+            q += lds_align16(static_cast<size_t>(n + 1) * 4);
+            int32_t *ii = reinterpret_cast<int32_t *>(q);
+            q += lds_align16(static_cast<size_t>(a.nv) * 4);
+            int32_t *to = reinterpret_cast<int32_t *>(q);
+            q += lds_align16(static_cast<size_t>(a.nterms + 1) * 4);
+            int32_t *tv = reinterpret_cast<int32_t *>(q);
+            q += lds_align16(static_cast<size_t>(a.nv) * 4);
+            float *tc = reinterpret_cast<float *>(q);
+            for (int e = lane; e <= n; e += 64) ip[e] = a.inc_ptr[e];
+            for (int e = lane; e < a.nv; e += 64) {
+                ii[e] = a.inc_idx[e];
+                tv[e] = a.t_vars[e];
+            }
+            for (int e = lane; e <= a.nterms; e += 64) to[e] = a.t_off[e];
+            for (int e = lane; e < a.nterms; e += 64) tc[e] = a.t_coef[e];
+            IP = ip;
+            II = ii;
+            TO = to;
+            TV = tv;
+            TC = tc;
         }
     }
     const int64_t off = static_cast<int64_t>(c) * n;
@@ -2347,29 +2393,16 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
         F[i] = a.gF[off + i];
     }
     for (int i = n + lane; i < rn; i += 64) F[i] = 0.0f;
-    // site k of phase ph is step tau = ph * n + k of the sweep: word tau & 3 of block tau >> 2
-    const uint32_t tau0 = static_cast<uint32_t>(a.ph) * static_cast<uint32_t>(n);
-    for (uint32_t b = (tau0 >> 2) + lane; b <= ((tau0 + n - 1) >> 2); b += 64) {
-        const Philox4 w = philox4x32_10(static_cast<uint32_t>(c), a.sweep_id, b, TAG_CHECKER, a.k0, a.k1);
-        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t tau = 4 * b + q;
-            if (tau >= tau0 && tau < tau0 + static_cast<uint32_t>(n)) W[tau - tau0] = ws[q];
-        }
-    }
-    __syncthreads();
     constexpr int NR = NQ > 0 ? NQ : 1;
     const f4 *J4 = reinterpret_cast<const f4 *>(a.J4rows);
     auto load_row = [&](f4 (&h)[NR], int k) {
         const f4 *row = J4 + static_cast<int64_t>(min(k, n - 1)) * ng;
-        static_for<NQ>([&](auto q) { h[q] = row[min(lane + 64 * q, ng - 1)]; });
+        static_for<(NQ > 0 ? NQ : 0)>([&](auto q) { h[q] = row[min(lane + 64 * q, ng - 1)]; });
     };
-    auto site = [&](const f4 (&h)[NR], int k) {
+    auto site = [&](const f4 (&h)[NR], int k, uint32_t word) {
         const float f = F[k];
         const int8_t s = S[k];
         const int nb = NB[k];
-        const uint32_t word = W[k];
         const float fs = static_cast<float>(s);
         const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(nb));
         const bool acc = de <= 0.0f || u01(word) < __expf(-de);
@@ -2383,11 +2416,25 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
                 } else if (lane == 0) {
                     for (int q = q0; q < q1; ++q) F[AI[q]] += AV[q] * ds;
                 }
+            } else if constexpr (NQ < 0) {
+                // the two-barrier kernel's HUBO update, in the same order (terms in incidence
+                // order, products over the term with the flipped spin's new value)
+                if (lane == 0) {
+                    for (int q = IP[k]; q < IP[k + 1]; ++q) {
+                        const int tt = II[q];
+                        float prod = TC[tt];
+                        for (int e = TO[tt]; e < TO[tt + 1]; ++e) prod *= S[TV[e]];
+                        for (int e = TO[tt]; e < TO[tt + 1]; ++e) {
+                            const int v = TV[e];
+                            if (v != k) F[v] += 2.0f * prod * S[v];
+                        }
+                    }
+                }
             } else {
                 f4 hd[NR];
                 if constexpr (kDemand) load_row(hd, k);
                 const f4 (&hr)[NR] = kDemand ? hd : h;
-                static_for<NQ>([&](auto q) {
+                static_for<(NQ > 0 ? NQ : 0)>([&](auto q) {
                     const int gi = lane + 64 * q;
                     if (gi < ng) {
                         const f4 x = hr[q], y = F4[gi];
@@ -2398,20 +2445,41 @@ __global__ void __launch_bounds__(64) chk_wave_kernel(ChkArgs a) {
         }
         __syncthreads();  // this site's field and spin writes before the next site's reads
     };
+    // site k of phase ph is step tau = ph * n + k of the sweep: word tau & 3 of Philox block tau >> 2;
+    // the words are evaluated a chunk of 64 blocks at a time (one per lane)
+    const uint32_t tau0 = static_cast<uint32_t>(a.ph) * static_cast<uint32_t>(n);
+    constexpr bool kStream = NQ > 0 && !kDemand;
     f4 A[NR], B[NR];
-    if constexpr (NQ == 0 || kDemand) {
-        for (int k = 0; k < n; ++k) site(A, k);
-    } else {
+    if constexpr (kStream) {
         load_row(A, 0);
         load_row(B, 1);
-        for (int k = 0; k < n; k += 2) {
-            site(A, k);
-            load_row(A, k + 2);
-            if (k + 1 < n) {
-                site(B, k + 1);
-                load_row(B, k + 3);
+    }
+    for (int k = 0; k < n;) {
+        const uint32_t b0 = (tau0 + static_cast<uint32_t>(k)) >> 2;
+        {
+            const Philox4 w = philox4x32_10(static_cast<uint32_t>(c), a.sweep_id, b0 + lane, TAG_CHECKER, a.k0, a.k1);
+            W[4 * lane] = w.x;
+            W[4 * lane + 1] = w.y;
+            W[4 * lane + 2] = w.z;
+            W[4 * lane + 3] = w.w;
+        }
+        __syncthreads();
+        const int kend = min(n, static_cast<int>(4 * b0 + kChkWords - tau0));
+        for (; k < kend; ++k) {
+            const uint32_t word = W[tau0 + k - 4 * b0];
+            if constexpr (kStream) {
+                if ((k & 1) == 0) {
+                    site(A, k, word);
+                    load_row(A, k + 2);
+                } else {
+                    site(B, k, word);
+                    load_row(B, k + 2);
+                }
+            } else {
+                site(A, k, word);
             }
         }
+        // (the last site's barrier orders this chunk's word reads before the next chunk's writes)
     }
     for (int i = lane; i < n; i += 64) {
         a.gS[off + i] = S[i];
@@ -2498,9 +2566,18 @@ __global__ void __launch_bounds__(64) chk_energy_kernel(ProblemView pv, const in
     const int8_t *sc = gS + static_cast<int64_t>(c) * n;
     const float *fc = gF + static_cast<int64_t>(c) * n;
     double acc = 0.0;
-    for (int i = lane; i < n; i += 64)
-        acc += static_cast<double>(sc[i]) * (static_cast<double>(pv.h32[i]) + static_cast<double>(fc[i]));
-    acc *= 0.5;
+    if (pv.kind != QA_KIND_HUBO) {
+        for (int i = lane; i < n; i += 64)
+            acc += static_cast<double>(sc[i]) * (static_cast<double>(pv.h32[i]) + static_cast<double>(fc[i]));
+        acc *= 0.5;
+    } else {  // direct sum, as group_tail
+        for (int i = lane; i < n; i += 64) acc += static_cast<double>(pv.h32[i]) * sc[i];
+        for (int t = lane; t < pv.nterms; t += 64) {
+            double prod = pv.t_coef32[t];
+            for (int q = pv.t_off[t]; q < pv.t_off[t + 1]; ++q) prod *= sc[pv.t_vars[q]];
+            acc += prod;
+        }
+    }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
     if (lane == 0) chain_e[c] = pv.c + acc;
 }
@@ -2540,54 +2617,78 @@ __global__ void __launch_bounds__(256) chk_tail_kernel(const double *chain_e, co
             best_state[static_cast<int64_t>(g) * n + i] = gS[(static_cast<int64_t>(r0) * P + arg) * n + i];
 }
 
-// Problems whose checkerboard sweeps run in chk_wave_kernel: dense up to this size, and, for the
-// production anneal only, sparse ones whose adjacency fits the wave's LDS. (The pilot scan runs
-// 64x more chains; a sparse wave's LDS footprint then leaves too few waves per CU, and the
-// two-barrier kernel's 64 chains per workgroup measured faster: MAX-CUT N=4096 529 ms vs 1718 ms.)
-constexpr int kChkWaveMaxN = 2048;
+// Problems whose checkerboard sweeps run in chk_wave_kernel: dense up to this size (the J rows are
+// loaded on demand past 2048 sites), sparse and HUBO problems whose wave state fits LDS. For the
+// pilot scan (64x more chains) sparse and HUBO coupling lists are read from global memory (L2)
+// instead of staged per wave, so that a wave's LDS stays small and many waves share a CU.
+constexpr int kChkWaveMaxN = 8192;
 // Dense J rows are loaded on demand (chk_wave_kernel<NQ, true>: only for accepted flips) rather
 // than streamed ahead of every site: the other waves in flight hide each load's latency and only
 // the accepted fraction of rows is read. Measured on SK at the C5 shape: pilot N=2048 243-354 ms
 // vs 503-597 ms streamed, N=512 13-17 vs 22-24 ms; a whole run_chi at N=2048 1.385 s vs 1.784 s.
-// QANNEAL_CHI_ROWS=stream selects the streamed rows.
+// QANNEAL_CHI_ROWS=stream selects the streamed rows (up to 2048 sites).
 bool chk_rows_on_demand() {
     const char *env = std::getenv("QANNEAL_CHI_ROWS");
     return !(env && env[0] == 's');
 }
 
+// LDS bytes of a wave's state plus, when staged, its coupling lists; 0 when lists are not staged.
+size_t chk_list_bytes_of(const qa_problem *p) {
+    return chk_list_bytes(p->kind, static_cast<int>(p->n), static_cast<int>(2 * p->nedges), static_cast<int>(p->nterms),
+                          static_cast<int>(p->t_vars.count));
+}
+
+bool chk_lists_staged(const qa_problem *p, bool pilot) {
+    const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4;
+    return !pilot && p->kind != QA_KIND_DENSE && chk_lds_bytes(n, rn) + chk_list_bytes_of(p) <= 160 * 1024;
+}
+
 bool chk_wave_applies(const qa_problem *p, bool pilot) {
     const char *env = std::getenv("QANNEAL_CHI_WAVE");
     if (env && env[0] == '0') return false;
-    const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4, nnz = static_cast<int>(2 * p->nedges);
+    const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4;
     if (p->kind == QA_KIND_DENSE) return n <= kChkWaveMaxN;
-    if (p->kind == QA_KIND_SPARSE) return !pilot && chk_lds_bytes(n, rn, nnz, true) <= 160 * 1024;
-    return false;
+    return chk_lds_bytes(n, rn) <= 160 * 1024;
 }
 
 // Launches the phases of one checkerboard sweep (sweep_id) of every chain; `demand`: dense rows
-// loaded only for accepted flips (chk_wave_kernel<NQ, true>).
-void chk_sweep(const qa_problem *p, ChkArgs &ca, int64_t chains, int P, hipStream_t s, bool demand = false) {
+// loaded only for accepted flips (chk_wave_kernel<NQ, true>); `pilot`: lists read from global memory.
+void chk_sweep(const qa_problem *p, ChkArgs &ca, int64_t chains, int P, hipStream_t s, bool demand, bool pilot) {
     const int n = static_cast<int>(p->n), rn = (n + 3) / 4 * 4, ngr = rn / 4;
     const int nnz = static_cast<int>(2 * p->nedges);
-    const bool sparse = p->kind == QA_KIND_SPARSE;
-    const void *kern = sparse ? reinterpret_cast<const void *>(chk_wave_kernel<0>)
+    const int kind = p->kind;
+    if (ngr > 512) demand = true;  // rows of more than 2048 floats are never streamed
+    QA_REQUIRE(kind != QA_KIND_DENSE || ngr <= 2048, "chk_wave_kernel: row too long");
+    const void *kern = kind == QA_KIND_SPARSE ? reinterpret_cast<const void *>(chk_wave_kernel<0>)
+                     : kind == QA_KIND_HUBO   ? reinterpret_cast<const void *>(chk_wave_kernel<-1>)
                      : demand ? (ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1, true>)
                                  : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2, true>)
                                  : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4, true>)
-                                              : reinterpret_cast<const void *>(chk_wave_kernel<8, true>))
+                                 : ngr <= 512 ? reinterpret_cast<const void *>(chk_wave_kernel<8, true>)
+                                 : ngr <= 1024 ? reinterpret_cast<const void *>(chk_wave_kernel<16, true>)
+                                               : reinterpret_cast<const void *>(chk_wave_kernel<32, true>))
                      : ngr <= 64 ? reinterpret_cast<const void *>(chk_wave_kernel<1>)
                      : ngr <= 128 ? reinterpret_cast<const void *>(chk_wave_kernel<2>)
                      : ngr <= 256 ? reinterpret_cast<const void *>(chk_wave_kernel<4>)
                                   : reinterpret_cast<const void *>(chk_wave_kernel<8>);
-    QA_REQUIRE(sparse || ngr <= 512, "chk_wave_kernel: row too long");
-    const size_t shmem = chk_lds_bytes(n, rn, nnz, sparse);
+    const bool staged = chk_lists_staged(p, pilot);
+    const size_t shmem = chk_lds_bytes(n, rn) + (staged ? chk_list_bytes_of(p) : 0);
+    QA_REQUIRE(shmem <= 160 * 1024, "chk_wave_kernel: wave state does not fit LDS");
     QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
     const ProblemView pv = view_of(p);
     ca.J4rows = pv.J32p;
     ca.adj_ptr = pv.adj_ptr;
     ca.adj_idx = pv.adj_idx;
     ca.adj_val = pv.adj_val32;
+    ca.inc_ptr = pv.inc_ptr;
+    ca.inc_idx = pv.inc_idx;
+    ca.t_off = pv.t_off;
+    ca.t_vars = pv.t_vars;
+    ca.t_coef = pv.t_coef32;
     ca.nnz = nnz;
+    ca.nterms = static_cast<int>(p->nterms);
+    ca.nv = static_cast<int>(p->t_vars.count);
+    ca.lists_lds = staged ? 1 : 0;
     ca.unique = p->adj_unique ? 1 : 0;
     ca.P = P;
     ca.n = n;
@@ -2678,7 +2779,7 @@ void sqa_fp32_pilot(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_t R, int6
         const bool demand = chk_rows_on_demand();
         for (int64_t sw = 0; sw < burn + sweeps; ++sw) {
             ca.sweep_id = static_cast<uint32_t>(sw);
-            chk_sweep(p, ca, NS * P, static_cast<int>(P), s, demand);
+            chk_sweep(p, ca, NS * P, static_cast<int>(P), s, demand, true);
             if (sw >= burn) {
                 hipLaunchKernelGGL(chk_mag_kernel, dim3(static_cast<unsigned>(NS)), dim3(256), 0, s, gS.ptr,
                                    static_cast<int>(P), static_cast<int>(n), dm2.ptr, dma.ptr);
@@ -2772,7 +2873,7 @@ void sqa_fp32_chi_production(qa_ctx *ctx, const qa_problem *p, int64_t P, int64_
             ca.pidx = static_cast<int>(k);
             for (int64_t sw = 0; sw < sweeps; ++sw) {
                 ca.sweep_id = static_cast<uint32_t>(k * sweeps + sw);
-                chk_sweep(p, ca, R * P, static_cast<int>(P), s, chk_rows_on_demand());
+                chk_sweep(p, ca, R * P, static_cast<int>(P), s, chk_rows_on_demand(), false);
             }
             hipLaunchKernelGGL(chk_energy_kernel, dim3(static_cast<unsigned>(R * P)), dim3(64), 0, s, a.pv, gS.ptr,
                                gF.ptr, ce.ptr);

@@ -360,6 +360,47 @@ def test_sqa_chi_production_wave_kernel_bit_identical(ctx, monkeypatch):
     monkeypatch.delenv("QANNEAL_CHI_WAVE", raising=False)
 
 
+def test_sqa_chi_wave_kernel_large_dense_hubo_and_sparse_pilot_bit_identical(ctx, monkeypatch):
+    """The wavefront checkerboard kernel past 2048 dense sites (rows loaded on demand, 16 and 32
+    float4 groups per lane), for HUBO problems (lists staged in LDS for production, read from
+    global memory for the pilot scan) and for the sparse pilot scan (lists from global memory):
+    bit-identical to the two-barrier kernel, and the returned energy is the oracle's FP64 energy."""
+    from qlatanneal_amd import abi
+    from oracle.orc import Oracle, Problem, ORACLE_SO
+    from tests.golden_io import consolidate_hubo
+    orc = Oracle(ORACLE_SO)
+    rng = np.random.default_rng(57)
+    kw = dict(beta=2.5, gamma_start=3.0, gamma_end=0.05, num_steps=4, sweeps_per_step=2, scan_points=4,
+              scan_sweeps=2, scan_burn=1, seed=61)
+    cases = []
+    for n, P, R in [(3001, 8, 2), (4096, 5, 2), (8192, 2, 1)]:
+        h, J = rng.normal(0, 0.2, n), _sk(rng, n)
+        cases.append((f"dense{n}", P, R, Problem.dense(h, J, 0.25), abi.DeviceProblem.dense(ctx, h, J, 0.25)))
+    for n, P, R in [(60, 7, 3), (200, 8, 2)]:
+        terms = _hubo_case(rng, n, 3 * n)
+        hh, off, var, co, dc = consolidate_hubo(n, terms)
+        cases.append((f"hubo{n}", P, R, Problem.hubo(n, terms, -0.25),
+                      abi.DeviceProblem.hubo(ctx, n, hh, off, var, co, -0.25 + dc)))
+    for name, P, R, po, pd in cases:
+        print("chi wave case", name, P, R, flush=True)
+        res = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_CHI_WAVE", v)
+            res.append(abi.sqa_chi_run(pd, P, R, **kw))
+        a, b = res
+        assert np.array_equal(a["best_state"], b["best_state"]), name
+        assert bits_equal([a["best_energy"]], [b["best_energy"]]), name
+        assert bits_equal(a["energy_trace"], b["energy_trace"]), name
+        assert bits_equal(a["scan_chi_B"], b["scan_chi_B"]), name
+        assert a["best_energy"] == orc.energy(po, a["best_state"])[0], name
+        pil = []
+        for v in ("1", "0"):
+            monkeypatch.setenv("QANNEAL_CHI_WAVE", v)
+            pil.append(abi.chi_pilot(pd, P, R, 2.0, 3.0, 1e-3, 8, 3, 2, seed=47))
+        assert bits_equal(pil[0]["scan_chi_B"], pil[1]["scan_chi_B"]), name
+    monkeypatch.delenv("QANNEAL_CHI_WAVE", raising=False)
+
+
 def test_sqa_fp32_distribution_vs_oracle(ctx):
     from scipy.stats import ks_2samp
     from qlatanneal_amd import abi
