@@ -9,7 +9,8 @@ information: a schedule linear in s).
 
 Pass criteria: the surrogate is used (no pilot fallback) on every instance; its mean best-energy
 gap to the pilot-scheduled run is at most 1 % (gap = (E_surrogate - E_pilot) / |E_pilot|); and its
-device time is at most 85 % of the pilot-scheduled runs' (the pilot is the removed cost). The flat
+device time is at most 90 % of the pilot-scheduled runs' (the pilot, the removed cost, is 12-20 %
+of run_chi at these sizes with the wavefront pilot kernel). The flat
 schedule's gap is reported beside it. Every returned energy is the FP64 energy of its state.
 """
 import os
@@ -60,4 +61,4 @@ def test_surrogate_schedule_matches_pilot_and_saves_the_pilot():
           f"{ms_p:.0f} ms, surrogate-scheduled {ms_s:.0f} ms ({1 - ms_s / ms_p:.1%} saved)", flush=True)
     assert all(used), "the surrogate fell back to the pilot on a held-out instance"
     assert g_sur.mean() <= 0.01
-    assert ms_s <= 0.85 * ms_p
+    assert ms_s <= 0.90 * ms_p
