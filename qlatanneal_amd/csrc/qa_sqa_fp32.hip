@@ -431,8 +431,13 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     // Dense: the flipped chains are taken eight at a time with a wave-uniform mask; every LDS
     // load of a batch (ring row value, field) is issued before any of its stores, so one batch
     // costs one LDS round trip instead of one per chain and column.
-    // One flip-mask word: chains cw .. cw + 63.
-    auto apply_word = [&](const unsigned long long m_all, const int cw) {
+    // One flip-mask word: chains cw .. cw + 63. skip (global-state slice sweeps): element SKIP[c]
+    // of a flipped chain's row keeps its value -- the chain's next decision has loaded it already
+    // and applies this flip to it itself (and writes it back).
+    int *SKIP = RSLOT;  // the row ring is never used with the state in global memory
+    // pre(): loads to issue right after the first batch's row loads (global-state slice sweeps:
+    // the next decisions' inputs), so both sets are in flight together; pre_done records it.
+    auto apply_word = [&](const unsigned long long m_all, const int cw, const bool skip, auto &&pre, bool &pre_done) {
         if (m_all == 0ull) return;
         if constexpr (!kLds && NJ > 0) {
             constexpr int B = NJ >= 8 ? 1 : 8 / NJ;
@@ -442,20 +447,28 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             const int n4 = n >> 2;
             f4 *F4g = reinterpret_cast<f4 *>(F);
             while (m) {
-                int cs[B];
+                // every load of the batch is issued unconditionally (an empty slot reads chain 0's
+                // row and J row 0, lanes past the row end reread its last group), so the batch's
+                // loads are in flight together with no wait between them; only the stores are masked
+                int cs[B], sk[B];
                 float dsv[B];
                 const f4 *rows[B];
+                const f4 *frow[B];
 #pragma unroll
                 for (int q = 0; q < B; ++q) {
                     cs[q] = -1;
+                    sk[q] = -1;
                     dsv[q] = 0.0f;
-                    rows[q] = nullptr;
+                    rows[q] = reinterpret_cast<const f4 *>(a.pv.J32);
+                    frow[q] = F4g;
                     if (m) {
                         const int c = cw + __ffsll(static_cast<long long>(m)) - 1;
                         m &= m - 1ull;
                         cs[q] = c;
                         dsv[q] = DS[c];
+                        if (skip) sk[q] = SKIP[c];
                         rows[q] = reinterpret_cast<const f4 *>(a.pv.J32 + static_cast<int64_t>(ROW[c]) * n);
+                        frow[q] = F4g + static_cast<int64_t>(c) * n4;
                     }
                 }
                 f4 rv[B][NJ], fv[B][NJ];
@@ -463,12 +476,14 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 for (int q = 0; q < B; ++q)
 #pragma unroll
                     for (int u = 0; u < NJ; ++u) {
-                        const int gi = tid + kThreads * u;
-                        if (cs[q] >= 0 && gi < n4) {
-                            rv[q][u] = rows[q][gi];
-                            fv[q][u] = F4g[static_cast<int64_t>(cs[q]) * n4 + gi];
-                        }
+                        const int gi = min(tid + kThreads * u, n4 - 1);
+                        rv[q][u] = rows[q][gi];
+                        fv[q][u] = frow[q][gi];
                     }
+                if (!pre_done) {
+                    pre();
+                    pre_done = true;
+                }
 #pragma unroll
                 for (int q = 0; q < B; ++q)
 #pragma unroll
@@ -476,9 +491,16 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                         const int gi = tid + kThreads * u;
                         if (cs[q] >= 0 && gi < n4) {
                             const f4 x = rv[q][u], y = fv[q][u];
-                            F4g[static_cast<int64_t>(cs[q]) * n4 + gi] =
-                                f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
-                                   fmaf(dsv[q], x.w, y.w)};
+                            f4 v = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                                      fmaf(dsv[q], x.w, y.w)};
+                            if (gi == (sk[q] >> 2)) {
+                                const int e = sk[q] & 3;
+                                v.x = e == 0 ? y.x : v.x;
+                                v.y = e == 1 ? y.y : v.y;
+                                v.z = e == 2 ? y.z : v.z;
+                                v.w = e == 3 ? y.w : v.w;
+                            }
+                            F4g[static_cast<int64_t>(cs[q]) * n4 + gi] = v;
                         }
                     }
             }
@@ -545,8 +567,16 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     };
     // Apply the field updates of every flagged chain: one mask word per deciding wavefront.
     const int nwords = a.cmax >> 6;
+    auto no_pre = []() {};
     auto apply_updates = [&]() {
-        for (int w = 0; w < nwords; ++w) apply_word(flipmask[w], w << 6);
+        bool done = true;
+        for (int w = 0; w < nwords; ++w) apply_word(flipmask[w], w << 6, false, no_pre, done);
+    };
+    // with skips, and pre() issued after the first batch's loads (or at the end when nothing flipped)
+    auto apply_updates_pre = [&](auto &&pre) {
+        bool done = false;
+        for (int w = 0; w < nwords; ++w) apply_word(flipmask[w], w << 6, true, pre, done);
+        if (!done) pre();
     };
 
     // Random permutation of 0..n-1 for this sweep (Fisher-Yates on one lane, Philox draws).
@@ -703,22 +733,24 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             Philox4 rnd{0, 0, 0, 0};
             const int nsteps = n + P - 1;
             if constexpr (!kLds && NJ > 0) {
-                // State in global memory: each chain loads the inputs of its next decision one
-                // lockstep step ahead (field, own and neighbour spins, and J[i_k][i_k+1] for its own
-                // flip), so a decision reads registers only and the loads' latency overlaps the
-                // field updates. The prefetched field lacks the chain's own flip at position k; the
-                // decision adds it with the fmaf the update applies to that element. The spin of the
-                // previous slice at the next site is decided in this very step (by lane c - 1), so it
-                // comes from that lane's decision through a shuffle instead of memory.
+                // State in global memory: each chain's next decision inputs (field, own and
+                // neighbour spins, and J[i_k][i_k+1] for its own flip) are loaded right after this
+                // step's decisions, while the field updates run, so the two latencies overlap and a
+                // decision reads registers only. The loads see every decision of this step (the
+                // previous slice's spin at the next site included) and the updates of earlier steps;
+                // this step's update of the chain's own row skips the loaded element, and the next
+                // decision adds that flip itself -- the fmaf the update would apply -- and writes the
+                // element back, so every field element gets the same fmaf sequence as in LDS.
                 float pf_f = 0.0f, pf_j = 0.0f, ds_prev = 0.0f;
-                int pf_s = 0, pf_sp = 0, pf_sn = 0, s_left = 0;
+                int pf_s = 0, pf_sp = 0, pf_sn = 0;
                 bool fl_prev = false;
+                int nx_i = 0, nx_i1 = -1;  // this step's site and the next one, for the loads
                 for (int tau = 0; tau < nsteps; ++tau) {
                     if (tid < a.cmax) {
                         const int k = tau - t_me;
                         const bool active = decider && k >= 0 && k < n;
                         bool acc = false;
-                        int snew = 0;
+                        nx_i1 = -1;
                         if ((tau & 3) == 0)
                             rnd = philox4x32_10(cbase + c_me, sweep_id, static_cast<uint32_t>(tau >> 2), TAG_SLICE,
                                                 a.k0, a.k1);
@@ -735,10 +767,9 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                             } else {
                                 s = pf_s;
                                 f = fl_prev ? fmaf(ds_prev, pf_j, pf_f) : pf_f;
-                                // (lane 0 of a later wavefront: lane c - 1 is in the previous
-                                // wavefront, so its spin, written last step, comes from memory)
-                                sp = t_me >= 1 ? (lane == 0 ? S[static_cast<int64_t>(cprev) * n + i] : s_left) : pf_sp;
-                                sn = (P == 2 && t_me == 1) ? s_left : pf_sn;
+                                if (fl_prev) F[static_cast<int64_t>(c_me) * ldf + i] = f;
+                                sp = pf_sp;
+                                sn = pf_sn;
                             }
                             const float fs = static_cast<float>(s);
                             const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(sp + sn));
@@ -748,25 +779,31 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                             DS[c_me] = -2.0f * fs;
                             ROW[c_me] = i;
                             if (acc) sc[i] = static_cast<int8_t>(-s);
-                            snew = acc ? -s : s;
                             ds_prev = -2.0f * fs;
                             fl_prev = acc;
                             if (k + 1 < n) {
-                                const int i1 = order[k + 1];
-                                pf_f = F[static_cast<int64_t>(c_me) * ldf + i1];
-                                pf_j = a.pv.J32[static_cast<int64_t>(i) * n + i1];
-                                pf_s = sc[i1];
-                                pf_sp = S[static_cast<int64_t>(cprev) * n + i1];
-                                pf_sn = S[static_cast<int64_t>(cnext) * n + i1];
+                                nx_i = i;
+                                nx_i1 = order[k + 1];
                             }
+                            SKIP[c_me] = nx_i1;
                         }
-                        s_left = __shfl_up(snew, 1);  // lane c - 1's spin at lane c's next site
                         const unsigned long long m = __ballot(acc);
                         if (lane == 0) flipmask[wave] = m;
                         nacc += acc ? 1ull : 0ull;
+                        // the field write-back and the spin store complete before the barrier: other
+                        // wavefronts load those words in the update phase
+                        __builtin_amdgcn_s_waitcnt(0);
                     }
                     __syncthreads();
-                    apply_updates();
+                    apply_updates_pre([&]() {
+                        if (tid < a.cmax && nx_i1 >= 0) {
+                            pf_f = F[static_cast<int64_t>(c_me) * ldf + nx_i1];
+                            pf_j = a.pv.J32[static_cast<int64_t>(nx_i) * n + nx_i1];
+                            pf_s = S[static_cast<int64_t>(c_me) * n + nx_i1];
+                            pf_sp = S[static_cast<int64_t>(cprev) * n + nx_i1];
+                            pf_sn = S[static_cast<int64_t>(cnext) * n + nx_i1];
+                        }
+                    });
                     __syncthreads();
                 }
                 if (a.bstat) bond_sample(true);
