@@ -440,36 +440,30 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     auto apply_word = [&](const unsigned long long m_all, const int cw, const bool skip, auto &&pre, bool &pre_done) {
         if (m_all == 0ull) return;
         if constexpr (!kLds && NJ > 0) {
-            constexpr int B = NJ >= 8 ? 1 : 8 / NJ;
+            constexpr int BMAX = NJ >= 8 ? 1 : 8 / NJ;
             uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
             uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
             unsigned long long m = (static_cast<unsigned long long>(mhi) << 32) | mlo;
             const int n4 = n >> 2;
             f4 *F4g = reinterpret_cast<f4 *>(F);
-            while (m) {
-                // every load of the batch is issued unconditionally (an empty slot reads chain 0's
-                // row and J row 0, lanes past the row end reread its last group), so the batch's
-                // loads are in flight together with no wait between them; only the stores are masked
+            // batches of exactly BMAX, ..., 2 or 1 flipped chains: every load of a batch is issued
+            // unconditionally (lanes past the row end reread its last group), so the batch's loads
+            // are in flight together with no wait between them; only the stores are masked
+            auto batch = [&](auto bsz) {
+                constexpr int B = decltype(bsz)::value;
                 int cs[B], sk[B];
                 float dsv[B];
                 const f4 *rows[B];
                 const f4 *frow[B];
 #pragma unroll
                 for (int q = 0; q < B; ++q) {
-                    cs[q] = -1;
-                    sk[q] = -1;
-                    dsv[q] = 0.0f;
-                    rows[q] = reinterpret_cast<const f4 *>(a.pv.J32);
-                    frow[q] = F4g;
-                    if (m) {
-                        const int c = cw + __ffsll(static_cast<long long>(m)) - 1;
-                        m &= m - 1ull;
-                        cs[q] = c;
-                        dsv[q] = DS[c];
-                        if (skip) sk[q] = SKIP[c];
-                        rows[q] = reinterpret_cast<const f4 *>(a.pv.J32 + static_cast<int64_t>(ROW[c]) * n);
-                        frow[q] = F4g + static_cast<int64_t>(c) * n4;
-                    }
+                    const int c = cw + __ffsll(static_cast<long long>(m)) - 1;
+                    m &= m - 1ull;
+                    cs[q] = c;
+                    dsv[q] = DS[c];
+                    sk[q] = skip ? SKIP[c] : -1;
+                    rows[q] = reinterpret_cast<const f4 *>(a.pv.J32 + static_cast<int64_t>(ROW[c]) * n);
+                    frow[q] = F4g + static_cast<int64_t>(c) * n4;
                 }
                 f4 rv[B][NJ], fv[B][NJ];
 #pragma unroll
@@ -489,7 +483,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 #pragma unroll
                     for (int u = 0; u < NJ; ++u) {
                         const int gi = tid + kThreads * u;
-                        if (cs[q] >= 0 && gi < n4) {
+                        if (gi < n4) {
                             const f4 x = rv[q][u], y = fv[q][u];
                             f4 v = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
                                       fmaf(dsv[q], x.w, y.w)};
@@ -503,6 +497,13 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                             F4g[static_cast<int64_t>(cs[q]) * n4 + gi] = v;
                         }
                     }
+            };
+            while (m) {
+                const int cnt = __popcll(m);
+                if (BMAX >= 8 && cnt >= 8) batch(std::integral_constant<int, (BMAX >= 8 ? 8 : 1)>{});
+                else if (BMAX >= 4 && cnt >= 4) batch(std::integral_constant<int, (BMAX >= 4 ? 4 : 1)>{});
+                else if (BMAX >= 2 && cnt >= 2) batch(std::integral_constant<int, (BMAX >= 2 ? 2 : 1)>{});
+                else batch(std::integral_constant<int, 1>{});
             }
             return;
         }
@@ -510,39 +511,42 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
             uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
             unsigned long long m = (static_cast<unsigned long long>(mhi) << 32) | mlo;
-            while (m) {
-                int cs[8];
+            // batches of exactly 8, 4, 2 or 1 flipped chains: every load of a column is issued
+            // unconditionally, and the J source is one address space per loop, so the loads of a
+            // column are in flight together
+            auto batch = [&](auto bsz) {
+                constexpr int B = decltype(bsz)::value;
+                float dsv[B];
+                int64_t roff[B], foff[B];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    if (m) {
-                        cs[q] = cw + __ffsll(static_cast<long long>(m)) - 1;
-                        m &= m - 1ull;
-                    } else {
-                        cs[q] = -1;
-                    }
+                for (int q = 0; q < B; ++q) {
+                    const int c = cw + __ffsll(static_cast<long long>(m)) - 1;
+                    m &= m - 1ull;
+                    dsv[q] = DS[c];
+                    roff[q] = static_cast<int64_t>(a.ring ? RSLOT[c] : ROW[c]) * n;
+                    foff[q] = static_cast<int64_t>(c) * ldf;
                 }
-                float dsv[8];
-                const float *rows[8];
+                auto columns = [&](const float *src) {
+                    for (int j = tid; j < n; j += kThreads) {
+                        float rv[B], fv[B];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    if (cs[q] >= 0) {
-                        dsv[q] = DS[cs[q]];
-                        rows[q] = a.ring ? Jring + static_cast<int64_t>(RSLOT[cs[q]]) * n
-                                         : a.pv.J32 + static_cast<int64_t>(ROW[cs[q]]) * n;
-                    }
-                }
-                for (int j = tid; j < n; j += kThreads) {
-                    float rv[8], fv[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (cs[q] >= 0) {
-                            rv[q] = rows[q][j];
-                            fv[q] = F[static_cast<int64_t>(cs[q]) * ldf + j];
+                        for (int q = 0; q < B; ++q) {
+                            rv[q] = src[roff[q] + j];
+                            fv[q] = F[foff[q] + j];
                         }
 #pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (cs[q] >= 0) F[static_cast<int64_t>(cs[q]) * ldf + j] = fmaf(dsv[q], rv[q], fv[q]);
-                }
+                        for (int q = 0; q < B; ++q) F[foff[q] + j] = fmaf(dsv[q], rv[q], fv[q]);
+                    }
+                };
+                if (a.ring) columns(Jring);
+                else columns(a.pv.J32);
+            };
+            while (m) {
+                const int cnt = __popcll(m);
+                if (cnt >= 8) batch(std::integral_constant<int, 8>{});
+                else if (cnt >= 4) batch(std::integral_constant<int, 4>{});
+                else if (cnt >= 2) batch(std::integral_constant<int, 2>{});
+                else batch(std::integral_constant<int, 1>{});
             }
         } else if (tid >= cw && tid < cw + 64 && tid < G && ((m_all >> (tid - cw)) & 1ull)) {
             const int c = tid, i = ROW[c];
