@@ -108,6 +108,7 @@ struct SqaArgs {
     int hubo_nv;           // HUBO: entries of t_vars (= of inc_idx)
     int adj_lds;           // sparse: adjacency (CSR) staged in LDS after the scratch
     int cmax;              // per-chain scratch entries: 64, or G rounded up to 64 when P > 64
+    int s_lds;             // global-state fields with the spins in LDS (dense float4 kernel)
 };
 
 
@@ -210,6 +211,10 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
         p += lds_align16(static_cast<size_t>(nnz) * 4);
         L.adj_val = reinterpret_cast<const float *>(p);
     }
+    if (!lds_state && a.s_lds) {  // the plan placed them here, after everything above
+        L.S = reinterpret_cast<int8_t *>(p);
+        p += lds_align16(static_cast<size_t>(a.rg) * a.P * n);
+    }
     if (a.hubo_lds) {  // the plan placed them here, after everything above
         L.inc_ptr = reinterpret_cast<const int32_t *>(p);
         p += lds_align16(static_cast<size_t>(n + 1) * 4);
@@ -274,6 +279,8 @@ __device__ void group_init(const SqaArgs &a, float *F, int8_t *S, int ldf, int G
                 S[e] = a.gS[static_cast<int64_t>(r0) * P * n + e];
                 F[static_cast<int64_t>(c) * ldf + i] = a.gF[static_cast<int64_t>(r0) * P * n + e];
             }
+        } else if (a.s_lds) {  // fields stay in global memory, spins come into LDS
+            for (int e = tid; e < G * n; e += nt) S[e] = a.gS[static_cast<int64_t>(r0) * P * n + e];
         }
     } else {
         for (int e = tid; e < G * n; e += nt) {
@@ -389,7 +396,7 @@ __device__ void group_save(const SqaArgs &a, const float *F, const int8_t *S, in
 // NJ > 0 (global state, dense, n % 4 == 0, n <= 1024 NJ): field updates in float4 groups, NJ per
 // thread, with every load of a batch of flipped chains issued before any of its stores, so a batch
 // costs one round trip to L2/MALL instead of one per column stride (same fmaf per element).
-template <bool kLds, int NJ = 0>
+template <bool kLds, int NJ = 0, bool kSL = false>
 __device__ void general_group(const SqaArgs &a, const int g) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x;
@@ -401,7 +408,8 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 
     const SqaLds L = carve_lds(a, smem, kLds);
     float *F = kLds ? L.F : a.gF + static_cast<int64_t>(r0) * P * n;
-    int8_t *S = kLds ? L.S : a.gS + static_cast<int64_t>(r0) * P * n;
+    // kSL: fields in global memory, spins in LDS (copied back at the end)
+    int8_t *S = (kLds || kSL) ? L.S : a.gS + static_cast<int64_t>(r0) * P * n;
     int *order = L.order;
     float *Jring = L.Jring;
     int *RSLOT = L.RSLOT;
@@ -434,7 +442,8 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     // One flip-mask word: chains cw .. cw + 63. skip (global-state slice sweeps): element SKIP[c]
     // of a flipped chain's row keeps its value -- the chain's next decision has loaded it already
     // and applies this flip to it itself (and writes it back).
-    int *SKIP = RSLOT;  // the row ring is never used with the state in global memory
+    int *SKIP = RSLOT;   // the row ring is never used with the state in global memory
+    int *SKIP2 = rflag;  // (worldline decisions; free during slice sweeps): kSL write-back element
     // pre(): loads to issue right after the first batch's row loads (global-state slice sweeps:
     // the next decisions' inputs), so both sets are in flight together; pre_done records it.
     auto apply_word = [&](const unsigned long long m_all, const int cw, const bool skip, auto &&pre, bool &pre_done) {
@@ -451,7 +460,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
             // are in flight together with no wait between them; only the stores are masked
             auto batch = [&](auto bsz) {
                 constexpr int B = decltype(bsz)::value;
-                int cs[B], sk[B];
+                int cs[B], sk[B], sk2[B];
                 float dsv[B];
                 const f4 *rows[B];
                 const f4 *frow[B];
@@ -462,6 +471,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                     cs[q] = c;
                     dsv[q] = DS[c];
                     sk[q] = skip ? SKIP[c] : -1;
+                    sk2[q] = (kSL && skip) ? SKIP2[c] : -1;
                     rows[q] = reinterpret_cast<const f4 *>(a.pv.J32 + static_cast<int64_t>(ROW[c]) * n);
                     frow[q] = F4g + static_cast<int64_t>(c) * n4;
                 }
@@ -485,16 +495,19 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                         const int gi = tid + kThreads * u;
                         if (gi < n4) {
                             const f4 x = rv[q][u], y = fv[q][u];
-                            f4 v = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
-                                      fmaf(dsv[q], x.w, y.w)};
-                            if (gi == (sk[q] >> 2)) {
-                                const int e = sk[q] & 3;
-                                v.x = e == 0 ? y.x : v.x;
-                                v.y = e == 1 ? y.y : v.y;
-                                v.z = e == 2 ? y.z : v.z;
-                                v.w = e == 3 ? y.w : v.w;
+                            const f4 v = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                                            fmaf(dsv[q], x.w, y.w)};
+                            if (gi == (sk[q] >> 2) || gi == (sk2[q] >> 2)) {
+                                // a group holding a skipped element: the other elements one by one
+                                float *d = reinterpret_cast<float *>(F4g + static_cast<int64_t>(cs[q]) * n4 + gi);
+                                const int b0 = 4 * gi;
+                                if (b0 != sk[q] && b0 != sk2[q]) d[0] = v.x;
+                                if (b0 + 1 != sk[q] && b0 + 1 != sk2[q]) d[1] = v.y;
+                                if (b0 + 2 != sk[q] && b0 + 2 != sk2[q]) d[2] = v.z;
+                                if (b0 + 3 != sk[q] && b0 + 3 != sk2[q]) d[3] = v.w;
+                            } else {
+                                F4g[static_cast<int64_t>(cs[q]) * n4 + gi] = v;
                             }
-                            F4g[static_cast<int64_t>(cs[q]) * n4 + gi] = v;
                         }
                     }
             };
@@ -745,16 +758,24 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                 // this step's update of the chain's own row skips the loaded element, and the next
                 // decision adds that flip itself -- the fmaf the update would apply -- and writes the
                 // element back, so every field element gets the same fmaf sequence as in LDS.
+                // With the spins in LDS (kSL) the decision reads them there, makes no global access
+                // and leaves the element write-back to the update phase: there the chain's own row
+                // update skips both the element just decided (written back with this step's flip,
+                // J_ii = 0, applied by the same fmaf) and the element loaded for the next decision.
                 float pf_f = 0.0f, pf_j = 0.0f, ds_prev = 0.0f;
                 int pf_s = 0, pf_sp = 0, pf_sn = 0;
                 bool fl_prev = false;
                 int nx_i = 0, nx_i1 = -1;  // this step's site and the next one, for the loads
+                bool wb = false;           // kSL: write wb_v to element wb_i of the chain's row
+                int wb_i = 0;
+                float wb_v = 0.0f;
                 for (int tau = 0; tau < nsteps; ++tau) {
                     if (tid < a.cmax) {
                         const int k = tau - t_me;
                         const bool active = decider && k >= 0 && k < n;
                         bool acc = false;
                         nx_i1 = -1;
+                        wb = false;
                         if ((tau & 3) == 0)
                             rnd = philox4x32_10(cbase + c_me, sweep_id, static_cast<uint32_t>(tau >> 2), TAG_SLICE,
                                                 a.k0, a.k1);
@@ -766,6 +787,11 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                             if (k == 0) {
                                 s = sc[i];
                                 f = F[static_cast<int64_t>(c_me) * ldf + i];
+                                sp = S[static_cast<int64_t>(cprev) * n + i];
+                                sn = S[static_cast<int64_t>(cnext) * n + i];
+                            } else if constexpr (kSL) {
+                                s = sc[i];
+                                f = fl_prev ? fmaf(ds_prev, pf_j, pf_f) : pf_f;
                                 sp = S[static_cast<int64_t>(cprev) * n + i];
                                 sn = S[static_cast<int64_t>(cnext) * n + i];
                             } else {
@@ -783,6 +809,7 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                             DS[c_me] = -2.0f * fs;
                             ROW[c_me] = i;
                             if (acc) sc[i] = static_cast<int8_t>(-s);
+                            const bool fl_prev_in = fl_prev;
                             ds_prev = -2.0f * fs;
                             fl_prev = acc;
                             if (k + 1 < n) {
@@ -790,22 +817,35 @@ __device__ void general_group(const SqaArgs &a, const int g) {
                                 nx_i1 = order[k + 1];
                             }
                             SKIP[c_me] = nx_i1;
+                            if constexpr (kSL) {
+                                SKIP2[c_me] = i;
+                                wb = fl_prev_in || acc;
+                                wb_i = i;
+                                wb_v = acc ? fmaf(-2.0f * fs, 0.0f, f) : f;
+                            }
                         }
                         const unsigned long long m = __ballot(acc);
                         if (lane == 0) flipmask[wave] = m;
                         nacc += acc ? 1ull : 0ull;
-                        // the field write-back and the spin store complete before the barrier: other
-                        // wavefronts load those words in the update phase
-                        __builtin_amdgcn_s_waitcnt(0);
+                        // (spins in global memory) the field write-back and the spin store complete
+                        // before the barrier: other wavefronts load those words in the update phase
+                        if constexpr (!kSL) __builtin_amdgcn_s_waitcnt(0);
                     }
                     __syncthreads();
                     apply_updates_pre([&]() {
-                        if (tid < a.cmax && nx_i1 >= 0) {
-                            pf_f = F[static_cast<int64_t>(c_me) * ldf + nx_i1];
-                            pf_j = a.pv.J32[static_cast<int64_t>(nx_i) * n + nx_i1];
-                            pf_s = S[static_cast<int64_t>(c_me) * n + nx_i1];
-                            pf_sp = S[static_cast<int64_t>(cprev) * n + nx_i1];
-                            pf_sn = S[static_cast<int64_t>(cnext) * n + nx_i1];
+                        if (tid < a.cmax) {
+                            if constexpr (kSL) {
+                                if (wb) F[static_cast<int64_t>(c_me) * ldf + wb_i] = wb_v;
+                            }
+                            if (nx_i1 >= 0) {
+                                pf_f = F[static_cast<int64_t>(c_me) * ldf + nx_i1];
+                                pf_j = a.pv.J32[static_cast<int64_t>(nx_i) * n + nx_i1];
+                                if constexpr (!kSL) {
+                                    pf_s = S[static_cast<int64_t>(c_me) * n + nx_i1];
+                                    pf_sp = S[static_cast<int64_t>(cprev) * n + nx_i1];
+                                    pf_sn = S[static_cast<int64_t>(cnext) * n + nx_i1];
+                                }
+                            }
                         }
                     });
                     __syncthreads();
@@ -1011,6 +1051,10 @@ __device__ void general_group(const SqaArgs &a, const int g) {
     }
     // final spins out (for the caller's state readback / diagnostics)
     if constexpr (kLds) group_save(a, F, S, ldf, G, r0);
+    if constexpr (kSL) {
+        __syncthreads();
+        for (int e = tid; e < G * n; e += kThreads) a.gS[static_cast<int64_t>(r0) * P * n + e] = S[e];
+    }
     if (a.cmax == kMaxChains) {
         if (tid < kMaxChains) {
             for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
@@ -1032,9 +1076,9 @@ __device__ void general_group(const SqaArgs &a, const int g) {
 }
 
 
-template <bool kLds, int NJ = 0>
+template <bool kLds, int NJ = 0, bool kSL = false>
 __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
-    general_group<kLds, NJ>(a, static_cast<int>(blockIdx.x));
+    general_group<kLds, NJ, kSL>(a, static_cast<int>(blockIdx.x));
 }
 
 // ---- pipelined slice sweeps (dense couplings, LDS-resident state) ---------------------------
@@ -1459,6 +1503,7 @@ struct SqaPlan {
     int ring = 0, pfd = 1, ldf = 0, ldr = 0, pipe = 0;
     int hubo_lds = 0, hubo_nv = 0, adj_lds = 0;
     int cmax = kMaxChains;
+    int s_lds = 0;  // global-state fields with the spins in LDS (dense float4 kernel)
     size_t shmem = 0;
 };
 
@@ -1529,6 +1574,18 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
             pl.shmem += sb;
         }
     }
+    // global-state dense problems whose spins fit LDS beside the scratch keep them there (the
+    // float4 kernel; QANNEAL_SQA_SPINS_LDS=0 keeps them in global memory: same results bit for bit)
+    {
+        const char *f4env = std::getenv("QANNEAL_SQA_GLOBAL_F4");
+        const char *slenv = std::getenv("QANNEAL_SQA_SPINS_LDS");
+        const size_t sb = lds_align16(static_cast<size_t>(pl.rg * P * n));
+        if (!pl.use_lds && p->kind == QA_KIND_DENSE && n % 4 == 0 && n <= 8192 && !(f4env && f4env[0] == '0') &&
+            !(slenv && slenv[0] == '0') && pl.shmem + sb <= lds_cap) {
+            pl.s_lds = 1;
+            pl.shmem += sb;
+        }
+    }
     pl.ldf = static_cast<int>(pl.use_lds ? rn : n);
     pl.ldr = static_cast<int>(rn);
     return pl;
@@ -1540,6 +1597,7 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     a.hubo_nv = pl.hubo_nv;
     a.adj_lds = pl.adj_lds;
     a.cmax = pl.cmax;
+    a.s_lds = pl.s_lds;
     a.ngroups = static_cast<int>(pl.ngroups);
     a.ldf = pl.ldf;
     a.ldr = pl.ldr;
@@ -1563,10 +1621,16 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
         const char *env = std::getenv("QANNEAL_SQA_GLOBAL_F4");
         const bool f4ok = !pl.use_lds && a.pv.kind == QA_KIND_DENSE && n % 4 == 0 && n <= 8192 &&
                           !(env && env[0] == '0');
+        QA_REQUIRE(!a.s_lds || f4ok, "spins in LDS need the float4 global-state kernel");
         if (pl.use_lds)
             kern = reinterpret_cast<const void *>(sqa_fp32_kernel<true>);
         else if (!f4ok)
             kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false>);
+        else if (a.s_lds)
+            kern = n <= 1024 ? reinterpret_cast<const void *>(sqa_fp32_kernel<false, 1, true>)
+                 : n <= 2048 ? reinterpret_cast<const void *>(sqa_fp32_kernel<false, 2, true>)
+                 : n <= 4096 ? reinterpret_cast<const void *>(sqa_fp32_kernel<false, 4, true>)
+                             : reinterpret_cast<const void *>(sqa_fp32_kernel<false, 8, true>);
         else if (n <= 1024)
             kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false, 1>);
         else if (n <= 2048)

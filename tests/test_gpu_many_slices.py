@@ -63,9 +63,10 @@ def test_many_slices_state_paths_bit_identical(ctx, monkeypatch):
         dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), _sk(rng, n), 0.25)
         betas, gammas = np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4)
         res = []
-        for state, f4 in (("lds", "1"), ("global", "1"), ("global", "0")):
+        for state, f4, sl in (("lds", "1", "1"), ("global", "1", "1"), ("global", "1", "0"), ("global", "0", "1")):
             monkeypatch.setenv("QANNEAL_SQA_STATE", state)
             monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", f4)
+            monkeypatch.setenv("QANNEAL_SQA_SPINS_LDS", sl)
             res.append(abi.sqa_run(dp, betas, gammas, P, R, 2, wl, cl, abi.MODE_FP32, 31))
         for r in res[1:]:
             assert r["accept_rate"] == res[0]["accept_rate"], (n, P, R)
@@ -74,6 +75,7 @@ def test_many_slices_state_paths_bit_identical(ctx, monkeypatch):
             assert bits_equal(r["energy_trace"], res[0]["energy_trace"]), (n, P, R)
     monkeypatch.delenv("QANNEAL_SQA_STATE", raising=False)
     monkeypatch.delenv("QANNEAL_SQA_GLOBAL_F4", raising=False)
+    monkeypatch.delenv("QANNEAL_SQA_SPINS_LDS", raising=False)
 
 
 def test_many_slices_distribution_vs_oracle(ctx):

@@ -142,9 +142,10 @@ def test_sqa_fp32_global_state_prefetch_bit_identical(ctx, monkeypatch):
         dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), _sk(rng, n), 0.25)
         betas, gammas = np.geomspace(0.3, 3.0, 5), np.geomspace(3.0, 0.05, 5)
         res = []
-        for state, f4 in (("lds", "1"), ("global", "1"), ("global", "0")):
+        for state, f4, sl in (("lds", "1", "1"), ("global", "1", "1"), ("global", "1", "0"), ("global", "0", "1")):
             monkeypatch.setenv("QANNEAL_SQA_STATE", state)
             monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", f4)
+            monkeypatch.setenv("QANNEAL_SQA_SPINS_LDS", sl)
             res.append(abi.sqa_run(dp, betas, gammas, P, R, 2, 0, 0, abi.MODE_FP32, 23))
         for r in res[1:]:
             assert r["accept_rate"] == res[0]["accept_rate"], (n, P, R)
@@ -154,17 +155,20 @@ def test_sqa_fp32_global_state_prefetch_bit_identical(ctx, monkeypatch):
     n, P = 512, 32
     dp = abi.DeviceProblem.dense(ctx, np.zeros(n), _sk(rng, n))
     res = []
-    for state in ("lds", "global"):
+    for state, sl in (("lds", "1"), ("global", "1"), ("global", "0")):
         monkeypatch.setenv("QANNEAL_SQA_STATE", state)
         monkeypatch.setenv("QANNEAL_SQA_GLOBAL_F4", "1")
+        monkeypatch.setenv("QANNEAL_SQA_SPINS_LDS", sl)
         res.append(abi.sqapt_run(dp, np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4), P, 3, 0, 4, 1, 0,
                                  ladders=3, mode=abi.MODE_FP32, seed=6))
-    for key in ("final_states", "best_state"):
-        assert np.array_equal(res[0][key], res[1][key]), key
-    for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
-        assert bits_equal(res[0][key].ravel(), res[1][key].ravel()), key
+    for r in res[1:]:
+        for key in ("final_states", "best_state"):
+            assert np.array_equal(res[0][key], r[key]), key
+        for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
+            assert bits_equal(res[0][key].ravel(), r[key].ravel()), key
     monkeypatch.delenv("QANNEAL_SQA_STATE", raising=False)
     monkeypatch.delenv("QANNEAL_SQA_GLOBAL_F4", raising=False)
+    monkeypatch.delenv("QANNEAL_SQA_SPINS_LDS", raising=False)
 
 
 def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
