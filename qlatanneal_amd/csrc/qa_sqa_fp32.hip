@@ -1380,9 +1380,10 @@ __global__ void reset_counters_kernel(int *c, int count) {
 template <int V>
 __global__ void __launch_bounds__(kThreads) sqa_batch_kernel(const SqaArgs *__restrict__ inst,
                                                              const BatchItem *__restrict__ items, int nitems,
-                                                             int *counter) {
+                                                             int *counter, int *done) {
     __shared__ int s_item;
-    for (;;) {
+    // at most nitems + 1 rounds: every round takes a fresh index from the counter
+    for (int round = 0; round <= nitems; ++round) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
         __syncthreads();
         const int it = s_item;
@@ -1392,6 +1393,7 @@ __global__ void __launch_bounds__(kThreads) sqa_batch_kernel(const SqaArgs *__re
         if constexpr (V > 0) pipe_group<V>(inst[b.inst], b.group);
         else if constexpr (V == 0) general_group<true>(inst[b.inst], b.group);
         else general_group<false>(inst[b.inst], b.group);
+        if (threadIdx.x == 0) atomicAdd(done, 1);  // items completed, checked by the host
     }
 }
 
@@ -1591,8 +1593,9 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     return pl;
 }
 
-void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
-    SqaArgs a = args;
+// The plan-derived fields of a launch (replica groups, LDS carve-up, row ring, staged lists): every
+// launch, the batched one included, takes them from here.
+void apply_plan(SqaArgs &a, const SqaPlan &pl) {
     a.hubo_lds = pl.hubo_lds;
     a.hubo_nv = pl.hubo_nv;
     a.adj_lds = pl.adj_lds;
@@ -1603,6 +1606,11 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     a.ldr = pl.ldr;
     a.ring = pl.ring;
     a.pfd = pl.pfd;
+}
+
+void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
+    SqaArgs a = args;
+    apply_plan(a, pl);
     // the pipelined kernel runs plain slice sweeps over dense couplings with an LDS state
     const bool pipe = pl.pipe && a.cmax == kMaxChains && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 &&
                       a.pv.kind == QA_KIND_DENSE && a.bstat == nullptr;
@@ -3195,12 +3203,8 @@ void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const
         a.best_chain = bchain.ptr + x.oG;
         a.best_state = bstate.ptr + x.oB;
         a.accepted = accd.ptr + x.oG;
-        a.ngroups = static_cast<int>(x.pl.ngroups);
-        a.ldf = x.pl.ldf;
-        a.ldr = x.pl.ldr;
-        a.ring = x.pl.ring;
-        a.pfd = x.pl.pfd;
-        a.cmax = x.pl.cmax;
+        apply_plan(a, x.pl);
+        a.s_lds = 0;  // the batch kernel's global-state body keeps the spins in global memory
         const double cost = static_cast<double>(x.pl.rg * P * x.n) * (probs[k]->kind == QA_KIND_DENSE ? x.n : 16);
         for (int64_t gi = 0; gi < x.pl.ngroups; ++gi)
             work[x.variant].push_back({cost, BatchItem{static_cast<int>(k), static_cast<int>(gi)}});
@@ -3229,17 +3233,20 @@ void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const
     }
     DevBuf<BatchItem> ditems(items.size());
     ditems.upload(items.data(), items.size(), s);
-    DevBuf<int> counters(launches.size());
+    // [0, L): item counters of the L launches; [L]: items completed over all launches
+    DevBuf<int> counters(launches.size() + 1);
 
     auto enqueue = [&]() {
-        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters.ptr, static_cast<int>(launches.size()));
+        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters.ptr,
+                           static_cast<int>(launches.size() + 1));
         QA_HIP(hipGetLastError());
         for (size_t li = 0; li < launches.size(); ++li) {
             const SqaArgs *pa = dargs.ptr;
             const BatchItem *pi = ditems.ptr + launches[li].first;
             int ni = static_cast<int>(launches[li].n);
             int *pc = counters.ptr + li;
-            void *params[] = {&pa, &pi, &ni, &pc};
+            int *pd = counters.ptr + launches.size();
+            void *params[] = {&pa, &pi, &ni, &pc, &pd};
             QA_HIP(hipLaunchKernel(launches[li].kern, dim3(launches[li].grid), dim3(kThreads), params,
                                    launches[li].shmem, s));
         }
@@ -3261,6 +3268,12 @@ void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const
         enqueue();
     }
     QA_HIP(hipEventRecord(ctx->ev1, s));
+    int done_items = 0;
+    QA_HIP(hipMemcpyAsync(&done_items, counters.ptr + launches.size(), sizeof(int), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipStreamSynchronize(s));
+    if (done_items != static_cast<int>(items.size()))
+        throw Error{QA_EDEVICE, "qa_sqa_batch_run: the batched launch completed " + std::to_string(done_items) +
+                                    " of " + std::to_string(items.size()) + " work items"};
 
     // per instance: best group (first strict minimum in (step, replica, slice) order), FP64 energy
     std::vector<double> hbe(nG);
