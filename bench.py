@@ -646,7 +646,7 @@ def main():
             line["other_workloads"] = {
                 "c1": _summary(run_c1(2, 1, False)),
                 "c2": _summary(run_c2(2, 1, False)),
-                "c4_sample": _summary(run_c4(1, 1, False, args.c4_instances, args.c4_streams)),
+                "c4_sample": _summary(run_c4(1, 2, False, args.c4_instances, args.c4_streams)),
                 "c5": _summary(run_c5(1, 1, False)),
             }
     print(json.dumps(line), flush=True)
