@@ -208,13 +208,14 @@ def run_c3(steps, warmup, cpu=True):
     attempts = L * rungs * P * n * spb * nsteps
     for w in range(warmup):
         abi.sqapt_run(dp, betas, gammas, P, spb, 0, 5, 1, 0, L, abi.MODE_FP32, 10 + w, refresh_interval=100)
-    kms, sws, drift, best = [], [], [], []
+    kms, sws, drift, best, accs = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(steps):
         r = abi.sqapt_run(dp, betas, gammas, P, spb, 0, nsteps, 1, 0, L, abi.MODE_FP32, 1000 + k,
                           refresh_interval=100)
         kms.append(r["kernel_ms"])
         sws.append(r["sweep_ms"])
+        accs.append(r["accept_rate"])
         drift.append(r["field_drift"].tolist())
         best.append(float(np.min(r["best_energy"])))
         log(f"c3 step {k}: device {kms[-1]:.1f} ms best {best[-1]:.4f} drift {drift[-1]}")
@@ -226,7 +227,8 @@ def run_c3(steps, warmup, cpu=True):
                   "trotter_slices": P, "rungs": rungs, "ladders": L, "sweeps": spb * nsteps,
                   "attempts_per_step": attempts, "device_ms_mean": float(np.mean(kms)),
                   "sweep_launches_per_step": nsteps, "sweep_launch_ms_mean": launch_s * 1e3,
-                  "field_drift_before_after_refresh": drift[-1], "best_energy_per_spin": float(np.min(best)) / n},
+                  "field_drift_before_after_refresh": drift[-1], "best_energy_per_spin": float(np.min(best)) / n,
+                  "accept_rate": float(np.mean(accs))},
                  _roof(bytes_per_launch, launch_s, traffic_from_profile("sqa_fp32_kernel_c3_traffic.md"),
                        lockstep=spb * (n + P - 1)))
     if cpu:

@@ -244,6 +244,8 @@ typedef struct qa_sqapt_result {
     double *sweep_ms;            /* optional (FP32): device time of the sweep launches alone,
                                     summed over the run (kernel_ms adds the exchange, refresh
                                     and drift kernels)                                        */
+    double *accept_rate;         /* optional (FP32): flipped (chain, site) visits over all
+                                    visits of all sweeps of the run                           */
 } qa_sqapt_result;
 
 qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm,

@@ -61,7 +61,8 @@ class SqaptParams(C.Structure):
 class SqaptResult(C.Structure):
     _fields_ = [("final_states", _bp), ("final_energies", _dp), ("best_state", _bp),
                 ("best_energy", _dp), ("average_energy_trace", _dp), ("swap_acceptance_trace", _dp),
-                ("swap_uniforms_used", _lp), ("kernel_ms", _dp), ("field_drift", _dp), ("sweep_ms", _dp)]
+                ("swap_uniforms_used", _lp), ("kernel_ms", _dp), ("field_drift", _dp), ("sweep_ms", _dp),
+                ("accept_rate", _dp)]
 
 
 class ChiParams(C.Structure):
@@ -334,13 +335,13 @@ def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_pe
     fs, fe = np.zeros((L, R, n), np.int8), np.zeros((L, R))
     bs, be = np.zeros((L, n), np.int8), np.zeros(L)
     at, st = np.zeros((L, steps)), np.zeros((L, steps))
-    used, ms, dr, sw = np.zeros(1, np.int64), np.zeros(1), np.zeros(2), np.zeros(1)
+    used, ms, dr, sw, ar = np.zeros(1, np.int64), np.zeros(1), np.zeros(2), np.zeros(1), np.full(1, np.nan)
     r = SqaptResult(_b(fs), _d(fe), _b(bs), _d(be), _d(at), _d(st), used.ctypes.data_as(_lp), _d(ms), _d(dr),
-                    _d(sw) if mode == MODE_FP32 else None)
+                    _d(sw) if mode == MODE_FP32 else None, _d(ar) if mode == MODE_FP32 else None)
     check(lib().qa_sqapt_run(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
     return dict(final_states=fs, final_energies=fe, best_state=bs, best_energy=be, average_energy_trace=at,
                 swap_acceptance_trace=st, swap_uniforms_used=int(used[0]), kernel_ms=float(ms[0]),
-                field_drift=dr, sweep_ms=float(sw[0]))
+                field_drift=dr, sweep_ms=float(sw[0]), accept_rate=float(ar[0]))
 
 
 def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: float, gamma_start: float,

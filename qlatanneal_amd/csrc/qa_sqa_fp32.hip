@@ -2242,7 +2242,8 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     DevBuf<double> dbp(R), djpd(R), chain_e(NS * P), be(L), at(L * steps), st(L * steps);
     DevBuf<int> chain_b(NS * P), dsor(NS), dros(NS);
     DevBuf<int8_t> gS(NS * P * n), bstate(L * n);
-    DevBuf<unsigned long long> accd(pl.ngroups), drift(2);
+    // accepted moves per replica group: one slot per launch when the caller asks for the rate
+    DevBuf<unsigned long long> accd(pl.ngroups * (res->accept_rate ? steps : 1)), drift(2);
     const int64_t ri = prm->refresh_interval;
     QA_REQUIRE(ri >= 0, "refresh_interval must be >= 0");
     const int64_t nchain = NS * P, nsample = std::min<int64_t>(nchain, 64);
@@ -2313,6 +2314,7 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
         a.resume = step > 0 ? 1 : 0;
         a.sweep_base = static_cast<uint32_t>(step) * per_step;
         a.step_base = static_cast<int>(step);
+        if (res->accept_rate) a.accepted = accd.ptr + step * pl.ngroups;
         if (res->sweep_ms) QA_HIP(hipEventRecord(sev[2 * step], s));
         launch_sqa(ctx, pl, a);
         if (res->sweep_ms) QA_HIP(hipEventRecord(sev[2 * step + 1], s));
@@ -2388,6 +2390,14 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
         }
         for (auto &e : sev) (void)hipEventDestroy(e);
         *res->sweep_ms = tot;
+    }
+    if (res->accept_rate) {
+        std::vector<unsigned long long> hacc(pl.ngroups * steps);
+        accd.download(hacc.data(), hacc.size(), s);
+        QA_HIP(hipStreamSynchronize(s));
+        double tot = 0.0;
+        for (unsigned long long v : hacc) tot += static_cast<double>(v);
+        *res->accept_rate = tot / (static_cast<double>(steps) * per_step * NS * P * n);
     }
 }
 

@@ -166,6 +166,7 @@ def test_sqa_fp32_global_state_prefetch_bit_identical(ctx, monkeypatch):
             assert np.array_equal(res[0][key], r[key]), key
         for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace"):
             assert bits_equal(res[0][key].ravel(), r[key].ravel()), key
+        assert 0.0 < r["accept_rate"] < 1.0 and r["accept_rate"] == res[0]["accept_rate"]
     monkeypatch.delenv("QANNEAL_SQA_STATE", raising=False)
     monkeypatch.delenv("QANNEAL_SQA_GLOBAL_F4", raising=False)
     monkeypatch.delenv("QANNEAL_SQA_SPINS_LDS", raising=False)
