@@ -3,8 +3,10 @@
 
 Default run (no --workload): the headline line is the largest single-GPU configuration that runs
 here -- SQA-PT with matrix-core field refresh on a dense SK Ising, N=2048, P=64, 32 rungs x 4
-ladders, 10 sweeps per step, 500 steps (8.39e10 single-spin Metropolis attempts per timed step),
-FP32 local fields refreshed every 100 sweeps with the FP64 drift check. The same JSON line carries
+ladders, 10 sweeps per PT step, an anneal of 500 PT steps, FP32 local fields refreshed every 100
+sweeps with the FP64 drift check. A bench step is 25 PT steps (4.19e9 single-spin Metropolis
+attempts); --steps 20 (the default) times exactly the configured 5000-sweep anneal. The same JSON
+line carries
   roofline        the sweep kernel's algorithmic bytes per launch over its average launch time
                   against the 8 TB/s HBM peak, the counter-measured HBM bytes per launch, and the
                   lockstep critical path (lockstep steps per launch and measured time per step);
@@ -192,43 +194,47 @@ def run_c2(steps, warmup, cpu=True, cpu_sample_steps=8):
 # ---- C3: SQA-PT with matrix-core field refresh (the headline) ---------------------------------
 def run_c3(steps, warmup, cpu=True):
     """SQA-PT with matrix-core field refresh: dense SK N=2048, P=64, 32 rungs x 4 ladders, 10 sweeps
-    per step, 500 steps (5000 sweeps), swaps every step, FP32 fields refreshed every 100 sweeps (bf16
-    hi+lo S.J, FP32 accumulate) with the FP64 drift check. Algorithmic bytes per sweep (128
+    per PT step, swaps every PT step, FP32 fields refreshed every 100 sweeps (bf16 hi+lo S.J, FP32
+    accumulate) with the FP64 drift check. The configured anneal is 500 PT steps (5000 sweeps); a
+    bench step is one twentieth of it, 25 PT steps (250 sweeps of all 128 replicas, 4.19e9 attempts),
+    and the K timed steps run as one anneal of 25 K PT steps (K = 20: exactly the 5000-sweep anneal),
+    the W warmup steps as one anneal of 25 W PT steps before it. The ladder's (beta, Gamma) points do
+    not change during the anneal, so every PT step is the same work. Algorithmic bytes per sweep (128
     replicas): fields r+w 134.2 MB + spins r+w 33.6 MB + J once 16.8 MB = 11.0 B/attempt. One sweep
     launch = 10 sweeps of all 128 replicas (one PT step); its average duration comes from hipEvent
     pairs the library records around each sweep launch on its stream (sweep_ms); the run's device
     time adds the exchange, refresh and drift kernels (3 %)."""
+    import torch
     from qlatanneal_amd import abi
-    n, P, rungs, L, spb, nsteps = 2048, 64, 32, 4, 10, 500
+    n, P, rungs, L, spb, per_step = 2048, 64, 32, 4, 10, 25
     J = sk_instance(n, 20261204)
     betas = np.geomspace(0.3, 5.0, rungs)
     gammas = np.geomspace(3.0, 0.05, rungs)
     ctx = abi.Context(0)
     dp = abi.DeviceProblem.dense(ctx, np.zeros(n), J)
-    attempts = L * rungs * P * n * spb * nsteps
-    for w in range(warmup):
-        abi.sqapt_run(dp, betas, gammas, P, spb, 0, 5, 1, 0, L, abi.MODE_FP32, 10 + w, refresh_interval=100)
-    kms, sws, drift, best, accs = [], [], [], [], []
+    attempts = L * rungs * P * n * spb * per_step  # per bench step
+    if warmup > 0:
+        abi.sqapt_run(dp, betas, gammas, P, spb, 0, per_step * warmup, 1, 0, L, abi.MODE_FP32, 10,
+                      refresh_interval=100)
+    nsteps = per_step * steps
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(steps):
-        r = abi.sqapt_run(dp, betas, gammas, P, spb, 0, nsteps, 1, 0, L, abi.MODE_FP32, 1000 + k,
-                          refresh_interval=100)
-        kms.append(r["kernel_ms"])
-        sws.append(r["sweep_ms"])
-        accs.append(r["accept_rate"])
-        drift.append(r["field_drift"].tolist())
-        best.append(float(np.min(r["best_energy"])))
-        log(f"c3 step {k}: device {kms[-1]:.1f} ms best {best[-1]:.4f} drift {drift[-1]}")
+    r = abi.sqapt_run(dp, betas, gammas, P, spb, 0, nsteps, 1, 0, L, abi.MODE_FP32, 1000, refresh_interval=100)
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    best = float(np.min(r["best_energy"]))
+    log(f"c3: {nsteps} PT steps, device {r['kernel_ms']:.1f} ms, wall {wall:.2f} s, best {best:.4f}, "
+        f"drift {r['field_drift'].tolist()}")
     bytes_per_launch = (2 * L * rungs * P * n * 4 + 2 * L * rungs * P * n + n * n * 4) * spb
-    launch_s = float(np.mean(sws)) / 1e3 / nsteps  # hipEvent pairs around each sweep launch
+    launch_s = float(r["sweep_ms"]) / 1e3 / nsteps  # hipEvent pairs around each sweep launch
     line = _line(steps * attempts / wall, steps, warmup, wall,
                  {"workload": "sqapt_dense_sk_n2048_p64_32rungs_x4_5000sweeps_refresh100_fp32", "N": n,
-                  "trotter_slices": P, "rungs": rungs, "ladders": L, "sweeps": spb * nsteps,
-                  "attempts_per_step": attempts, "device_ms_mean": float(np.mean(kms)),
-                  "sweep_launches_per_step": nsteps, "sweep_launch_ms_mean": launch_s * 1e3,
-                  "field_drift_before_after_refresh": drift[-1], "best_energy_per_spin": float(np.min(best)) / n,
-                  "accept_rate": float(np.mean(accs))},
+                  "trotter_slices": P, "rungs": rungs, "ladders": L, "sweeps_per_pt_step": spb,
+                  "pt_steps_per_bench_step": per_step, "pt_steps_timed": nsteps,
+                  "anneal_pt_steps_configured": 500, "attempts_per_step": attempts,
+                  "device_ms": float(r["kernel_ms"]), "sweep_launch_ms_mean": launch_s * 1e3,
+                  "field_drift_before_after_refresh": r["field_drift"].tolist(),
+                  "best_energy_per_spin": best / n, "accept_rate": float(r["accept_rate"])},
                  _roof(bytes_per_launch, launch_s, traffic_from_profile("sqa_fp32_kernel_c3_traffic.md"),
                        lockstep=spb * (n + P - 1)))
     if cpu:
@@ -241,7 +247,7 @@ def run_c3(steps, warmup, cpu=True):
         dt = time.perf_counter() - t
         att = rungs * P * n * spb
         line["cpu_baseline"] = {"value": att / dt, "unit": "attempts/s", "cores": cores, "kind": kind,
-                                "sample": (f"one ladder (32 rungs), 1 of {nsteps} steps (10 sweeps) of the same "
+                                "sample": (f"one ladder (32 rungs), one PT step (10 sweeps) of the same "
                                            f"instance ({dt:.1f} s); the reference's cost per step does not depend "
                                            f"on the acceptance rate (O(N) delta_energy per attempt)")}
     return line
@@ -592,8 +598,10 @@ def _summary(line):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (headline C3: 25 PT steps each, default 20 = the whole 5000-sweep "
+                         "anneal; other workloads: whole runs, default 2)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (C3: 2, others: 1)")
     ap.add_argument("--cpu-sample-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -628,6 +636,11 @@ def main():
         return
     cpu = not args.no_cpu
     w = args.workload
+    headline = w in (None, "c3")
+    if args.steps is None:
+        args.steps = 20 if headline else 2
+    if args.warmup is None:
+        args.warmup = 2 if headline else 1
     if w == "c1":
         line = run_c1(args.steps, args.warmup, cpu)
     elif w == "c2":
