@@ -109,6 +109,7 @@ struct SqaArgs {
     int adj_lds;           // sparse: adjacency (CSR) staged in LDS after the scratch
     int cmax;              // per-chain scratch entries: 64, or G rounded up to 64 when P > 64
     int s_lds;             // global-state fields with the spins in LDS (dense float4 kernel)
+    int la;                // look-ahead slice sweep: its LDS exchange area follows the spins
 };
 
 
@@ -132,6 +133,8 @@ struct SqaLds {
     unsigned long long *flipmask;  // two-barrier: [w] flips of wave w's chains; pipelined: [b] flips, [2 + b] flips from s = +1
     int *scal;         // [0]: best chain of the step
     float *junk;       // [64][4]: target of stores a lane must not make (never read)
+    float *la;         // look-ahead sweep exchange (kLaBytes): PF[2][64], JC[4][2][64], DS[2][64],
+                       // ROW[2][64], flip masks [2]
     const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO arrays (LDS copies or global)
     const float *t_coef;
     const int32_t *adj_ptr, *adj_idx;                     // sparse CSR (LDS copy or global)
@@ -150,6 +153,10 @@ __host__ __device__ constexpr size_t hubo_lds_bytes(int n, int nterms, int nv) {
 }
 
 // Flip-mask words: one per deciding wavefront (cmax / 64), at least the pipelined kernel's four.
+// LDS of the look-ahead sweep's exchange area (floats: PF 2x64, JC 8x64, DS 2x64, ROW 2x64, then
+// two 64-bit flip masks)
+constexpr size_t kLaBytes = 896 * 4 + 16;
+
 __host__ __device__ constexpr size_t flipmask_bytes(int cmax) {
     return lds_align16(static_cast<size_t>(cmax / 64 > 4 ? cmax / 64 : 4) * 8);
 }
@@ -214,6 +221,10 @@ __device__ SqaLds carve_lds(const SqaArgs &a, unsigned char *p, bool lds_state) 
     if (!lds_state && a.s_lds) {  // the plan placed them here, after everything above
         L.S = reinterpret_cast<int8_t *>(p);
         p += lds_align16(static_cast<size_t>(a.rg) * a.P * n);
+    }
+    if (a.la) {  // after the spins
+        L.la = reinterpret_cast<float *>(p);
+        p += kLaBytes;
     }
     if (a.hubo_lds) {  // the plan placed them here, after everything above
         L.inc_ptr = reinterpret_cast<const int32_t *>(p);
@@ -1081,6 +1092,328 @@ __global__ void __launch_bounds__(kThreads) sqa_fp32_kernel(SqaArgs a) {
     general_group<kLds, NJ, kSL>(a, static_cast<int>(blockIdx.x));
 }
 
+// ---- look-ahead slice sweeps (dense, fields in global memory, spins in LDS) ------------------
+// One replica group of plain slice sweeps (no worldline, cluster or checkerboard moves, no bond
+// statistics) with one wavefront of chains: the same visit orders, random words and FP32
+// arithmetic as the two-barrier kernel, so the same results bit for bit. NJ: 4-float groups per
+// updating lane in a row (n <= 1024 NJ).
+template <int NJ>
+__device__ void la_group(const SqaArgs &a, const int g) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int n = a.pv.n, P = a.P;
+    const int r0 = g * a.rg;
+    const int nrep = min(a.rg, a.R - r0);
+    const int G = nrep * P;
+    const SqaLds L = carve_lds(a, smem, false);
+    float *F = a.gF + static_cast<int64_t>(r0) * P * n;
+    int8_t *S = L.S;
+    int *order = L.order;
+    const uint32_t cbase = static_cast<uint32_t>(r0 * P);
+    group_init(a, F, S, a.ldf, G, r0, false);
+    double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
+    unsigned long long nacc = 0;
+    const bool decider = tid < G;
+    const int c_me = tid;
+    const int rr_me = c_me / P, t_me = c_me - (c_me / P) * P;
+    const int cprev = rr_me * P + (t_me == 0 ? P - 1 : t_me - 1);
+    const int cnext = rr_me * P + (t_me + 1 == P ? 0 : t_me + 1);
+    uint32_t sweep_id = a.sweep_base;
+    for (int step = 0; step < a.steps; ++step) {
+        const int pidx = a.param_of_rep ? (decider ? a.param_of_rep[r0 + rr_me] : 0) : step;
+        const float bs = a.bs[pidx], jp = a.jp[pidx];
+        for (int sw = 0; sw < a.spb; ++sw, ++sweep_id) {
+            if (tid == 0) shuffle_order(order, n, cbase, sweep_id, a.k0, a.k1);
+            __syncthreads();
+            const int nsteps = n + P - 1;
+            // Fields in global memory, spins in LDS, one wavefront of chains (chain c = lane c).
+            // Roles, each running its own loop with two barriers per lockstep step tau:
+            //   wave 0 (decisions): decides step tau from LDS only -- spins, the field element
+            //     PF[c] and the couplings JC0..3[c] between the site and the chain's last four
+            //     sites -- and publishes the flips (DS, ROW, flip mask) in the set tau & 1;
+            //   waves 1-4 (updates, after the first barrier): finish the row updates of step
+            //     tau - 1 (rows loaded during the previous update phase, the chains beyond those
+            //     loaded and stored now), then load the rows of up to 8 / NJ flipped chains of
+            //     step tau, which the next update phase finishes;
+            //   wave 5 (helper, before the first barrier): stores into LDS, for decision tau + 1,
+            //     what it loaded at step tau - 2, then loads for decision tau + 3 the field
+            //     element and the four couplings (unconditional loads, clamped indexes).
+            // Every flip of step s is stored during update phase s + 1, and no field element is
+            // stored during a decision phase, so the element loaded at step tau holds exactly the
+            // chain's flips up to step tau - 2. Decision tau + 3 adds its flips at steps tau - 1
+            // .. tau + 2 with the fmaf the row update applies, in the same order, and so decides
+            // on the same bits as the element in memory (and as every other sweep kernel). A row
+            // update of chain c loads F[c] after the store of c's previous flip by the same lane.
+            // Three leading steps make the first loads and a trailing step without decisions
+            // lets the last flips be stored; the step count is even (the helper's registers
+            // alternate with the step parity).
+            constexpr int kUpdT = 256;                          // threads of waves 1-4
+            constexpr int kPreB = 8 / NJ;  // chains whose rows are loaded ahead (64 registers)
+            constexpr int kNowB = NJ >= 4 ? 1 : 4 / NJ;  // chains per synchronous batch
+            float *const LA = L.la;
+            const int n4 = n >> 2;
+            f4 *F4g = reinterpret_cast<f4 *>(F);
+            const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+            const int tend = nsteps + 1 + ((nsteps + 4) & 1);
+            // the first update phase reads the flip-mask set of step -4: no flips
+            if (tid < 2) reinterpret_cast<unsigned long long *>(LA + 896)[tid] = 0ull;
+            __syncthreads();
+            __builtin_amdgcn_s_waitcnt(0);
+            if (wv == 0) {
+                // ---- decisions --------------------------------------------------------------
+                bool fl1 = false, fl2 = false, fl3 = false, fl4 = false;
+                float ds1 = 0.0f, ds2 = 0.0f, ds3 = 0.0f, ds4 = 0.0f;
+                uint32_t pw0 = 0, pw1 = 0, pw2 = 0, pw3 = 0;
+                for (int tau = -3; tau < tend; ++tau) {
+                    const int set = tau & 1;
+                    const int k = tau - t_me;
+                    const bool active = decider && k >= 0 && k < n;
+                    bool acc = false;
+                    // this step's word of the chain's Philox block (tau >> 2), shifted out one
+                    // per step so no word is picked by a computed index
+                    if ((tau & 3) == 0) {
+                        const Philox4 r4 = philox4x32_10(cbase + c_me, sweep_id, static_cast<uint32_t>(tau >> 2),
+                                                         TAG_SLICE, a.k0, a.k1);
+                        pw0 = r4.x;
+                        pw1 = r4.y;
+                        pw2 = r4.z;
+                        pw3 = r4.w;
+                    }
+                    const uint32_t word = pw0;
+                    pw0 = pw1;
+                    pw1 = pw2;
+                    pw2 = pw3;
+                    if (active) {
+                        const int i = order[k];
+                        int8_t *sc = S + static_cast<int64_t>(c_me) * n;
+                        const int s = sc[i];
+                        float f = LA[set * 64 + c_me];
+                        if (fl4) f = fmaf(ds4, LA[128 + (0 * 2 + set) * 64 + c_me], f);
+                        if (fl3) f = fmaf(ds3, LA[128 + (1 * 2 + set) * 64 + c_me], f);
+                        if (fl2) f = fmaf(ds2, LA[128 + (2 * 2 + set) * 64 + c_me], f);
+                        if (fl1) f = fmaf(ds1, LA[128 + (3 * 2 + set) * 64 + c_me], f);
+                        const int sp = S[static_cast<int64_t>(cprev) * n + i];
+                        const int sn = S[static_cast<int64_t>(cnext) * n + i];
+                        const float fs = static_cast<float>(s);
+                        const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(sp + sn));
+                        acc = de <= 0.0f || u01(word) < __expf(-de);
+                        LA[640 + set * 64 + c_me] = -2.0f * fs;
+                        reinterpret_cast<int *>(LA + 768)[set * 64 + c_me] = i;
+                        if (acc) sc[i] = static_cast<int8_t>(-s);
+                        fl4 = fl3;
+                        ds4 = ds3;
+                        fl3 = fl2;
+                        ds3 = ds2;
+                        fl2 = fl1;
+                        ds2 = ds1;
+                        fl1 = acc;
+                        ds1 = -2.0f * fs;
+                    }
+                    const unsigned long long m = __ballot(acc);
+                    if (lane == 0) reinterpret_cast<unsigned long long *>(LA + 896)[set] = m;
+                    nacc += acc ? 1ull : 0ull;
+                    __syncthreads();
+                    __syncthreads();
+                }
+            } else if (wv == kHelperWave) {
+                // ---- helper: field elements and couplings three steps ahead ------------------
+                struct Ahead {
+                    float f, j0, j1, j2, j3;
+                };
+                const int t_l = lane % P;
+                const bool ch = lane < G;
+                const int cc = ch ? lane : 0;
+                auto helper = [&](const int tau, Ahead &h, const int set) {
+                    const int k1 = tau + 1 - t_l;
+                    if (ch && k1 >= 0 && k1 < n) {
+                        LA[set * 64 + lane] = h.f;
+                        LA[128 + (0 * 2 + set) * 64 + lane] = h.j0;
+                        LA[128 + (1 * 2 + set) * 64 + lane] = h.j1;
+                        LA[128 + (2 * 2 + set) * 64 + lane] = h.j2;
+                        LA[128 + (3 * 2 + set) * 64 + lane] = h.j3;
+                    }
+                    const int k3 = tau + 3 - t_l;
+                    const int64_t col = order[min(max(k3, 0), n - 1)];
+                    auto jrow = [&](int d) {
+                        return static_cast<int64_t>(order[min(max(k3 - d, 0), n - 1)]) * n;
+                    };
+                    h.f = F[static_cast<int64_t>(cc) * n + col];
+                    h.j0 = a.pv.J32[jrow(4) + col];
+                    h.j1 = a.pv.J32[jrow(3) + col];
+                    h.j2 = a.pv.J32[jrow(2) + col];
+                    h.j3 = a.pv.J32[jrow(1) + col];
+                    __syncthreads();
+                    __syncthreads();
+                };
+                Ahead ha{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, hb{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+                for (int tau = -3; tau < tend; tau += 2) {
+                    helper(tau, ha, 0);      // tau odd: the set decision tau + 1 reads
+                    helper(tau + 1, hb, 1);
+                }
+            } else {
+                // ---- row updates ------------------------------------------------------------
+                const int ut = tid - 64;  // 0 .. 255
+                int nu_w = 0;             // 4-float groups per lane this wave holds (wave-uniform)
+#pragma unroll
+                for (int u = 0; u < NJ; ++u)
+                    if ((wv - 1) * 64 + kUpdT * u < n4) nu_w = u + 1;
+                nu_w = __builtin_amdgcn_readfirstlane(nu_w);
+                // rows of the chains loaded ahead: J and F groups per chain and group index
+                f4 prj[kPreB][NJ], prf[kPreB][NJ];
+                int pcs[kPreB];
+                int pcnt = 0;
+                // synchronous update of B chains (the ones beyond those loaded ahead)
+                auto upd_now = [&](auto bsz, auto nu, const int *cs, const int set) {
+                    constexpr int B = decltype(bsz)::value;
+                    constexpr int NU = decltype(nu)::value;
+                    float dsv[B];
+                    const f4 *rows[B];
+                    f4 *frow[B];
+#pragma unroll
+                    for (int q = 0; q < B; ++q) {
+                        dsv[q] = LA[640 + set * 64 + cs[q]];
+                        rows[q] = reinterpret_cast<const f4 *>(
+                            a.pv.J32 + static_cast<int64_t>(reinterpret_cast<const int *>(LA + 768)[set * 64 + cs[q]]) * n);
+                        frow[q] = F4g + static_cast<int64_t>(cs[q]) * n4;
+                    }
+                    f4 rv[B][NU], fv[B][NU];
+#pragma unroll
+                    for (int u = 0; u < NU; ++u) {
+                        const int gi = min(ut + kUpdT * u, n4 - 1);
+#pragma unroll
+                        for (int q = 0; q < B; ++q) {
+                            rv[q][u] = rows[q][gi];
+                            fv[q][u] = frow[q][gi];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < NU; ++u) {
+                        const int gi = ut + kUpdT * u;
+#pragma unroll
+                        for (int q = 0; q < B; ++q) {
+                            const f4 x = rv[q][u], y = fv[q][u];
+                            f4 v = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                                      fmaf(dsv[q], x.w, y.w)};
+                            // computed on every path (not only under the store's mask): the
+                            // loads are then waited for here, never later
+                            asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+                            if (gi < n4) frow[q][gi] = v;
+                        }
+                    }
+                };
+                auto upd_now_nu = [&](auto bsz, const int *cs, const int set) {
+                    static_for<NJ>([&](auto u) {
+                        if (nu_w == decltype(u)::value + 1)
+                            upd_now(bsz, std::integral_constant<int, decltype(u)::value + 1>{}, cs, set);
+                    });
+                };
+                // finish the chains loaded ahead (their flips are in set `set`)
+                auto finish_pre = [&](const int set) {
+                    // every register a load of the previous phase may have written is read here
+                    // on every path, so the wait for those loads sits here and nowhere later
+                    static_for<kPreB>([&](auto qq) {
+#pragma unroll
+                        for (int u = 0; u < NJ; ++u) {
+                            const f4 x = prj[decltype(qq)::value][u], y = prf[decltype(qq)::value][u];
+                            asm volatile("" ::"v"(x.x), "v"(x.y), "v"(x.z), "v"(x.w), "v"(y.x), "v"(y.y), "v"(y.z),
+                                         "v"(y.w));
+                        }
+                    });
+                    static_for<kPreB>([&](auto qq) {
+                        constexpr int q = decltype(qq)::value;
+                        if (q < pcnt) {
+                            const float ds = LA[640 + set * 64 + pcs[q]];
+                            f4 *frow = F4g + static_cast<int64_t>(pcs[q]) * n4;
+#pragma unroll
+                            for (int u = 0; u < NJ; ++u) {
+                                if (u < nu_w) {
+                                    const int gi = ut + kUpdT * u;
+                                    const f4 x = prj[q][u], y = prf[q][u];
+                                    f4 v = f4{fmaf(ds, x.x, y.x), fmaf(ds, x.y, y.y), fmaf(ds, x.z, y.z),
+                                              fmaf(ds, x.w, y.w)};
+                                    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+                                    if (gi < n4) frow[gi] = v;
+                                }
+                            }
+                        }
+                    });
+                };
+                // load the rows of the first flipped chains of set `set`
+                auto load_pre = [&](uint64_t &m, const int set) {
+                    pcnt = 0;
+                    static_for<kPreB>([&](auto qq) {
+                        constexpr int q = decltype(qq)::value;
+                        if (m) {
+                            const int c = __builtin_ctzll(m);
+                            m &= m - 1u;
+                            pcs[q] = c;
+                            pcnt = q + 1;
+                            const f4 *row = reinterpret_cast<const f4 *>(
+                                a.pv.J32 + static_cast<int64_t>(reinterpret_cast<const int *>(LA + 768)[set * 64 + c]) * n);
+                            const f4 *frow = F4g + static_cast<int64_t>(c) * n4;
+#pragma unroll
+                            for (int u = 0; u < NJ; ++u) {
+                                if (u < nu_w) {
+                                    const int gi = min(ut + kUpdT * u, n4 - 1);
+                                    prj[q][u] = row[gi];
+                                    prf[q][u] = frow[gi];
+                                }
+                            }
+                        }
+                    });
+                };
+                auto mask_of = [&](const int set) -> uint64_t {
+                    const unsigned long long m_all = reinterpret_cast<const unsigned long long *>(LA + 896)[set];
+                    const uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
+                    const uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
+                    return (static_cast<uint64_t>(mhi) << 32) | mlo;
+                };
+                for (int tau = -3; tau < tend; ++tau) {
+                    const int set = tau & 1;
+                    __syncthreads();
+                    if (nu_w > 0) {
+                        // step tau - 1: the chains loaded ahead, then the rest of its flips
+                        finish_pre(set ^ 1);
+                        uint64_t mp = mask_of(set ^ 1);
+                        for (int q = 0; q < pcnt; ++q) mp &= mp - 1u;
+                        while (mp) {
+                            const int cnt = __popcll(mp);
+                            int cs[4] = {0, 0, 0, 0};
+                            const int b = (kNowB >= 4 && cnt >= 4) ? 4 : (kNowB >= 2 && cnt >= 2) ? 2 : 1;
+                            static_for<4>([&](auto qq) {
+                                if (decltype(qq)::value < b) {
+                                    cs[decltype(qq)::value] = __builtin_ctzll(mp);
+                                    mp &= mp - 1u;
+                                }
+                            });
+                            if (kNowB >= 4 && b == 4) upd_now_nu(std::integral_constant<int, (kNowB >= 4 ? 4 : 1)>{}, cs, set ^ 1);
+                            else if (kNowB >= 2 && b == 2) upd_now_nu(std::integral_constant<int, (kNowB >= 2 ? 2 : 1)>{}, cs, set ^ 1);
+                            else upd_now_nu(std::integral_constant<int, 1>{}, cs, set ^ 1);
+                        }
+                        // step tau: load ahead
+                        uint64_t mc = mask_of(set);
+                        load_pre(mc, set);
+                    }
+                    __syncthreads();
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        if (a.tail_mode != 2) group_tail(a, F, S, a.ldf, G, r0, g, step, L.ered, L.scal, gbest);
+    }
+    __syncthreads();
+    for (int e = tid; e < G * n; e += kThreads) a.gS[static_cast<int64_t>(r0) * P * n + e] = S[e];
+    if (tid < kMaxChains) {
+        for (int o = 32; o > 0; o >>= 1) nacc += __shfl_xor(nacc, o);
+        if (tid == 0) a.accepted[g] = nacc;
+    }
+}
+
+template <int NJ>
+__global__ void __launch_bounds__(kThreads) sqa_la_kernel(SqaArgs a) {
+    la_group<NJ>(a, static_cast<int>(blockIdx.x));
+}
+
 // ---- pipelined slice sweeps (dense couplings, LDS-resident state) ---------------------------
 // The same visit orders, random words and FP32 arithmetic as the two-barrier slice sweep of
 // sqa_fp32_kernel, hence bit-identical results, with one barrier per lockstep step and the
@@ -1506,6 +1839,7 @@ struct SqaPlan {
     int hubo_lds = 0, hubo_nv = 0, adj_lds = 0;
     int cmax = kMaxChains;
     int s_lds = 0;  // global-state fields with the spins in LDS (dense float4 kernel)
+    int la = 0;     // look-ahead slice sweep (s_lds, one wavefront of chains)
     size_t shmem = 0;
 };
 
@@ -1587,6 +1921,13 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
             pl.s_lds = 1;
             pl.shmem += sb;
         }
+        // with one wavefront of chains the slice sweeps take the look-ahead kernel
+        // (QANNEAL_SQA_LOOKAHEAD=0: the one-step prefetch, the same results bit for bit)
+        const char *laenv = std::getenv("QANNEAL_SQA_LOOKAHEAD");
+        if (pl.s_lds && pl.cmax == kMaxChains && !(laenv && laenv[0] == '0') && pl.shmem + kLaBytes <= lds_cap) {
+            pl.la = 1;
+            pl.shmem += kLaBytes;
+        }
     }
     pl.ldf = static_cast<int>(pl.use_lds ? rn : n);
     pl.ldr = static_cast<int>(rn);
@@ -1601,6 +1942,7 @@ void apply_plan(SqaArgs &a, const SqaPlan &pl) {
     a.adj_lds = pl.adj_lds;
     a.cmax = pl.cmax;
     a.s_lds = pl.s_lds;
+    a.la = pl.la;
     a.ngroups = static_cast<int>(pl.ngroups);
     a.ldf = pl.ldf;
     a.ldr = pl.ldr;
@@ -1634,6 +1976,11 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
             kern = reinterpret_cast<const void *>(sqa_fp32_kernel<true>);
         else if (!f4ok)
             kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false>);
+        else if (a.s_lds && a.la && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 && a.bstat == nullptr)
+            kern = n <= 1024 ? reinterpret_cast<const void *>(sqa_la_kernel<1>)
+                 : n <= 2048 ? reinterpret_cast<const void *>(sqa_la_kernel<2>)
+                 : n <= 4096 ? reinterpret_cast<const void *>(sqa_la_kernel<4>)
+                             : reinterpret_cast<const void *>(sqa_la_kernel<8>);
         else if (a.s_lds)
             kern = n <= 1024 ? reinterpret_cast<const void *>(sqa_fp32_kernel<false, 1, true>)
                  : n <= 2048 ? reinterpret_cast<const void *>(sqa_fp32_kernel<false, 2, true>)
@@ -3215,6 +3562,7 @@ void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const
         a.accepted = accd.ptr + x.oG;
         apply_plan(a, x.pl);
         a.s_lds = 0;  // the batch kernel's global-state body keeps the spins in global memory
+        a.la = 0;
         const double cost = static_cast<double>(x.pl.rg * P * x.n) * (probs[k]->kind == QA_KIND_DENSE ? x.n : 16);
         for (int64_t gi = 0; gi < x.pl.ngroups; ++gi)
             work[x.variant].push_back({cost, BatchItem{static_cast<int>(k), static_cast<int>(gi)}});

@@ -172,6 +172,49 @@ def test_sqa_fp32_global_state_prefetch_bit_identical(ctx, monkeypatch):
     monkeypatch.delenv("QANNEAL_SQA_SPINS_LDS", raising=False)
 
 
+def test_sqa_fp32_lookahead_sweep_bit_identical(ctx, monkeypatch):
+    """Global fields with the spins in LDS: the look-ahead slice sweep (decisions from LDS, field
+    elements and couplings loaded three steps ahead by the helper wavefront, the chain's own last
+    three flips added by the decision) against the one-step-prefetch sweep and the LDS-state kernel,
+    bit for bit: row lengths that end inside a wavefront's share (n = 300, 1100), one to 64 chains
+    per group, worldline and cluster moves between the slice sweeps, and SQA-PT at P = 64."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(45)
+    env = ("QANNEAL_SQA_STATE", "QANNEAL_SQA_GLOBAL_F4", "QANNEAL_SQA_SPINS_LDS", "QANNEAL_SQA_LOOKAHEAD")
+    configs = (("lds", "1", "1", "1"), ("global", "1", "1", "0"), ("global", "1", "1", "1"))
+    for n, P, R, wl, cl in [(64, 1, 5, 0, 0), (132, 3, 4, 0, 0), (300, 16, 8, 0, 0), (256, 64, 2, 1, 1),
+                            (1100, 8, 3, 0, 0), (64, 16, 1024, 0, 0), (2048, 64, 2, 0, 0)]:
+        print("lookahead case", n, P, R, wl, cl, flush=True)
+        dp = abi.DeviceProblem.dense(ctx, rng.normal(0, 0.2, n), _sk(rng, n), 0.25)
+        betas, gammas = np.geomspace(0.3, 3.0, 4), np.geomspace(3.0, 0.05, 4)
+        res = []
+        for cfg in configs:
+            for k, v in zip(env, cfg):
+                monkeypatch.setenv(k, v)
+            res.append(abi.sqa_run(dp, betas, gammas, P, R, 2, wl, cl, abi.MODE_FP32, 29))
+        for r in res[1:]:
+            assert r["accept_rate"] == res[0]["accept_rate"], (n, P, R)
+            assert np.array_equal(r["best_state"], res[0]["best_state"]), (n, P, R)
+            assert bits_equal([r["best_energy"]], [res[0]["best_energy"]]), (n, P, R)
+            assert bits_equal(r["energy_trace"], res[0]["energy_trace"]), (n, P, R)
+    n, P = 2048, 64
+    print("lookahead case sqapt", n, P, flush=True)
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), _sk(rng, n))
+    res = []
+    for cfg in configs[1:]:
+        for k, v in zip(env, cfg):
+            monkeypatch.setenv(k, v)
+        res.append(abi.sqapt_run(dp, np.geomspace(0.3, 5.0, 8), np.geomspace(3.0, 0.05, 8), P, 10, 0, 3, 1, 0,
+                                 ladders=2, mode=abi.MODE_FP32, seed=8, refresh_interval=10))
+    for key in ("final_states", "best_state"):
+        assert np.array_equal(res[0][key], res[1][key]), key
+    for key in ("final_energies", "best_energy", "average_energy_trace", "swap_acceptance_trace", "field_drift"):
+        assert bits_equal(res[0][key].ravel(), res[1][key].ravel()), key
+    assert 0.0 < res[1]["accept_rate"] < 1.0 and res[1]["accept_rate"] == res[0]["accept_rate"]
+    for k in env:
+        monkeypatch.delenv(k, raising=False)
+
+
 def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
     """The one-barrier pipelined sweep (decisions of step t overlapped with the field updates of
     step t-1) must reproduce the two-barrier lockstep sweep bit for bit: same spins, same FP32
