@@ -134,7 +134,7 @@ struct SqaLds {
     int *scal;         // [0]: best chain of the step
     float *junk;       // [64][4]: target of stores a lane must not make (never read)
     float *la;         // look-ahead sweep exchange (kLaBytes): PF[2][64], JC[4][2][64], DS[2][64],
-                       // ROW[2][64], flip masks [2]
+                       // ROW[2][64], flip masks [2], Philox words RW[2][4][64]
     const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO arrays (LDS copies or global)
     const float *t_coef;
     const int32_t *adj_ptr, *adj_idx;                     // sparse CSR (LDS copy or global)
@@ -153,9 +153,9 @@ __host__ __device__ constexpr size_t hubo_lds_bytes(int n, int nterms, int nv) {
 }
 
 // Flip-mask words: one per deciding wavefront (cmax / 64), at least the pipelined kernel's four.
-// LDS of the look-ahead sweep's exchange area (floats: PF 2x64, JC 8x64, DS 2x64, ROW 2x64, then
-// two 64-bit flip masks)
-constexpr size_t kLaBytes = 896 * 4 + 16;
+// LDS of the look-ahead sweep's exchange area (32-bit words: PF 2x64, JC 8x64, DS 2x64, ROW 2x64,
+// two 64-bit flip masks, then the Philox words RW 2x4x64)
+constexpr size_t kLaBytes = 896 * 4 + 16 + 512 * 4;
 
 __host__ __device__ constexpr size_t flipmask_bytes(int cmax) {
     return lds_align16(static_cast<size_t>(cmax / 64 > 4 ? cmax / 64 : 4) * 8);
@@ -1163,37 +1163,33 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                 // ---- decisions --------------------------------------------------------------
                 bool fl1 = false, fl2 = false, fl3 = false, fl4 = false;
                 float ds1 = 0.0f, ds2 = 0.0f, ds3 = 0.0f, ds4 = 0.0f;
-                uint32_t pw0 = 0, pw1 = 0, pw2 = 0, pw3 = 0;
+                const uint32_t *RWd = reinterpret_cast<const uint32_t *>(LA + 900);
+                int i_nx = order[0];  // the site of the next step, read one step ahead
                 for (int tau = -3; tau < tend; ++tau) {
                     const int set = tau & 1;
                     const int k = tau - t_me;
                     const bool active = decider && k >= 0 && k < n;
                     bool acc = false;
-                    // this step's word of the chain's Philox block (tau >> 2), shifted out one
-                    // per step so no word is picked by a computed index
-                    if ((tau & 3) == 0) {
-                        const Philox4 r4 = philox4x32_10(cbase + c_me, sweep_id, static_cast<uint32_t>(tau >> 2),
-                                                         TAG_SLICE, a.k0, a.k1);
-                        pw0 = r4.x;
-                        pw1 = r4.y;
-                        pw2 = r4.z;
-                        pw3 = r4.w;
-                    }
-                    const uint32_t word = pw0;
-                    pw0 = pw1;
-                    pw1 = pw2;
-                    pw2 = pw3;
+                    const int i = i_nx;
+                    i_nx = order[min(max(k + 1, 0), n - 1)];
+                    // word tau & 3 of the chain's Philox block tau >> 2 (counter (chain, sweep,
+                    // block, TAG_SLICE)), evaluated by the helper wave at least one step earlier
+                    const uint32_t word = RWd[(((tau >> 2) & 1) * 4 + (tau & 3)) * 64 + c_me];
                     if (active) {
-                        const int i = order[k];
                         int8_t *sc = S + static_cast<int64_t>(c_me) * n;
+                        // every LDS read of the decision is issued before the arithmetic starts
                         const int s = sc[i];
-                        float f = LA[set * 64 + c_me];
-                        if (fl4) f = fmaf(ds4, LA[128 + (0 * 2 + set) * 64 + c_me], f);
-                        if (fl3) f = fmaf(ds3, LA[128 + (1 * 2 + set) * 64 + c_me], f);
-                        if (fl2) f = fmaf(ds2, LA[128 + (2 * 2 + set) * 64 + c_me], f);
-                        if (fl1) f = fmaf(ds1, LA[128 + (3 * 2 + set) * 64 + c_me], f);
                         const int sp = S[static_cast<int64_t>(cprev) * n + i];
                         const int sn = S[static_cast<int64_t>(cnext) * n + i];
+                        float f = LA[set * 64 + c_me];
+                        const float jc0 = LA[128 + (0 * 2 + set) * 64 + c_me];
+                        const float jc1 = LA[128 + (1 * 2 + set) * 64 + c_me];
+                        const float jc2 = LA[128 + (2 * 2 + set) * 64 + c_me];
+                        const float jc3 = LA[128 + (3 * 2 + set) * 64 + c_me];
+                        f = fl4 ? fmaf(ds4, jc0, f) : f;
+                        f = fl3 ? fmaf(ds3, jc1, f) : f;
+                        f = fl2 ? fmaf(ds2, jc2, f) : f;
+                        f = fl1 ? fmaf(ds1, jc3, f) : f;
                         const float fs = static_cast<float>(s);
                         const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(sp + sn));
                         acc = de <= 0.0f || u01(word) < __expf(-de);
@@ -1223,7 +1219,20 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                 const int t_l = lane % P;
                 const bool ch = lane < G;
                 const int cc = ch ? lane : 0;
+                uint32_t *RWh = reinterpret_cast<uint32_t *>(LA + 900);
                 auto helper = [&](const int tau, Ahead &h, const int set) {
+                    // Philox block b of chain = lane for steps 4b .. 4b + 3: block 0 at tau = -3,
+                    // block b + 1 at tau = 4b, into the word set b & 1 the decisions of those
+                    // steps read (the decisions of steps 4b .. 4b + 3 read the other set)
+                    if (tau == -3 || (tau >= 0 && (tau & 3) == 0)) {
+                        const uint32_t b = tau < 0 ? 0u : static_cast<uint32_t>(tau >> 2) + 1u;
+                        const Philox4 r4 = philox4x32_10(cbase + lane, sweep_id, b, TAG_SLICE, a.k0, a.k1);
+                        uint32_t *dst = RWh + (b & 1u) * 256 + lane;
+                        dst[0] = r4.x;
+                        dst[64] = r4.y;
+                        dst[128] = r4.z;
+                        dst[192] = r4.w;
+                    }
                     const int k1 = tau + 1 - t_l;
                     if (ch && k1 >= 0 && k1 < n) {
                         LA[set * 64 + lane] = h.f;
