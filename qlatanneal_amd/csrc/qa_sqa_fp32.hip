@@ -133,8 +133,7 @@ struct SqaLds {
     unsigned long long *flipmask;  // two-barrier: [w] flips of wave w's chains; pipelined: [b] flips, [2 + b] flips from s = +1
     int *scal;         // [0]: best chain of the step
     float *junk;       // [64][4]: target of stores a lane must not make (never read)
-    float *la;         // look-ahead sweep exchange (kLaBytes): PF[2][64], JC[4][2][64], DS[2][64],
-                       // ROW[2][64], flip masks [2], Philox words RW[2][4][64]
+    float *la;         // look-ahead sweep exchange area (kLaBytes)
     const int32_t *inc_ptr, *inc_idx, *t_off, *t_vars;  // HUBO arrays (LDS copies or global)
     const float *t_coef;
     const int32_t *adj_ptr, *adj_idx;                     // sparse CSR (LDS copy or global)
@@ -153,9 +152,19 @@ __host__ __device__ constexpr size_t hubo_lds_bytes(int n, int nterms, int nv) {
 }
 
 // Flip-mask words: one per deciding wavefront (cmax / 64), at least the pipelined kernel's four.
-// LDS of the look-ahead sweep's exchange area (32-bit words: PF 2x64, JC 8x64, DS 2x64, ROW 2x64,
-// two 64-bit flip masks, then the Philox words RW 2x4x64)
-constexpr size_t kLaBytes = 896 * 4 + 16 + 512 * 4;
+// Look-ahead sweep: the helper wave loads a decision's field element kLaLead lockstep steps before
+// it (kLaLead - 1 steps for the loads to land), and the decision adds the chain's flips of the
+// kLaCorr = kLaLead + 1 steps the element does not hold yet. Exchange area in LDS, in 32-bit words:
+// PF [2][64] at 0, JC [kLaCorr][2][64], DS [2][64], ROW [2][64], two 64-bit flip masks, the Philox
+// words RW [2][4][64].
+constexpr int kLaLead = 3;
+constexpr int kLaCorr = kLaLead + 1;
+constexpr int kLaJC = 128;
+constexpr int kLaDS = kLaJC + kLaCorr * 128;
+constexpr int kLaROW = kLaDS + 128;
+constexpr int kLaFM = kLaROW + 128;
+constexpr int kLaRW = kLaFM + 4;
+constexpr size_t kLaBytes = static_cast<size_t>(kLaRW + 512) * 4;
 
 __host__ __device__ constexpr size_t flipmask_bytes(int cmax) {
     return lds_align16(static_cast<size_t>(cmax / 64 > 4 ? cmax / 64 : 4) * 8);
@@ -1129,43 +1138,52 @@ __device__ void la_group(const SqaArgs &a, const int g) {
             // Fields in global memory, spins in LDS, one wavefront of chains (chain c = lane c).
             // Roles, each running its own loop with two barriers per lockstep step tau:
             //   wave 0 (decisions): decides step tau from LDS only -- spins, the field element
-            //     PF[c] and the couplings JC0..3[c] between the site and the chain's last four
+            //     PF[c] and the couplings JC[.][c] between the site and the chain's last kLaCorr
             //     sites -- and publishes the flips (DS, ROW, flip mask) in the set tau & 1;
             //   waves 1-4 (updates, after the first barrier): finish the row updates of step
             //     tau - 1 (rows loaded during the previous update phase, the chains beyond those
             //     loaded and stored now), then load the rows of up to 8 / NJ flipped chains of
             //     step tau, which the next update phase finishes;
             //   wave 5 (helper, before the first barrier): stores into LDS, for decision tau + 1,
-            //     what it loaded at step tau - 2, then loads for decision tau + 3 the field
-            //     element and the four couplings (unconditional loads, clamped indexes).
+            //     what it loaded at step tau + 1 - kLaLead, then loads for decision tau + kLaLead
+            //     the field element and the kLaCorr couplings (unconditional loads, clamped
+            //     indexes), and evaluates the chains' Philox blocks.
             // Every flip of step s is stored during update phase s + 1, and no field element is
             // stored during a decision phase, so the element loaded at step tau holds exactly the
-            // chain's flips up to step tau - 2. Decision tau + 3 adds its flips at steps tau - 1
-            // .. tau + 2 with the fmaf the row update applies, in the same order, and so decides
-            // on the same bits as the element in memory (and as every other sweep kernel). A row
-            // update of chain c loads F[c] after the store of c's previous flip by the same lane.
-            // Three leading steps make the first loads and a trailing step without decisions
-            // lets the last flips be stored; the step count is even (the helper's registers
-            // alternate with the step parity).
+            // chain's flips up to step tau - 2. Decision tau + kLaLead adds its flips at steps
+            // tau - 1 .. tau + kLaLead - 1 with the fmaf the row update applies, in the same order,
+            // and so decides on the same bits as the element in memory (and as every other sweep
+            // kernel). A row update of chain c loads F[c] after the store of c's previous flip by
+            // the same lane. kLaLead leading steps make the first loads and trailing steps without
+            // decisions let the last flips be stored; the step count is a multiple of kLaLead - 1
+            // (the helper's register sets rotate with it).
             constexpr int kUpdT = 256;                          // threads of waves 1-4
             constexpr int kPreB = 8 / NJ;  // chains whose rows are loaded ahead (64 registers)
             constexpr int kNowB = NJ >= 4 ? 1 : 4 / NJ;  // chains per synchronous batch
+            constexpr int kSets = kLaLead - 1;             // helper register sets in flight
             float *const LA = L.la;
             const int n4 = n >> 2;
             f4 *F4g = reinterpret_cast<f4 *>(F);
             const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-            const int tend = nsteps + 1 + ((nsteps + 4) & 1);
-            // the first update phase reads the flip-mask set of step -4: no flips
-            if (tid < 2) reinterpret_cast<unsigned long long *>(LA + 896)[tid] = 0ull;
+            const int tbeg = -kLaLead;
+            const int tend = nsteps + 1 + (kSets - (nsteps + 1 - tbeg) % kSets) % kSets;
+            // the first update phase reads the flip-mask set of step tbeg - 1: no flips
+            if (tid < 2) reinterpret_cast<unsigned long long *>(LA + kLaFM)[tid] = 0ull;
             __syncthreads();
             __builtin_amdgcn_s_waitcnt(0);
             if (wv == 0) {
                 // ---- decisions --------------------------------------------------------------
-                bool fl1 = false, fl2 = false, fl3 = false, fl4 = false;
-                float ds1 = 0.0f, ds2 = 0.0f, ds3 = 0.0f, ds4 = 0.0f;
-                const uint32_t *RWd = reinterpret_cast<const uint32_t *>(LA + 900);
+                // fl[j], dsh[j]: the chain's attempt at position k - 1 - j (flipped?, -2 s_old)
+                bool fl[kLaCorr];
+                float dsh[kLaCorr];
+#pragma unroll
+                for (int j = 0; j < kLaCorr; ++j) {
+                    fl[j] = false;
+                    dsh[j] = 0.0f;
+                }
+                const uint32_t *RWd = reinterpret_cast<const uint32_t *>(LA + kLaRW);
                 int i_nx = order[0];  // the site of the next step, read one step ahead
-                for (int tau = -3; tau < tend; ++tau) {
+                for (int tau = tbeg; tau < tend; ++tau) {
                     const int set = tau & 1;
                     const int k = tau - t_me;
                     const bool active = decider && k >= 0 && k < n;
@@ -1182,49 +1200,49 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                         const int sp = S[static_cast<int64_t>(cprev) * n + i];
                         const int sn = S[static_cast<int64_t>(cnext) * n + i];
                         float f = LA[set * 64 + c_me];
-                        const float jc0 = LA[128 + (0 * 2 + set) * 64 + c_me];
-                        const float jc1 = LA[128 + (1 * 2 + set) * 64 + c_me];
-                        const float jc2 = LA[128 + (2 * 2 + set) * 64 + c_me];
-                        const float jc3 = LA[128 + (3 * 2 + set) * 64 + c_me];
-                        f = fl4 ? fmaf(ds4, jc0, f) : f;
-                        f = fl3 ? fmaf(ds3, jc1, f) : f;
-                        f = fl2 ? fmaf(ds2, jc2, f) : f;
-                        f = fl1 ? fmaf(ds1, jc3, f) : f;
+                        float jc[kLaCorr];
+#pragma unroll
+                        for (int d = 0; d < kLaCorr; ++d) jc[d] = LA[kLaJC + (d * 2 + set) * 64 + c_me];
+                        // JC slot d couples the site to position k - kLaCorr + d: oldest flip first
+#pragma unroll
+                        for (int d = 0; d < kLaCorr; ++d) {
+                            const int j = kLaCorr - 1 - d;
+                            f = fl[j] ? fmaf(dsh[j], jc[d], f) : f;
+                        }
                         const float fs = static_cast<float>(s);
                         const float de = fmaf(bs, -2.0f * fs * f, 2.0f * jp * fs * static_cast<float>(sp + sn));
                         acc = de <= 0.0f || u01(word) < __expf(-de);
-                        LA[640 + set * 64 + c_me] = -2.0f * fs;
-                        reinterpret_cast<int *>(LA + 768)[set * 64 + c_me] = i;
+                        LA[kLaDS + set * 64 + c_me] = -2.0f * fs;
+                        reinterpret_cast<int *>(LA + kLaROW)[set * 64 + c_me] = i;
                         if (acc) sc[i] = static_cast<int8_t>(-s);
-                        fl4 = fl3;
-                        ds4 = ds3;
-                        fl3 = fl2;
-                        ds3 = ds2;
-                        fl2 = fl1;
-                        ds2 = ds1;
-                        fl1 = acc;
-                        ds1 = -2.0f * fs;
+#pragma unroll
+                        for (int j = kLaCorr - 1; j > 0; --j) {
+                            fl[j] = fl[j - 1];
+                            dsh[j] = dsh[j - 1];
+                        }
+                        fl[0] = acc;
+                        dsh[0] = -2.0f * fs;
                     }
                     const unsigned long long m = __ballot(acc);
-                    if (lane == 0) reinterpret_cast<unsigned long long *>(LA + 896)[set] = m;
+                    if (lane == 0) reinterpret_cast<unsigned long long *>(LA + kLaFM)[set] = m;
                     nacc += acc ? 1ull : 0ull;
                     __syncthreads();
                     __syncthreads();
                 }
             } else if (wv == kHelperWave) {
-                // ---- helper: field elements and couplings three steps ahead ------------------
+                // ---- helper: field elements and couplings kLaLead steps ahead, Philox words ---
                 struct Ahead {
-                    float f, j0, j1, j2, j3;
+                    float f, j[kLaCorr];
                 };
                 const int t_l = lane % P;
                 const bool ch = lane < G;
                 const int cc = ch ? lane : 0;
-                uint32_t *RWh = reinterpret_cast<uint32_t *>(LA + 900);
-                auto helper = [&](const int tau, Ahead &h, const int set) {
-                    // Philox block b of chain = lane for steps 4b .. 4b + 3: block 0 at tau = -3,
-                    // block b + 1 at tau = 4b, into the word set b & 1 the decisions of those
+                uint32_t *RWh = reinterpret_cast<uint32_t *>(LA + kLaRW);
+                auto helper = [&](const int tau, Ahead &h) {
+                    // Philox block b of chain = lane for steps 4b .. 4b + 3: block 0 at the first
+                    // step, block b + 1 at tau = 4b, into the word set b & 1 the decisions of those
                     // steps read (the decisions of steps 4b .. 4b + 3 read the other set)
-                    if (tau == -3 || (tau >= 0 && (tau & 3) == 0)) {
+                    if (tau == tbeg || (tau >= 0 && (tau & 3) == 0)) {
                         const uint32_t b = tau < 0 ? 0u : static_cast<uint32_t>(tau >> 2) + 1u;
                         const Philox4 r4 = philox4x32_10(cbase + lane, sweep_id, b, TAG_SLICE, a.k0, a.k1);
                         uint32_t *dst = RWh + (b & 1u) * 256 + lane;
@@ -1233,32 +1251,31 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                         dst[128] = r4.z;
                         dst[192] = r4.w;
                     }
+                    const int set = (tau + 1) & 1;
                     const int k1 = tau + 1 - t_l;
                     if (ch && k1 >= 0 && k1 < n) {
                         LA[set * 64 + lane] = h.f;
-                        LA[128 + (0 * 2 + set) * 64 + lane] = h.j0;
-                        LA[128 + (1 * 2 + set) * 64 + lane] = h.j1;
-                        LA[128 + (2 * 2 + set) * 64 + lane] = h.j2;
-                        LA[128 + (3 * 2 + set) * 64 + lane] = h.j3;
+#pragma unroll
+                        for (int d = 0; d < kLaCorr; ++d) LA[kLaJC + (d * 2 + set) * 64 + lane] = h.j[d];
                     }
-                    const int k3 = tau + 3 - t_l;
-                    const int64_t col = order[min(max(k3, 0), n - 1)];
-                    auto jrow = [&](int d) {
-                        return static_cast<int64_t>(order[min(max(k3 - d, 0), n - 1)]) * n;
-                    };
+                    const int kq = tau + kLaLead - t_l;
+                    const int64_t col = order[min(max(kq, 0), n - 1)];
                     h.f = F[static_cast<int64_t>(cc) * n + col];
-                    h.j0 = a.pv.J32[jrow(4) + col];
-                    h.j1 = a.pv.J32[jrow(3) + col];
-                    h.j2 = a.pv.J32[jrow(2) + col];
-                    h.j3 = a.pv.J32[jrow(1) + col];
+#pragma unroll
+                    for (int d = 0; d < kLaCorr; ++d)
+                        h.j[d] = a.pv.J32[static_cast<int64_t>(order[min(max(kq - kLaCorr + d, 0), n - 1)]) * n + col];
                     __syncthreads();
                     __syncthreads();
                 };
-                Ahead ha{0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, hb{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-                for (int tau = -3; tau < tend; tau += 2) {
-                    helper(tau, ha, 0);      // tau odd: the set decision tau + 1 reads
-                    helper(tau + 1, hb, 1);
+                Ahead hs[kSets];
+#pragma unroll
+                for (int q = 0; q < kSets; ++q) {
+                    hs[q].f = 0.0f;
+#pragma unroll
+                    for (int d = 0; d < kLaCorr; ++d) hs[q].j[d] = 0.0f;
                 }
+                for (int tau = tbeg; tau < tend; tau += kSets)
+                    static_for<kSets>([&](auto q) { helper(tau + decltype(q)::value, hs[decltype(q)::value]); });
             } else {
                 // ---- row updates ------------------------------------------------------------
                 const int ut = tid - 64;  // 0 .. 255
@@ -1280,9 +1297,9 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                     f4 *frow[B];
 #pragma unroll
                     for (int q = 0; q < B; ++q) {
-                        dsv[q] = LA[640 + set * 64 + cs[q]];
+                        dsv[q] = LA[kLaDS + set * 64 + cs[q]];
                         rows[q] = reinterpret_cast<const f4 *>(
-                            a.pv.J32 + static_cast<int64_t>(reinterpret_cast<const int *>(LA + 768)[set * 64 + cs[q]]) * n);
+                            a.pv.J32 + static_cast<int64_t>(reinterpret_cast<const int *>(LA + kLaROW)[set * 64 + cs[q]]) * n);
                         frow[q] = F4g + static_cast<int64_t>(cs[q]) * n4;
                     }
                     f4 rv[B][NU], fv[B][NU];
@@ -1331,7 +1348,7 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                     static_for<kPreB>([&](auto qq) {
                         constexpr int q = decltype(qq)::value;
                         if (q < pcnt) {
-                            const float ds = LA[640 + set * 64 + pcs[q]];
+                            const float ds = LA[kLaDS + set * 64 + pcs[q]];
                             f4 *frow = F4g + static_cast<int64_t>(pcs[q]) * n4;
 #pragma unroll
                             for (int u = 0; u < NJ; ++u) {
@@ -1358,7 +1375,7 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                             pcs[q] = c;
                             pcnt = q + 1;
                             const f4 *row = reinterpret_cast<const f4 *>(
-                                a.pv.J32 + static_cast<int64_t>(reinterpret_cast<const int *>(LA + 768)[set * 64 + c]) * n);
+                                a.pv.J32 + static_cast<int64_t>(reinterpret_cast<const int *>(LA + kLaROW)[set * 64 + c]) * n);
                             const f4 *frow = F4g + static_cast<int64_t>(c) * n4;
 #pragma unroll
                             for (int u = 0; u < NJ; ++u) {
@@ -1372,12 +1389,12 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                     });
                 };
                 auto mask_of = [&](const int set) -> uint64_t {
-                    const unsigned long long m_all = reinterpret_cast<const unsigned long long *>(LA + 896)[set];
+                    const unsigned long long m_all = reinterpret_cast<const unsigned long long *>(LA + kLaFM)[set];
                     const uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
                     const uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
                     return (static_cast<uint64_t>(mhi) << 32) | mlo;
                 };
-                for (int tau = -3; tau < tend; ++tau) {
+                for (int tau = tbeg; tau < tend; ++tau) {
                     const int set = tau & 1;
                     __syncthreads();
                     if (nu_w > 0) {
