@@ -235,7 +235,7 @@ def run_c3(steps, warmup, cpu=True):
                   "device_ms": float(r["kernel_ms"]), "sweep_launch_ms_mean": launch_s * 1e3,
                   "field_drift_before_after_refresh": r["field_drift"].tolist(),
                   "best_energy_per_spin": best / n, "accept_rate": float(r["accept_rate"])},
-                 _roof(bytes_per_launch, launch_s, traffic_from_profile("sqa_fp32_kernel_c3_traffic.md"),
+                 _roof(bytes_per_launch, launch_s, traffic_from_profile("sqa_la_kernel_c3_traffic.md"),
                        lockstep=spb * (n + P - 1)))
     if cpu:
         from oracle.orc import Problem
