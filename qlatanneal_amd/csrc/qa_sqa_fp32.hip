@@ -1159,7 +1159,7 @@ __device__ void la_group(const SqaArgs &a, const int g) {
             // (the helper's register sets rotate with it).
             constexpr int kUpdT = 256;                          // threads of waves 1-4
             constexpr int kPreB = 8 / NJ;  // chains whose rows are loaded ahead (64 registers)
-            constexpr int kNowB = NJ >= 4 ? 1 : 4 / NJ;  // chains per synchronous batch
+            constexpr int kNowB = NJ >= 8 ? 1 : 8 / NJ;  // chains per synchronous batch
             constexpr int kSets = kLaLead - 1;             // helper register sets in flight
             float *const LA = L.la;
             const int n4 = n >> 2;
