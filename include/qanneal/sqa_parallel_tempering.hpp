@@ -41,11 +41,15 @@ public:
                                    std::size_t swap_interval = 1, std::size_t cluster_sweeps = 0,
                                    std::size_t continuous_time_slices = 0);
 
-    // Independent ladders in one device launch (FP32); element l is ladder l's result.
+    // Independent ladders in one device launch (FP32); element l is ladder l's result. With
+    // step_betas / step_gammas (steps rows of one (beta, Gamma) pair per rung) every PT step runs
+    // its own row (an annealed ladder); empty keeps the constructor's ladder at every step.
     std::vector<SQAParallelTemperingResult> run_ladders(std::size_t ladders, std::size_t sweeps_per_step,
                                                         std::size_t worldline_sweeps, std::size_t steps,
                                                         std::size_t swap_interval = 1,
-                                                        std::size_t cluster_sweeps = 0);
+                                                        std::size_t cluster_sweeps = 0,
+                                                        const std::vector<std::vector<double>> &step_betas = {},
+                                                        const std::vector<std::vector<double>> &step_gammas = {});
 
     // Adaptive shared-J_perp schedule (sqa_parallel_tempering.cpp:348-882) on the device.
     SQAParallelTemperingResult run_optimal(std::size_t num_steps, std::size_t sweeps_per_step,

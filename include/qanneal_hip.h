@@ -229,6 +229,14 @@ typedef struct qa_sqapt_params {
     const double *swap_uniforms; /* validation: master-engine uniform stream for swaps       */
     int64_t swap_uniforms_len;
     int64_t refresh_interval;    /* FP32: as in qa_sqa_params (sweeps between field refreshes) */
+    /* FP32 mode, optional (NULL: the fixed ladder betas/gammas at every step): a ladder per PT
+     * step, [steps*rungs] each. Rung r sweeps step k at (step_betas[k*rungs + r],
+     * step_gammas[k*rungs + r]) and the exchange after step k is decided with the same pairs, so
+     * every rung anneals (an annealed ladder, e.g. on the schedule qa_chi_schedule returns) while
+     * replicas still move between neighbouring rungs. The reference's ladder is fixed for a run
+     * (sqa_parallel_tempering.cpp:97-200); with every row equal to betas/gammas the run is
+     * bit-identical to the fixed-ladder run. betas/gammas must still hold a valid ladder. */
+    const double *step_betas, *step_gammas;
 } qa_sqapt_params;
 
 typedef struct qa_sqapt_result {
@@ -365,6 +373,16 @@ typedef struct qa_chi_result {
 
 qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm,
                          qa_chi_result *res);
+
+/* Stages 1-2 of run_chi without the production anneal (sqa_chi_annealer.cpp:146-330), FP32 mode:
+ * the chi_B profile (chi_override, else the surrogate, else the pilot scan, exactly as
+ * qa_sqa_chi_run chooses it) and the beta/gamma/s schedules built from it, bit-identical to the
+ * schedules qa_sqa_chi_run returns for the same prm. Fills beta_schedule, gamma_schedule,
+ * s_schedule, scan_s, scan_gamma, scan_chi_B, scalars and the optional kernel_ms (pilot time),
+ * surrogate_used and surrogate_spread; best_state, best_energy and energy_trace are not used and
+ * may be NULL. For callers that anneal on that schedule themselves (an SQA-PT ladder per step,
+ * qa_sqapt_params.step_betas / step_gammas). */
+qa_status qa_chi_schedule(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
 
 /* Stage 1 of run_chi alone (the pilot chi_B scan, sqa_chi_annealer.cpp:232-281), FP32 mode: the
  * profile the production schedule is built from, and the training label of the surrogate.

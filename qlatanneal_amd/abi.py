@@ -55,7 +55,8 @@ class SqaptParams(C.Structure):
                 ("trotter_slices", _i64), ("sweeps_per_step", _i64), ("worldline_sweeps", _i64),
                 ("cluster_sweeps", _i64), ("steps", _i64), ("swap_interval", _i64), ("mode", _i32),
                 ("seed", _u64), ("init_spins", _bp), ("rung_seeds", _up), ("swap_uniforms", _dp),
-                ("swap_uniforms_len", _i64), ("refresh_interval", _i64)]
+                ("swap_uniforms_len", _i64), ("refresh_interval", _i64), ("step_betas", _dp),
+                ("step_gammas", _dp)]
 
 
 class SqaptResult(C.Structure):
@@ -121,7 +122,8 @@ def lib() -> C.CDLL:
         for name in ("qa_ctx_create", "qa_ctx_destroy", "qa_problem_dense", "qa_problem_sparse",
                      "qa_problem_hubo", "qa_problem_destroy", "qa_energy", "qa_local_fields", "qa_sa_run", "qa_sqa_run",
                      "qa_sqa_run_noalloc",
-                     "qa_sqapt_run", "qa_sqa_chi_run", "qa_sqa_batch_run", "qa_chi_pilot", "qa_chi_features",
+                     "qa_sqapt_run", "qa_sqa_chi_run", "qa_chi_schedule", "qa_sqa_batch_run", "qa_chi_pilot",
+                     "qa_chi_features",
                      "qa_mlp_create", "qa_mlp_destroy", "qa_mlp_info", "qa_mlp_get_state", "qa_mlp_set_state",
                      "qa_mlp_forward", "qa_mlp_loss_grad", "qa_mlp_train"):
             getattr(L, name).restype = C.c_int
@@ -310,7 +312,9 @@ def sqa_run_noalloc(prob: DeviceProblem, betas, gammas, trotter_slices: int, rep
 def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_per_step: int,
               worldline_sweeps: int, steps: int, swap_interval: int = 1, cluster_sweeps: int = 0,
               ladders: int = 1, mode: int = MODE_FP32, seed: int = 0, init_spins=None, rung_seeds=None,
-              swap_uniforms=None, refresh_interval: int = 0):
+              swap_uniforms=None, refresh_interval: int = 0, step_betas=None, step_gammas=None):
+    """SQA parallel tempering (qa_sqapt_run). step_betas / step_gammas ([steps, rungs], FP32 mode)
+    give every PT step its own ladder; None keeps betas/gammas fixed."""
     betas = np.ascontiguousarray(betas, np.float64)
     gammas = np.ascontiguousarray(gammas, np.float64)
     R, n = betas.shape[0], prob.n
@@ -318,6 +322,16 @@ def sqapt_run(prob: DeviceProblem, betas, gammas, trotter_slices: int, sweeps_pe
                     cluster_sweeps, steps, swap_interval, mode, seed & (2**64 - 1))
     p.refresh_interval = refresh_interval
     keep = []
+    if (step_betas is None) != (step_gammas is None):
+        raise ValueError("step_betas and step_gammas go together")
+    if step_betas is not None:
+        step_betas = np.ascontiguousarray(step_betas, np.float64)
+        step_gammas = np.ascontiguousarray(step_gammas, np.float64)
+        if step_betas.shape != (steps, R) or step_gammas.shape != (steps, R):
+            raise ValueError(f"step_betas / step_gammas must have shape ({steps}, {R})")
+        keep += [step_betas, step_gammas]
+        p.step_betas = _d(step_betas)
+        p.step_gammas = _d(step_gammas)
     if init_spins is not None:
         init_spins = np.ascontiguousarray(init_spins, np.int8)
         keep.append(init_spins)
@@ -382,6 +396,36 @@ def sqa_chi_run(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: f
                 s_schedule=ssch, scan_s=ss, scan_gamma=sg, scan_chi_B=sc, s_star=sc5[0], gamma_star=sc5[1],
                 j_perp_star=sc5[2], chi_floor=sc5[3], driver_A0=sc5[4], kernel_ms=float(ms[0]),
                 surrogate_used=bool(used[0]), surrogate_spread=float(spread[0]))
+
+
+def chi_schedule(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: float, gamma_start: float,
+                 gamma_end: float, num_steps: int = 100, scan_points: int = 16, scan_sweeps: int = 30,
+                 scan_burn: int = 10, chi_floor_fraction: float = 1e-6, driver_A0: float = 0.0,
+                 beta_ramp_fraction: float = 0.3, beta_ramp_start: float = 0.1, seed: int = 0, chi_override=None,
+                 surrogate: "Optional[Mlp]" = None, surrogate_max_spread: float = 0.05):
+    """run_chi's chi_B profile and schedules without its production anneal (qa_chi_schedule, FP32):
+    the beta/gamma/s schedules sqa_chi_run would anneal on for the same arguments."""
+    p = ChiParams(trotter_slices, replicas, beta, gamma_start, gamma_end, num_steps, 1, scan_points, scan_sweeps,
+                  scan_burn, chi_floor_fraction, driver_A0, beta_ramp_fraction, beta_ramp_start, MODE_FP32,
+                  seed & (2**64 - 1))
+    keep = []
+    if chi_override is not None:
+        chi_override = np.ascontiguousarray(chi_override, np.float64)
+        keep.append(chi_override)
+        p.chi_override = _d(chi_override)
+    if surrogate is not None:
+        p.surrogate = surrogate.h
+        p.surrogate_max_spread = float(surrogate_max_spread)
+    bsch, gsch, ssch = (np.zeros(num_steps) for _ in range(3))
+    ss, sg, sc = (np.zeros(scan_points) for _ in range(3))
+    sc5, ms = np.zeros(5), np.zeros(1)
+    used, spread = np.zeros(1, np.int32), np.zeros(1)
+    r = ChiResult(None, None, None, _d(bsch), _d(gsch), _d(ssch), _d(ss), _d(sg), _d(sc), _d(sc5), _d(ms),
+                  used.ctypes.data_as(C.POINTER(_i32)), _d(spread))
+    check(lib().qa_chi_schedule(prob.ctx.h, prob.h, C.byref(p), C.byref(r)))
+    return dict(beta_schedule=bsch, gamma_schedule=gsch, s_schedule=ssch, scan_s=ss, scan_gamma=sg, scan_chi_B=sc,
+                s_star=sc5[0], gamma_star=sc5[1], j_perp_star=sc5[2], chi_floor=sc5[3], driver_A0=sc5[4],
+                kernel_ms=float(ms[0]), surrogate_used=bool(used[0]), surrogate_spread=float(spread[0]))
 
 
 def chi_pilot(prob: DeviceProblem, trotter_slices: int, replicas: int, beta: float, gamma_start: float,

@@ -375,7 +375,8 @@ PYBIND11_MODULE(_qanneal, m) {
              py::arg("continuous_time_slices") = 0, py::call_guard<py::gil_scoped_release>())
         .def("run_ladders", &SQAParallelTemperingAnnealer::run_ladders, py::arg("ladders"),
              py::arg("sweeps_per_step"), py::arg("worldline_sweeps"), py::arg("steps"), py::arg("swap_interval") = 1,
-             py::arg("cluster_sweeps") = 0, py::call_guard<py::gil_scoped_release>())
+             py::arg("cluster_sweeps") = 0, py::arg("step_betas") = std::vector<std::vector<double>>{},
+             py::arg("step_gammas") = std::vector<std::vector<double>>{}, py::call_guard<py::gil_scoped_release>())
         .def("run_optimal", &SQAParallelTemperingAnnealer::run_optimal, py::arg("num_steps"),
              py::arg("sweeps_per_step"), py::arg("worldline_sweeps") = 0, py::arg("eps_tilde") = 0.0,
              py::arg("alpha") = 15.0 / 14.0, py::arg("j_perp_end") = 0.0, py::arg("cluster_sweeps") = 0,

@@ -700,7 +700,8 @@ SQAParallelTemperingResult SQAParallelTemperingAnnealer::run(std::size_t sweeps_
 
 std::vector<SQAParallelTemperingResult> SQAParallelTemperingAnnealer::run_ladders(
     std::size_t ladders, std::size_t sweeps_per_step, std::size_t worldline_sweeps, std::size_t steps,
-    std::size_t swap_interval, std::size_t cluster_sweeps) {
+    std::size_t swap_interval, std::size_t cluster_sweeps, const std::vector<std::vector<double>> &step_betas,
+    const std::vector<std::vector<double>> &step_gammas) {
     if (ladders == 0) throw std::invalid_argument("ladders must be > 0.");
     if (steps == 0) throw std::invalid_argument("steps must be > 0.");
     if (swap_interval == 0) throw std::invalid_argument("swap_interval must be > 0.");
@@ -708,6 +709,19 @@ std::vector<SQAParallelTemperingResult> SQAParallelTemperingAnnealer::run_ladder
     if (hb.precision() != DevicePrecision::FP32)
         throw std::invalid_argument("run_ladders is an FP32-mode entry point");
     const std::size_t n = hb.size(), R = betas_.size(), L = ladders;
+    if (step_betas.empty() != step_gammas.empty())
+        throw std::invalid_argument("step_betas and step_gammas go together");
+    std::vector<double> sb, sg;  // [steps*R] row-major
+    if (!step_betas.empty()) {
+        if (step_betas.size() != steps || step_gammas.size() != steps)
+            throw std::invalid_argument("step_betas / step_gammas need one row per step");
+        for (std::size_t k = 0; k < steps; ++k) {
+            if (step_betas[k].size() != R || step_gammas[k].size() != R)
+                throw std::invalid_argument("every step_betas / step_gammas row needs one value per rung");
+            sb.insert(sb.end(), step_betas[k].begin(), step_betas[k].end());
+            sg.insert(sg.end(), step_gammas[k].begin(), step_gammas[k].end());
+        }
+    }
     qa_sqapt_params prm{};
     prm.rungs = static_cast<int64_t>(R);
     prm.betas = betas_.data();
@@ -721,6 +735,10 @@ std::vector<SQAParallelTemperingResult> SQAParallelTemperingAnnealer::run_ladder
     prm.swap_interval = static_cast<int64_t>(swap_interval);
     prm.mode = QA_MODE_FP32;
     prm.seed = rng_();
+    if (!sb.empty()) {
+        prm.step_betas = sb.data();
+        prm.step_gammas = sg.data();
+    }
     std::vector<int8_t> fs(L * R * n), bs(L * n);
     std::vector<double> fe(L * R), be(L), at(L * steps), st(L * steps);
     qa_sqapt_result out{fs.data(), fe.data(), bs.data(), be.data(), at.data(), st.data(), nullptr, nullptr};

@@ -436,13 +436,13 @@ void chi_pilot_profile(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *pr
     }
 }
 
-void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
-    const int64_t P = prm->trotter_slices, SP = prm->scan_points, R = prm->replicas;
+void chi_fp32_schedule(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
+    const int64_t P = prm->trotter_slices, SP = prm->scan_points;
     const ChiGrid g = make_grid(prm);
     std::vector<double> ss, sg, sjp;
     scan_grid(prm, g, P, ss, sg, sjp);
     std::vector<double> chi(SP, 0.0);
-    float ms_pilot = 0.0f, ms_prod = 0.0f;
+    float ms_pilot = 0.0f;
     int32_t used = 0;
     double spread = std::numeric_limits<double>::quiet_NaN();
     if (prm->chi_override) {
@@ -453,12 +453,21 @@ void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi
         pilot_chi(ctx, p, prm, sjp, chi, &ms_pilot);
     }
     build_schedule(prm, g, P, ss, sg, chi, res);
-    sqa_fp32_chi_production(ctx, p, P, R, prm->num_steps, res->beta_schedule, res->gamma_schedule,
-                            prm->sweeps_per_step, prm->seed ^ 0x9E3779B97F4A7C15ull, res->best_state,
-                            res->best_energy, res->energy_trace, &ms_prod);
-    if (res->kernel_ms) *res->kernel_ms = ms_pilot + ms_prod;
+    if (res->kernel_ms) *res->kernel_ms = ms_pilot;
     if (res->surrogate_used) *res->surrogate_used = used;
     if (res->surrogate_spread) *res->surrogate_spread = spread;
+}
+
+void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
+    double ms_pilot = 0.0;
+    qa_chi_result head = *res;
+    head.kernel_ms = &ms_pilot;
+    chi_fp32_schedule(ctx, p, prm, &head);
+    float ms_prod = 0.0f;
+    sqa_fp32_chi_production(ctx, p, prm->trotter_slices, prm->replicas, prm->num_steps, res->beta_schedule,
+                            res->gamma_schedule, prm->sweeps_per_step, prm->seed ^ 0x9E3779B97F4A7C15ull,
+                            res->best_state, res->best_energy, res->energy_trace, &ms_prod);
+    if (res->kernel_ms) *res->kernel_ms = static_cast<float>(ms_pilot) + ms_prod;
 }
 
 }  // namespace qa

@@ -795,6 +795,15 @@ qa_status qa_sqapt_run(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *
                    "At least one of sweeps_per_step/worldline_sweeps/cluster_sweeps must be > 0.");
         check_finite(prm->betas, prm->rungs, "betas");
         check_finite(prm->gammas, prm->rungs, "gammas");
+        QA_REQUIRE(!prm->step_betas == !prm->step_gammas, "step_betas and step_gammas go together");
+        if (prm->step_betas) {
+            if (prm->mode != QA_MODE_FP32) throw Error{QA_EUNSUPPORTED, "a per-step ladder is an FP32-mode input"};
+            check_finite(prm->step_betas, prm->steps * prm->rungs, "step_betas");
+            check_finite(prm->step_gammas, prm->steps * prm->rungs, "step_gammas");
+            for (int64_t k = 0; k < prm->steps * prm->rungs; ++k)
+                QA_REQUIRE(prm->step_betas[k] > 0.0 && prm->step_gammas[k] > 0.0,
+                           "per-step ladder betas and gammas must be > 0");
+        }
         QA_HIP(hipSetDevice(ctx->device));
         if (prm->mode == QA_MODE_FP64_VALIDATION) sqapt_validation(ctx, p, prm, res);
         else if (prm->mode == QA_MODE_FP32) sqapt_fp32(ctx, p, prm, res);
@@ -873,6 +882,29 @@ qa_status qa_sqa_chi_run(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *
         if (prm->mode == QA_MODE_FP64_VALIDATION) chi_validation(ctx, p, prm, res);
         else if (prm->mode == QA_MODE_FP32) chi_fp32(ctx, p, prm, res);
         else throw Error{QA_EINVAL, "unknown mode"};
+    });
+}
+
+qa_status qa_chi_schedule(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res) {
+    return guarded([&] {
+        RunScope scope(ctx);
+        QA_REQUIRE(ctx && p && prm && res, "qa_chi_schedule: NULL argument");
+        QA_REQUIRE(p->ctx == ctx, "qa_chi_schedule: problem belongs to another context");
+        QA_REQUIRE(res->beta_schedule && res->gamma_schedule && res->s_schedule && res->scan_s && res->scan_gamma &&
+                       res->scan_chi_B && res->scalars, "qa_chi_schedule: NULL result array");
+        if (prm->mode != QA_MODE_FP32) throw Error{QA_EUNSUPPORTED, "qa_chi_schedule is an FP32-mode entry point"};
+        QA_REQUIRE(prm->trotter_slices > 0, "trotter_slices must be > 0.");
+        QA_REQUIRE(prm->replicas > 0, "replicas must be > 0.");
+        QA_REQUIRE(prm->num_steps >= 2, "num_steps must be >= 2.");
+        QA_REQUIRE(prm->sweeps_per_step > 0, "sweeps_per_step must be > 0.");
+        QA_REQUIRE(prm->beta > 0.0, "beta must be > 0.");
+        QA_REQUIRE(prm->gamma_end > 0.0 && prm->gamma_start > prm->gamma_end, "Require gamma_start > gamma_end > 0.");
+        QA_REQUIRE(prm->scan_points >= 4, "scan_points must be >= 4.");
+        QA_REQUIRE(prm->scan_sweeps > 0, "scan_sweeps must be > 0.");
+        QA_REQUIRE(prm->scan_burn >= 0, "scan_burn must be >= 0.");
+        QA_REQUIRE(prm->chi_floor_fraction >= 0.0, "chi_floor_fraction must be >= 0.");
+        QA_HIP(hipSetDevice(ctx->device));
+        chi_fp32_schedule(ctx, p, prm, res);
     });
 }
 

@@ -284,6 +284,7 @@ void sqapt_validation(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *p
 void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa_sqapt_result *res);
 void chi_validation(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
 void chi_fp32(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
+void chi_fp32_schedule(qa_ctx *ctx, const qa_problem *p, const qa_chi_params *prm, qa_chi_result *res);
 // instance statistics from per-spin coupling sums (see qa_problem::st_*)
 struct StatsAcc {
     std::vector<double> deg, r2;

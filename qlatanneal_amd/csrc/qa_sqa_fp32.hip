@@ -2596,11 +2596,15 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     const int64_t NS = L * R;  // state slots, one replica each
     const SqaPlan pl = plan_sqa(ctx, p, NS, P);
     hipStream_t s = ctx->stream;
-    std::vector<float> hbs(R), hjp(R), hjoin(R);
-    std::vector<double> hbp(R), hjpd(R);
-    for (int64_t r = 0; r < R; ++r) {
-        hjpd[r] = trotter_coupling(prm->betas[r], prm->gammas[r], P);
-        hbp[r] = prm->betas[r] / static_cast<double>(P);
+    // ladder rows: one (fixed ladder) or one per step (step_betas / step_gammas), R pairs each
+    const int64_t rows = prm->step_betas ? steps : 1, RL = rows * R;
+    const double *lb = prm->step_betas ? prm->step_betas : prm->betas;
+    const double *lg = prm->step_gammas ? prm->step_gammas : prm->gammas;
+    std::vector<float> hbs(RL), hjp(RL), hjoin(RL);
+    std::vector<double> hbp(RL), hjpd(RL);
+    for (int64_t r = 0; r < RL; ++r) {
+        hjpd[r] = trotter_coupling(lb[r], lg[r], P);
+        hbp[r] = lb[r] / static_cast<double>(P);
         hbs[r] = static_cast<float>(hbp[r]);
         hjp[r] = static_cast<float>(hjpd[r]);
         hjoin[r] = static_cast<float>(1.0 - std::exp(-2.0 * hjpd[r]));
@@ -2611,8 +2615,8 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
             sor[l * R + r] = static_cast<int>(l * R + r);
             ros[l * R + r] = static_cast<int>(r);
         }
-    DevBuf<float> dbs(R), djp(R), djoin(R), gF(NS * P * n);
-    DevBuf<double> dbp(R), djpd(R), chain_e(NS * P), be(L), at(L * steps), st(L * steps);
+    DevBuf<float> dbs(RL), djp(RL), djoin(RL), gF(NS * P * n);
+    DevBuf<double> dbp(RL), djpd(RL), chain_e(NS * P), be(L), at(L * steps), st(L * steps);
     DevBuf<int> chain_b(NS * P), dsor(NS), dros(NS);
     DevBuf<int8_t> gS(NS * P * n), bstate(L * n);
     // accepted moves per replica group: one slot per launch when the caller asks for the rate
@@ -2623,11 +2627,11 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
     const bool mfma_refresh = p->kind == QA_KIND_DENSE && p->symmetric;
     DevBuf<double> scratch(ri > 0 ? (mfma_refresh ? nsample : nchain) * n : 0);
     QA_HIP(hipMemsetAsync(drift.ptr, 0, 2 * sizeof(unsigned long long), s));
-    dbs.upload(hbs.data(), R, s);
-    djp.upload(hjp.data(), R, s);
-    djoin.upload(hjoin.data(), R, s);
-    dbp.upload(hbp.data(), R, s);
-    djpd.upload(hjpd.data(), R, s);
+    dbs.upload(hbs.data(), RL, s);
+    djp.upload(hjp.data(), RL, s);
+    djoin.upload(hjoin.data(), RL, s);
+    dbp.upload(hbp.data(), RL, s);
+    djpd.upload(hjpd.data(), RL, s);
     dsor.upload(sor.data(), NS, s);
     dros.upload(ros.data(), NS, s);
     std::vector<double> inf(L, std::numeric_limits<double>::infinity());
@@ -2687,6 +2691,13 @@ void sqapt_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqapt_params *prm, qa
         a.resume = step > 0 ? 1 : 0;
         a.sweep_base = static_cast<uint32_t>(step) * per_step;
         a.step_base = static_cast<int>(step);
+        if (rows > 1) {
+            a.bs = dbs.ptr + step * R;
+            a.jp = djp.ptr + step * R;
+            a.join = djoin.ptr + step * R;
+            w.bp = dbp.ptr + step * R;
+            w.jpr = djpd.ptr + step * R;
+        }
         if (res->accept_rate) a.accepted = accd.ptr + step * pl.ngroups;
         if (res->sweep_ms) QA_HIP(hipEventRecord(sev[2 * step], s));
         launch_sqa(ctx, pl, a);

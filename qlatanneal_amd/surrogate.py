@@ -239,6 +239,24 @@ def chi_ladder(profile: np.ndarray, cfg: PilotConfig, rungs: int, floor_fraction
     return np.clip(g, cfg.gamma_end, cfg.gamma_start)
 
 
+def chi_annealed_ladder(beta_schedule, gamma_schedule, rungs: int, beta_ratio: float = 0.5):
+    """Per-step SQA-PT ladders that follow a run_chi schedule (qa_chi_schedule's beta and gamma per
+    step): at step k every rung shares Gamma_k, and rung r runs at beta_k * beta_ratio**((R-1-r)/(R-1)),
+    so rung R-1 anneals exactly on the schedule and the hotter rungs trail it by a geometric
+    factor down to beta_ratio. Returns (step_betas, step_gammas), each [steps, rungs], the
+    qa_sqapt_params.step_betas / step_gammas layout."""
+    if rungs < 2:
+        raise ValueError("an SQA-PT ladder needs >= 2 rungs")
+    if not 0.0 < beta_ratio <= 1.0:
+        raise ValueError("beta_ratio must lie in (0, 1]")
+    b = np.asarray(beta_schedule, np.float64)
+    g = np.asarray(gamma_schedule, np.float64)
+    if b.ndim != 1 or b.shape != g.shape:
+        raise ValueError("beta_schedule and gamma_schedule must be 1-D of equal length")
+    fr = beta_ratio ** ((rungs - 1 - np.arange(rungs)) / (rungs - 1))
+    return b[:, None] * fr[None, :], np.repeat(g[:, None], rungs, axis=1)
+
+
 def evaluate(ctx: abi.Context, sur: ChiSurrogate, specs: Sequence[InstanceSpec], num_steps: int = 100,
              sweeps_per_step: int = 10, max_spread: float = 0.05, seed: int = 12345, log=None) -> dict:
     """Production run_chi on each instance under four schedules, same production seed unless noted:

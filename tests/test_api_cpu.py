@@ -145,3 +145,24 @@ def test_classical_argument_errors():
     for args, msg in (((0, 1, 1), "sweeps_per_step"), ((1, 0, 1), "steps"), ((1, 1, 0), "swap_interval")):
         with pytest.raises(ValueError, match=msg):
             t.run(*args)
+
+
+def test_chi_annealed_ladder_layout():
+    """Per-step SQA-PT ladders from a run_chi schedule: the coldest rung anneals on the schedule,
+    the hottest trails it by beta_ratio, rungs between are geometric, Gamma is shared per step."""
+    from qlatanneal_amd.surrogate import chi_annealed_ladder
+    b = np.array([0.1, 1.0, 4.0, 16.0, 16.0])
+    g = np.array([3.0, 3.0, 1.0, 0.1, 0.001])
+    sb, sg = chi_annealed_ladder(b, g, 3, 0.25)
+    assert sb.shape == sg.shape == (5, 3)
+    assert np.array_equal(sb[:, 2], b) and np.allclose(sb[:, 0], 0.25 * b) and np.allclose(sb[:, 1], 0.5 * b)
+    assert np.all(sg == g[:, None])
+    sb1, _ = chi_annealed_ladder(b, g, 4, 1.0)
+    assert np.all(sb1 == b[:, None])
+    for bad in (dict(rungs=1), dict(beta_ratio=0.0), dict(beta_ratio=1.5)):
+        kw = dict(rungs=3, beta_ratio=0.5)
+        kw.update(bad)
+        with pytest.raises(ValueError):
+            chi_annealed_ladder(b, g, **kw)
+    with pytest.raises(ValueError):
+        chi_annealed_ladder(b, g[:-1], 3)
