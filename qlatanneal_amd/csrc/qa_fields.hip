@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -75,6 +76,51 @@ __global__ void __launch_bounds__(256) fields_mfma_kernel(const int8_t *__restri
             const int64_t i = i0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
             if (i < M) F[i * ldf + jb] = acc[v] + hj;
         }
+    }
+}
+
+// FP64 fields of `count` states for a symmetric dense J, each element in the reference's order
+// (DenseIsing::compute_local_fields, dense_ising.cpp:64-83): h_i, then + J_ij s_j for j = 0 .. n-1,
+// then - J_ii s_i. Every product is exact (s = +-1), so each element is the same sequence of FP64
+// additions as the one-state-per-workgroup fields_kernel, bit for bit. Lane = element i of a block
+// of 64; with J symmetric, term j reads J[j][i], so a wave's loads of one j are contiguous, and
+// kFieldsCG states share each load; their spins are staged in LDS as +-1.0, kFieldsJT rows at a time.
+constexpr int kFieldsCG = 8, kFieldsJT = 256;
+__global__ void __launch_bounds__(64) fields64_sym_kernel(const double *__restrict__ J64,
+                                                          const double *__restrict__ h64,
+                                                          const int8_t *__restrict__ S, int n, int64_t count,
+                                                          double *__restrict__ out) {
+    __shared__ double sd[kFieldsJT][kFieldsCG];
+    const int lane = threadIdx.x;
+    const int i = static_cast<int>(blockIdx.x) * 64 + lane;
+    const int iv = i < n ? i : n - 1;  // loads stay in the matrix; stores are guarded
+    const int64_t c0 = static_cast<int64_t>(blockIdx.y) * kFieldsCG;
+    const int nc = static_cast<int>(min<int64_t>(kFieldsCG, count - c0));
+    double acc[kFieldsCG];
+    const double h = h64[iv];
+#pragma unroll
+    for (int c = 0; c < kFieldsCG; ++c) acc[c] = h;
+    for (int jb = 0; jb < n; jb += kFieldsJT) {
+        const int jn = min(kFieldsJT, n - jb);
+        __syncthreads();
+        for (int e = lane; e < kFieldsJT * kFieldsCG; e += 64) {
+            const int jj = e / kFieldsCG, c = e - jj * kFieldsCG;
+            sd[jj][c] = (jj < jn && c < nc) ? static_cast<double>(S[(c0 + c) * n + jb + jj]) : 0.0;
+        }
+        __syncthreads();
+        const double *col = J64 + static_cast<int64_t>(jb) * n + iv;
+#pragma unroll 8
+        for (int jj = 0; jj < jn; ++jj) {
+            const double v = col[static_cast<int64_t>(jj) * n];
+#pragma unroll
+            for (int c = 0; c < kFieldsCG; ++c) acc[c] += v * sd[jj][c];
+        }
+    }
+    if (i >= n) return;
+    const double d = J64[static_cast<int64_t>(i) * n + i];
+    for (int c = 0; c < nc; ++c) {
+        const double si = static_cast<double>(S[(c0 + c) * n + i]);
+        out[(c0 + c) * n + i] = acc[c] - d * si;
     }
 }
 
@@ -156,7 +202,17 @@ void field_drift_launch(const qa_problem *p, const int8_t *d_spins, const float 
                         double *d_scratch, unsigned long long *d_max, hipStream_t s) {
     const int64_t n = p->n;
     if (count <= 0) return;
-    fields_fp64_launch(p, d_spins, d_scratch, count, s);
+    // QANNEAL_DRIFT_FIELDS=reference: the one-state-per-workgroup kernel (same values bit for bit;
+    // a test switch)
+    const char *env = std::getenv("QANNEAL_DRIFT_FIELDS");
+    if (p->kind == QA_KIND_DENSE && p->symmetric && !(env && std::strcmp(env, "reference") == 0)) {
+        const dim3 grid(static_cast<unsigned>((n + 63) / 64), static_cast<unsigned>((count + kFieldsCG - 1) / kFieldsCG));
+        hipLaunchKernelGGL(fields64_sym_kernel, grid, dim3(64), 0, s, p->J64.ptr, p->h64.ptr, d_spins,
+                           static_cast<int>(n), count, d_scratch);
+        QA_HIP(hipGetLastError());
+    } else {
+        fields_fp64_launch(p, d_spins, d_scratch, count, s);
+    }
     hipLaunchKernelGGL(drift_kernel, dim3(static_cast<unsigned>(std::min<int64_t>((count * n + 255) / 256, 1024))),
                        dim3(256), 0, s, d_fields, ldf, d_scratch, static_cast<int>(n), count, d_max);
     QA_HIP(hipGetLastError());

@@ -149,3 +149,30 @@ def test_refresh_bounds_drift_and_keeps_samples_exact(ctx):
     prob = Problem.dense(np.zeros(n), J)
     for l in range(4):
         assert p["best_energy"][l] == orc.energy(prob, p["best_state"][l])[0]
+
+
+@pytest.mark.parametrize("n,P,R", [(300, 3, 2), (2048, 64, 2)])
+def test_drift_fields_kernel_equals_one_state_per_workgroup(ctx, n, P, R, monkeypatch):
+    """The drift check's FP64 fields (fields64_sym_kernel: 64 elements x 8 states per workgroup,
+    J read down columns) are the same additions in the same order as the one-state-per-workgroup
+    kernel, so the drift values of two otherwise identical runs are bit-equal. Nonzero h and a
+    nonzero diagonal (kept in the FP64 couplings, removed from every field) exercise the first and
+    last terms; n = 300 leaves a partial element block and 6 sampled chains a partial state block."""
+    from qlatanneal_amd import abi
+    rng = np.random.default_rng(n)
+    J = _sk(rng, n) + np.diag(rng.normal(0, 1, n))
+    h = rng.normal(0, 0.3, n)
+    dp = abi.DeviceProblem.dense(ctx, h, J, 0.0)
+    betas, gammas = np.full(6, 1.5), np.linspace(2.0, 0.1, 6)
+    out = {}
+    for mode in ("reference", "tiled"):
+        if mode == "reference":
+            monkeypatch.setenv("QANNEAL_DRIFT_FIELDS", "reference")
+        else:
+            monkeypatch.delenv("QANNEAL_DRIFT_FIELDS", raising=False)
+        out[mode] = abi.sqa_run(dp, betas, gammas, P, R, 2, 0, 0, abi.MODE_FP32, 9, refresh_interval=4)
+    a, b = out["reference"], out["tiled"]
+    print("drift", a["field_drift"], b["field_drift"], flush=True)
+    assert np.all(np.isfinite(a["field_drift"])) and a["field_drift"][1] <= 2e-4
+    assert np.array_equal(a["field_drift"], b["field_drift"])
+    assert np.array_equal(a["best_state"], b["best_state"]) and a["best_energy"] == b["best_energy"]
