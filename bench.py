@@ -35,6 +35,15 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
+# Hardware queues of this process, set before anything initializes HIP. The concurrent parts of the
+# bench (the C4 sample's runs on the library's 4 contexts; the tts99 methods' 4 host threads, each
+# with a context of its own, beside the library's 4) open about ten streams; with HIP's default of
+# 4 queues, two busy streams can land on one queue and run one after the other (measured: 16
+# sqa_chi runs on 4 threads 7.2 s with 4 queues, 4.9 s with 16; one run alone 1.19 s). A larger
+# value set by the caller is kept.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 

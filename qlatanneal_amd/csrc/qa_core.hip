@@ -18,6 +18,16 @@ thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
+void ensure_dynamic_lds(const void *kern, size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void *, size_t> limit;
+    std::lock_guard<std::mutex> g(mu);
+    size_t &cur = limit[kern];
+    if (bytes <= cur) return;
+    QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    cur = bytes;
+}
+
 // ---- run-scoped device buffer cache ----------------------------------------------------------
 struct BufCache {
     std::mutex mu;

@@ -422,13 +422,11 @@ void sa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sa_params *prm, qa_sa_re
     if (wave) {
         const size_t jsh = static_cast<size_t>(n) * n * sizeof(float);
         auto kern = n <= 64 ? sa_wave_kernel<1> : sa_wave_kernel<2>;
-        QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   static_cast<int>(jsh)));
+        ensure_dynamic_lds(reinterpret_cast<const void *>(kern), jsh);
         hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((R + 3) / 4)), dim3(256), jsh, s, a);
     } else {
         auto kern = use_lds ? sa_fp32_kernel<true> : sa_fp32_kernel<false>;
-        QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   static_cast<int>(shmem)));
+        ensure_dynamic_lds(reinterpret_cast<const void *>(kern), shmem);
         const unsigned grid = static_cast<unsigned>((R + kMaxReads - 1) / kMaxReads);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), shmem, s, a);
     }

@@ -2021,7 +2021,7 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
         else
             kern = reinterpret_cast<const void *>(sqa_fp32_kernel<false, 8>);
     }
-    QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(pl.shmem)));
+    ensure_dynamic_lds(kern, pl.shmem);
     void *params[] = {&a};
     QA_HIP(hipLaunchKernel(kern, dim3(static_cast<unsigned>(pl.ngroups)), dim3(threads), params, pl.shmem,
                            ctx->stream));
@@ -3056,8 +3056,7 @@ __global__ void __launch_bounds__(256) chk_init_kernel(ProblemView pv, int chain
 void chk_init(const qa_problem *p, int64_t chains, uint32_t k0, uint32_t k1, int8_t *gS, float *gF, hipStream_t s) {
     const int n = static_cast<int>(p->n);
     const size_t shmem = static_cast<size_t>(kChkInitChains) * n;
-    QA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(chk_init_kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    ensure_dynamic_lds(reinterpret_cast<const void *>(chk_init_kernel), shmem);
     hipLaunchKernelGGL(chk_init_kernel, dim3(static_cast<unsigned>((n + 255) / 256),
                                              static_cast<unsigned>((chains + kChkInitChains - 1) / kChkInitChains)),
                        dim3(256), shmem, s, view_of(p), static_cast<int>(chains), k0, k1, gS, gF);
@@ -3181,7 +3180,7 @@ void chk_sweep(const qa_problem *p, ChkArgs &ca, int64_t chains, int P, hipStrea
     const bool staged = chk_lists_staged(p, pilot);
     const size_t shmem = chk_lds_bytes(n, rn) + (staged ? chk_list_bytes_of(p) : 0);
     QA_REQUIRE(shmem <= 160 * 1024, "chk_wave_kernel: wave state does not fit LDS");
-    QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shmem)));
+    ensure_dynamic_lds(kern, shmem);
     const ProblemView pv = view_of(p);
     ca.J4rows = pv.J32p;
     ca.adj_ptr = pv.adj_ptr;
@@ -3623,7 +3622,7 @@ void sqa_batch(qa_ctx *ctx, int64_t count, const qa_problem *const *probs, const
         for (auto &w : v) items.push_back(w.second);
         const int64_t per_cu = std::max<int64_t>(1, static_cast<int64_t>((160 * 1024) / (L.shmem + 64)));
         L.grid = static_cast<unsigned>(std::min<int64_t>(L.n, per_cu * ctx->num_cus));
-        QA_HIP(hipFuncSetAttribute(L.kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(L.shmem)));
+        ensure_dynamic_lds(L.kern, L.shmem);
         launches.push_back(L);
     }
     DevBuf<BatchItem> ditems(items.size());
