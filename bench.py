@@ -439,8 +439,8 @@ def tts99(t_read: float, success: float):
     return t_read * math.log(0.01) / math.log(1.0 - success)
 
 
-def run_tts(instances=2, reads_a=4, reads_b=8, ladders_pt=32, rungs_pt=4, gap=0.01, beta_b=16.0, workers=4,
-            seed=20261210):
+def run_tts(instances=2, reads_a=4, reads_b=8, ladders_pt=32, rungs_pt=4, pt_gamma_max=1.3, gap=0.01, beta_b=16.0,
+            workers=4, seed=20261210):
     """Time to reach, with 99 % probability, an energy within `gap` of the best FP64-verified energy
     any method found, on dense SK N=2048 instances (J_ij ~ N(0, 1/N)). Every method runs on the
     same `workers` host threads, each set up before any timing (its HIP context, device copies of
@@ -456,11 +456,12 @@ def run_tts(instances=2, reads_a=4, reads_b=8, ladders_pt=32, rungs_pt=4, gap=0.
       R=4, beta=16, 100 x 10 production sweeps on the wavefront kernel -- one read per run, runs of
       all instances concurrent on the same host threads; its time includes the surrogate or pilot.
     B_pt (batched FP32 SQA-PT with the chi_B surrogate): `ladders_pt` ladders of `rungs_pt` rungs in
-      one launch per instance, every rung annealed on the surrogate's Gamma schedule at beta=16
-      (qa_chi_schedule, no beta ramp) with its beta trailing by a geometric factor down to 0.5
-      (beta 8 -> 16), replica exchange between neighbouring rungs after each of 100 x 10 sweeps,
-      FP32 fields refreshed on the matrix cores every 100 sweeps; one read per ladder, its time
-      includes the schedule.
+      one launch per instance, every rung annealed on the surrogate's 100-step Gamma schedule at
+      beta=16 (qa_chi_schedule, no beta ramp) from its first step with Gamma <= `pt_gamma_max` on
+      (the steps above it are paramagnetic: random states already sample them, and they accept the
+      most flips), its beta trailing by a geometric factor down to 0.5 (beta 8 -> 16), replica
+      exchange between neighbouring rungs after each PT step of 10 sweeps, FP32 fields refreshed on
+      the matrix cores every 100 sweeps; one read per ladder, its time includes the schedule.
     C (B's quality reference): the same sqa_chi with the pilot scan, one run per instance.
     Every returned energy is checked against the FP64 energy of its state (host, reference order).
     Success of a read: E <= E* + gap |E*|, E* the best verified energy over A, B, B_pt and C. A
@@ -566,14 +567,16 @@ def run_tts(instances=2, reads_a=4, reads_b=8, ladders_pt=32, rungs_pt=4, gap=0.
                                num_steps=100, scan_points=cfg.scan_points, scan_sweeps=cfg.scan_sweeps,
                                scan_burn=cfg.scan_burn, driver_A0=cfg.driver_A0, beta_ramp_fraction=0.0,
                                surrogate=(local.sur.mlp if local.sur is not None else None), seed=seed + 7 * k)
-        sb, sg = S.chi_annealed_ladder(sch["beta_schedule"], sch["gamma_schedule"], rungs_pt, 0.5)
-        r = abi.sqapt_run(local.dps[k], sb[-1], sg[-1], P, 10, 0, 100, 1, 0, ladders=ladders_pt,
+        sb, sg = S.chi_annealed_ladder(sch["beta_schedule"], sch["gamma_schedule"], rungs_pt, 0.5,
+                                       gamma_max=pt_gamma_max)
+        r = abi.sqapt_run(local.dps[k], sb[-1], sg[-1], P, 10, 0, sb.shape[0], 1, 0, ladders=ladders_pt,
                           seed=seed + 13 * k, step_betas=sb, step_gammas=sg, refresh_interval=100)
-        return k, r["best_state"], r["best_energy"], bool(sch["surrogate_used"])
+        return k, r["best_state"], r["best_energy"], bool(sch["surrogate_used"]), sb.shape[0]
 
     rp, wall_p = timed_pool(ladder_pt, list(range(instances)))
     pool.shutdown()
-    ep = [(k, [verify(k, st[l], e[l]) for l in range(ladders_pt)], u) for k, st, e, u in rp]
+    ep = [(k, [verify(k, st[l], e[l]) for l in range(ladders_pt)], u) for k, st, e, u, _ in rp]
+    pt_steps = [m for *_, m in rp]
     t_read_p = wall_p / (instances * ladders_pt)
     log(f"tts B_pt: {instances * ladders_pt} ladders in {wall_p:.1f} s")
 
@@ -632,9 +635,10 @@ def run_tts(instances=2, reads_a=4, reads_b=8, ladders_pt=32, rungs_pt=4, gap=0.
         "B": (f"FP32 sqa_chi on the chi_B surrogate's schedule ("
               f"{'model ' + os.path.relpath(SURROGATE_MODEL, ROOT) if sur is not None else 'no model file: pilot scan'}"
               f"), P=32, R=4, beta={beta_b}, 100 x 10 sweeps, {reads_b} runs/instance on {workers} contexts"),
-        "B_pt": (f"batched FP32 SQA-PT annealed on the chi_B surrogate's Gamma schedule: {ladders_pt} ladders x "
-                 f"{rungs_pt} rungs (beta 8 -> 16) per launch, P=32, 100 x 10 sweeps, exchange every step, "
-                 f"MFMA field refresh every 100 sweeps"),
+        "B_pt": (f"batched FP32 SQA-PT annealed on the chi_B surrogate's Gamma schedule from Gamma <= "
+                 f"{pt_gamma_max} on: {ladders_pt} ladders x {rungs_pt} rungs (beta 8 -> 16) per launch, P=32, "
+                 f"{'/'.join(str(m) for m in pt_steps)} PT steps x 10 sweeps, exchange every step, MFMA field "
+                 f"refresh every 100 sweeps"),
         "t_read_A_s": t_read_a, "t_read_B_s": t_read_b, "t_read_B_pt_s": t_read_p,
         "tts99_A_s_median": med("tts99_A_s"), "tts99_B_s_median": med("tts99_B_s"),
         "tts99_B_pt_s_median": med("tts99_B_pt_s"),

@@ -239,12 +239,14 @@ def chi_ladder(profile: np.ndarray, cfg: PilotConfig, rungs: int, floor_fraction
     return np.clip(g, cfg.gamma_end, cfg.gamma_start)
 
 
-def chi_annealed_ladder(beta_schedule, gamma_schedule, rungs: int, beta_ratio: float = 0.5):
+def chi_annealed_ladder(beta_schedule, gamma_schedule, rungs: int, beta_ratio: float = 0.5,
+                        gamma_max: Optional[float] = None):
     """Per-step SQA-PT ladders that follow a run_chi schedule (qa_chi_schedule's beta and gamma per
     step): at step k every rung shares Gamma_k, and rung r runs at beta_k * beta_ratio**((R-1-r)/(R-1)),
     so rung R-1 anneals exactly on the schedule and the hotter rungs trail it by a geometric
-    factor down to beta_ratio. Returns (step_betas, step_gammas), each [steps, rungs], the
-    qa_sqapt_params.step_betas / step_gammas layout."""
+    factor down to beta_ratio. With gamma_max the ladder starts at the first step whose Gamma is at
+    most gamma_max (the steps before it are dropped). Returns (step_betas, step_gammas), each
+    [steps, rungs], the qa_sqapt_params.step_betas / step_gammas layout."""
     if rungs < 2:
         raise ValueError("an SQA-PT ladder needs >= 2 rungs")
     if not 0.0 < beta_ratio <= 1.0:
@@ -253,6 +255,11 @@ def chi_annealed_ladder(beta_schedule, gamma_schedule, rungs: int, beta_ratio: f
     g = np.asarray(gamma_schedule, np.float64)
     if b.ndim != 1 or b.shape != g.shape:
         raise ValueError("beta_schedule and gamma_schedule must be 1-D of equal length")
+    if gamma_max is not None:
+        below = np.nonzero(g <= gamma_max)[0]
+        if below.size == 0:
+            raise ValueError("no step of the schedule has Gamma <= gamma_max")
+        b, g = b[below[0]:], g[below[0]:]
     fr = beta_ratio ** ((rungs - 1 - np.arange(rungs)) / (rungs - 1))
     return b[:, None] * fr[None, :], np.repeat(g[:, None], rungs, axis=1)
 

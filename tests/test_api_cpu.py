@@ -166,3 +166,8 @@ def test_chi_annealed_ladder_layout():
             chi_annealed_ladder(b, g, **kw)
     with pytest.raises(ValueError):
         chi_annealed_ladder(b, g[:-1], 3)
+    # gamma_max: the ladder starts at the first step with Gamma <= gamma_max
+    sb2, sg2 = chi_annealed_ladder(b, g, 3, 0.25, gamma_max=1.0)
+    assert sb2.shape == (3, 3) and np.array_equal(sb2, sb[2:]) and np.array_equal(sg2, sg[2:])
+    with pytest.raises(ValueError):
+        chi_annealed_ladder(b, g, 3, gamma_max=1e-4)
