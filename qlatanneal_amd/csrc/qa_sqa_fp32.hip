@@ -1723,6 +1723,264 @@ __global__ void __launch_bounds__(kThreads) sqa_pipe_kernel(SqaArgs a) {
     pipe_group<NQ>(a, static_cast<int>(blockIdx.x));
 }
 
+// ---- pipelined worldline sweeps (dense couplings, LDS-resident state) ------------------------
+// The worldline move of the two-barrier kernel (sqa_annealer.cpp:294-315: every slice of a replica
+// flips site i together, accepted on beta/P times the summed classical change) with one barrier
+// per site instead of four, the same visit orders, random words and FP32 arithmetic, hence
+// bit-identical results. Roles:
+//   wave 0     decides site k for every replica of the group: lane c = chain c computes
+//              -2 s f of its slice, then every lane sums its replica's slices from LDS in slice
+//              order (the two-barrier kernel's order), so the replica's lanes reach one decision;
+//   waves 1-4  apply the field updates of site k-1 (one J row for every flipped chain, loaded
+//              once per lane), the flipped chains dealt round-robin to the four waves;
+//   wave 5     stages the J row of order position k + pfd into the LDS ring, evaluates the
+//              decisions' Philox words a block of sites ahead and takes one Fisher-Yates step of
+//              the next sweep's visit order.
+// The decision at site k needs f_c[i_k] including the flip of site k-1: the updaters skip the
+// 4-float group holding i_k (the same group for every chain) and wave 0 applies that flip's fmaf
+// to the group itself, reading J[i_{k-1}] from the ring, so no LDS word is written by two waves in
+// one step. Runs a.wl worldline sweeps per step (no slice or cluster sweeps), then the step tail.
+template <int NQ>
+__device__ void pipe_wl_group(const SqaArgs &a, const int g) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int kB = NQ >= 8 ? 1 : 8 / NQ;  // flipped chains per update batch
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n = a.pv.n, P = a.P;
+    const int r0 = g * a.rg;
+    const int nrep = min(a.rg, a.R - r0);
+    const int G = nrep * P;
+    const int ldf = a.ldf;  // == a.ldr, a multiple of 4
+    const int ngr = ldf >> 2;
+    const int W = a.ring;
+    const int kb = 64 / a.rg;  // sites per block of decision words (>= 4)
+    const SqaLds L = carve_lds(a, smem, true);
+    // decision words of the current and the next block, [block parity][replica * kb + site]
+    uint32_t *RW = reinterpret_cast<uint32_t *>(L.RSLOT);
+    float *wred = L.wred;
+    float *F = L.F;
+    int8_t *S = L.S;
+    f4 *F4 = reinterpret_cast<f4 *>(L.F);
+    const f4 *J4 = reinterpret_cast<const f4 *>(L.Jring);
+    f4 *Jring4 = reinterpret_cast<f4 *>(L.Jring);
+    unsigned long long *flipmask = L.flipmask;
+    const uint32_t cbase = static_cast<uint32_t>(r0 * P);
+
+    group_init(a, F, S, ldf, G, r0, true);
+    double gbest = (tid == 0 && a.tail_mode == 0) ? a.best_e[g] : 0.0;
+
+    const bool decider = tid < G;
+    const int c_me = decider ? tid : 0;
+    const int rr_me = c_me / P;
+
+    // wave 5: one staged row, NQ groups per lane (clamped into the row, so hpre stays in registers)
+    f4 hpre[NQ];
+    auto ring_fetch = [&](const int *ord, int pos) {
+        const f4 *row = reinterpret_cast<const f4 *>(a.pv.J32p + static_cast<int64_t>(ord[min(pos, n - 1)]) * ldf);
+        static_for<NQ>([&](auto r) { hpre[r] = row[min(lane + 64 * r, ngr - 1)]; });
+    };
+    auto ring_store = [&](int pos, int slot) {
+        if (pos >= n) return;
+        static_for<NQ>([&](auto r) {
+            const int gi = lane + 64 * r;
+            if (gi < ngr) Jring4[slot * ngr + gi] = hpre[r];
+        });
+    };
+
+    // waves 1-4: the updates of site k-1 (flip masks of parity bb, ring slot `slot`); group gskip
+    // (that of site k, or -1) is left to wave 0
+    f4 *junk4 = reinterpret_cast<f4 *>(L.junk) + lane;
+    auto apply = [&](int bb, int slot, int gskip) {
+        const unsigned long long m_all = flipmask[bb];
+        const unsigned long long u_all = flipmask[2 + bb];
+        const uint32_t mlo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all));
+        const uint32_t mhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(m_all >> 32));
+        const uint32_t ulo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u_all));
+        const uint32_t uhi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u_all >> 32));
+        const unsigned long long up = (static_cast<unsigned long long>(uhi) << 32) | ulo;
+        unsigned long long mine = ((static_cast<unsigned long long>(mhi) << 32) | mlo) & (0x1111111111111111ull << (wave - 1));
+        if (!mine) return;
+        f4 rv[NQ];
+        static_for<NQ>([&](auto r) { rv[r] = J4[slot * ngr + min(lane + 64 * r, ngr - 1)]; });
+        auto batch = [&](auto bsz) {
+            constexpr int B = decltype(bsz)::value;
+            float dsv[B];
+            int foff[B];
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                const int c = __ffsll(static_cast<long long>(mine)) - 1;
+                mine &= mine - 1ull;
+                dsv[q] = ((up >> c) & 1ull) ? -2.0f : 2.0f;
+                foff[q] = c * ngr;
+            }
+            f4 fv[B][NQ];
+            static_for<B>([&](auto q) {
+                static_for<NQ>([&](auto r) { fv[q][r] = F4[foff[q] + min(lane + 64 * r, ngr - 1)]; });
+            });
+            static_for<B>([&](auto q) {
+                static_for<NQ>([&](auto r) {
+                    const int gi = lane + 64 * r;
+                    const f4 x = rv[r], y = fv[q][r];
+                    f4 *dst = (gi < ngr && gi != gskip) ? F4 + foff[q] + gi : junk4;
+                    *dst = f4{fmaf(dsv[q], x.x, y.x), fmaf(dsv[q], x.y, y.y), fmaf(dsv[q], x.z, y.z),
+                              fmaf(dsv[q], x.w, y.w)};
+                });
+            });
+        };
+        while (mine) {
+            const int cnt = __popcll(mine);
+            if (kB >= 8 && cnt >= 8) batch(std::integral_constant<int, (kB >= 8 ? 8 : 1)>{});
+            else if (kB >= 4 && cnt >= 4) batch(std::integral_constant<int, (kB >= 4 ? 4 : 1)>{});
+            else if (kB >= 2 && cnt >= 2) batch(std::integral_constant<int, (kB >= 2 ? 2 : 1)>{});
+            else batch(std::integral_constant<int, 1>{});
+        }
+    };
+
+    uint32_t sweep_id = a.sweep_base;
+    bool have_next = false;  // order buffer cur already holds the order of sweep next_id
+    uint32_t next_id = 0;
+    int cur = 0;
+    for (int step = 0; step < a.steps; ++step) {
+        const int pidx = a.param_of_rep ? (decider ? a.param_of_rep[r0 + rr_me] : 0) : step;
+        const float bs = a.bs[pidx];
+        for (int sw = 0; sw < a.wl; ++sw, ++sweep_id) {
+            int *ocur = L.order + cur * L.obstride;
+            int *onext = L.order + (cur ^ 1) * L.obstride;
+            if (!(have_next && next_id == sweep_id)) {
+                if (tid == 0) shuffle_order(ocur, n, cbase, sweep_id, a.k0, a.k1);
+                __syncthreads();
+            }
+            const bool pre_next = sw + 1 < a.wl || step + 1 < a.steps;
+            if (pre_next)
+                for (int i = tid; i < n; i += kThreads) onext[i] = i;
+            if (wave == kHelperWave) {  // decision words of block 0: replica lane / kb, site lane % kb
+                const int rr = lane / kb, q = lane - (lane / kb) * kb;
+                RW[lane] = philox4x32_10(cbase + rr * P, sweep_id, static_cast<uint32_t>(q), TAG_WORLD, a.k0, a.k1).x;
+            }
+            // ring rows of positions 0 .. pfd-1
+            for (int e = tid; e < min(a.pfd, n) * ngr; e += kThreads) {
+                const int q = e / ngr, gi = e - q * ngr;
+                Jring4[q * ngr + gi] = reinterpret_cast<const f4 *>(a.pv.J32p + static_cast<int64_t>(ocur[q]) * ldf)[gi];
+            }
+            __syncthreads();
+
+            bool fl_prev = false;        // wave 0: the lane's replica flipped the previous site
+            float ds_prev = 0.0f;        // wave 0: -2 s_old of that flip
+            int i_cur = ocur[0];         // wave 0: site of this step
+            int sl = W - 1;              // waves 0-4: ring slot of position tau - 1
+            PhiloxRun nxt{};                   // wave 5: decision word of the next block for lane
+            int sh_ii = pre_next ? n - 1 : 0;  // wave 5: next Fisher-Yates iteration
+            Philox4 shb{0, 0, 0, 0};            // wave 5: this lane's draw block of the batch
+            int hslot = a.pfd % W;              // wave 5: slot of position tau - 1 + pfd
+            for (int tau = 0; tau <= n; ++tau) {
+                const int b = tau & 1;
+                if (wave == 0) {
+                    if (tau < n) {
+                        const int i = i_cur;
+                        const int gq = c_me * ngr + (i >> 2);
+                        const f4 fv = F4[gq];
+                        const f4 jv = J4[sl * ngr + (i >> 2)];
+                        const int8_t s = S[c_me * n + i];
+                        const int kq = tau / kb;
+                        const uint32_t word = RW[(kq & 1) * 64 + rr_me * kb + (tau - kq * kb)];
+                        const int i_nx = ocur[min(tau + 1, n - 1)];
+                        const f4 fc = f4{fmaf(ds_prev, jv.x, fv.x), fmaf(ds_prev, jv.y, fv.y), fmaf(ds_prev, jv.z, fv.z),
+                                         fmaf(ds_prev, jv.w, fv.w)};
+                        const f4 fu = fl_prev ? fc : fv;
+                        const int e = i & 3;
+                        const float f = e == 0 ? fu.x : e == 1 ? fu.y : e == 2 ? fu.z : fu.w;
+                        if (decider) wred[c_me] = -2.0f * static_cast<float>(s) * f;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        float dcl = 0.0f;  // the replica's slices in slice order
+                        const float *wr = wred + rr_me * P;
+#pragma unroll 8
+                        for (int t = 0; t < P; ++t) dcl += wr[t];
+                        const float de = bs * dcl;
+                        const bool acc = decider && (de <= 0.0f || u01(word) < __expf(-de));
+                        if (decider && fl_prev) F4[gq] = fc;
+                        if (acc) S[c_me * n + i] = static_cast<int8_t>(-s);
+                        i_cur = i_nx;
+                        ds_prev = -2.0f * static_cast<float>(s);
+                        fl_prev = acc;
+                        sl = sl + 1 == W ? 0 : sl + 1;
+                        const unsigned long long m = __ballot(acc);
+                        const unsigned long long mu = __ballot(acc && s > 0);
+                        if (tid == 0) {
+                            flipmask[b] = m;
+                            flipmask[2 + b] = mu;
+                        }
+                    }
+                } else if (wave <= 4) {
+                    if (tau > 0) apply(b ^ 1, sl, tau < n ? (ocur[tau] >> 2) : -1);
+                    sl = sl + 1 == W ? 0 : sl + 1;
+                } else {
+                    // decision words of block kq + 1 (counter (replica's chain 0, sweep, site,
+                    // TAG_WORLD)), evaluated 3+3+2+2 rounds over the first four steps of block kq
+                    {
+                        const int kq = tau / kb, q = tau - kq * kb;
+                        if (q < 4 && (kq + 1) * kb < n) {
+                            if (q == 0) {
+                                const int rr = lane / kb, qs = lane - (lane / kb) * kb;
+                                nxt.start(cbase + rr * P, sweep_id, static_cast<uint32_t>((kq + 1) * kb + qs), TAG_WORLD,
+                                          a.k0, a.k1);
+                                nxt.rounds<3>();
+                            } else if (q == 1) {
+                                nxt.rounds<3>();
+                            } else if (q == 2) {
+                                nxt.rounds<2>();
+                            } else {
+                                nxt.rounds<2>();
+                                RW[((kq + 1) & 1) * 64 + lane] = nxt.c0;
+                            }
+                        }
+                    }
+                    // store the row fetched last step, shuffle, fetch this step's row
+                    if (tau > 0) {
+                        ring_store(tau - 1 + a.pfd, hslot);
+                        hslot = hslot + 1 == W ? 0 : hslot + 1;
+                    }
+                    if (sh_ii > 0) {
+                        // iteration sh_ii of shuffle_order(onext, ..., sweep_id + 1, ...)
+                        const int it = n - 1 - sh_ii;
+                        if ((it & 255) == 0) {
+                            const int i0 = n - 1 - 4 * ((it >> 2) + lane);
+                            if (i0 > 0)
+                                shb = philox4x32_10(cbase, sweep_id + 1, static_cast<uint32_t>(i0), TAG_ORDER, a.k0, a.k1);
+                        }
+                        const int qq = it & 3;
+                        const uint32_t wv = qq == 0 ? shb.x : qq == 1 ? shb.y : qq == 2 ? shb.z : shb.w;
+                        const uint32_t d = __builtin_amdgcn_readlane(wv, (it >> 2) & 63);
+                        if (lane == 0) {
+                            const int j = static_cast<int>(below(d, static_cast<uint32_t>(sh_ii + 1)));
+                            const int tmp = onext[sh_ii];
+                            onext[sh_ii] = onext[j];
+                            onext[j] = tmp;
+                        }
+                        --sh_ii;
+                    }
+                    ring_fetch(ocur, tau + a.pfd);
+                }
+                __syncthreads();
+            }
+            have_next = pre_next;
+            if (pre_next) {
+                next_id = sweep_id + 1;
+                cur ^= 1;
+            }
+        }
+        if (a.tail_mode != 2) group_tail(a, F, S, ldf, G, r0, g, step, L.ered, L.scal, gbest);
+    }
+    group_save(a, F, S, ldf, G, r0);
+    if (tid == 0) a.accepted[g] = 0ull;  // worldline flips are not counted (as in the two-barrier kernel)
+}
+
+template <int NQ>
+__global__ void __launch_bounds__(kThreads) sqa_pipe_wl_kernel(SqaArgs a) {
+    pipe_wl_group<NQ>(a, static_cast<int>(blockIdx.x));
+}
+
 // ---- batched runs: persistent workgroups over (instance, replica group) items -----------------
 struct BatchItem {
     int inst;   // index into the per-instance SqaArgs
@@ -1982,16 +2240,28 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     // the pipelined kernel runs plain slice sweeps over dense couplings with an LDS state
     const bool pipe = pl.pipe && a.cmax == kMaxChains && a.sweep_kind == 0 && a.wl == 0 && a.cl == 0 &&
                       a.pv.kind == QA_KIND_DENSE && a.bstat == nullptr;
+    // worldline sweeps alone (the second launch of a split step, or wl-only runs) on the same
+    // layout: the pipelined worldline kernel (QANNEAL_SQA_WL_PIPE=0: the two-barrier kernel)
+    const char *wl_env = std::getenv("QANNEAL_SQA_WL_PIPE");
+    const bool wl_pipe = pl.pipe && a.cmax == kMaxChains && a.sweep_kind == 0 && a.spb == 0 && a.cl == 0 &&
+                         a.wl > 0 && a.pv.kind == QA_KIND_DENSE && a.bstat == nullptr && 64 / a.rg >= 4 &&
+                         !(wl_env && wl_env[0] == '0');
     const void *kern;
     unsigned threads = kThreads;
-    if (pipe) {
+    if (pipe || wl_pipe) {
         QA_REQUIRE(a.ring >= a.P + 2 && a.ldf == a.ldr && a.ldf % 4 == 0 && n_fits_ring(a.pv.n),
                    "pipelined sweep needs the dense row ring");
         const int ngr = a.ldf / 4;
-        kern = ngr <= 64 ? reinterpret_cast<const void *>(sqa_pipe_kernel<1>)
-             : ngr <= 128 ? reinterpret_cast<const void *>(sqa_pipe_kernel<2>)
-             : ngr <= 256 ? reinterpret_cast<const void *>(sqa_pipe_kernel<4>)
-                          : reinterpret_cast<const void *>(sqa_pipe_kernel<8>);
+        if (pipe)
+            kern = ngr <= 64 ? reinterpret_cast<const void *>(sqa_pipe_kernel<1>)
+                 : ngr <= 128 ? reinterpret_cast<const void *>(sqa_pipe_kernel<2>)
+                 : ngr <= 256 ? reinterpret_cast<const void *>(sqa_pipe_kernel<4>)
+                              : reinterpret_cast<const void *>(sqa_pipe_kernel<8>);
+        else
+            kern = ngr <= 64 ? reinterpret_cast<const void *>(sqa_pipe_wl_kernel<1>)
+                 : ngr <= 128 ? reinterpret_cast<const void *>(sqa_pipe_wl_kernel<2>)
+                 : ngr <= 256 ? reinterpret_cast<const void *>(sqa_pipe_wl_kernel<4>)
+                              : reinterpret_cast<const void *>(sqa_pipe_wl_kernel<8>);
     } else {
         const int n = a.pv.n;
         const char *env = std::getenv("QANNEAL_SQA_GLOBAL_F4");
