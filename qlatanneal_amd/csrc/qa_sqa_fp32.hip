@@ -2124,6 +2124,8 @@ struct SqaPlan {
     int cmax = kMaxChains;
     int s_lds = 0;  // global-state fields with the spins in LDS (dense float4 kernel)
     int la = 0;     // look-ahead slice sweep (s_lds, one wavefront of chains)
+    int wl_ring = 0;  // LDS state without a slice row ring: rows of a worldline-only ring that fits
+    int wlr = 0;      // this layout holds that worldline ring (a launch of worldline sweeps only)
     size_t shmem = 0;
 };
 
@@ -2174,6 +2176,13 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     pl.ngroups = (R + pl.rg - 1) / pl.rg;
     pl.shmem = fixed(pl.rg, pl.use_lds) + lds_align16(static_cast<size_t>(pl.ring) * rn * 4);
     QA_REQUIRE(pl.shmem <= lds_cap, "problem too large for the FP32 sweep kernel's LDS layout");
+    // worldline sweeps read one row per site (every slice at the same site), not one per slice:
+    // where the slice ring does not fit beside the state, a ring of 3-8 rows may
+    if (pl.use_lds && dense_ring && pl.ring == 0 && pl.cmax == kMaxChains) {
+        const size_t f = fixed(pl.rg, true);
+        const int64_t w = f + 16 < lds_cap ? std::min<int64_t>(8, static_cast<int64_t>((lds_cap - f - 16) / (static_cast<size_t>(rn) * 4))) : 0;
+        if (w >= 3) pl.wl_ring = static_cast<int>(w);
+    }
     // the coupling lists the field updates walk (sparse adjacency, HUBO incidence and terms) are
     // staged in LDS after everything above when they fit
     const char *lists_env = std::getenv("QANNEAL_SQA_LISTS_LDS");
@@ -2218,6 +2227,19 @@ SqaPlan plan_sqa(const qa_ctx *ctx, const qa_problem *p, int64_t R, int64_t P, i
     return pl;
 }
 
+// The layout of a launch of worldline sweeps alone for a plan whose slice ring does not fit: the
+// same state with the plan's worldline ring (prefetch distance ring - 2).
+SqaPlan wl_plan_of(const SqaPlan &pl) {
+    SqaPlan w = pl;
+    if (!pl.pipe && pl.wl_ring > 0) {
+        w.ring = pl.wl_ring;
+        w.pfd = pl.wl_ring - 2;
+        w.wlr = 1;
+        w.shmem = pl.shmem + lds_align16(static_cast<size_t>(pl.wl_ring) * pl.ldr * 4);
+    }
+    return w;
+}
+
 // The plan-derived fields of a launch (replica groups, LDS carve-up, row ring, staged lists): every
 // launch, the batched one included, takes them from here.
 void apply_plan(SqaArgs &a, const SqaPlan &pl) {
@@ -2243,13 +2265,15 @@ void launch_sqa(const qa_ctx *ctx, const SqaPlan &pl, const SqaArgs &args) {
     // worldline sweeps alone (the second launch of a split step, or wl-only runs) on the same
     // layout: the pipelined worldline kernel (QANNEAL_SQA_WL_PIPE=0: the two-barrier kernel)
     const char *wl_env = std::getenv("QANNEAL_SQA_WL_PIPE");
-    const bool wl_pipe = pl.pipe && a.cmax == kMaxChains && a.sweep_kind == 0 && a.spb == 0 && a.cl == 0 &&
-                         a.wl > 0 && a.pv.kind == QA_KIND_DENSE && a.bstat == nullptr && 64 / a.rg >= 4 &&
-                         !(wl_env && wl_env[0] == '0');
+    const bool wl_pipe = (pl.pipe || pl.wlr) && a.cmax == kMaxChains && a.sweep_kind == 0 && a.spb == 0 &&
+                         a.cl == 0 && a.wl > 0 && a.pv.kind == QA_KIND_DENSE && a.bstat == nullptr &&
+                         64 / a.rg >= 4 && !(wl_env && wl_env[0] == '0');
+    QA_REQUIRE(!pl.wlr || wl_pipe, "the worldline ring layout runs only the pipelined worldline kernel");
     const void *kern;
     unsigned threads = kThreads;
     if (pipe || wl_pipe) {
-        QA_REQUIRE(a.ring >= a.P + 2 && a.ldf == a.ldr && a.ldf % 4 == 0 && n_fits_ring(a.pv.n),
+        QA_REQUIRE((pipe ? a.ring >= a.P + 2 : (a.pfd >= 1 && a.ring >= a.pfd + 2)) && a.ldf == a.ldr &&
+                       a.ldf % 4 == 0 && n_fits_ring(a.pv.n),
                    "pipelined sweep needs the dense row ring");
         const int ngr = a.ldf / 4;
         if (pipe)
@@ -2353,8 +2377,13 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     // cluster and worldline sweeps and the step tail in the two-barrier kernel, the LDS state
     // passed through gS/gF. Sweep ids, Philox counters and arithmetic are those of one launch.
     const char *split_env = std::getenv("QANNEAL_SQA_SPLIT");
-    const bool split = pl.pipe && prm->sweeps_per_beta > 0 &&
+    const char *wl_env = std::getenv("QANNEAL_SQA_WL_PIPE");
+    // without the slice ring, steps split only for the pipelined worldline kernel on its own ring
+    const bool wl_split = !pl.pipe && pl.wl_ring > 0 && p->kind == QA_KIND_DENSE && pl.rg <= 16 &&
+                          prm->worldline_sweeps > 0 && prm->cluster_sweeps == 0 && !(wl_env && wl_env[0] == '0');
+    const bool split = (pl.pipe || wl_split) && prm->sweeps_per_beta > 0 &&
                        (prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0) && !(split_env && split_env[0] == '0');
+    const SqaPlan pl_rest = wl_split ? wl_plan_of(pl) : pl;
     const int64_t nacc_slots = split ? 2 * prm->steps : nchunks;
     DevBuf<int8_t> gS(R * P * n), bstate(ngroups * n);
     DevBuf<float> gF((use_lds && nchunks == 1 && !split) ? 1 : R * P * n);
@@ -2432,7 +2461,7 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
                 rest.save_fields = k + 1 < prm->steps ? 1 : 0;
                 rest.sweep_base = static_cast<uint32_t>(k * per_step + prm->sweeps_per_beta);
                 rest.accepted = accd.ptr + (2 * k + 1) * ngroups;
-                launch_sqa(ctx, pl, rest);
+                launch_sqa(ctx, pl_rest, rest);
             }
         }
         if (ci + 1 < nchunks) {  // drift of the running fields, matrix-core refresh, drift after it

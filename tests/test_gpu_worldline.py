@@ -32,7 +32,10 @@ KEYS = ("best_state", "best_energy", "energy_trace")
 # worldline-only runs, a row that ends inside a 4-float group, P = 5 / 32 / 64 (12, 2 and 1
 # replicas per group), a row of 150 groups (4 per lane)
 CASES = [(300, 16, 64, 4, 2, 3), (130, 8, 24, 0, 3, 4), (257, 16, 8, 3, 2, 3), (200, 5, 30, 2, 2, 3),
-         (96, 32, 4, 2, 2, 2), (64, 64, 2, 1, 2, 2), (600, 16, 4, 2, 2, 2), (40, 16, 3, 0, 4, 2)]
+         (96, 32, 4, 2, 2, 2), (64, 64, 2, 1, 2, 2), (600, 16, 4, 2, 2, 2), (40, 16, 3, 0, 4, 2),
+         # no slice ring beside the state: slice sweeps in the two-barrier kernel, worldline sweeps in
+         # the pipelined kernel on a ring of a few rows
+         (1100, 16, 16, 2, 2, 2), (1465, 16, 4, 2, 2, 2)]
 
 
 @pytest.mark.parametrize("n,P,R,spb,wl,steps", CASES)
