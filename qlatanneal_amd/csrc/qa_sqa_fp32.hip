@@ -2381,7 +2381,10 @@ void sqa_fp32(qa_ctx *ctx, const qa_problem *p, const qa_sqa_params *prm, qa_sqa
     // without the slice ring, steps split only for the pipelined worldline kernel on its own ring
     const bool wl_split = !pl.pipe && pl.wl_ring > 0 && p->kind == QA_KIND_DENSE && pl.rg <= 16 &&
                           prm->worldline_sweeps > 0 && prm->cluster_sweeps == 0 && !(wl_env && wl_env[0] == '0');
-    const bool split = (pl.pipe || wl_split) && prm->sweeps_per_beta > 0 &&
+    // global-state fields with the spins in LDS: the slice sweeps of a step run in the look-ahead
+    // kernel, the cluster and worldline sweeps and the tail in the one-step-prefetch kernel
+    const bool la_split = !pl.use_lds && pl.s_lds && pl.la && p->kind == QA_KIND_DENSE;
+    const bool split = (pl.pipe || wl_split || la_split) && prm->sweeps_per_beta > 0 &&
                        (prm->worldline_sweeps > 0 || prm->cluster_sweeps > 0) && !(split_env && split_env[0] == '0');
     const SqaPlan pl_rest = wl_split ? wl_plan_of(pl) : pl;
     const int64_t nacc_slots = split ? 2 * prm->steps : nchunks;

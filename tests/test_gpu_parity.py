@@ -262,11 +262,14 @@ def test_sqa_fp32_sweep_paths_bit_identical(ctx, monkeypatch):
 def test_sqa_fp32_split_step_launches_bit_identical(ctx, monkeypatch):
     """Steps with cluster or worldline sweeps run their slice sweeps in the pipelined kernel and the
     rest in the two-barrier kernel (two launches per step, state passed through global memory); the
-    result must equal the single two-barrier launch bit for bit, with and without field refreshes."""
+    result must equal the single launch bit for bit, with and without field refreshes."""
     from qlatanneal_amd import abi
     rng = np.random.default_rng(47)
+    # ... and global-state fields (spins in LDS): slice sweeps in the look-ahead kernel, the rest in
+    # the one-step-prefetch kernel (solve()'s defaults at N=2048, P=32, one replica per read)
     cases = [(200, 16, 64, 2, 1, 0, 0), (97, 5, 7, 3, 1, 1, 0), (550, 16, 64, 4, 2, 0, 0), (64, 64, 2, 1, 0, 1, 0),
-             (301, 8, 9, 2, 1, 1, 6), (1465, 16, 8, 1, 1, 0, 0)]
+             (301, 8, 9, 2, 1, 1, 6), (1465, 16, 8, 1, 1, 0, 0), (2048, 32, 1, 2, 1, 0, 0), (1024, 16, 4, 2, 1, 1, 0),
+             (2048, 32, 2, 2, 1, 0, 9)]
     for n, P, R, spb, wl, cl, ri in cases:
         print("split-step case", n, P, R, spb, wl, cl, ri, flush=True)
         J = _sk(rng, n)
