@@ -175,3 +175,28 @@ def test_annealed_ladder_sk2048_within_one_percent_of_pilot_sqa_chi(ctx):
         assert e[l] == dp.energy(r["best_state"][l])[0]
     assert e.mean() <= pilot["best_energy"] + 0.01 * abs(pilot["best_energy"])
     assert np.all(np.isfinite(r["field_drift"])) and r["field_drift"][1] <= 5e-4
+
+
+def test_step_ladder_argument_errors(ctx):
+    """A per-step ladder is an FP32 input with positive values and one row per step; the schedule
+    entry point is FP32-only."""
+    from qlatanneal_amd import abi
+    n, P, R, steps = 32, 4, 3, 4
+    dp = abi.DeviceProblem.dense(ctx, np.zeros(n), _sk(n, 1))
+    b, g = np.geomspace(0.5, 2.0, R), np.geomspace(1.0, 0.1, R)
+    sb, sg = np.tile(b, (steps, 1)), np.tile(g, (steps, 1))
+    bad = sb.copy()
+    bad[2, 1] = 0.0
+    with pytest.raises(abi.QaInvalidArgument):
+        abi.sqapt_run(dp, b, g, P, 1, 0, steps, 1, 0, seed=1, step_betas=bad, step_gammas=sg)
+    bad = sg.copy()
+    bad[1, 0] = np.nan
+    with pytest.raises(abi.QaInvalidArgument):
+        abi.sqapt_run(dp, b, g, P, 1, 0, steps, 1, 0, seed=1, step_betas=sb, step_gammas=bad)
+    with pytest.raises(ValueError):
+        abi.sqapt_run(dp, b, g, P, 1, 0, steps, 1, 0, seed=1, step_betas=sb[:-1], step_gammas=sg[:-1])
+    with pytest.raises(abi.QaError) as e:
+        abi.sqapt_run(dp, b, g, P, 1, 0, steps, 1, 0, mode=abi.MODE_FP64_VALIDATION, seed=1, step_betas=sb,
+                      step_gammas=sg, init_spins=np.ones(R * P * n, np.int8), rung_seeds=np.arange(R, dtype=np.uint64),
+                      swap_uniforms=np.full(steps * (R - 1), 0.5))
+    assert e.value.status == abi.QA_EUNSUPPORTED
