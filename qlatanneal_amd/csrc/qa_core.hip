@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -19,10 +20,13 @@ thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
 void ensure_dynamic_lds(const void *kern, size_t bytes) {
+    // the attribute is set on the current device: every device keeps its own limit per kernel
+    int dev = 0;
+    QA_HIP(hipGetDevice(&dev));
     static std::mutex mu;
-    static std::map<const void *, size_t> limit;
+    static std::map<std::pair<int, const void *>, size_t> limit;
     std::lock_guard<std::mutex> g(mu);
-    size_t &cur = limit[kern];
+    size_t &cur = limit[{dev, kern}];
     if (bytes <= cur) return;
     QA_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
     cur = bytes;

@@ -18,9 +18,10 @@ namespace qa {
 // ---- error plumbing ------------------------------------------------------------------------
 void set_error(const std::string &msg);
 // Raises a kernel's dynamic-LDS limit (hipFuncAttributeMaxDynamicSharedMemorySize) to at least
-// `bytes`. The attribute belongs to the kernel, process-wide: here it only ever grows, under one
-// lock, so runs on concurrent host threads never lower it below a size another thread is about to
-// launch with, and a launch whose size is already covered makes no runtime call.
+// `bytes` on the current device. The attribute belongs to the kernel on that device, for the whole
+// process: here it only ever grows, under one lock, so runs on concurrent host threads never lower
+// it below a size another thread is about to launch with, and a launch whose size is already
+// covered on its device makes no runtime call.
 void ensure_dynamic_lds(const void *kern, size_t bytes);
 
 struct Error {
