@@ -1289,7 +1289,7 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                 int pcs[kPreB];
                 int pcnt = 0;
                 // synchronous update of B chains (the ones beyond those loaded ahead)
-                auto upd_now = [&](auto bsz, auto nu, const int *cs, const int set) {
+                auto upd_now = [&](auto bsz, auto nu, const int *cs, const int set, const int real) {
                     constexpr int B = decltype(bsz)::value;
                     constexpr int NU = decltype(nu)::value;
                     float dsv[B];
@@ -1323,14 +1323,14 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                             // computed on every path (not only under the store's mask): the
                             // loads are then waited for here, never later
                             asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
-                            if (gi < n4) frow[q][gi] = v;
+                            if (q < real && gi < n4) frow[q][gi] = v;
                         }
                     }
                 };
-                auto upd_now_nu = [&](auto bsz, const int *cs, const int set) {
+                auto upd_now_nu = [&](auto bsz, const int *cs, const int set, const int real) {
                     static_for<NJ>([&](auto u) {
                         if (nu_w == decltype(u)::value + 1)
-                            upd_now(bsz, std::integral_constant<int, decltype(u)::value + 1>{}, cs, set);
+                            upd_now(bsz, std::integral_constant<int, decltype(u)::value + 1>{}, cs, set, real);
                     });
                 };
                 // finish the chains loaded ahead (their flips are in set `set`)
@@ -1403,18 +1403,25 @@ __device__ void la_group(const SqaArgs &a, const int g) {
                         uint64_t mp = mask_of(set ^ 1);
                         for (int q = 0; q < pcnt; ++q) mp &= mp - 1u;
                         while (mp) {
+                            // batches of 4 while at least 3 chains remain: one round trip per 4
+                            // chains; a slot beyond the remaining chains loads chain cs[0]'s rows
+                            // again and stores nothing
                             const int cnt = __popcll(mp);
                             int cs[4] = {0, 0, 0, 0};
-                            const int b = (kNowB >= 4 && cnt >= 4) ? 4 : (kNowB >= 2 && cnt >= 2) ? 2 : 1;
+                            const int b = (kNowB >= 4 && cnt >= 3) ? 4 : (kNowB >= 2 && cnt >= 2) ? 2 : 1;
+                            const int real = min(b, cnt);
                             static_for<4>([&](auto qq) {
-                                if (decltype(qq)::value < b) {
-                                    cs[decltype(qq)::value] = __builtin_ctzll(mp);
+                                constexpr int q = decltype(qq)::value;
+                                if (q < real) {
+                                    cs[q] = __builtin_ctzll(mp);
                                     mp &= mp - 1u;
+                                } else if (q < b) {
+                                    cs[q] = cs[0];
                                 }
                             });
-                            if (kNowB >= 4 && b == 4) upd_now_nu(std::integral_constant<int, (kNowB >= 4 ? 4 : 1)>{}, cs, set ^ 1);
-                            else if (kNowB >= 2 && b == 2) upd_now_nu(std::integral_constant<int, (kNowB >= 2 ? 2 : 1)>{}, cs, set ^ 1);
-                            else upd_now_nu(std::integral_constant<int, 1>{}, cs, set ^ 1);
+                            if (kNowB >= 4 && b == 4) upd_now_nu(std::integral_constant<int, (kNowB >= 4 ? 4 : 1)>{}, cs, set ^ 1, real);
+                            else if (kNowB >= 2 && b == 2) upd_now_nu(std::integral_constant<int, (kNowB >= 2 ? 2 : 1)>{}, cs, set ^ 1, real);
+                            else upd_now_nu(std::integral_constant<int, 1>{}, cs, set ^ 1, real);
                         }
                         // step tau: load ahead
                         uint64_t mc = mask_of(set);
