@@ -185,18 +185,61 @@ def run_c2(steps, warmup, cpu=True, cpu_sample_steps=8):
                  _roof(bytes_per_launch, kern_s, traffic_from_profile("sqa_pipe_kernel_c2_traffic.md"),
                        lockstep=SW * (N + P - 1)))
     if cpu:
-        from oracle.orc import Problem
-        cores = host_cores()
-        lib, kind = cpu_lib(cores)
-        p = Problem.dense(np.zeros(N), J)
-        t = time.perf_counter()
-        lib.sqa_run(p, betas[:cpu_sample_steps], gammas[:cpu_sample_steps], P, R, 1, 0, 0, 0, 1)
-        dt = time.perf_counter() - t
-        att = cpu_sample_steps * R * P * N
-        line["cpu_baseline"] = {
-            "value": att / dt, "unit": "attempts/s", "cores": cores, "kind": kind,
-            "sample": (f"same N=512 P=32 R=256 instance, first {cpu_sample_steps} of {SW} schedule steps "
+        line["cpu_baseline"] = c2_cpu_baseline(J, betas, gammas, cpu_sample_steps)
+    return line
+
+
+def c2_cpu_baseline(J, betas, gammas, sample_steps):
+    """The CPU engine on the first `sample_steps` schedule steps of the C2 instance."""
+    from oracle.orc import Problem
+    N, P, R = C2_N, C2_P, C2_R
+    cores = host_cores()
+    lib, kind = cpu_lib(cores)
+    p = Problem.dense(np.zeros(N), J)
+    t = time.perf_counter()
+    lib.sqa_run(p, betas[:sample_steps], gammas[:sample_steps], P, R, 1, 0, 0, 0, 1)
+    dt = time.perf_counter() - t
+    return {"value": sample_steps * R * P * N / dt, "unit": "attempts/s", "cores": cores, "kind": kind,
+            "sample": (f"same N=512 P=32 R=256 instance, first {sample_steps} of {C2_SWEEPS} schedule steps "
                        f"(slice sweeps + the reference's serial per-step FP64 slice energies), {dt:.1f} s")}
+
+
+def run_c2v(steps, warmup, cpu=True, sweeps=20, cpu_sample_steps=8):
+    """C2's instance and shape (dense SK N=512, P=32, 256 replicas) in the FP64 validation mode
+    (backend "hip_fp64": the reference's random draws and FP64 arithmetic, results bit-equal to
+    the reference's), over the first `sweeps` steps of C2's schedule, one sweep each. Algorithmic
+    bytes 18.5 B/attempt at N=512: FP64 fields r+w 16 B, spins r+w 2 B, FP64 J once per sweep 0.5 B."""
+    import qlatanneal_amd as qa
+    N, P, R = C2_N, C2_P, C2_R
+    attempts = sweeps * R * P * N
+    bytes_per_launch = (2 * R * P * N * 8 + 2 * R * P * N + N * N * 8) * sweeps
+    J = sk_instance(N, 20261203)
+    betas, gammas = np.full(C2_SWEEPS, C2_BETA), np.linspace(3.0, 1e-3, C2_SWEEPS)
+    ham = qa.DenseIsing(np.zeros(N), J, 0.0)
+    ann = qa.SQAAnnealer(ham, qa.SQASchedule.from_vectors(betas[:sweeps].tolist(), gammas[:sweeps].tolist()), P, R,
+                         backend="hip_fp64")
+    for w in range(warmup):
+        ann.set_seed(100 + w)
+        ann.run(1, 0)
+    kms = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ann.set_seed(1000 + k)
+        res = ann.run(1, 0)
+        kms.append(ann.last_kernel_ms())
+        log(f"c2v step {k}: kernel {kms[-1]:.1f} ms best {res.best_energy:.6f}")
+    wall = time.perf_counter() - t0
+    # outside the timed region: the returned best energy is the host FP64 energy of its state
+    checked = ham.energy([int(x) for x in res.best_state.spins]) == res.best_energy
+    kern_s = float(np.mean(kms)) / 1e3
+    line = _line(steps * attempts / wall, steps, warmup, wall,
+                 {"workload": f"sqa_dense_sk_n512_p32_r256_first{sweeps}sweeps_fp64_validation", "N": N,
+                  "trotter_slices": P, "replicas": R, "sweeps": sweeps, "beta": C2_BETA,
+                  "attempts_per_step": attempts, "kernel_ms_mean": kern_s * 1e3,
+                  "best_energy_equals_fp64_recompute": bool(checked)},
+                 _roof(bytes_per_launch, kern_s), dtype="f64")
+    if cpu:
+        line["cpu_baseline"] = c2_cpu_baseline(J, betas, gammas, cpu_sample_steps)
     return line
 
 
@@ -673,7 +716,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="default run: only the headline line (no tts99 block, no other workloads)")
-    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5", "tts"), default=None,
+    ap.add_argument("--workload", choices=("c1", "c2", "c2v", "c3", "c4", "c5", "tts"), default=None,
                     help="run one workload alone (default: the headline C3 line with tts99 and the others)")
     ap.add_argument("--c4-instances", type=int, default=20)
     ap.add_argument("--c4-streams", type=int, default=4,
@@ -712,6 +755,8 @@ def main():
         line = run_c1(args.steps, args.warmup, cpu)
     elif w == "c2":
         line = run_c2(args.steps, args.warmup, cpu, args.cpu_sample_steps)
+    elif w == "c2v":
+        line = run_c2v(args.steps, args.warmup, cpu, cpu_sample_steps=args.cpu_sample_steps)
     elif w == "c3":
         line = run_c3(args.steps, args.warmup, cpu)
     elif w == "c4":
